@@ -1,0 +1,151 @@
+/*
+ * rot2d3c.h -- C ABI of the MI355X (gfx950) implementation of the Rot2D3C
+ * pseudospectral right-hand side and its 4-stage Runge-Kutta integrator.
+ *
+ * Every entry point takes plain pointers and sizes.  Device arrays use the
+ * reference's layout for one MPI rank, so a reference array and a device
+ * buffer hold identical bytes:
+ *   spectral field : complex128, (n/2+1) x n, index [kx][ky], ky fastest, ky
+ *                    in FFT order, values unnormalised (n^2 x coefficient)
+ *                    == Fortran DOUBLE COMPLEX C(n, 1:n/2+1)
+ *                    (src/Rot2D3C.para.f90:107-117, src/fftp2D.f90:153)
+ *   real field     : float64, n x n, index [y][x], x fastest
+ *                    == Fortran DOUBLE PRECISION R(n, 1:n)
+ * Complex arrays are passed as `double *` to interleaved (re, im) pairs.
+ *
+ * All device work is asynchronous on the caller's HIP stream (passed as
+ * `void *`, i.e. a hipStream_t; NULL = the default stream).  No entry point
+ * allocates device memory except r2d3c_plan_create.  A plan is not
+ * thread-safe: use one host thread per plan.  Every function returns 0 on
+ * success, a negative R2D3C_E* code on an argument error, or
+ * -(1000 + hipError_t) when the HIP runtime fails; r2d3c_last_error()
+ * returns a description of the last failure on the calling thread.
+ */
+#ifndef ROT2D3C_H
+#define ROT2D3C_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define R2D3C_OK 0
+#define R2D3C_EINVAL (-1)     /* bad argument (size, pointer, exponent) */
+#define R2D3C_ENOTSUP (-2)    /* grid size without a compiled kernel    */
+#define R2D3C_EHIP_BASE 1000  /* -(1000 + hipError_t)                   */
+
+typedef struct r2d3c_plan r2d3c_plan;
+
+/* Dissipation and rotation parameters, parameter.inp lines 12-20
+ * (src/Rot2D3C.para.f90:147-155). */
+typedef struct r2d3c_params {
+  double nu, hnu;   /* viscosity and hypo-viscosity of psi          */
+  double nuv, hnuv; /* viscosity and hypo-viscosity of vz           */
+  double omega;     /* rotation rate                                */
+  int nn, mm;       /* orders: nu*k^(2nn), hnu*k^(-2mm) for psi     */
+  int nnv, mmv;     /* same for vz                                  */
+} r2d3c_params;
+
+/* One draw of the iflow=3 random forcing for one field, as produced by the
+ * host part of rand_force (src/pseudospec2D_Rot2D3C.f90:1086-1097):
+ * wavevector (kx, ky) rounded to integers, phase in radians, injection rate
+ * f0 (fp0 or fz0) and kupk = kup**kin.  The amplitude sqrt(2*f0)/(sqrt(dt)*kupk)
+ * is formed on the device from the step's dt.  enabled = 0 means no forcing. */
+typedef struct r2d3c_force_draw {
+  double kx, ky, phase, f0, kupk;
+  int enabled;
+} r2d3c_force_draw;
+
+typedef struct r2d3c_forcing {
+  r2d3c_force_draw psi; /* fp: kin = 1 */
+  r2d3c_force_draw vz;  /* fz: kin = 0 */
+} r2d3c_forcing;
+
+#define R2D3C_DT_FIXED 0 /* dt given by the caller (no reference counterpart) */
+#define R2D3C_DT_CFL 1   /* dt from CFL_condition on the pre-step state     */
+
+/* ---- plan ------------------------------------------------------------- */
+
+/* Creates the plan of an n x n grid on the current HIP device: twiddle tables
+ * and the workspace of the fused step.  Replaces fftp2d_create_plan
+ * (src/fftp2D.f90:30-63) and the wavenumber set-up (src/Rot2D3C.para.f90:213-228).
+ * n: power of two, 16 <= n <= 4096. */
+int r2d3c_plan_create(r2d3c_plan **plan, int n);
+/* Replaces fftp2d_destroy_plan (src/fftp2D.f90:66-86). */
+int r2d3c_plan_destroy(r2d3c_plan *plan);
+/* Device bytes held by the plan. */
+long long r2d3c_plan_workspace_bytes(const r2d3c_plan *plan);
+int r2d3c_plan_size(const r2d3c_plan *plan);
+const char *r2d3c_last_error(void);
+
+/* ---- transforms ------------------------------------------------------- */
+
+/* Unnormalised inverse 2D FFT, e^{+i}; the imaginary parts at kx = 0 and
+ * kx = n/2 are ignored.  Replaces fftp2d_complex_to_real
+ * (src/fftp2D.f90:214-290); unlike the reference the input is preserved. */
+int r2d3c_c2r(r2d3c_plan *plan, const double *in_spec, double *out_real, void *stream);
+/* Unnormalised forward 2D FFT, e^{-i}.  Replaces fftp2d_real_to_complex
+ * (src/fftp2D.f90:133-211). */
+int r2d3c_r2c(r2d3c_plan *plan, const double *in_real, double *out_spec, void *stream);
+
+/* ---- spectral operators ----------------------------------------------- */
+
+/* dir 1: i*kx*a, dir 2: i*ky*a, dir 0: copy; zero where kx^2+ky^2 > (n/3)^2.
+ * Replaces derivk2 (src/pseudospec2D_Rot2D3C.f90:6-68). */
+int r2d3c_derivk2(r2d3c_plan *plan, const double *a, double *b, int dir, void *stream);
+/* b = -k^2 a.  Replaces laplak2 (src/pseudospec2D_Rot2D3C.f90:71-95). */
+int r2d3c_laplak2(r2d3c_plan *plan, const double *a, double *b, void *stream);
+/* b = -a/k^2 (k != 0), 0 at k = 0.  Replaces inv_laplak2
+ * (src/pseudospec2D_Rot2D3C.f90:98-127). */
+int r2d3c_inv_laplak2(r2d3c_plan *plan, const double *a, double *b, void *stream);
+/* c = {a, b} = r2c[(da/dx db/dy - da/dy db/dx)/n^4] with dealiased inputs.
+ * Replaces poisson (src/pseudospec2D_Rot2D3C.f90:169-226). */
+int r2d3c_poisson(r2d3c_plan *plan, const double *a, const double *b, double *c,
+                  void *stream);
+
+/* ---- reductions (result written to device memory, deterministic) ------ */
+
+/* sum two*|a|^2*k^(2 kin)/n^4.  Replaces energy (src/pseudospec2D_Rot2D3C.f90:336-396). */
+int r2d3c_energy(r2d3c_plan *plan, const double *a, int kin, double *d_out, void *stream);
+/* sum two*k^(2 kin)*Re(b conj a)/n^4 over the dealiased disc.  Replaces
+ * inerprod (src/pseudospec2D_Rot2D3C.f90:400-451). */
+int r2d3c_inerprod(r2d3c_plan *plan, const double *a, const double *b, int kin,
+                   double *d_out, void *stream);
+
+/* ---- time step -------------------------------------------------------- */
+
+/* dt = cfl/(kcut*max|grad psi| + nu*kcut^(2nn) + nuv*kcut^(2nnv) + 2*omega),
+ * kcut = n/3.  Replaces CFL_condition (src/pseudospec2D_Rot2D3C.f90:840-899). */
+int r2d3c_cfl(r2d3c_plan *plan, const r2d3c_params *p, double cfl, const double *psi,
+              double *d_dt, void *stream);
+
+/* One time step of the 4-stage scheme, in place on psi and vz: C1=psi, C2=vz;
+ * for o = 4..1 the semi-implicit update with the Poisson brackets
+ * J(psi, lap psi), J(psi, vz), the 2*Omega*d/dy coupling and the forcing;
+ * then psi = mask(C1), vz = mask(C2).  Replaces the loop body
+ * src/Rot2D3C.para.f90:399 (CFL) and :545-611 (stages).
+ * dt_mode R2D3C_DT_CFL: dt_or_cfl is the CFL number; R2D3C_DT_FIXED: dt.
+ * The dt used is written to d_dt (device, may be NULL).  forcing may be NULL. */
+int r2d3c_step(r2d3c_plan *plan, const r2d3c_params *p, double *psi, double *vz,
+               const r2d3c_forcing *forcing, int dt_mode, double dt_or_cfl,
+               double *d_dt, void *stream);
+
+/* Dense forcing arrays fp (kin = 1) and fz (kin = 0) of one draw for a step
+ * of length dt (host value), as rand_force fills them
+ * (src/pseudospec2D_Rot2D3C.f90:1100-1122).  Either output may be NULL. */
+int r2d3c_forcing_fill(r2d3c_plan *plan, const r2d3c_forcing *forcing, double dt,
+                       double *fp, double *fz, void *stream);
+
+/* ---- host random numbers (no device work) ----------------------------- */
+
+/* Park-Miller minimal standard with the reference's XOR mask; float32 value in
+ * [-1, 1).  Replaces randu (src/pseudospec2D_mod.f90:52-71). */
+float r2d3c_randu(int *idum);
+/* The draw of rand_force: theta = randu(seed)*pi (advances *seed),
+ * phase = randu(seed+1)*pi (does not), kx/ky = floor(kup*cos/sin(theta)+0.5)
+ * (src/pseudospec2D_Rot2D3C.f90:1086-1091). */
+void r2d3c_rand_force_draw(double kup, int *seed, double *kx, double *ky, double *phase);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ROT2D3C_H */

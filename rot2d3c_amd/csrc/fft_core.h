@@ -1,0 +1,144 @@
+// fft_core.h -- FP64 radix-8 Stockham FFT building blocks for gfx950.
+//
+// One length-N transform is done by T = N/8 threads.  Between passes the data
+// goes through LDS; before the first and after the last pass it lives in
+// registers in the "canonical" distribution: thread b holds elements
+// b + T*q, q = 0..7, in v[q].  The first pass of a forward transform and the
+// last pass of an inverse one (and vice versa) therefore see the same
+// distribution, so a kernel can feed the output of one transform into the next
+// without touching LDS or global memory.
+//
+// Pass schedule: radix 8, then radix 8 while at least 16 points remain, then
+// one radix 2/4/8 pass.  2048 = 8*8*8*4, 256 = 8*8*4, 1024 = 8*8*8*2.
+// Twiddles come from a table tw[t] = exp(-2 pi i t / N) built on the host in
+// extended precision; the inverse uses its conjugate.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace r2d {
+
+typedef double2 cplx;
+
+__device__ __forceinline__ cplx mk(double x, double y) { return make_double2(x, y); }
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return mk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return mk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ cplx cconj(cplx a) { return mk(a.x, -a.y); }
+__device__ __forceinline__ cplx cscale(cplx a, double s) { return mk(a.x * s, a.y * s); }
+// a * (S i)
+template <int S> __device__ __forceinline__ cplx mul_si(cplx a) {
+  return S > 0 ? mk(-a.y, a.x) : mk(a.y, -a.x);
+}
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
+
+template <int N> struct Shape {
+  static_assert(N >= 16 && (N & (N - 1)) == 0, "N must be a power of two >= 16");
+  static constexpr int T = N / 8;                          // threads per transform
+  static constexpr int LOG = ilog2(N);
+  static constexpr int NMID = (LOG - 4) / 3;               // middle radix-8 passes
+  static constexpr int RL = 1 << (LOG - 3 - 3 * NMID);     // last radix: 2, 4 or 8
+  static constexpr int LDSN = N + N / 16;                  // padded LDS slots
+};
+
+// LDS slot of element i: one pad slot every 16 entries keeps the strided
+// Stockham writes spread over the banks.
+__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
+
+template <int S> __device__ __forceinline__ cplx twid(const cplx *__restrict__ tw, int t) {
+  cplx w = tw[t];
+  if (S > 0) w.y = -w.y;
+  return w;
+}
+
+template <int S> __device__ __forceinline__ void dft2(cplx &a, cplx &b) {
+  cplx t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <int S> __device__ __forceinline__ void dft4(cplx &x0, cplx &x1, cplx &x2, cplx &x3) {
+  cplx a0 = cadd(x0, x2), a1 = csub(x0, x2), a2 = cadd(x1, x3);
+  cplx a3 = mul_si<S>(csub(x1, x3));
+  x0 = cadd(a0, a2);
+  x2 = csub(a0, a2);
+  x1 = cadd(a1, a3);
+  x3 = csub(a1, a3);
+}
+
+// y_k = sum_m x_m exp(S 2 pi i m k / 8)
+template <int S> __device__ __forceinline__ void dft8(cplx *x) {
+  cplx e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
+  cplx o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
+  dft4<S>(e0, e1, e2, e3);
+  dft4<S>(o0, o1, o2, o3);
+  const double c = 0.70710678118654752440;
+  o1 = mk(c * (o1.x - S * o1.y), c * (o1.y + S * o1.x));   // * exp(S i pi/4)
+  o2 = mul_si<S>(o2);                                        // * exp(S i pi/2)
+  o3 = mk(c * (-o3.x - S * o3.y), c * (S * o3.x - o3.y));  // * exp(3 S i pi/4)
+  x[0] = cadd(e0, o0); x[4] = csub(e0, o0);
+  x[1] = cadd(e1, o1); x[5] = csub(e1, o1);
+  x[2] = cadd(e2, o2); x[6] = csub(e2, o2);
+  x[3] = cadd(e3, o3); x[7] = csub(e3, o3);
+}
+
+template <int R, int S> __device__ __forceinline__ void dftR(cplx *x) {
+  if constexpr (R == 2) dft2<S>(x[0], x[1]);
+  else if constexpr (R == 4) dft4<S>(x[0], x[1], x[2], x[3]);
+  else dft8<S>(x);
+}
+
+// Unnormalised DFT of length N, sign S (-1 forward, +1 inverse), of the
+// canonical register set v of thread b (0 <= b < N/8).  lds points at this
+// transform's LDS area; consecutive slots are IL entries apart, so IL
+// transforms can share one interleaved LDS array.  All threads of the block
+// must call this together (it contains block barriers); the LDS area is free
+// again on return.
+template <int N, int S, int IL>
+__device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
+                                          const cplx *__restrict__ tw) {
+  using F = Shape<N>;
+  constexpr int T = F::T;
+  // pass 1: radix 8, span 1; inputs b + m*T are exactly v[m]
+  dft8<S>(v);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) lds[pad(b * 8 + m) * IL] = v[m];
+  __syncthreads();
+  int L = 8;
+#pragma unroll
+  for (int p = 0; p < F::NMID; ++p) {
+    cplx x[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) x[m] = lds[pad(b + m * T) * IL];
+    __syncthreads();
+    const int j = b & (L - 1);
+    const int st = (N / (8 * L)) * j;
+#pragma unroll
+    for (int m = 1; m < 8; ++m) x[m] = cmul(x[m], twid<S>(tw, st * m));
+    dft8<S>(x);
+    const int base = (b - j) * 8 + j;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) lds[pad(base + m * L) * IL] = x[m];
+    __syncthreads();
+    L *= 8;
+  }
+  // last pass: radix RL, span N/RL; outputs j + m*N/RL land in canonical order
+  constexpr int RL = F::RL, U = 8 / RL;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int j = b + T * u;
+    cplx x[RL];
+#pragma unroll
+    for (int m = 0; m < RL; ++m) x[m] = lds[pad(j + m * (N / RL)) * IL];
+#pragma unroll
+    for (int m = 1; m < RL; ++m) x[m] = cmul(x[m], twid<S>(tw, j * m));
+    dftR<RL, S>(x);
+#pragma unroll
+    for (int m = 0; m < RL; ++m) v[u + m * U] = x[m];
+  }
+  __syncthreads();
+}
+
+}  // namespace r2d

@@ -1,0 +1,988 @@
+// rot2d3c.hip -- gfx950 kernels and C ABI of the Rot2D3C pseudospectral step.
+//
+// Layouts
+//   spectral field  S[i][j]   i = kx column 0..n/2, j = ky in FFT order
+//                             (reference layout, src/fftp2D.f90:153)
+//   real field      R[y][x]
+//   mixed field     M(kx, y): the column transform of a spectral field, kept
+//                   y-tiled as M[(y/4)][kx][y%4] with a column pitch kp.  A
+//                   column kernel touches 64-byte chunks (4 y of one kx) and a
+//                   row kernel reads or writes 4 whole rows as one contiguous
+//                   span.
+//
+// The fused step (r2d3c_step) uses only kx <= K = floor(n/3): every field that
+// enters a Poisson bracket is dealiased (derivk2 masks kx^2+ky^2 > (n/3)^2,
+// src/pseudospec2D_Rot2D3C.f90:35,48,58) and the stage update keeps only
+// tiny < k^2 <= kmax (src/Rot2D3C.para.f90:580), so columns kx > K are zero in
+// every mixed field and in the result.
+//
+// Per step: one column-inverse kernel, then per stage one row kernel and one
+// fused kernel (column forward + semi-implicit update + next stage's column
+// inverse, or the state write after the last stage).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rot2d3c.h"
+#include "fft_core.h"
+
+using namespace r2d;
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int kyof(int j, int n) { return j < n / 2 ? j : j - n; }
+// half-plane index i -> ka(i): 0..n/2-1, and -n/2 for i = n/2 (src/Rot2D3C.para.f90:220-223)
+__device__ __forceinline__ int kxof(int i, int n) { return i < n / 2 ? i : i - n; }
+__device__ __forceinline__ size_t tix(int y, int kx, int kp) {
+  return (size_t)(y >> 2) * (size_t)(kp * 4) + (size_t)kx * 4 + (y & 3);
+}
+// integer power by binary exponentiation (Fortran x**n with integer n >= 0)
+__host__ __device__ __forceinline__ double ipow(double x, int e) {
+  double r = 1.0;
+  while (e > 0) {
+    if (e & 1) r *= x;
+    x *= x;
+    e >>= 1;
+  }
+  return r;
+}
+// Workgroup ids are dispatched round-robin over the 8 XCDs.  Map hardware id
+// hb to logical id lb so that logical pairs (2p, 2p+1) -- neighbouring kx
+// columns whose 64-byte chunks share 128-byte lines -- run on the same XCD in
+// consecutive slots.  Requires gridDim.x % 16 == 0.
+__device__ __forceinline__ int xcd_pairs(int hb) {
+  const int x = hb & 7, s = hb >> 3;
+  return ((s >> 1) << 4) + (x << 1) + (s & 1);
+}
+__host__ __forceinline__ int round16(int x) { return (x + 15) & ~15; }
+
+// spectral operators applied on load (derivk2 / laplak2 / inv_laplak2)
+enum { OP_RAW = 0, OP_DX = 1, OP_DY = 2, OP_MASK = 3, OP_LAP = 4, OP_INVLAP = 5 };
+
+struct Geo {
+  int n;
+  int nc, kp;    // fused step: columns kx <= K, pitch (even)
+  int ncf, kpf;  // full transforms: n/2+1 columns, pitch
+  double kmax, tiny;
+};
+
+__device__ __forceinline__ cplx apply_op(int op, cplx a, int kx, int ky, double kmax,
+                                         double tiny) {
+  const double ka2 = (double)kx * kx + (double)ky * ky;
+  switch (op) {
+    case OP_DX: return ka2 <= kmax ? mk(-(double)kx * a.y, (double)kx * a.x) : mk(0.0, 0.0);
+    case OP_DY: return ka2 <= kmax ? mk(-(double)ky * a.y, (double)ky * a.x) : mk(0.0, 0.0);
+    case OP_MASK: return ka2 <= kmax ? a : mk(0.0, 0.0);
+    case OP_LAP: return mk(-ka2 * a.x, -ka2 * a.y);
+    case OP_INVLAP: return ka2 > tiny ? mk(-a.x / ka2, -a.y / ka2) : mk(0.0, 0.0);
+    default: return a;
+  }
+}
+
+// Packed half-spectrum loader for a row transform.  Two real rows a, b with
+// half spectra A, B (kx = 0..n/2) are transformed as one complex row
+// z = a + i b, whose full spectrum is Z[k] = A[k] + i B[k] (k <= n/2) and
+// Z[n-k] = conj(A[k]) + i conj(B[k]).  Imaginary parts at k = 0 and k = n/2
+// are dropped first, as FFTW's c2r does.  Columns k >= nc are zero.
+template <int N>
+__device__ __forceinline__ cplx packed_z(int k, cplx A, cplx B, bool mirrored) {
+  if (!mirrored) {
+    if (k == 0 || k == N / 2) { A.y = 0.0; B.y = 0.0; }
+    return mk(A.x - B.y, A.y + B.x);
+  }
+  return mk(A.x + B.y, B.x - A.y);
+}
+
+// ---------------------------------------------------------------------------
+// generic transforms (full kx range, for r2d3c_c2r / r2c / poisson / cfl)
+// ---------------------------------------------------------------------------
+
+// column inverse: S[i][:] (with op) -> M(i, y)
+template <int N>
+__global__ __launch_bounds__(N / 8) void k_col_inv_full(const cplx *__restrict__ spec,
+                                                         cplx *__restrict__ mix, int op,
+                                                         double scale, Geo g,
+                                                         const cplx *__restrict__ tw) {
+  constexpr int T = N / 8;
+  __shared__ cplx lds[Shape<N>::LDSN];
+  const int i = blockIdx.x, b = threadIdx.x;
+  const int kx = kxof(i, N);
+  cplx v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int j = b + T * q;
+    cplx a = spec[(size_t)i * N + j];
+    a = apply_op(op, a, kx, kyof(j, N), g.kmax, g.tiny);
+    v[q] = cscale(a, scale);
+  }
+  fft_canon<N, +1, 1>(v, lds, b, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) mix[tix(b + T * q, i, g.kpf)] = v[q];
+}
+
+// column forward: M(i, y) -> S[i][:]
+template <int N>
+__global__ __launch_bounds__(N / 8) void k_col_fwd_full(const cplx *__restrict__ mix,
+                                                         cplx *__restrict__ spec, Geo g,
+                                                         const cplx *__restrict__ tw) {
+  constexpr int T = N / 8;
+  __shared__ cplx lds[Shape<N>::LDSN];
+  const int i = blockIdx.x, b = threadIdx.x;
+  cplx v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[q] = mix[tix(b + T * q, i, g.kpf)];
+  fft_canon<N, -1, 1>(v, lds, b, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) spec[(size_t)i * N + b + T * q] = v[q];
+}
+
+// rows per block of the full-range row kernels: 2 packed transforms = 4 rows
+template <int N> struct RowFull {
+  static constexpr int P = (N / 8) * 2 <= 1024 ? 2 : 1;  // packed transforms per block
+  static constexpr int ROWS = 2 * P;
+};
+
+// row c2r: M(:, y), M(:, y+1) -> R[y][:], R[y+1][:]
+template <int N>
+__global__ __launch_bounds__(N / 8 * RowFull<N>::P) void k_row_c2r(const cplx *__restrict__ mix,
+                                                                    double *__restrict__ out,
+                                                                    Geo g,
+                                                                    const cplx *__restrict__ tw) {
+  constexpr int T = N / 8, P = RowFull<N>::P;
+  __shared__ cplx lds[P * Shape<N>::LDSN];
+  const int r = threadIdx.x % P, b = threadIdx.x / P;
+  const int y = blockIdx.x * RowFull<N>::ROWS + 2 * r;
+  cplx v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int k = b + T * q;
+    const bool mir = k > N / 2;
+    const int kk = mir ? N - k : k;
+    const cplx A = mix[tix(y, kk, g.kpf)], B = mix[tix(y + 1, kk, g.kpf)];
+    v[q] = packed_z<N>(k, A, B, mir);
+  }
+  fft_canon<N, +1, P>(v, lds + r, b, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int x = b + T * q;
+    out[(size_t)y * N + x] = v[q].x;
+    out[(size_t)(y + 1) * N + x] = v[q].y;
+  }
+}
+
+// Unpack a packed forward row transform held in LDS (canonical k order) into
+// the half spectra of its two real rows, writing kx = k for k < ncols.
+template <int N, int IL>
+__device__ __forceinline__ void unpack_store(const cplx *lds, int b, int ncols, int kp,
+                                             cplx *__restrict__ mixa, int ya,
+                                             cplx *__restrict__ mixb, int yb_) {
+  constexpr int T = N / 8;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int k = b + T * q;
+    if (k < ncols) {
+      const cplx w = lds[pad(k) * IL], wm = lds[pad((N - k) & (N - 1)) * IL];
+      const cplx A = mk(0.5 * (w.x + wm.x), 0.5 * (w.y - wm.y));
+      const cplx B = mk(0.5 * (w.y + wm.y), -0.5 * (w.x - wm.x));
+      mixa[tix(ya, k, kp)] = A;
+      mixb[tix(yb_, k, kp)] = B;
+    }
+  }
+}
+
+// row r2c: R[y][:], R[y+1][:] -> M(:, y), M(:, y+1)
+template <int N>
+__global__ __launch_bounds__(N / 8 * RowFull<N>::P) void k_row_r2c(const double *__restrict__ in,
+                                                                    cplx *__restrict__ mix, Geo g,
+                                                                    const cplx *__restrict__ tw) {
+  constexpr int T = N / 8, P = RowFull<N>::P;
+  __shared__ cplx lds[P * Shape<N>::LDSN];
+  const int r = threadIdx.x % P, b = threadIdx.x / P;
+  const int y = blockIdx.x * RowFull<N>::ROWS + 2 * r;
+  cplx v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int x = b + T * q;
+    v[q] = mk(in[(size_t)y * N + x], in[(size_t)(y + 1) * N + x]);
+  }
+  fft_canon<N, -1, P>(v, lds + r, b, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) lds[pad(b + T * q) * P + r] = v[q];
+  __syncthreads();
+  unpack_store<N, P>(lds + r, b, g.ncf, g.kpf, mix, y, mix, y + 1);
+}
+
+// rows per block of the step row kernel and the single-field full-range
+// bracket/max kernels: up to 4 (one y tile)
+template <int N> struct RowStep {
+  static constexpr int ROWS = (N / 8) * 4 <= 1024 ? 4 : ((N / 8) * 2 <= 1024 ? 2 : 1);
+};
+
+// Poisson bracket rows (full range): z1 = ax + i ay, z2 = bx + i by from four
+// mixed fields; out[y][x] = (ax*by - ay*bx)/n^4  (src/pseudospec2D_Rot2D3C.f90:195-216)
+template <int N>
+__global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_row_bracket(
+    const cplx *__restrict__ m0, const cplx *__restrict__ m1, const cplx *__restrict__ m2,
+    const cplx *__restrict__ m3, double *__restrict__ out, double n4, Geo g,
+    const cplx *__restrict__ tw) {
+  constexpr int T = N / 8, RW = RowStep<N>::ROWS;
+  __shared__ cplx lds[RW * Shape<N>::LDSN];
+  const int r = threadIdx.x % RW, b = threadIdx.x / RW;
+  const int y = blockIdx.x * RW + r;
+  cplx z1[8], z2[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int k = b + T * q;
+    const bool mir = k > N / 2;
+    const int kk = mir ? N - k : k;
+    z1[q] = packed_z<N>(k, m0[tix(y, kk, g.kpf)], m1[tix(y, kk, g.kpf)], mir);
+    z2[q] = packed_z<N>(k, m2[tix(y, kk, g.kpf)], m3[tix(y, kk, g.kpf)], mir);
+  }
+  fft_canon<N, +1, RW>(z1, lds + r, b, tw);
+  fft_canon<N, +1, RW>(z2, lds + r, b, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    double r3 = z1[q].x * z2[q].y;
+    r3 = (r3 - z1[q].y * z2[q].x) / n4;
+    out[(size_t)y * N + b + T * q] = r3;
+  }
+}
+
+// max over the grid of |z|^2 with z = c2r(m0) + i c2r(m1)  (CFL, full range)
+template <int N>
+__global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_row_gradmax(
+    const cplx *__restrict__ m0, const cplx *__restrict__ m1, unsigned long long *maxbits, Geo g,
+    const cplx *__restrict__ tw) {
+  constexpr int T = N / 8, RW = RowStep<N>::ROWS;
+  __shared__ cplx lds[RW * Shape<N>::LDSN];
+  const int r = threadIdx.x % RW, b = threadIdx.x / RW;
+  const int y = blockIdx.x * RW + r;
+  cplx z[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int k = b + T * q;
+    const bool mir = k > N / 2;
+    const int kk = mir ? N - k : k;
+    z[q] = packed_z<N>(k, m0[tix(y, kk, g.kpf)], m1[tix(y, kk, g.kpf)], mir);
+  }
+  fft_canon<N, +1, RW>(z, lds + r, b, tw);
+  double m = 0.0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) m = fmax(m, z[q].x * z[q].x + z[q].y * z[q].y);
+  for (int off = 32; off > 0; off >>= 1)
+    if (off < (int)blockDim.x) m = fmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(m));
+}
+
+// ---------------------------------------------------------------------------
+// elementwise spectral operators and reductions
+// ---------------------------------------------------------------------------
+__global__ void k_spec_op(const cplx *__restrict__ a, cplx *__restrict__ b, int op, int n,
+                          double kmax, double tiny) {
+  const size_t total = (size_t)(n / 2 + 1) * n;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    b[e] = apply_op(op, a[e], kxof(i, n), kyof(j, n), kmax, tiny);
+  }
+}
+
+// mode 0: energy(a, kin); mode 1: inerprod(a, b, kin).  Per-block partial
+// sums in a fixed order (deterministic), finished by k_reduce_finish.
+__global__ __launch_bounds__(256) void k_reduce_partial(const cplx *__restrict__ a,
+                                                         const cplx *__restrict__ bb, int mode,
+                                                         int kin, int n, double kmax, double tiny,
+                                                         double *__restrict__ partial) {
+  __shared__ double red[256];
+  const size_t total = (size_t)(n / 2 + 1) * n;
+  const double inv_n4 = 1.0 / ((double)n * n * (double)n * n);
+  double acc = 0.0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    const double kx = kxof(i, n), ky = kyof(j, n);
+    const double ka2 = kx * kx + ky * ky;
+    const double two = (i == 0) ? 1.0 : 2.0;
+    const double kp = kin >= 0 ? ipow(ka2, kin) : (ka2 > 0.0 ? 1.0 / ipow(ka2, -kin) : 0.0);
+    const cplx x = a[e];
+    if (mode == 0) {
+      if (kin >= 0 || ka2 >= tiny) acc += two * (x.x * x.x + x.y * x.y) * kp * inv_n4;
+    } else {
+      const cplx y = bb[e];
+      const bool in = ka2 <= kmax && (kin >= 0 || ka2 >= tiny);
+      if (in) acc += two * kp * (y.x * x.x + y.y * x.y) * inv_n4;  // Re(b conj(a))
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_reduce_finish(const double *__restrict__ partial, int np,
+                                                        double *__restrict__ out) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int e = threadIdx.x; e < np; e += 256) acc += partial[e];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+
+struct DtConst {
+  double cfl, kcut, nrm, a, bq, two_omega;
+};
+// dt = cfl/(kcut*sqrt(max)/nrm + nu*kcut^(2nn) + nuv*kcut^(2nnv) + 2*omega)
+// (src/pseudospec2D_Rot2D3C.f90:892-895), summed left to right as written.
+__device__ __forceinline__ double dt_from_max(const DtConst &c, unsigned long long bits) {
+  const double tmp = sqrt(__longlong_as_double((long long)bits)) / c.nrm;
+  return c.cfl / (((c.kcut * tmp + c.a) + c.bq) + c.two_omega);
+}
+__global__ void k_dt_from_max(const unsigned long long *maxbits, DtConst c, double *dt) {
+  *dt = dt_from_max(c, *maxbits);
+}
+
+// dense forcing fill (rand_force, src/pseudospec2D_Rot2D3C.f90:1100-1122)
+struct ForceEnt {
+  int i, j;         // half-plane column, full index
+  double sgn;       // +1, or -1 for the conjugate partner
+  double cph, sph;  // cos(phase), sin(phase)
+};
+struct ForceSet {
+  int nent;
+  ForceEnt e[2];
+  double num, kupk;  // sqrt(2*f0), kup**kin
+};
+__device__ __forceinline__ cplx force_value(const ForceSet &f, int e, double dt, double n2) {
+  const double tmp = f.num / (sqrt(dt) * f.kupk);
+  const double t1 = tmp * f.e[e].cph, t2 = tmp * f.e[e].sph;
+  return mk(t1 * n2, f.e[e].sgn * (t2 * n2));
+}
+__global__ void k_force_fill(cplx *__restrict__ fp, ForceSet f, double dt, int n) {
+  const size_t total = (size_t)(n / 2 + 1) * n;
+  const double n2 = (double)n * n;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    cplx v = mk(0.0, 0.0);
+    for (int q = 0; q < f.nent; ++q)
+      if (f.e[q].i == i && f.e[q].j == j) v = force_value(f, q, dt, n2);
+    fp[e] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// the fused step
+// ---------------------------------------------------------------------------
+struct StepConst {
+  Geo g;
+  size_t fs;  // complex elements per mixed field
+  double nu, hnu, nuv, hnuv, two_omega;
+  int nn, mm, nnv, mmv;
+  int cfl_mode;
+  double dt_fixed;
+  DtConst dc;
+  ForceSet fp, fz;
+  double two_omega_n2, inv_n4, n2;
+};
+
+// Column inverse of the stage-1 state.  Logical blocks [0, nc): psi column kx
+// -> M0 = psi~, M1 = d_y psi~, M2 = w~, M3 = d_y w~ with w = -k^2 psi
+// (laplak2, src/Rot2D3C.para.f90:571); blocks [ncp, ncp+nc): vz column kx ->
+// M4 = vz~, M5 = d_y vz~.  All inputs dealiased as derivk2 does.
+template <int N>
+__global__ __launch_bounds__(N / 8) void k_step_colinv(const cplx *__restrict__ psi,
+                                                        const cplx *__restrict__ vz,
+                                                        cplx *__restrict__ M, StepConst c,
+                                                        unsigned long long *maxbits,
+                                                        const cplx *__restrict__ tw) {
+  constexpr int T = N / 8;
+  __shared__ cplx lds[Shape<N>::LDSN];
+  const int lb = xcd_pairs(blockIdx.x), b = threadIdx.x;
+  if (lb == 0 && b == 0) *maxbits = 0ull;
+  const int ncp = (c.g.nc + 1) & ~1;
+  int field, kx;
+  if (lb < c.g.nc) { field = 0; kx = lb; }
+  else if (lb >= ncp && lb < ncp + c.g.nc) { field = 1; kx = lb - ncp; }
+  else return;
+  const cplx *src = (field == 0 ? psi : vz) + (size_t)kx * N;
+  cplx s[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int j = b + T * q, ky = kyof(j, N);
+    const double ka2 = (double)kx * kx + (double)ky * ky;
+    s[q] = ka2 <= c.g.kmax ? src[j] : mk(0.0, 0.0);
+  }
+  const int nf = field == 0 ? 4 : 2;
+  cplx *dst0 = M + (size_t)(field == 0 ? 0 : 4) * c.fs;
+  for (int f = 0; f < nf; ++f) {
+    cplx v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int ky = kyof(b + T * q, N);
+      const double ka2 = (double)kx * kx + (double)ky * ky;
+      cplx a = s[q];
+      if (f >= 2) a = mk(-ka2 * a.x, -ka2 * a.y);
+      if (f & 1) a = mk(-(double)ky * a.y, (double)ky * a.x);
+      v[q] = a;
+    }
+    fft_canon<N, +1, 1>(v, lds, b, tw);
+    cplx *dst = dst0 + (size_t)f * c.fs;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) dst[tix(b + T * q, kx, c.g.kp)] = v[q];
+  }
+}
+
+// Row kernel of one stage: for RW rows, z1 = psi_x + i psi_y,
+// z2 = w_x + i w_y, z3 = vz_x + i vz_y (x derivatives as i*kx on the mixed
+// fields), then
+//   P1 = (-(psi_x w_y - psi_y w_x) + 2 Omega n^2 vz_y)/n^4
+//   P2 = ( (psi_x vz_y - psi_y vz_x) + 2 Omega n^2 psi_y)/n^4
+// whose forward transforms are -C5 + 2 Omega C4 and C6 + 2 Omega C3 of
+// src/Rot2D3C.para.f90:571-576,586,591.  P1 + i P2 is forward transformed as
+// one packed row and unpacked into Q0, Q1 for kx < nc.  With maxbits set, the
+// max of psi_x^2 + psi_y^2 is accumulated (CFL on the pre-step state).
+template <int N>
+__device__ __forceinline__ void load_packed_step(cplx v[8], const cplx *__restrict__ FA,
+                                                 const cplx *__restrict__ FB, int y, int b, int nc,
+                                                 int kp) {
+  constexpr int T = N / 8;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int k = b + T * q;
+    cplx z = mk(0.0, 0.0);
+    if (k < nc) {
+      const cplx f = FA[tix(y, k, kp)], B = FB[tix(y, k, kp)];
+      const cplx A = mk(-(double)k * f.y, (double)k * f.x);  // i*kx*f
+      z = packed_z<N>(k, A, B, false);
+    } else if (k > N - nc) {
+      const int kk = N - k;
+      const cplx f = FA[tix(y, kk, kp)], B = FB[tix(y, kk, kp)];
+      const cplx A = mk(-(double)kk * f.y, (double)kk * f.x);
+      z = packed_z<N>(k, A, B, true);
+    }
+    v[q] = z;
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
+    const cplx *__restrict__ M, cplx *__restrict__ Q, StepConst c, unsigned long long *maxbits,
+    const cplx *__restrict__ tw) {
+  constexpr int T = N / 8, RW = RowStep<N>::ROWS;
+  __shared__ cplx lds[RW * Shape<N>::LDSN];
+  const int r = threadIdx.x % RW, b = threadIdx.x / RW;
+  const int y = blockIdx.x * RW + r;
+  const int nc = c.g.nc, kp = c.g.kp;
+  const size_t fs = c.fs;
+  cplx v[8];
+  double px[8], py[8], j1[8];
+  load_packed_step<N>(v, M, M + fs, y, b, nc, kp);
+  fft_canon<N, +1, RW>(v, lds + r, b, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) { px[q] = v[q].x; py[q] = v[q].y; }
+  if (maxbits) {
+    double m = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) m = fmax(m, px[q] * px[q] + py[q] * py[q]);
+    for (int off = 32; off > 0; off >>= 1)
+      if (off < (int)blockDim.x) m = fmax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(m));
+  }
+  load_packed_step<N>(v, M + 2 * fs, M + 3 * fs, y, b, nc, kp);
+  fft_canon<N, +1, RW>(v, lds + r, b, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) j1[q] = px[q] * v[q].y - py[q] * v[q].x;
+  load_packed_step<N>(v, M + 4 * fs, M + 5 * fs, y, b, nc, kp);
+  fft_canon<N, +1, RW>(v, lds + r, b, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const double vx = v[q].x, vy = v[q].y;
+    const double p1 = (-j1[q] + c.two_omega_n2 * vy) * c.inv_n4;
+    const double p2 = ((px[q] * vy - py[q] * vx) + c.two_omega_n2 * py[q]) * c.inv_n4;
+    v[q] = mk(p1, p2);
+  }
+  fft_canon<N, -1, RW>(v, lds + r, b, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) lds[pad(b + T * q) * RW + r] = v[q];
+  __syncthreads();
+  unpack_store<N, RW>(lds + r, b, nc, kp, Q, y, Q + fs, y);
+}
+
+// Fused column kernel of stage o: column forward of Q (psi blocks: Q0, vz
+// blocks: Q1), the semi-implicit update of src/Rot2D3C.para.f90:578-599 with
+// the forcing, then either the next stage's column inverse (as
+// k_step_colinv) or, after the last stage, the state write psi = C1, vz = C2
+// (src/Rot2D3C.para.f90:610-611) including zero columns kx > K.
+template <int N>
+__global__ __launch_bounds__(N / 8) void k_step_colfused(
+    const cplx *__restrict__ Q, cplx *psi, cplx *vz, cplx *__restrict__ M, StepConst c, int o,
+    int last, const unsigned long long *__restrict__ maxbits, double *dt_out,
+    const cplx *__restrict__ tw) {
+  constexpr int T = N / 8;
+  __shared__ cplx lds[Shape<N>::LDSN];
+  const int lb = xcd_pairs(blockIdx.x), b = threadIdx.x;
+  const int nc = c.g.nc, ncp = (nc + 1) & ~1;
+  int field, kx;
+  if (lb < nc) { field = 0; kx = lb; }
+  else if (lb >= ncp && lb < ncp + nc) { field = 1; kx = lb - ncp; }
+  else if (last && lb >= 2 * ncp && lb < 2 * ncp + (N / 2 + 1 - nc)) {
+    kx = nc + (lb - 2 * ncp);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      psi[(size_t)kx * N + b + T * q] = mk(0.0, 0.0);
+      vz[(size_t)kx * N + b + T * q] = mk(0.0, 0.0);
+    }
+    return;
+  } else return;
+
+  const double dt = c.cfl_mode ? dt_from_max(c.dc, *maxbits) : c.dt_fixed;
+  if (lb == 0 && b == 0 && o == 4 && dt_out) *dt_out = dt;
+
+  cplx v[8];
+  const cplx *Qf = Q + (size_t)field * c.fs;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[q] = Qf[tix(b + T * q, kx, c.g.kp)];
+  fft_canon<N, -1, 1>(v, lds, b, tw);
+
+  const ForceSet &fo = field == 0 ? c.fp : c.fz;
+  cplx fval[2] = {mk(0.0, 0.0), mk(0.0, 0.0)};
+  for (int e = 0; e < fo.nent; ++e)
+    if (fo.e[e].i == kx) fval[e] = force_value(fo, e, dt, c.n2);
+
+  cplx *state = field == 0 ? psi : vz;
+  const double tmp1 = dt / (double)o;
+  const double nu = field == 0 ? c.nu : c.nuv, hnu = field == 0 ? c.hnu : c.hnuv;
+  const int nn = field == 0 ? c.nn : c.nnv, mm = field == 0 ? c.mm : c.mmv;
+  cplx s[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int j = b + T * q, ky = kyof(j, N);
+    const double ka2 = (double)kx * kx + (double)ky * ky;
+    cplx cc = mk(0.0, 0.0);
+    if (ka2 <= c.g.kmax && ka2 > c.g.tiny) {
+      const double tmp2 = 1.0 / ka2;
+      const double tmp3 = 1.0 + (nu * ipow(ka2, nn) + hnu * ipow(tmp2, mm)) * tmp1;
+      cplx f = mk(0.0, 0.0);
+      for (int e = 0; e < fo.nent; ++e)
+        if (fo.e[e].i == kx && fo.e[e].j == j) f = fval[e];
+      const cplx s0 = state[(size_t)kx * N + j];
+      cplx t = field == 0 ? cscale(v[q], tmp2) : v[q];
+      t = cadd(t, f);
+      t = cadd(s0, cscale(t, tmp1));
+      cc = mk(t.x / tmp3, t.y / tmp3);
+    }
+    s[q] = cc;
+  }
+  if (last) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) state[(size_t)kx * N + b + T * q] = s[q];
+    return;
+  }
+  const int nf = field == 0 ? 4 : 2;
+  cplx *dst0 = M + (size_t)(field == 0 ? 0 : 4) * c.fs;
+  for (int f = 0; f < nf; ++f) {
+    cplx w[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int ky = kyof(b + T * q, N);
+      const double ka2 = (double)kx * kx + (double)ky * ky;
+      cplx a = s[q];
+      if (f >= 2) a = mk(-ka2 * a.x, -ka2 * a.y);
+      if (f & 1) a = mk(-(double)ky * a.y, (double)ky * a.x);
+      w[q] = a;
+    }
+    fft_canon<N, +1, 1>(w, lds, b, tw);
+    cplx *dst = dst0 + (size_t)f * c.fs;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) dst[tix(b + T * q, kx, c.g.kp)] = w[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side: plan, dispatch, C ABI
+// ---------------------------------------------------------------------------
+struct r2d3c_plan {
+  int n = 0, device = 0;
+  Geo g{};
+  size_t fs = 0, fsf = 0;         // complex elements per step / full mixed field
+  cplx *tw = nullptr;             // n twiddles exp(-2 pi i t/n)
+  cplx *mstep = nullptr;          // 6 mixed fields + 2 bracket fields (fused step)
+  cplx *mfull = nullptr;          // 4 full-range mixed fields
+  double *real = nullptr;         // n*n real scratch
+  unsigned long long *maxbits = nullptr;
+  double *dtslot = nullptr;
+  double *partial = nullptr;      // reduction partials
+  long long bytes = 0;
+};
+
+static thread_local std::string g_err;
+static int set_err(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+static int hip_err(hipError_t e, const char *where) {
+  return set_err(-(R2D3C_EHIP_BASE + (int)e), std::string(where) + ": " + hipGetErrorString(e));
+}
+#define HIPCHK(expr)                                  \
+  do {                                                \
+    hipError_t e_ = (expr);                           \
+    if (e_ != hipSuccess) return hip_err(e_, #expr);  \
+  } while (0)
+#define LAUNCHCHK(where)                              \
+  do {                                                \
+    hipError_t e_ = hipGetLastError();                \
+    if (e_ != hipSuccess) return hip_err(e_, where);  \
+  } while (0)
+
+static constexpr int kReduceBlocks = 256;
+
+#define R2D_DISPATCH(n, FN, ...)                                   \
+  switch (n) {                                                     \
+    case 16: return FN<16>(__VA_ARGS__);                           \
+    case 32: return FN<32>(__VA_ARGS__);                           \
+    case 64: return FN<64>(__VA_ARGS__);                           \
+    case 128: return FN<128>(__VA_ARGS__);                         \
+    case 256: return FN<256>(__VA_ARGS__);                         \
+    case 512: return FN<512>(__VA_ARGS__);                         \
+    case 1024: return FN<1024>(__VA_ARGS__);                       \
+    case 2048: return FN<2048>(__VA_ARGS__);                       \
+    case 4096: return FN<4096>(__VA_ARGS__);                       \
+    default: return set_err(R2D3C_ENOTSUP, "unsupported grid size"); \
+  }
+
+template <int N> static int run_c2r(r2d3c_plan *p, const cplx *in, double *out, hipStream_t s) {
+  k_col_inv_full<N><<<N / 2 + 1, N / 8, 0, s>>>(in, p->mfull, OP_RAW, 1.0, p->g, p->tw);
+  k_row_c2r<N><<<N / RowFull<N>::ROWS, N / 8 * RowFull<N>::P, 0, s>>>(p->mfull, out, p->g, p->tw);
+  LAUNCHCHK("c2r");
+  return 0;
+}
+
+template <int N> static int run_r2c(r2d3c_plan *p, const double *in, cplx *out, hipStream_t s) {
+  k_row_r2c<N><<<N / RowFull<N>::ROWS, N / 8 * RowFull<N>::P, 0, s>>>(in, p->mfull, p->g, p->tw);
+  k_col_fwd_full<N><<<N / 2 + 1, N / 8, 0, s>>>(p->mfull, out, p->g, p->tw);
+  LAUNCHCHK("r2c");
+  return 0;
+}
+
+template <int N>
+static int run_poisson(r2d3c_plan *p, const cplx *a, const cplx *b, cplx *c, hipStream_t s) {
+  cplx *m0 = p->mfull, *m1 = m0 + p->fsf, *m2 = m1 + p->fsf, *m3 = m2 + p->fsf;
+  k_col_inv_full<N><<<N / 2 + 1, N / 8, 0, s>>>(a, m0, OP_DX, 1.0, p->g, p->tw);
+  k_col_inv_full<N><<<N / 2 + 1, N / 8, 0, s>>>(a, m1, OP_DY, 1.0, p->g, p->tw);
+  k_col_inv_full<N><<<N / 2 + 1, N / 8, 0, s>>>(b, m2, OP_DX, 1.0, p->g, p->tw);
+  k_col_inv_full<N><<<N / 2 + 1, N / 8, 0, s>>>(b, m3, OP_DY, 1.0, p->g, p->tw);
+  const double n4 = (double)N * N * (double)N * N;
+  k_row_bracket<N><<<N / RowStep<N>::ROWS, N / 8 * RowStep<N>::ROWS, 0, s>>>(m0, m1, m2, m3,
+                                                                            p->real, n4, p->g,
+                                                                            p->tw);
+  LAUNCHCHK("poisson");
+  return run_r2c<N>(p, p->real, c, s);
+}
+
+static DtConst make_dt_const(int n, const r2d3c_params *q, double cfl) {
+  DtConst d;
+  d.cfl = cfl;
+  d.kcut = (double)n / 3.0;
+  d.nrm = (double)n * (double)n;
+  d.a = q->nu * ipow(d.kcut, 2 * q->nn);
+  d.bq = q->nuv * ipow(d.kcut, 2 * q->nnv);
+  d.two_omega = 2 * q->omega;
+  return d;
+}
+
+template <int N>
+static int run_cfl(r2d3c_plan *p, const r2d3c_params *q, double cfl, const cplx *psi, double *d_dt,
+                   hipStream_t s) {
+  cplx *m0 = p->mfull, *m1 = m0 + p->fsf;
+  k_set_u64<<<1, 1, 0, s>>>(p->maxbits, 0ull);
+  k_col_inv_full<N><<<N / 2 + 1, N / 8, 0, s>>>(psi, m0, OP_DX, 1.0, p->g, p->tw);
+  k_col_inv_full<N><<<N / 2 + 1, N / 8, 0, s>>>(psi, m1, OP_DY, 1.0, p->g, p->tw);
+  k_row_gradmax<N><<<N / RowStep<N>::ROWS, N / 8 * RowStep<N>::ROWS, 0, s>>>(m0, m1, p->maxbits,
+                                                                            p->g, p->tw);
+  k_dt_from_max<<<1, 1, 0, s>>>(p->maxbits, make_dt_const(N, q, cfl), d_dt);
+  LAUNCHCHK("cfl");
+  return 0;
+}
+
+// Placement of one rand_force draw, bug-compatible with
+// src/pseudospec2D_Rot2D3C.f90:1100-1122: in columns i >= 1 the half-plane
+// wavenumber ka(i) is compared with +-ky and the full index with kx; only the
+// kx = 0 column (i = 1 in Fortran) gets a conjugate partner.
+static int make_force_set(const r2d3c_force_draw *d, int n, ForceSet *f) {
+  std::memset(f, 0, sizeof(*f));
+  if (d == nullptr || !d->enabled) return 0;
+  f->num = std::sqrt(2.0 * d->f0);
+  f->kupk = d->kupk;
+  const double cph = std::cos(d->phase), sph = std::sin(d->phase);
+  auto ka = [n](int i) { return (double)(i < n / 2 ? i : i - n); };
+  auto add = [&](int i, int j, double sgn) {
+    if (f->nent >= 2) return false;
+    f->e[f->nent++] = ForceEnt{i, j, sgn, cph, sph};
+    return true;
+  };
+  for (int i = 1; i <= n / 2; ++i)
+    if (ka(i) == d->ky || ka(i) == -d->ky)
+      for (int j = 0; j < n; ++j)
+        if (ka(j) == d->kx && !add(i, j, 1.0))
+          return set_err(R2D3C_EINVAL, "forcing placement overflow");
+  if (ka(0) == d->ky)
+    for (int j = 1; j < n / 2; ++j)
+      if (ka(j) == d->kx && !(add(0, j, 1.0) && add(0, n - j, -1.0)))
+        return set_err(R2D3C_EINVAL, "forcing placement overflow");
+  return 0;
+}
+
+template <int N>
+static int run_step(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
+                    const r2d3c_forcing *fo, int dt_mode, double dt_or_cfl, double *d_dt,
+                    hipStream_t s) {
+  StepConst c;
+  std::memset(&c, 0, sizeof(c));
+  c.g = p->g;
+  c.fs = p->fs;
+  c.nu = q->nu; c.hnu = q->hnu; c.nuv = q->nuv; c.hnuv = q->hnuv;
+  c.nn = q->nn; c.mm = q->mm; c.nnv = q->nnv; c.mmv = q->mmv;
+  c.two_omega = 2 * q->omega;
+  c.cfl_mode = dt_mode == R2D3C_DT_CFL;
+  c.dt_fixed = dt_or_cfl;
+  c.dc = make_dt_const(N, q, dt_or_cfl);
+  c.n2 = (double)N * (double)N;
+  c.two_omega_n2 = c.two_omega * c.n2;
+  c.inv_n4 = 1.0 / (c.n2 * c.n2);
+  if (fo) {
+    int rc = make_force_set(&fo->psi, N, &c.fp);
+    if (rc) return rc;
+    rc = make_force_set(&fo->vz, N, &c.fz);
+    if (rc) return rc;
+  }
+  const int nc = p->g.nc, ncp = (nc + 1) & ~1;
+  cplx *M = p->mstep, *Q = p->mstep + 6 * p->fs;
+  constexpr int RW = RowStep<N>::ROWS;
+  double *dt_out = d_dt ? d_dt : p->dtslot;
+  k_step_colinv<N><<<round16(ncp + nc), N / 8, 0, s>>>(psi, vz, M, c, p->maxbits, p->tw);
+  for (int o = 4; o >= 1; --o) {
+    k_step_row<N><<<N / RW, N / 8 * RW, 0, s>>>(M, Q, c, (o == 4 && c.cfl_mode) ? p->maxbits : nullptr,
+                                                p->tw);
+    const int last = o == 1;
+    const int nblk = round16(2 * ncp + (last ? (N / 2 + 1 - nc) : 0));
+    k_step_colfused<N><<<nblk, N / 8, 0, s>>>(Q, psi, vz, M, c, o, last, p->maxbits, dt_out, p->tw);
+  }
+  LAUNCHCHK("step");
+  return 0;
+}
+
+static int launch_grid(size_t total) {
+  size_t g = (total + 255) / 256;
+  return (int)(g > 2048 ? 2048 : (g == 0 ? 1 : g));
+}
+
+extern "C" {
+
+const char *r2d3c_last_error(void) { return g_err.c_str(); }
+
+int r2d3c_plan_create(r2d3c_plan **out, int n) {
+  if (!out) return set_err(R2D3C_EINVAL, "plan pointer is NULL");
+  *out = nullptr;
+  if (n < 16 || n > 4096 || (n & (n - 1))) return set_err(R2D3C_ENOTSUP, "n must be a power of two in [16, 4096]");
+  r2d3c_plan *p = new r2d3c_plan();
+  p->n = n;
+  HIPCHK(hipGetDevice(&p->device));
+  // dealiasing: ka2 <= kmax = (n/3)^2 (src/Rot2D3C.para.f90:213-214)
+  const double kmax = ((double)n / 3.0) * ((double)n / 3.0);
+  int K = 0;
+  while ((double)(K + 1) * (K + 1) <= kmax) ++K;
+  p->g.n = n;
+  p->g.nc = K + 1;
+  p->g.kp = (K + 2) & ~1;
+  p->g.ncf = n / 2 + 1;
+  p->g.kpf = n / 2 + 2;
+  p->g.kmax = kmax;
+  p->g.tiny = 0.000001;
+  p->fs = (size_t)n * p->g.kp;
+  p->fsf = (size_t)n * p->g.kpf;
+  // twiddles from the extended-precision circle, rounded once to double
+  std::vector<cplx> tw(n);
+  const long double twopi = 6.283185307179586476925286766559005768L;
+  for (int t = 0; t < n; ++t) {
+    const long double a = twopi * (long double)t / (long double)n;
+    tw[t] = make_double2((double)cosl(a), (double)(-sinl(a)));
+  }
+  size_t b_tw = sizeof(cplx) * n, b_ms = sizeof(cplx) * 8 * p->fs, b_mf = sizeof(cplx) * 4 * p->fsf;
+  size_t b_re = sizeof(double) * (size_t)n * n;
+  hipError_t e;
+  if ((e = hipMalloc(&p->tw, b_tw)) != hipSuccess || (e = hipMalloc(&p->mstep, b_ms)) != hipSuccess ||
+      (e = hipMalloc(&p->mfull, b_mf)) != hipSuccess || (e = hipMalloc(&p->real, b_re)) != hipSuccess ||
+      (e = hipMalloc(&p->maxbits, 256)) != hipSuccess || (e = hipMalloc(&p->dtslot, 256)) != hipSuccess ||
+      (e = hipMalloc(&p->partial, sizeof(double) * kReduceBlocks)) != hipSuccess ||
+      (e = hipMemcpy(p->tw, tw.data(), b_tw, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemset(p->mstep, 0, b_ms)) != hipSuccess || (e = hipMemset(p->mfull, 0, b_mf)) != hipSuccess) {
+    int rc = hip_err(e, "plan allocation");
+    r2d3c_plan_destroy(p);
+    return rc;
+  }
+  p->bytes = (long long)(b_tw + b_ms + b_mf + b_re + 512 + sizeof(double) * kReduceBlocks);
+  *out = p;
+  return 0;
+}
+
+int r2d3c_plan_destroy(r2d3c_plan *p) {
+  if (!p) return 0;
+  (void)hipFree(p->tw);
+  (void)hipFree(p->mstep);
+  (void)hipFree(p->mfull);
+  (void)hipFree(p->real);
+  (void)hipFree(p->maxbits);
+  (void)hipFree(p->dtslot);
+  (void)hipFree(p->partial);
+  delete p;
+  return 0;
+}
+
+long long r2d3c_plan_workspace_bytes(const r2d3c_plan *p) { return p ? p->bytes : 0; }
+int r2d3c_plan_size(const r2d3c_plan *p) { return p ? p->n : 0; }
+
+#define NEED(x) \
+  if (!(x)) return set_err(R2D3C_EINVAL, "NULL argument: " #x)
+
+int r2d3c_c2r(r2d3c_plan *p, const double *in, double *out, void *stream) {
+  NEED(p); NEED(in); NEED(out);
+  R2D_DISPATCH(p->n, run_c2r, p, (const cplx *)in, out, (hipStream_t)stream);
+}
+
+int r2d3c_r2c(r2d3c_plan *p, const double *in, double *out, void *stream) {
+  NEED(p); NEED(in); NEED(out);
+  R2D_DISPATCH(p->n, run_r2c, p, in, (cplx *)out, (hipStream_t)stream);
+}
+
+static int spec_op(r2d3c_plan *p, const double *a, double *b, int op, void *stream) {
+  NEED(p); NEED(a); NEED(b);
+  const size_t total = (size_t)(p->n / 2 + 1) * p->n;
+  k_spec_op<<<launch_grid(total), 256, 0, (hipStream_t)stream>>>((const cplx *)a, (cplx *)b, op, p->n,
+                                                                   p->g.kmax, p->g.tiny);
+  LAUNCHCHK("spectral operator");
+  return 0;
+}
+
+int r2d3c_derivk2(r2d3c_plan *p, const double *a, double *b, int dir, void *stream) {
+  return spec_op(p, a, b, dir == 1 ? OP_DX : (dir == 2 ? OP_DY : OP_MASK), stream);
+}
+int r2d3c_laplak2(r2d3c_plan *p, const double *a, double *b, void *stream) {
+  return spec_op(p, a, b, OP_LAP, stream);
+}
+int r2d3c_inv_laplak2(r2d3c_plan *p, const double *a, double *b, void *stream) {
+  return spec_op(p, a, b, OP_INVLAP, stream);
+}
+
+int r2d3c_poisson(r2d3c_plan *p, const double *a, const double *b, double *c, void *stream) {
+  NEED(p); NEED(a); NEED(b); NEED(c);
+  R2D_DISPATCH(p->n, run_poisson, p, (const cplx *)a, (const cplx *)b, (cplx *)c,
+               (hipStream_t)stream);
+}
+
+static int reduce(r2d3c_plan *p, const double *a, const double *b, int mode, int kin, double *out,
+                  void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  k_reduce_partial<<<kReduceBlocks, 256, 0, s>>>((const cplx *)a, (const cplx *)b, mode, kin, p->n,
+                                                  p->g.kmax, p->g.tiny, p->partial);
+  k_reduce_finish<<<1, 256, 0, s>>>(p->partial, kReduceBlocks, out);
+  LAUNCHCHK("reduction");
+  return 0;
+}
+
+int r2d3c_energy(r2d3c_plan *p, const double *a, int kin, double *d_out, void *stream) {
+  NEED(p); NEED(a); NEED(d_out);
+  return reduce(p, a, a, 0, kin, d_out, stream);
+}
+
+int r2d3c_inerprod(r2d3c_plan *p, const double *a, const double *b, int kin, double *d_out,
+                   void *stream) {
+  NEED(p); NEED(a); NEED(b); NEED(d_out);
+  return reduce(p, a, b, 1, kin, d_out, stream);
+}
+
+static int check_params(const r2d3c_params *q) {
+  NEED(q);
+  if (q->nn < 0 || q->mm < 0 || q->nnv < 0 || q->mmv < 0)
+    return set_err(R2D3C_EINVAL, "dissipation orders must be >= 0");
+  return 0;
+}
+
+int r2d3c_cfl(r2d3c_plan *p, const r2d3c_params *q, double cfl, const double *psi, double *d_dt,
+              void *stream) {
+  NEED(p); NEED(psi); NEED(d_dt);
+  int rc = check_params(q);
+  if (rc) return rc;
+  R2D_DISPATCH(p->n, run_cfl, p, q, cfl, (const cplx *)psi, d_dt, (hipStream_t)stream);
+}
+
+int r2d3c_step(r2d3c_plan *p, const r2d3c_params *q, double *psi, double *vz,
+               const r2d3c_forcing *forcing, int dt_mode, double dt_or_cfl, double *d_dt,
+               void *stream) {
+  NEED(p); NEED(psi); NEED(vz);
+  int rc = check_params(q);
+  if (rc) return rc;
+  if (dt_mode != R2D3C_DT_CFL && dt_mode != R2D3C_DT_FIXED)
+    return set_err(R2D3C_EINVAL, "dt_mode must be R2D3C_DT_CFL or R2D3C_DT_FIXED");
+  if (psi == vz) return set_err(R2D3C_EINVAL, "psi and vz must not alias");
+  R2D_DISPATCH(p->n, run_step, p, q, (cplx *)psi, (cplx *)vz, forcing, dt_mode, dt_or_cfl, d_dt,
+               (hipStream_t)stream);
+}
+
+int r2d3c_forcing_fill(r2d3c_plan *p, const r2d3c_forcing *fo, double dt, double *fp, double *fz,
+                       void *stream) {
+  NEED(p); NEED(fo);
+  if (!(dt > 0.0)) return set_err(R2D3C_EINVAL, "dt must be positive");
+  const size_t total = (size_t)(p->n / 2 + 1) * p->n;
+  ForceSet a, b;
+  int rc = make_force_set(&fo->psi, p->n, &a);
+  if (rc) return rc;
+  rc = make_force_set(&fo->vz, p->n, &b);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (fp) k_force_fill<<<launch_grid(total), 256, 0, s>>>((cplx *)fp, a, dt, p->n);
+  if (fz) k_force_fill<<<launch_grid(total), 256, 0, s>>>((cplx *)fz, b, dt, p->n);
+  LAUNCHCHK("forcing fill");
+  return 0;
+}
+
+// randu: src/pseudospec2D_mod.f90:52-71 (REAL results, am = 1./im in REAL)
+float r2d3c_randu(int *idum) {
+  const int iq = 127773, ir = 2836, mask = 123459876, ia = 16807, im = 2147483647;
+  const float am = 1.0f / (float)im;
+  int v = *idum ^ mask;
+  const int k = v / iq;
+  v = ia * (v - k * iq) - ir * k;
+  if (v < 0) v += im;
+  volatile float r = am * (float)v;  // no contraction with the next line
+  r = (r - 0.5f) * 2.0f;
+  *idum = v ^ mask;
+  return r;
+}
+
+void r2d3c_rand_force_draw(double kup, int *seed, double *kx, double *ky, double *phase) {
+  const double pi = 3.1415926535897932384;  // src/pseudospec2D_mod.f90:35
+  const double theta = (double)r2d3c_randu(seed) * pi;
+  int tmp = *seed + 1;
+  const double ph = (double)r2d3c_randu(&tmp) * pi;
+  *kx = std::floor(kup * std::cos(theta) + 0.5);
+  *ky = std::floor(kup * std::sin(theta) + 0.5);
+  *phase = ph;
+}
+
+}  // extern "C"
