@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Benchmark: 2048^2 FP64 Rot2D3C RK4 steps on one MI355X.
+
+Workload (the single-GPU run the headline metric is quoted on): n = 2048,
+fixed dt = 2e-4, nu = nuv = 5e-5, hnu = hnuv = 0.5 (nn = nnv = 1, mm = mmv = 2),
+Omega = 5, random forcing (iflow = 3) on the shell kf = 40 for psi and vz with
+injection rate 1.0, forcing seed 0; seeded random-phase initial condition.
+One step = CFL-free RK4 step = 1 column-inverse + 4 x (row + fused column)
+kernels, inputs resident in HBM.
+
+Prints one JSON line.  `value` is Mpts*RK4-steps/s = n^2*steps/s/1e6.  The
+`roofline` object describes the kernel class with the largest share of the
+step time, timed live with HIP events stamped at kernel start/end
+(hipExtLaunchKernelGGL) over a second timed region of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
+
+
+def algorithmic_bytes(n):
+    """Compulsory HBM bytes per launch of each step kernel (dealiased schedule).
+
+    M = bytes of one mixed field restricted to the kx <= K columns every
+    dealiased field lives in, D = number of (kx, ky) half-plane modes inside
+    the dealiasing disc, S = bytes of one full spectral field."""
+    kmax = (n / 3.0) ** 2
+    K = 0
+    while (K + 1) ** 2 <= kmax:
+        K += 1
+    M = n * (K + 1) * 16
+    ky = np.fft.fftfreq(n, 1.0 / n)
+    kx = np.arange(K + 1, dtype=np.float64)
+    D = int(np.count_nonzero(kx[:, None] ** 2 + ky[None, :] ** 2 <= kmax))
+    S = 16 * n * (n // 2 + 1)
+    per = {
+        "k_step_colinv": 2 * 16 * D + 6 * M,
+        "k_step_row": 8 * M,
+        "k_step_colfused": 2 * M + 2 * 16 * (D - 1) + 6 * M,
+        "k_step_colfused_last": 2 * M + 2 * 16 * (D - 1) + 2 * S,
+    }
+    step = per["k_step_colinv"] + 4 * per["k_step_row"] + 3 * per["k_step_colfused"] \
+        + per["k_step_colfused_last"]
+    return per, step, {"K": K, "M": M, "D": D, "S": S, "B76S": 76 * S}
+
+
+def cpu_baseline(n, steps, p, kf, f0, seed, ps0, vz0, dt):
+    """The oracle (numpy, one core) on the same workload for `steps` steps,
+    forcing drawn from the same seed; returns the elapsed seconds."""
+    from oracle import spectral as O
+    op = O.Params(p["nu"], p["hnu"], p["nn"], p["mm"], p["nuv"], p["hnuv"], p["nnv"],
+                  p["mmv"], p["omega"])
+    t0 = time.perf_counter()
+    O.run_random_forcing(ps0.copy(), vz0.copy(), steps, 0.0, op, kf, kf, f0, f0, seed,
+                         fixed_dt=dt)
+    return time.perf_counter() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="steps of the oracle timed on the host (0 disables)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one 512 MiB device copy (PMC byte calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from rot2d3c_amd import _lib
+    from rot2d3c_amd import ic, spectral as S
+
+    n = args.n
+    dt = 2e-4
+    p = dict(nu=5e-5, hnu=0.5, nn=1, mm=2, nuv=5e-5, hnuv=0.5, nnv=1, mmv=2, omega=5.0)
+    kf, f0, seed = 40.0, 1.0, 0
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    ps0, vz0 = ic.random_phase_ic(n, seed=1234)
+    total = args.warmup + args.steps + min(args.steps, 100)
+    sched = ic.forcing_schedule(kf, f0, f0, seed, total, S.rand_force_draw)
+    draws = [(S.ForceDraw(*a), S.ForceDraw(*b)) for a, b in sched]
+    sp = S.Params(**p)
+    ps = torch.from_numpy(ps0).to(dev)
+    vz = torch.from_numpy(vz0).to(dev)
+    plan = S.Plan.get(n, dev)
+
+    def run(k0, k):
+        for s in range(k0, k0 + k):
+            S.rk4_step(ps, vz, sp, draws[s][0], draws[s][1], dt=dt)
+
+    if args.copy_probe:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    run(0, args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(args.warmup, args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = n * n * args.steps / wall / 1e6
+
+    # second timed region: per-kernel durations from start/stop-stamped events
+    ktimed = min(args.steps, 100)
+    lib = _lib.lib()
+    _lib.check(lib.r2d3c_timing_enable(plan.handle, 1), "timing_enable")
+    run(args.warmup + args.steps, ktimed)
+    ms = (_lib.ctypes.c_double * _lib.R2D3C_NKERNELS)()
+    cnt = (_lib.ctypes.c_longlong * _lib.R2D3C_NKERNELS)()
+    _lib.check(lib.r2d3c_timing_read(plan.handle, ms, cnt), "timing_read")
+    _lib.check(lib.r2d3c_timing_enable(plan.handle, 0), "timing_disable")
+    torch.cuda.synchronize()
+    finite = bool(torch.isfinite(torch.view_as_real(ps)).all() and torch.isfinite(torch.view_as_real(vz)).all())
+
+    per, step_bytes, geo = algorithmic_bytes(n)
+    kern = {}
+    for k, name in enumerate(_lib.KERNEL_NAMES):
+        if cnt[k]:
+            avg_ms = ms[k] / cnt[k]
+            kern[name] = {"avg_us": 1e3 * avg_ms, "launches_per_step": cnt[k] / ktimed,
+                          "share": ms[k] / sum(ms), "bytes_per_launch": per[name],
+                          "GBps": per[name] / (avg_ms * 1e-3) / 1e9}
+    dom = max(kern, key=lambda k: kern[k]["share"])
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_step2048.json")
+    if os.path.exists(tfile) and n == 2048:
+        with open(tfile) as f:
+            tj = json.load(f)
+        if dom in tj.get("kernels", {}):
+            traffic = tj["kernels"][dom]["hbm_bytes_per_launch"]
+    achieved = kern[dom]["GBps"]
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic, "kernel": dom,
+            "bytes_per_launch": per[dom], "avg_launch_us": round(kern[dom]["avg_us"], 2)}
+
+    cpu = None
+    if args.cpu_steps > 0:
+        el = cpu_baseline(n, args.cpu_steps, p, kf, f0, seed, ps0, vz0, dt)
+        cpu = {"value": round(n * n * args.cpu_steps / el / 1e6, 4), "unit": "Mpts·RK4-steps/s",
+               "cores": 1, "kind": "port",
+               "sample": "%d steps of the same %d^2 workload through the numpy oracle "
+                         "(pocketfft, single thread): %.1f s" % (args.cpu_steps, n, el)}
+
+    out = {
+        "metric": "Mpts·RK4-steps/s at 2048² FP64 (%HBM peak); closure-vs-DNS speedup at 5% err",
+        "value": round(value, 2),
+        "unit": "Mpts·RK4-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random-phase initial condition, seeded random forcing)",
+        "config": {"workload": "Rot2D3C RK4 step, %d^2 FP64, dt=2e-4 fixed, nu=5e-5, nu+=0.5, "
+                               "Omega=5, forcing kf=40 (psi and vz), injection 1.0" % n,
+                   "grid": n, "dt": dt, "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "step_hbm": {"algorithmic_bytes_per_step": step_bytes,
+                     "achieved_GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+                     "frac_of_peak": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+                     "model_76S_bytes_per_step": geo["B76S"],
+                     "model_76S_frac_of_peak": round(geo["B76S"] / (ms_per_step * 1e-3) / 1e9
+                                                     / HBM_PEAK_GBPS, 4)},
+        "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in kern.items()},
+        "finite": finite,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

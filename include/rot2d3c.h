@@ -135,6 +135,20 @@ int r2d3c_step(r2d3c_plan *plan, const r2d3c_params *p, double *psi, double *vz,
 int r2d3c_forcing_fill(r2d3c_plan *plan, const r2d3c_forcing *forcing, double dt,
                        double *fp, double *fz, void *stream);
 
+/* ---- kernel timing of r2d3c_step (no reference counterpart) ------------ */
+
+/* Kernel classes of r2d3c_step: 0 column inverse of the state, 1 row
+ * (brackets) kernel, 2 fused column kernel of stages 1-3, 3 fused column
+ * kernel of the last stage (state write). */
+#define R2D3C_NKERNELS 4
+/* While enabled, every kernel launched by r2d3c_step is bracketed by HIP
+ * events stamped at the kernel's start and end.  Enabling or disabling
+ * discards what was recorded. */
+int r2d3c_timing_enable(r2d3c_plan *plan, int enable);
+/* Waits for the recorded events, returns the summed milliseconds and the
+ * launch count per kernel class (arrays of R2D3C_NKERNELS), and clears them. */
+int r2d3c_timing_read(r2d3c_plan *plan, double *ms, long long *launches);
+
 /* ---- host random numbers (no device work) ----------------------------- */
 
 /* Park-Miller minimal standard with the reference's XOR mask; float32 value in

@@ -10,6 +10,8 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librot2d3c.
 
 R2D3C_DT_FIXED = 0
 R2D3C_DT_CFL = 1
+R2D3C_NKERNELS = 4
+KERNEL_NAMES = ("k_step_colinv", "k_step_row", "k_step_colfused", "k_step_colfused_last")
 
 _c_int, _c_double, _vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
 
@@ -49,6 +51,8 @@ SIGNATURES = {
     "r2d3c_step": (_c_int, [_vp, ctypes.POINTER(Params), _vp, _vp, ctypes.POINTER(Forcing),
                             _c_int, _c_double, _vp, _vp]),
     "r2d3c_forcing_fill": (_c_int, [_vp, ctypes.POINTER(Forcing), _c_double, _vp, _vp, _vp]),
+    "r2d3c_timing_enable": (_c_int, [_vp, _c_int]),
+    "r2d3c_timing_read": (_c_int, [_vp, ctypes.POINTER(_c_double), ctypes.POINTER(ctypes.c_longlong)]),
     "r2d3c_randu": (ctypes.c_float, [ctypes.POINTER(_c_int)]),
     "r2d3c_rand_force_draw": (None, [_c_double, ctypes.POINTER(_c_int),
                                      ctypes.POINTER(_c_double), ctypes.POINTER(_c_double),

@@ -19,6 +19,7 @@
 // Per step: one column-inverse kernel, then per stage one row kernel and one
 // fused kernel (column forward + semi-implicit update + next stage's column
 // inverse, or the state write after the last stage).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -629,7 +630,28 @@ struct r2d3c_plan {
   double *dtslot = nullptr;
   double *partial = nullptr;      // reduction partials
   long long bytes = 0;
+  // kernel timing of r2d3c_step (r2d3c_timing_enable / r2d3c_timing_read)
+  int timing = 0;
+  std::vector<hipEvent_t> evpool;
+  size_t evused = 0;
+  struct Rec { int kind; hipEvent_t a, b; };
+  std::vector<Rec> recs;
 };
+
+// Start/stop events of one timed launch (null events when timing is off);
+// hipExtLaunchKernelGGL stamps them at the kernel's own start and end.
+static void ev_pair(r2d3c_plan *p, int kind, hipEvent_t *a, hipEvent_t *b) {
+  *a = *b = nullptr;
+  if (!p->timing) return;
+  while (p->evused + 2 > p->evpool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    p->evpool.push_back(e);
+  }
+  *a = p->evpool[p->evused++];
+  *b = p->evpool[p->evused++];
+  p->recs.push_back({kind, *a, *b});
+}
 
 static thread_local std::string g_err;
 static int set_err(int code, const std::string &msg) {
@@ -775,13 +797,22 @@ static int run_step(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
   cplx *M = p->mstep, *Q = p->mstep + 6 * p->fs;
   constexpr int RW = RowStep<N>::ROWS;
   double *dt_out = d_dt ? d_dt : p->dtslot;
-  k_step_colinv<N><<<round16(ncp + nc), N / 8, 0, s>>>(psi, vz, M, c, p->maxbits, p->tw);
+  const cplx *tw = p->tw;
+  hipEvent_t ea, eb;
+  ev_pair(p, 0, &ea, &eb);
+  hipExtLaunchKernelGGL(k_step_colinv<N>, dim3(round16(ncp + nc)), dim3(N / 8), 0, s, ea, eb, 0,
+                        (const cplx *)psi, (const cplx *)vz, M, c, p->maxbits, tw);
   for (int o = 4; o >= 1; --o) {
-    k_step_row<N><<<N / RW, N / 8 * RW, 0, s>>>(M, Q, c, (o == 4 && c.cfl_mode) ? p->maxbits : nullptr,
-                                                p->tw);
+    unsigned long long *mb = (o == 4 && c.cfl_mode) ? p->maxbits : nullptr;
+    ev_pair(p, 1, &ea, &eb);
+    hipExtLaunchKernelGGL(k_step_row<N>, dim3(N / RW), dim3(N / 8 * RW), 0, s, ea, eb, 0,
+                          (const cplx *)M, Q, c, mb, tw);
     const int last = o == 1;
     const int nblk = round16(2 * ncp + (last ? (N / 2 + 1 - nc) : 0));
-    k_step_colfused<N><<<nblk, N / 8, 0, s>>>(Q, psi, vz, M, c, o, last, p->maxbits, dt_out, p->tw);
+    ev_pair(p, last ? 3 : 2, &ea, &eb);
+    hipExtLaunchKernelGGL(k_step_colfused<N>, dim3(nblk), dim3(N / 8), 0, s, ea, eb, 0,
+                          (const cplx *)Q, psi, vz, M, c, o, last,
+                          (const unsigned long long *)p->maxbits, dt_out, tw);
   }
   LAUNCHCHK("step");
   return 0;
@@ -843,6 +874,7 @@ int r2d3c_plan_create(r2d3c_plan **out, int n) {
 
 int r2d3c_plan_destroy(r2d3c_plan *p) {
   if (!p) return 0;
+  for (hipEvent_t e : p->evpool) (void)hipEventDestroy(e);
   (void)hipFree(p->tw);
   (void)hipFree(p->mstep);
   (void)hipFree(p->mfull);
@@ -855,6 +887,29 @@ int r2d3c_plan_destroy(r2d3c_plan *p) {
 }
 
 long long r2d3c_plan_workspace_bytes(const r2d3c_plan *p) { return p ? p->bytes : 0; }
+
+int r2d3c_timing_enable(r2d3c_plan *p, int enable) {
+  if (!p) return set_err(R2D3C_EINVAL, "NULL plan");
+  p->timing = enable != 0;
+  p->evused = 0;
+  p->recs.clear();
+  return 0;
+}
+
+int r2d3c_timing_read(r2d3c_plan *p, double *ms, long long *launches) {
+  if (!p || !ms || !launches) return set_err(R2D3C_EINVAL, "NULL argument");
+  for (int k = 0; k < R2D3C_NKERNELS; ++k) { ms[k] = 0.0; launches[k] = 0; }
+  for (const auto &r : p->recs) {
+    HIPCHK(hipEventSynchronize(r.b));
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+    ms[r.kind] += t;
+    launches[r.kind] += 1;
+  }
+  p->evused = 0;
+  p->recs.clear();
+  return 0;
+}
 int r2d3c_plan_size(const r2d3c_plan *p) { return p ? p->n : 0; }
 
 #define NEED(x) \
