@@ -6,7 +6,8 @@ modules fails with an error that says how to build it.
 import ctypes
 import os
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librot2d3c.so")
+LIB_PATH = os.environ.get("R2D3C_LIB") or os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), "librot2d3c.so")
 
 R2D3C_DT_FIXED = 0
 R2D3C_DT_CFL = 1

@@ -11,9 +11,15 @@
 // Pass schedule: radix 8, then radix 8 while at least 16 points remain, then
 // one radix 2/4/8 pass.  2048 = 8*8*8*4, 256 = 8*8*4, 1024 = 8*8*8*2.
 // Twiddles come from a table tw[t] = exp(-2 pi i t / N) built on the host in
-// extended precision; the inverse uses its conjugate.
+// extended precision; the inverse uses its conjugate.  A radix-8 butterfly
+// loads w, w^2, w^4 and forms w^3, w^5, w^6, w^7 by products (at most two
+// extra roundings), which cuts the twiddle loads from 7 to 3.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#ifndef R2D_TW_DERIVE
+#define R2D_TW_DERIVE 1
+#endif
 
 namespace r2d {
 
@@ -74,14 +80,18 @@ template <int S> __device__ __forceinline__ void dft8(cplx *x) {
   cplx o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
   dft4<S>(e0, e1, e2, e3);
   dft4<S>(o0, o1, o2, o3);
+  // exp(S i pi/4) = c (1 + S i) and exp(3 S i pi/4) = c (-1 + S i): the factor
+  // c is folded into the final additions as fused multiply-adds
   const double c = 0.70710678118654752440;
-  o1 = mk(c * (o1.x - S * o1.y), c * (o1.y + S * o1.x));   // * exp(S i pi/4)
-  o2 = mul_si<S>(o2);                                        // * exp(S i pi/2)
-  o3 = mk(c * (-o3.x - S * o3.y), c * (S * o3.x - o3.y));  // * exp(3 S i pi/4)
+  const cplx t1 = mk(o1.x - S * o1.y, o1.y + S * o1.x);
+  const cplx t3 = mk(-o3.x - S * o3.y, S * o3.x - o3.y);
+  o2 = mul_si<S>(o2);  // * exp(S i pi/2)
   x[0] = cadd(e0, o0); x[4] = csub(e0, o0);
-  x[1] = cadd(e1, o1); x[5] = csub(e1, o1);
+  x[1] = mk(fma(c, t1.x, e1.x), fma(c, t1.y, e1.y));
+  x[5] = mk(fma(-c, t1.x, e1.x), fma(-c, t1.y, e1.y));
   x[2] = cadd(e2, o2); x[6] = csub(e2, o2);
-  x[3] = cadd(e3, o3); x[7] = csub(e3, o3);
+  x[3] = mk(fma(c, t3.x, e3.x), fma(c, t3.y, e3.y));
+  x[7] = mk(fma(-c, t3.x, e3.x), fma(-c, t3.y, e3.y));
 }
 
 template <int R, int S> __device__ __forceinline__ void dftR(cplx *x) {
@@ -115,8 +125,18 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
     __syncthreads();
     const int j = b & (L - 1);
     const int st = (N / (8 * L)) * j;
+#if R2D_TW_DERIVE
+    {  // three table loads; the other powers by products (error <= 2 ulp more)
+      const cplx w1 = twid<S>(tw, st), w2 = twid<S>(tw, 2 * st), w4 = twid<S>(tw, 4 * st);
+      const cplx w3 = cmul(w1, w2);
+      x[1] = cmul(x[1], w1); x[2] = cmul(x[2], w2); x[3] = cmul(x[3], w3);
+      x[4] = cmul(x[4], w4); x[5] = cmul(x[5], cmul(w1, w4));
+      x[6] = cmul(x[6], cmul(w2, w4)); x[7] = cmul(x[7], cmul(w3, w4));
+    }
+#else
 #pragma unroll
     for (int m = 1; m < 8; ++m) x[m] = cmul(x[m], twid<S>(tw, st * m));
+#endif
     dft8<S>(x);
     const int base = (b - j) * 8 + j;
 #pragma unroll
@@ -132,8 +152,23 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
     cplx x[RL];
 #pragma unroll
     for (int m = 0; m < RL; ++m) x[m] = lds[pad(j + m * (N / RL)) * IL];
+#if R2D_TW_DERIVE
+    if constexpr (RL == 2) {
+      x[1] = cmul(x[1], twid<S>(tw, j));
+    } else if constexpr (RL == 4) {
+      const cplx w1 = twid<S>(tw, j), w2 = twid<S>(tw, 2 * j);
+      x[1] = cmul(x[1], w1); x[2] = cmul(x[2], w2); x[3] = cmul(x[3], cmul(w1, w2));
+    } else {
+      const cplx w1 = twid<S>(tw, j), w2 = twid<S>(tw, 2 * j), w4 = twid<S>(tw, 4 * j);
+      const cplx w3 = cmul(w1, w2);
+      x[1] = cmul(x[1], w1); x[2] = cmul(x[2], w2); x[3] = cmul(x[3], w3);
+      x[4] = cmul(x[4], w4); x[5] = cmul(x[5], cmul(w1, w4));
+      x[6] = cmul(x[6], cmul(w2, w4)); x[7] = cmul(x[7], cmul(w3, w4));
+    }
+#else
 #pragma unroll
     for (int m = 1; m < RL; ++m) x[m] = cmul(x[m], twid<S>(tw, j * m));
+#endif
     dftR<RL, S>(x);
 #pragma unroll
     for (int m = 0; m < RL; ++m) v[u + m * U] = x[m];

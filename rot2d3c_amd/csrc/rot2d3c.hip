@@ -56,8 +56,10 @@ __host__ __device__ __forceinline__ double ipow(double x, int e) {
 // Workgroup ids are dispatched round-robin over the 8 XCDs.  Map hardware id
 // hb to logical id lb so that logical pairs (2p, 2p+1) -- neighbouring kx
 // columns whose 64-byte chunks share 128-byte lines -- run on the same XCD in
-// consecutive slots.  Requires gridDim.x % 16 == 0.
+// consecutive slots.  A bijection when gridDim.x % 16 == 0 (the launches
+// round their grids up with round16); other grids keep the identity.
 __device__ __forceinline__ int xcd_pairs(int hb) {
+  if (gridDim.x & 15) return hb;
   const int x = hb & 7, s = hb >> 3;
   return ((s >> 1) << 4) + (x << 1) + (s & 1);
 }
@@ -220,10 +222,25 @@ __global__ __launch_bounds__(N / 8 * RowFull<N>::P) void k_row_r2c(const double 
 }
 
 // rows per block of the step row kernel and the single-field full-range
-// bracket/max kernels: up to 4 (one y tile)
+// bracket/max kernels: up to R2D_ROW_RW (a divisor of the 4-row y tile)
+#ifndef R2D_ROW_RW
+#define R2D_ROW_RW 2
+#endif
 template <int N> struct RowStep {
-  static constexpr int ROWS = (N / 8) * 4 <= 1024 ? 4 : ((N / 8) * 2 <= 1024 ? 2 : 1);
+  static constexpr int ROWS = (N / 8) * R2D_ROW_RW <= 1024 ? R2D_ROW_RW
+                              : ((N / 8) * 2 <= 1024 ? 2 : 1);
+  static constexpr int G = 4 / ROWS;  // blocks sharing one y tile
 };
+// Blocks of one y tile read interleaved 16-byte slots of the same 64-byte
+// chunks; map G consecutive logical blocks onto one XCD so they share L2.
+// The map is a bijection only when gridDim.x % (8 G) == 0; smaller grids keep
+// the identity.
+template <int G> __device__ __forceinline__ int xcd_group(int hb) {
+  if constexpr (G == 1) return hb;
+  if (gridDim.x % (8 * G) != 0) return hb;
+  const int x = hb & 7, s = hb >> 3;
+  return (s / G) * (8 * G) + x * G + (s % G);
+}
 
 // Poisson bracket rows (full range): z1 = ax + i ay, z2 = bx + i by from four
 // mixed fields; out[y][x] = (ax*by - ay*bx)/n^4  (src/pseudospec2D_Rot2D3C.f90:195-216)
@@ -481,14 +498,17 @@ __device__ __forceinline__ void load_packed_step(cplx v[8], const cplx *__restri
   }
 }
 
+#ifndef R2D_ROW_MINW
+#define R2D_ROW_MINW 1
+#endif
 template <int N>
-__global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
+__global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS, R2D_ROW_MINW) void k_step_row(
     const cplx *__restrict__ M, cplx *__restrict__ Q, StepConst c, unsigned long long *maxbits,
     const cplx *__restrict__ tw) {
   constexpr int T = N / 8, RW = RowStep<N>::ROWS;
   __shared__ cplx lds[RW * Shape<N>::LDSN];
   const int r = threadIdx.x % RW, b = threadIdx.x / RW;
-  const int y = blockIdx.x * RW + r;
+  const int y = xcd_group<RowStep<N>::G>(blockIdx.x) * RW + r;
   const int nc = c.g.nc, kp = c.g.kp;
   const size_t fs = c.fs;
   cplx v[8];
@@ -523,6 +543,156 @@ __global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
   for (int q = 0; q < 8; ++q) lds[pad(b + T * q) * RW + r] = v[q];
   __syncthreads();
   unpack_store<N, RW>(lds + r, b, nc, kp, Q, y, Q + fs, y);
+}
+
+// Persistent variant of the row kernel for n >= 256.  One block per CU walks
+// its y tiles; the inputs of each packed transform (a field pair of RW rows)
+// are copied straight into LDS with global_load_lds_dwordx4 while the previous
+// transform runs, so the block never waits on HBM between its four transforms.
+// LDS: RW*(n + n/16) complex exchange slots + one staging buffer of 2*nc*RW.
+#ifndef R2D_ROW2_RW
+#define R2D_ROW2_RW 2
+#endif
+#ifndef R2D_ROW2_BPC
+#define R2D_ROW2_BPC 1
+#endif
+template <int N> struct RowV2 {
+  static constexpr int T = N / 8;
+  static constexpr int NCMAX = N / 3 + 2;
+  static constexpr int fits(int rw) {
+    return ((2 * NCMAX * rw + 63) / 64 * 64 + rw * Shape<N>::LDSN) * 16 <= 160 * 1024 &&
+           T * rw <= 1024;
+  }
+  static constexpr int RW = (R2D_ROW2_RW == 2 && fits(2)) ? 2 : 1;
+  static constexpr int THREADS = T * RW;
+  static constexpr int SBN = (2 * NCMAX * RW + 63) / 64 * 64;
+  static constexpr int G = 4 / RW;  // blocks per y tile
+};
+
+__device__ __forceinline__ void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Stage field pair p of the RW rows (tile t, sub-block sub) into sb as
+// sb[f2*nc*RW + kx*RW + rr] = M_{2p+f2}(kx, 4t + sub*RW + rr).  Each wave
+// instruction fills 64 consecutive slots (LDS address = wave base + lane*16).
+template <int N>
+__device__ __forceinline__ void stage_pair(cplx *sb, const cplx *__restrict__ M, size_t fs, int p,
+                                           int t, int sub, int nc, int kp) {
+  using RV = RowV2<N>;
+  constexpr int RW = RV::RW, NW = RV::THREADS / 64;
+  // opaque copy of the thread index: keeps the per-lane index math below from
+  // being hoisted out of the caller's tile loop (it would pin ~40 VGPRs)
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int wave = tid >> 6, lane = tid & 63;
+  const int per = nc * RW, total = 2 * per;
+  const cplx *base = M + (size_t)(2 * p) * fs + (size_t)t * (size_t)(kp * 4) + sub * RW;
+  for (int w0 = wave * 64; w0 < total; w0 += NW * 64) {
+    const int e = w0 + lane;
+    if (e < total) {
+      const int f2 = e >= per, rem = e - f2 * per;
+      const int kx = rem / RW, rr = rem - kx * RW;
+      const cplx *g = base + (size_t)f2 * fs + kx * 4 + rr;
+      __builtin_amdgcn_global_load_lds((const void *)g,
+                                       (__attribute__((address_space(3))) void *)(sb + w0), 16, 0,
+                                       0);
+    }
+  }
+}
+
+// z = c2r(i kx A) + i c2r(B) of row r from the staged pair (see load_packed_step)
+template <int N>
+__device__ __forceinline__ void build_packed(cplx v[8], const cplx *sb, int r, int b, int nc) {
+  constexpr int T = N / 8, RW = RowV2<N>::RW;
+  const int per = nc * RW;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int k = b + T * q;
+    cplx z = mk(0.0, 0.0);
+    const bool lo = k < nc, hi = k > N - nc;
+    if (lo || hi) {
+      const int kk = lo ? k : N - k;
+      const cplx f = sb[kk * RW + r], B = sb[per + kk * RW + r];
+      const cplx A = mk(-(double)kk * f.y, (double)kk * f.x);
+      z = packed_z<N>(k, A, B, hi);
+    }
+    v[q] = z;
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(RowV2<N>::THREADS) void k_step_row2(
+    const cplx *__restrict__ M, cplx *__restrict__ Q, StepConst c, unsigned long long *maxbits,
+    const cplx *__restrict__ tw) {
+  using RV = RowV2<N>;
+  constexpr int T = N / 8, RW = RV::RW, G = RV::G;
+  __shared__ cplx xb[RW * Shape<N>::LDSN];
+  __shared__ cplx sb[RV::SBN];
+  const int r = threadIdx.x % RW, b = threadIdx.x / RW;
+  const int nc = c.g.nc, kp = c.g.kp;
+  const size_t fs = c.fs;
+  // tile of iteration k: G consecutive slots of one XCD share a tile
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int tslots = (int)(gridDim.x >> 3) / G, sub = slot % G;
+  auto tile_of = [&](int k) { return (k * tslots + slot / G) * 8 + xcd; };
+  constexpr int NTILES = N / 4;
+  int k = 0, t = tile_of(0);
+  if (t >= NTILES) return;
+  double m = 0.0;
+  stage_pair<N>(sb, M, fs, 0, t, sub, nc, kp);
+  wait_vm();
+  __syncthreads();
+  for (;;) {
+    const int y = 4 * t + sub * RW + r;
+    cplx v[8];
+    double px[8], py[8], j1[8];
+    build_packed<N>(v, sb, r, b, nc);
+    __syncthreads();
+    stage_pair<N>(sb, M, fs, 1, t, sub, nc, kp);
+    fft_canon<N, +1, RW>(v, xb + r, b, tw);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      px[q] = v[q].x;
+      py[q] = v[q].y;
+      m = fmax(m, px[q] * px[q] + py[q] * py[q]);
+    }
+    wait_vm();
+    __syncthreads();
+    build_packed<N>(v, sb, r, b, nc);
+    __syncthreads();
+    stage_pair<N>(sb, M, fs, 2, t, sub, nc, kp);
+    fft_canon<N, +1, RW>(v, xb + r, b, tw);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) j1[q] = px[q] * v[q].y - py[q] * v[q].x;
+    wait_vm();
+    __syncthreads();
+    build_packed<N>(v, sb, r, b, nc);
+    __syncthreads();
+    const int tn = tile_of(k + 1);
+    if (tn < NTILES) stage_pair<N>(sb, M, fs, 0, tn, sub, nc, kp);
+    fft_canon<N, +1, RW>(v, xb + r, b, tw);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const double vx = v[q].x, vy = v[q].y;
+      const double p1 = (-j1[q] + c.two_omega_n2 * vy) * c.inv_n4;
+      const double p2 = ((px[q] * vy - py[q] * vx) + c.two_omega_n2 * py[q]) * c.inv_n4;
+      v[q] = mk(p1, p2);
+    }
+    fft_canon<N, -1, RW>(v, xb + r, b, tw);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) xb[pad(b + T * q) * RW + r] = v[q];
+    __syncthreads();
+    unpack_store<N, RW>(xb + r, b, nc, kp, Q, y, Q + fs, y);
+    if (tn >= NTILES) break;
+    wait_vm();
+    __syncthreads();
+    t = tn;
+    ++k;
+  }
+  if (maxbits) {
+    for (int off = 32; off > 0; off >>= 1)
+      if (off < (int)blockDim.x) m = fmax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(m));
+  }
 }
 
 // Fused column kernel of stage o: column forward of Q (psi blocks: Q0, vz
@@ -619,7 +789,7 @@ __global__ __launch_bounds__(N / 8) void k_step_colfused(
 // host side: plan, dispatch, C ABI
 // ---------------------------------------------------------------------------
 struct r2d3c_plan {
-  int n = 0, device = 0;
+  int n = 0, device = 0, cus = 256;
   Geo g{};
   size_t fs = 0, fsf = 0;         // complex elements per step / full mixed field
   cplx *tw = nullptr;             // n twiddles exp(-2 pi i t/n)
@@ -770,6 +940,17 @@ static int make_force_set(const r2d3c_force_draw *d, int n, ForceSet *f) {
   return 0;
 }
 
+// persistent row kernel grid: one block per CU, a multiple of 8*G, no more
+// blocks than tile slots
+template <int N> static int row2_grid(int cus) {
+  constexpr int G = RowV2<N>::G, Q8 = 8 * G;
+  int g = cus * R2D_ROW2_BPC / Q8 * Q8;
+  const int need = (N / 4) * G;
+  if (g > need) g = need;
+  if (g < Q8) g = Q8;
+  return g;
+}
+
 template <int N>
 static int run_step(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
                     const r2d3c_forcing *fo, int dt_mode, double dt_or_cfl, double *d_dt,
@@ -805,8 +986,13 @@ static int run_step(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
   for (int o = 4; o >= 1; --o) {
     unsigned long long *mb = (o == 4 && c.cfl_mode) ? p->maxbits : nullptr;
     ev_pair(p, 1, &ea, &eb);
-    hipExtLaunchKernelGGL(k_step_row<N>, dim3(N / RW), dim3(N / 8 * RW), 0, s, ea, eb, 0,
-                          (const cplx *)M, Q, c, mb, tw);
+    if constexpr (RowV2<N>::THREADS >= 64) {
+      hipExtLaunchKernelGGL(k_step_row2<N>, dim3(row2_grid<N>(p->cus)), dim3(RowV2<N>::THREADS), 0,
+                            s, ea, eb, 0, (const cplx *)M, Q, c, mb, tw);
+    } else {
+      hipExtLaunchKernelGGL(k_step_row<N>, dim3(N / RW), dim3(N / 8 * RW), 0, s, ea, eb, 0,
+                            (const cplx *)M, Q, c, mb, tw);
+    }
     const int last = o == 1;
     const int nblk = round16(2 * ncp + (last ? (N / 2 + 1 - nc) : 0));
     ev_pair(p, last ? 3 : 2, &ea, &eb);
@@ -834,6 +1020,7 @@ int r2d3c_plan_create(r2d3c_plan **out, int n) {
   r2d3c_plan *p = new r2d3c_plan();
   p->n = n;
   HIPCHK(hipGetDevice(&p->device));
+  HIPCHK(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device));
   // dealiasing: ka2 <= kmax = (n/3)^2 (src/Rot2D3C.para.f90:213-214)
   const double kmax = ((double)n / 3.0) * ((double)n / 3.0);
   int K = 0;
