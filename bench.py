@@ -5,8 +5,10 @@ Workload (the single-GPU run the headline metric is quoted on): n = 2048,
 fixed dt = 2e-4, nu = nuv = 5e-5, hnu = hnuv = 0.5 (nn = nnv = 1, mm = mmv = 2),
 Omega = 5, random forcing (iflow = 3) on the shell kf = 40 for psi and vz with
 injection rate 1.0, forcing seed 0; seeded random-phase initial condition.
-One step = CFL-free RK4 step = 1 column-inverse + 4 x (row + fused column)
-kernels, inputs resident in HBM.
+One step = 4 x (row + fused column) kernels; the steps of the timed region
+run as one r2d3c_run call, so the column transforms of each step's state come
+from the previous step's last kernel (one column-inverse kernel per call).
+Inputs resident in HBM.
 
 Prints one JSON line.  `value` is Mpts*RK4-steps/s = n^2*steps/s/1e6.  The
 `roofline` object describes the kernel class with the largest share of the
@@ -28,12 +30,19 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
 
 
-def algorithmic_bytes(n):
-    """Compulsory HBM bytes per launch of each step kernel (dealiased schedule).
+def algorithmic_bytes(n, k):
+    """Compulsory HBM bytes per launch of each step kernel class for a run of k
+    chained steps (one r2d3c_run call), dealiased schedule.
 
     M = bytes of one mixed field restricted to the kx <= K columns every
-    dealiased field lives in, D = number of (kx, ky) half-plane modes inside
-    the dealiasing disc, S = bytes of one full spectral field."""
+    dealiased field lives in, D' = number of (kx, ky) half-plane modes inside
+    the update mask tiny < k^2 <= kmax (D = D' + 1 with the k = 0 mode),
+    S = bytes of one full spectral field.
+      column inverse (once per run): read psi, vz on the disc, write 6 M
+      row kernel: read 6 M, write 2 M
+      fused column, stages 1-3: read 2 M + psi0, vz0 on the disc, write 6 M
+      fused column, last stage: read 2 M + psi0, vz0, write psi, vz on the disc
+        and, except after the run's last step, the next step's 6 M."""
     kmax = (n / 3.0) ** 2
     K = 0
     while (K + 1) ** 2 <= kmax:
@@ -43,13 +52,16 @@ def algorithmic_bytes(n):
     kx = np.arange(K + 1, dtype=np.float64)
     D = int(np.count_nonzero(kx[:, None] ** 2 + ky[None, :] ** 2 <= kmax))
     S = 16 * n * (n // 2 + 1)
+    disc = 2 * 16 * (D - 1)
+    last_chained = 2 * M + disc + disc + 6 * M
+    last_final = 2 * M + disc + disc
     per = {
         "k_step_colinv": 2 * 16 * D + 6 * M,
         "k_step_row": 8 * M,
-        "k_step_colfused": 2 * M + 2 * 16 * (D - 1) + 6 * M,
-        "k_step_colfused_last": 2 * M + 2 * 16 * (D - 1) + 2 * S,
+        "k_step_colfused": 2 * M + disc + 6 * M,
+        "k_step_colfused_last": ((k - 1) * last_chained + last_final) / k,
     }
-    step = per["k_step_colinv"] + 4 * per["k_step_row"] + 3 * per["k_step_colfused"] \
+    step = per["k_step_colinv"] / k + 4 * per["k_step_row"] + 3 * per["k_step_colfused"] \
         + per["k_step_colfused_last"]
     return per, step, {"K": K, "M": M, "D": D, "S": S, "B76S": 76 * S}
 
@@ -102,8 +114,8 @@ def main():
     plan = S.Plan.get(n, dev)
 
     def run(k0, k):
-        for s in range(k0, k0 + k):
-            S.rk4_step(ps, vz, sp, draws[s][0], draws[s][1], dt=dt)
+        # r2d3c_run: k chained steps (bitwise equal to k single steps)
+        S.rk4_run(ps, vz, sp, draws[k0:k0 + k], dt=dt)
 
     if args.copy_probe:
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=dev)
@@ -134,7 +146,8 @@ def main():
     torch.cuda.synchronize()
     finite = bool(torch.isfinite(torch.view_as_real(ps)).all() and torch.isfinite(torch.view_as_real(vz)).all())
 
-    per, step_bytes, geo = algorithmic_bytes(n)
+    per, step_bytes, geo = algorithmic_bytes(n, args.steps)
+    per, _, _ = algorithmic_bytes(n, ktimed)  # the event-timed region is one call of ktimed steps
     kern = {}
     for k, name in enumerate(_lib.KERNEL_NAMES):
         if cnt[k]:

@@ -129,6 +129,19 @@ int r2d3c_step(r2d3c_plan *plan, const r2d3c_params *p, double *psi, double *vz,
                const r2d3c_forcing *forcing, int dt_mode, double dt_or_cfl,
                double *d_dt, void *stream);
 
+/* n_steps consecutive time steps in place, equal bit for bit to n_steps calls
+ * of r2d3c_step.  forcing: array of n_steps draws (one per step, NULL = no
+ * forcing); d_dt: device array of n_steps doubles receiving each step's dt
+ * (may be NULL).  Consecutive steps share work: the last stage of a step also
+ * produces the next step's column transforms, and after the first step only
+ * the dealiased disc of the state is written back (the rest is already
+ * zero).  No reference counterpart: the reference loops over single steps
+ * (src/Rot2D3C.para.f90:393-615); use this when no diagnostics or output
+ * fall between the steps. */
+int r2d3c_run(r2d3c_plan *plan, const r2d3c_params *p, double *psi, double *vz,
+              const r2d3c_forcing *forcing, int n_steps, int dt_mode, double dt_or_cfl,
+              double *d_dt, void *stream);
+
 /* Dense forcing arrays fp (kin = 1) and fz (kin = 0) of one draw for a step
  * of length dt (host value), as rand_force fills them
  * (src/pseudospec2D_Rot2D3C.f90:1100-1122).  Either output may be NULL. */

@@ -176,4 +176,143 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
   __syncthreads();
 }
 
+// ---------------------------------------------------------------------------
+// Four-step transforms for N >= 512: N = 512 * W.  Each wavefront (64 lanes x
+// 8 points) does a 512-point sub-transform whose two exchanges stay inside
+// the wave (ordered LDS traffic of one wave needs no s_barrier), and one
+// radix-W exchange crosses the W waves of the transform.  Two block barriers
+// per transform instead of six, so waves drift apart and overlap their VALU
+// and LDS work.
+//   DIT: input  v[q] of thread b = 64 w + l holds x[w + W (l + 64 q)]
+//        ("interleaved"), output canonical (X[b + (N/8) q]).
+//   DIF: input canonical, output interleaved.
+// A forward DIF followed by an inverse DIT (or the reverse) therefore needs
+// no redistribution between the two.  LDS: W regions of 544 slots (512 + pad)
+// = N + N/16 slots, contiguous per transform.
+// ---------------------------------------------------------------------------
+
+// Orders this wave's LDS writes before its later LDS reads (compiler and
+// memory model); LDS instructions of one wave execute in order.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int S>
+__device__ __forceinline__ void twiddle8(cplx *x, const cplx *__restrict__ tw, int st) {
+  const cplx w1 = twid<S>(tw, st), w2 = twid<S>(tw, 2 * st), w4 = twid<S>(tw, 4 * st);
+  const cplx w3 = cmul(w1, w2);
+  x[1] = cmul(x[1], w1); x[2] = cmul(x[2], w2); x[3] = cmul(x[3], w3);
+  x[4] = cmul(x[4], w4); x[5] = cmul(x[5], cmul(w1, w4));
+  x[6] = cmul(x[6], cmul(w2, w4)); x[7] = cmul(x[7], cmul(w3, w4));
+}
+
+// 512-point transform of one wavefront: lane l holds elements l + 64 q in
+// v[q] on entry and exit.  L: this wave's 544-slot region.  Twiddles
+// W_512^t = tw[t * TS] (tw is the table of length 512 * TS).
+template <int S, int TS>
+__device__ __forceinline__ void fft_wave512(cplx v[8], cplx *L, int l,
+                                            const cplx *__restrict__ tw) {
+  dft8<S>(v);  // span 1
+#pragma unroll
+  for (int m = 0; m < 8; ++m) L[pad(l * 8 + m)] = v[m];
+  wave_lds_sync();
+  {  // span 8
+    cplx x[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) x[m] = L[pad(l + 64 * m)];
+    wave_lds_sync();
+    const int j = l & 7;
+    twiddle8<S>(x, tw, 8 * j * TS);
+    dft8<S>(x);
+    const int base = (l - j) * 8 + j;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) L[pad(base + 8 * m)] = x[m];
+    wave_lds_sync();
+  }
+  {  // span 64, last
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = L[pad(l + 64 * m)];
+    twiddle8<S>(v, tw, l * TS);
+    dft8<S>(v);
+    wave_lds_sync();
+  }
+}
+
+template <int N, int S>
+__device__ __forceinline__ void fft_dit4(cplx v[8], cplx *P, int b, const cplx *__restrict__ tw) {
+  constexpr int W = N / 512, T = N / 8, U = 8 / W;
+  const int w = b >> 6, l = b & 63;
+  cplx *L = P + w * 544;
+  fft_wave512<S, W>(v, L, l, tw);
+  if constexpr (W > 1) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int k2 = l + 64 * q;
+      L[pad(k2)] = w ? cmul(v[q], twid<S>(tw, w * k2)) : v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k2 = b + T * u;
+      cplx x[W];
+#pragma unroll
+      for (int w2 = 0; w2 < W; ++w2) x[w2] = P[w2 * 544 + pad(k2)];
+      dftR<W, S>(x);
+#pragma unroll
+      for (int k1 = 0; k1 < W; ++k1) v[u + U * k1] = x[k1];
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int S>
+__device__ __forceinline__ void fft_dif4(cplx v[8], cplx *P, int b, const cplx *__restrict__ tw) {
+  constexpr int W = N / 512, T = N / 8, U = 8 / W;
+  const int w = b >> 6, l = b & 63;
+  if constexpr (W > 1) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int n2 = b + T * u;
+      cplx x[W];
+#pragma unroll
+      for (int n1 = 0; n1 < W; ++n1) x[n1] = v[u + U * n1];
+      dftR<W, S>(x);
+#pragma unroll
+      for (int k1 = 0; k1 < W; ++k1)
+        P[k1 * 544 + pad(n2)] = k1 ? cmul(x[k1], twid<S>(tw, n2 * k1)) : x[k1];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = P[w * 544 + pad(l + 64 * q)];
+  }
+  fft_wave512<S, W>(v, P + w * 544, l, tw);
+  __syncthreads();
+}
+
+// Element index held in v[q] by thread b in the interleaved distribution
+// (the canonical one for N < 512, where fft_canon is used both ways).
+template <int N> __device__ __forceinline__ int ilv(int b, int q) {
+  if constexpr (N >= 512) {
+    return (b >> 6) + (N / 512) * ((b & 63) + 64 * q);
+  } else {
+    return b + (N / 8) * q;
+  }
+}
+
+// DIF / DIT dispatch: four-step for N >= 512, fft_canon (canonical both ways)
+// below.  P: the transform's LDS area with slot stride IL (IL = 1 required
+// for N >= 512).
+template <int N, int S, int IL>
+__device__ __forceinline__ void fft_dif(cplx v[8], cplx *P, int b, const cplx *__restrict__ tw) {
+  if constexpr (N >= 512) fft_dif4<N, S>(v, P, b, tw);
+  else fft_canon<N, S, IL>(v, P, b, tw);
+}
+template <int N, int S, int IL>
+__device__ __forceinline__ void fft_dit(cplx v[8], cplx *P, int b, const cplx *__restrict__ tw) {
+  if constexpr (N >= 512) fft_dit4<N, S>(v, P, b, tw);
+  else fft_canon<N, S, IL>(v, P, b, tw);
+}
+
 }  // namespace r2d

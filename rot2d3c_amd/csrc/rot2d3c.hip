@@ -441,8 +441,8 @@ __global__ __launch_bounds__(N / 8) void k_step_colinv(const cplx *__restrict__ 
   const cplx *src = (field == 0 ? psi : vz) + (size_t)kx * N;
   cplx s[8];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int j = b + T * q, ky = kyof(j, N);
+  for (int q = 0; q < 8; ++q) {  // inverse transforms take the interleaved order
+    const int j = ilv<N>(b, q), ky = kyof(j, N);
     const double ka2 = (double)kx * kx + (double)ky * ky;
     s[q] = ka2 <= c.g.kmax ? src[j] : mk(0.0, 0.0);
   }
@@ -452,14 +452,14 @@ __global__ __launch_bounds__(N / 8) void k_step_colinv(const cplx *__restrict__ 
     cplx v[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const int ky = kyof(b + T * q, N);
+      const int ky = kyof(ilv<N>(b, q), N);
       const double ka2 = (double)kx * kx + (double)ky * ky;
       cplx a = s[q];
       if (f >= 2) a = mk(-ka2 * a.x, -ka2 * a.y);
       if (f & 1) a = mk(-(double)ky * a.y, (double)ky * a.x);
       v[q] = a;
     }
-    fft_canon<N, +1, 1>(v, lds, b, tw);
+    fft_dit<N, +1, 1>(v, lds, b, tw);
     cplx *dst = dst0 + (size_t)f * c.fs;
 #pragma unroll
     for (int q = 0; q < 8; ++q) dst[tix(b + T * q, kx, c.g.kp)] = v[q];
@@ -553,6 +553,9 @@ __global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS, R2D_ROW_MINW) void k_step
 #ifndef R2D_ROW2_RW
 #define R2D_ROW2_RW 2
 #endif
+#ifndef R2D_ROW_FOURSTEP
+#define R2D_ROW_FOURSTEP 0
+#endif
 #ifndef R2D_ROW2_BPC
 #define R2D_ROW2_BPC 1
 #endif
@@ -567,12 +570,16 @@ template <int N> struct RowV2 {
   static constexpr int THREADS = T * RW;
   static constexpr int SBN = (2 * NCMAX * RW + 63) / 64 * 64;
   static constexpr int G = 4 / RW;  // blocks per y tile
+  // four-step transforms give each row whole waves: stage rows one after the
+  // other; otherwise rows alternate lane by lane and are staged interleaved
+  static constexpr bool ROW_MAJOR = N >= 512 && R2D_ROW_FOURSTEP;
 };
 
 __device__ __forceinline__ void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 // Stage field pair p of the RW rows (tile t, sub-block sub) into sb as
-// sb[f2*nc*RW + kx*RW + rr] = M_{2p+f2}(kx, 4t + sub*RW + rr).  Each wave
+// sb[f2*nc*RW + slot(kx, rr)] = M_{2p+f2}(kx, 4t + sub*RW + rr), with
+// slot = rr*nc + kx (ROW_MAJOR) or kx*RW + rr.  Each wave
 // instruction fills 64 consecutive slots (LDS address = wave base + lane*16).
 template <int N>
 __device__ __forceinline__ void stage_pair(cplx *sb, const cplx *__restrict__ M, size_t fs, int p,
@@ -590,7 +597,9 @@ __device__ __forceinline__ void stage_pair(cplx *sb, const cplx *__restrict__ M,
     const int e = w0 + lane;
     if (e < total) {
       const int f2 = e >= per, rem = e - f2 * per;
-      const int kx = rem / RW, rr = rem - kx * RW;
+      int rr, kx;
+      if constexpr (RowV2<N>::ROW_MAJOR) { rr = rem / nc; kx = rem - rr * nc; }
+      else { kx = rem / RW; rr = rem - kx * RW; }
       const cplx *g = base + (size_t)f2 * fs + kx * 4 + rr;
       __builtin_amdgcn_global_load_lds((const void *)g,
                                        (__attribute__((address_space(3))) void *)(sb + w0), 16, 0,
@@ -611,7 +620,8 @@ __device__ __forceinline__ void build_packed(cplx v[8], const cplx *sb, int r, i
     const bool lo = k < nc, hi = k > N - nc;
     if (lo || hi) {
       const int kk = lo ? k : N - k;
-      const cplx f = sb[kk * RW + r], B = sb[per + kk * RW + r];
+      const int o = RowV2<N>::ROW_MAJOR ? r * nc + kk : kk * RW + r;
+      const cplx f = sb[o], B = sb[per + o];
       const cplx A = mk(-(double)kk * f.y, (double)kk * f.x);
       z = packed_z<N>(k, A, B, hi);
     }
@@ -625,9 +635,17 @@ __global__ __launch_bounds__(RowV2<N>::THREADS) void k_step_row2(
     const cplx *__restrict__ tw) {
   using RV = RowV2<N>;
   constexpr int T = N / 8, RW = RV::RW, G = RV::G;
+  // n >= 512: each hardware wave serves one row (four-step transforms need a
+  // whole wave per 512-point part) and the rows' LDS areas are contiguous;
+  // below, rows interleave lane by lane and slot by slot.
+  constexpr bool WV = N >= 512 && R2D_ROW_FOURSTEP;
+  constexpr int IL = WV ? 1 : RW;
   __shared__ cplx xb[RW * Shape<N>::LDSN];
   __shared__ cplx sb[RV::SBN];
-  const int r = threadIdx.x % RW, b = threadIdx.x / RW;
+  const int tid = threadIdx.x;
+  const int r = WV ? (tid >> 6) % RW : tid % RW;
+  const int b = WV ? ((tid >> 6) / RW) * 64 + (tid & 63) : tid / RW;
+  cplx *P = WV ? xb + r * Shape<N>::LDSN : xb + r;
   const int nc = c.g.nc, kp = c.g.kp;
   const size_t fs = c.fs;
   // tile of iteration k: G consecutive slots of one XCD share a tile
@@ -648,7 +666,7 @@ __global__ __launch_bounds__(RowV2<N>::THREADS) void k_step_row2(
     build_packed<N>(v, sb, r, b, nc);
     __syncthreads();
     stage_pair<N>(sb, M, fs, 1, t, sub, nc, kp);
-    fft_canon<N, +1, RW>(v, xb + r, b, tw);
+    if constexpr (WV) fft_dif<N, +1, IL>(v, P, b, tw); else fft_canon<N, +1, IL>(v, P, b, tw);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       px[q] = v[q].x;
@@ -660,7 +678,7 @@ __global__ __launch_bounds__(RowV2<N>::THREADS) void k_step_row2(
     build_packed<N>(v, sb, r, b, nc);
     __syncthreads();
     stage_pair<N>(sb, M, fs, 2, t, sub, nc, kp);
-    fft_canon<N, +1, RW>(v, xb + r, b, tw);
+    if constexpr (WV) fft_dif<N, +1, IL>(v, P, b, tw); else fft_canon<N, +1, IL>(v, P, b, tw);
 #pragma unroll
     for (int q = 0; q < 8; ++q) j1[q] = px[q] * v[q].y - py[q] * v[q].x;
     wait_vm();
@@ -669,7 +687,7 @@ __global__ __launch_bounds__(RowV2<N>::THREADS) void k_step_row2(
     __syncthreads();
     const int tn = tile_of(k + 1);
     if (tn < NTILES) stage_pair<N>(sb, M, fs, 0, tn, sub, nc, kp);
-    fft_canon<N, +1, RW>(v, xb + r, b, tw);
+    if constexpr (WV) fft_dif<N, +1, IL>(v, P, b, tw); else fft_canon<N, +1, IL>(v, P, b, tw);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const double vx = v[q].x, vy = v[q].y;
@@ -677,11 +695,11 @@ __global__ __launch_bounds__(RowV2<N>::THREADS) void k_step_row2(
       const double p2 = ((px[q] * vy - py[q] * vx) + c.two_omega_n2 * py[q]) * c.inv_n4;
       v[q] = mk(p1, p2);
     }
-    fft_canon<N, -1, RW>(v, xb + r, b, tw);
+    if constexpr (WV) fft_dit<N, -1, IL>(v, P, b, tw); else fft_canon<N, -1, IL>(v, P, b, tw);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) xb[pad(b + T * q) * RW + r] = v[q];
+    for (int q = 0; q < 8; ++q) P[pad(b + T * q) * IL] = v[q];
     __syncthreads();
-    unpack_store<N, RW>(xb + r, b, nc, kp, Q, y, Q + fs, y);
+    unpack_store<N, IL>(P, b, nc, kp, Q, y, Q + fs, y);
     if (tn >= NTILES) break;
     wait_vm();
     __syncthreads();
@@ -697,14 +715,26 @@ __global__ __launch_bounds__(RowV2<N>::THREADS) void k_step_row2(
 
 // Fused column kernel of stage o: column forward of Q (psi blocks: Q0, vz
 // blocks: Q1), the semi-implicit update of src/Rot2D3C.para.f90:578-599 with
-// the forcing, then either the next stage's column inverse (as
-// k_step_colinv) or, after the last stage, the state write psi = C1, vz = C2
-// (src/Rot2D3C.para.f90:610-611) including zero columns kx > K.
+// the forcing, then, by `mode`:
+//   FM_EMIT   the inverse column transforms of the updated fields (the next
+//             stage's, or the next step's first stage's, mixed fields);
+//   FM_STATE  the state write psi = C1, vz = C2 (src/Rot2D3C.para.f90:610-611),
+//             every ky of the column, or with FM_DISK only the entries inside
+//             the update mask (the rest is known to be zero already);
+//   FM_ZERO   extra blocks zero the columns kx > K of both fields.
+// The forward transform is always the same (DIF, leaving the interleaved order
+// the inverse transforms take), so every path rounds alike: a run of chained
+// steps equals single steps bit for bit.  The state goes back through one LDS
+// exchange to the canonical order for coalesced stores.  Blocks with o == 4
+// write dt and clear the CFL slot of the next step (mreset).
+enum { FM_EMIT = 1, FM_STATE = 2, FM_DISK = 4, FM_ZERO = 8 };
+
 template <int N>
 __global__ __launch_bounds__(N / 8) void k_step_colfused(
     const cplx *__restrict__ Q, cplx *psi, cplx *vz, cplx *__restrict__ M, StepConst c, int o,
-    int last, const unsigned long long *__restrict__ maxbits, double *dt_out,
+    int mode, const unsigned long long *mread, unsigned long long *mreset, double *dt_out,
     const cplx *__restrict__ tw) {
+  auto pos = [](int b, int q) { return ilv<N>(b, q); };
   constexpr int T = N / 8;
   __shared__ cplx lds[Shape<N>::LDSN];
   const int lb = xcd_pairs(blockIdx.x), b = threadIdx.x;
@@ -712,7 +742,7 @@ __global__ __launch_bounds__(N / 8) void k_step_colfused(
   int field, kx;
   if (lb < nc) { field = 0; kx = lb; }
   else if (lb >= ncp && lb < ncp + nc) { field = 1; kx = lb - ncp; }
-  else if (last && lb >= 2 * ncp && lb < 2 * ncp + (N / 2 + 1 - nc)) {
+  else if ((mode & FM_ZERO) && lb >= 2 * ncp && lb < 2 * ncp + (N / 2 + 1 - nc)) {
     kx = nc + (lb - 2 * ncp);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -722,19 +752,24 @@ __global__ __launch_bounds__(N / 8) void k_step_colfused(
     return;
   } else return;
 
-  const double dt = c.cfl_mode ? dt_from_max(c.dc, *maxbits) : c.dt_fixed;
-  if (lb == 0 && b == 0 && o == 4 && dt_out) *dt_out = dt;
+  const double dt = c.cfl_mode ? dt_from_max(c.dc, *mread) : c.dt_fixed;
+  if (lb == 0 && b == 0 && o == 4) {
+    if (dt_out) *dt_out = dt;
+    if (mreset) *mreset = 0ull;
+  }
 
   cplx v[8];
   const cplx *Qf = Q + (size_t)field * c.fs;
 #pragma unroll
   for (int q = 0; q < 8; ++q) v[q] = Qf[tix(b + T * q, kx, c.g.kp)];
-  fft_canon<N, -1, 1>(v, lds, b, tw);
+  fft_dif<N, -1, 1>(v, lds, b, tw);  // v[q] = spectrum at ky index pos(b, q)
 
   const ForceSet &fo = field == 0 ? c.fp : c.fz;
-  cplx fval[2] = {mk(0.0, 0.0), mk(0.0, 0.0)};
-  for (int e = 0; e < fo.nent; ++e)
-    if (fo.e[e].i == kx) fval[e] = force_value(fo, e, dt, c.n2);
+  // at most two forced modes per field (scalars: no private-memory array)
+  const bool f0 = fo.nent > 0 && fo.e[0].i == kx, f1 = fo.nent > 1 && fo.e[1].i == kx;
+  const cplx fv0 = f0 ? force_value(fo, 0, dt, c.n2) : mk(0.0, 0.0);
+  const cplx fv1 = f1 ? force_value(fo, 1, dt, c.n2) : mk(0.0, 0.0);
+  const int fj0 = f0 ? fo.e[0].j : -1, fj1 = f1 ? fo.e[1].j : -1;
 
   cplx *state = field == 0 ? psi : vz;
   const double tmp1 = dt / (double)o;
@@ -743,15 +778,13 @@ __global__ __launch_bounds__(N / 8) void k_step_colfused(
   cplx s[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const int j = b + T * q, ky = kyof(j, N);
+    const int j = pos(b, q), ky = kyof(j, N);
     const double ka2 = (double)kx * kx + (double)ky * ky;
     cplx cc = mk(0.0, 0.0);
     if (ka2 <= c.g.kmax && ka2 > c.g.tiny) {
       const double tmp2 = 1.0 / ka2;
       const double tmp3 = 1.0 + (nu * ipow(ka2, nn) + hnu * ipow(tmp2, mm)) * tmp1;
-      cplx f = mk(0.0, 0.0);
-      for (int e = 0; e < fo.nent; ++e)
-        if (fo.e[e].i == kx && fo.e[e].j == j) f = fval[e];
+      const cplx f = j == fj0 ? fv0 : (j == fj1 ? fv1 : mk(0.0, 0.0));
       const cplx s0 = state[(size_t)kx * N + j];
       cplx t = field == 0 ? cscale(v[q], tmp2) : v[q];
       t = cadd(t, f);
@@ -760,28 +793,41 @@ __global__ __launch_bounds__(N / 8) void k_step_colfused(
     }
     s[q] = cc;
   }
-  if (last) {
+  if (mode & FM_STATE) {
+    // interleaved -> canonical through LDS, then coalesced stores
 #pragma unroll
-    for (int q = 0; q < 8; ++q) state[(size_t)kx * N + b + T * q] = s[q];
-    return;
-  }
-  const int nf = field == 0 ? 4 : 2;
-  cplx *dst0 = M + (size_t)(field == 0 ? 0 : 4) * c.fs;
-  for (int f = 0; f < nf; ++f) {
-    cplx w[8];
+    for (int q = 0; q < 8; ++q) lds[pad(pos(b, q))] = s[q];
+    __syncthreads();
+    const bool disk = (mode & FM_DISK) != 0;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const int ky = kyof(b + T * q, N);
+      const int j = b + T * q, ky = kyof(j, N);
       const double ka2 = (double)kx * kx + (double)ky * ky;
-      cplx a = s[q];
-      if (f >= 2) a = mk(-ka2 * a.x, -ka2 * a.y);
-      if (f & 1) a = mk(-(double)ky * a.y, (double)ky * a.x);
-      w[q] = a;
+      if (!disk || (ka2 <= c.g.kmax && ka2 > c.g.tiny))
+        state[(size_t)kx * N + j] = lds[pad(j)];
     }
-    fft_canon<N, +1, 1>(w, lds, b, tw);
-    cplx *dst = dst0 + (size_t)f * c.fs;
+    __syncthreads();
+  }
+  {
+    if (!(mode & FM_EMIT)) return;
+    const int nf = field == 0 ? 4 : 2;
+    cplx *dst0 = M + (size_t)(field == 0 ? 0 : 4) * c.fs;
+    for (int f = 0; f < nf; ++f) {
+      cplx w[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) dst[tix(b + T * q, kx, c.g.kp)] = w[q];
+      for (int q = 0; q < 8; ++q) {
+        const int ky = kyof(ilv<N>(b, q), N);
+        const double ka2 = (double)kx * kx + (double)ky * ky;
+        cplx a = s[q];
+        if (f >= 2) a = mk(-ka2 * a.x, -ka2 * a.y);
+        if (f & 1) a = mk(-(double)ky * a.y, (double)ky * a.x);
+        w[q] = a;
+      }
+      fft_dit<N, +1, 1>(w, lds, b, tw);
+      cplx *dst = dst0 + (size_t)f * c.fs;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) dst[tix(b + T * q, kx, c.g.kp)] = w[q];
+    }
   }
 }
 
@@ -951,56 +997,86 @@ template <int N> static int row2_grid(int cus) {
   return g;
 }
 
+static int make_step_const(r2d3c_plan *p, const r2d3c_params *q, int dt_mode, double dt_or_cfl,
+                           StepConst *c) {
+  std::memset(c, 0, sizeof(*c));
+  const int n = p->n;
+  c->g = p->g;
+  c->fs = p->fs;
+  c->nu = q->nu; c->hnu = q->hnu; c->nuv = q->nuv; c->hnuv = q->hnuv;
+  c->nn = q->nn; c->mm = q->mm; c->nnv = q->nnv; c->mmv = q->mmv;
+  c->two_omega = 2 * q->omega;
+  c->cfl_mode = dt_mode == R2D3C_DT_CFL;
+  c->dt_fixed = dt_or_cfl;
+  c->dc = make_dt_const(n, q, dt_or_cfl);
+  c->n2 = (double)n * (double)n;
+  c->two_omega_n2 = c->two_omega * c->n2;
+  c->inv_n4 = 1.0 / (c->n2 * c->n2);
+  return 0;
+}
+
+// Kernels of one time step.  first: the state's column transforms are made
+// here (otherwise the previous step's last stage left them in M); chain: the
+// last stage also emits the next step's column transforms; masked: the state
+// outside the update mask is known to be zero (written by an earlier step of
+// the same run), so only the disc is written back.  slot: CFL maximum slot of
+// this step (steps alternate, each clears the other's at its first stage).
 template <int N>
-static int run_step(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
-                    const r2d3c_forcing *fo, int dt_mode, double dt_or_cfl, double *d_dt,
-                    hipStream_t s) {
-  StepConst c;
-  std::memset(&c, 0, sizeof(c));
-  c.g = p->g;
-  c.fs = p->fs;
-  c.nu = q->nu; c.hnu = q->hnu; c.nuv = q->nuv; c.hnuv = q->hnuv;
-  c.nn = q->nn; c.mm = q->mm; c.nnv = q->nnv; c.mmv = q->mmv;
-  c.two_omega = 2 * q->omega;
-  c.cfl_mode = dt_mode == R2D3C_DT_CFL;
-  c.dt_fixed = dt_or_cfl;
-  c.dc = make_dt_const(N, q, dt_or_cfl);
-  c.n2 = (double)N * (double)N;
-  c.two_omega_n2 = c.two_omega * c.n2;
-  c.inv_n4 = 1.0 / (c.n2 * c.n2);
-  if (fo) {
-    int rc = make_force_set(&fo->psi, N, &c.fp);
-    if (rc) return rc;
-    rc = make_force_set(&fo->vz, N, &c.fz);
-    if (rc) return rc;
-  }
+static int step_seq(r2d3c_plan *p, StepConst &c, cplx *psi, cplx *vz, hipStream_t s,
+                    double *dt_out, bool first, bool chain, bool masked, int slot) {
   const int nc = p->g.nc, ncp = (nc + 1) & ~1;
   cplx *M = p->mstep, *Q = p->mstep + 6 * p->fs;
   constexpr int RW = RowStep<N>::ROWS;
-  double *dt_out = d_dt ? d_dt : p->dtslot;
   const cplx *tw = p->tw;
+  unsigned long long *mb = p->maxbits + slot, *mother = p->maxbits + (slot ^ 1);
   hipEvent_t ea, eb;
-  ev_pair(p, 0, &ea, &eb);
-  hipExtLaunchKernelGGL(k_step_colinv<N>, dim3(round16(ncp + nc)), dim3(N / 8), 0, s, ea, eb, 0,
-                        (const cplx *)psi, (const cplx *)vz, M, c, p->maxbits, tw);
+  if (first) {
+    ev_pair(p, 0, &ea, &eb);
+    hipExtLaunchKernelGGL(k_step_colinv<N>, dim3(round16(ncp + nc)), dim3(N / 8), 0, s, ea, eb, 0,
+                          (const cplx *)psi, (const cplx *)vz, M, c, mb, tw);
+  }
   for (int o = 4; o >= 1; --o) {
-    unsigned long long *mb = (o == 4 && c.cfl_mode) ? p->maxbits : nullptr;
+    unsigned long long *macc = (o == 4 && c.cfl_mode) ? mb : nullptr;
     ev_pair(p, 1, &ea, &eb);
     if constexpr (RowV2<N>::THREADS >= 64) {
       hipExtLaunchKernelGGL(k_step_row2<N>, dim3(row2_grid<N>(p->cus)), dim3(RowV2<N>::THREADS), 0,
-                            s, ea, eb, 0, (const cplx *)M, Q, c, mb, tw);
+                            s, ea, eb, 0, (const cplx *)M, Q, c, macc, tw);
     } else {
       hipExtLaunchKernelGGL(k_step_row<N>, dim3(N / RW), dim3(N / 8 * RW), 0, s, ea, eb, 0,
-                            (const cplx *)M, Q, c, mb, tw);
+                            (const cplx *)M, Q, c, macc, tw);
     }
-    const int last = o == 1;
-    const int nblk = round16(2 * ncp + (last ? (N / 2 + 1 - nc) : 0));
-    ev_pair(p, last ? 3 : 2, &ea, &eb);
+    int mode = FM_EMIT;
+    if (o == 1) mode = FM_STATE | (masked ? FM_DISK : FM_ZERO) | (chain ? FM_EMIT : 0);
+    const int nblk = round16(2 * ncp + ((mode & FM_ZERO) ? (N / 2 + 1 - nc) : 0));
+    unsigned long long *mres = o == 4 ? mother : nullptr;
+    ev_pair(p, o == 1 ? 3 : 2, &ea, &eb);
     hipExtLaunchKernelGGL(k_step_colfused<N>, dim3(nblk), dim3(N / 8), 0, s, ea, eb, 0,
-                          (const cplx *)Q, psi, vz, M, c, o, last,
-                          (const unsigned long long *)p->maxbits, dt_out, tw);
+                          (const cplx *)Q, psi, vz, M, c, o, mode,
+                          (const unsigned long long *)mb, mres, dt_out, tw);
   }
   LAUNCHCHK("step");
+  return 0;
+}
+
+template <int N>
+static int run_steps(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
+                     const r2d3c_forcing *fo, int nsteps, int dt_mode, double dt_or_cfl,
+                     double *d_dt, hipStream_t s) {
+  StepConst c;
+  make_step_const(p, q, dt_mode, dt_or_cfl, &c);
+  for (int st = 0; st < nsteps; ++st) {
+    std::memset(&c.fp, 0, sizeof(c.fp));
+    std::memset(&c.fz, 0, sizeof(c.fz));
+    if (fo) {
+      int rc = make_force_set(&fo[st].psi, N, &c.fp);
+      if (rc) return rc;
+      rc = make_force_set(&fo[st].vz, N, &c.fz);
+      if (rc) return rc;
+    }
+    double *dt_out = d_dt ? d_dt + st : p->dtslot;
+    int rc = step_seq<N>(p, c, psi, vz, s, dt_out, st == 0, st + 1 < nsteps, st > 0, st & 1);
+    if (rc) return rc;
+  }
   return 0;
 }
 
@@ -1182,8 +1258,23 @@ int r2d3c_step(r2d3c_plan *p, const r2d3c_params *q, double *psi, double *vz,
   if (dt_mode != R2D3C_DT_CFL && dt_mode != R2D3C_DT_FIXED)
     return set_err(R2D3C_EINVAL, "dt_mode must be R2D3C_DT_CFL or R2D3C_DT_FIXED");
   if (psi == vz) return set_err(R2D3C_EINVAL, "psi and vz must not alias");
-  R2D_DISPATCH(p->n, run_step, p, q, (cplx *)psi, (cplx *)vz, forcing, dt_mode, dt_or_cfl, d_dt,
-               (hipStream_t)stream);
+  R2D_DISPATCH(p->n, run_steps, p, q, (cplx *)psi, (cplx *)vz, forcing, 1, dt_mode, dt_or_cfl,
+               d_dt, (hipStream_t)stream);
+}
+
+int r2d3c_run(r2d3c_plan *p, const r2d3c_params *q, double *psi, double *vz,
+              const r2d3c_forcing *forcing, int n_steps, int dt_mode, double dt_or_cfl,
+              double *d_dt, void *stream) {
+  NEED(p); NEED(psi); NEED(vz);
+  int rc = check_params(q);
+  if (rc) return rc;
+  if (dt_mode != R2D3C_DT_CFL && dt_mode != R2D3C_DT_FIXED)
+    return set_err(R2D3C_EINVAL, "dt_mode must be R2D3C_DT_CFL or R2D3C_DT_FIXED");
+  if (psi == vz) return set_err(R2D3C_EINVAL, "psi and vz must not alias");
+  if (n_steps < 0) return set_err(R2D3C_EINVAL, "n_steps must be >= 0");
+  if (n_steps == 0) return 0;
+  R2D_DISPATCH(p->n, run_steps, p, q, (cplx *)psi, (cplx *)vz, forcing, n_steps, dt_mode,
+               dt_or_cfl, d_dt, (hipStream_t)stream);
 }
 
 int r2d3c_forcing_fill(r2d3c_plan *p, const r2d3c_forcing *fo, double dt, double *fp, double *fz,

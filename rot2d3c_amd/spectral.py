@@ -253,6 +253,39 @@ def rk4_step(ps, vz, p, fp_draw=None, fz_draw=None, dt=None, cfl=None, dt_out=No
                                 mode, float(val), optr, _stream(ps.device)), "r2d3c_step")
 
 
+def rk4_run(ps, vz, p, draws=None, nsteps=None, dt=None, cfl=None, dt_out=None):
+    """`nsteps` time steps in place, bitwise equal to as many rk4_step calls.
+
+    draws: sequence of (fp_draw, fz_draw) per step (None entries allowed), or
+    None for no forcing.  dt_out: float64 CUDA tensor with at least nsteps
+    elements, receiving each step's dt."""
+    n = _spec_n(ps)
+    if _spec_n(vz) != n:
+        raise ValueError("size mismatch")
+    if (dt is None) == (cfl is None):
+        raise ValueError("give exactly one of dt and cfl")
+    if nsteps is None:
+        if draws is None:
+            raise ValueError("give nsteps or draws")
+        nsteps = len(draws)
+    if draws is not None and len(draws) < nsteps:
+        raise ValueError("fewer forcing draws than steps")
+    if dt_out is not None and (dt_out.numel() < nsteps or dt_out.dtype != torch.float64):
+        raise ValueError("dt_out needs nsteps float64 elements")
+    pl = Plan.get(n, ps.device)
+    cp = p.c()
+    arr = None
+    if draws is not None:
+        arr = (_lib.Forcing * nsteps)()
+        for k in range(nsteps):
+            a, b = draws[k]
+            arr[k] = _forcing_c(a, b)
+    mode, val = (R2D3C_DT_FIXED, dt) if dt is not None else (R2D3C_DT_CFL, cfl)
+    optr = _ptr(dt_out) if dt_out is not None else ctypes.c_void_p(0)
+    check(_lib.lib().r2d3c_run(pl.handle, ctypes.byref(cp), _ptr(ps), _ptr(vz), arr, int(nsteps),
+                               mode, float(val), optr, _stream(ps.device)), "r2d3c_run")
+
+
 def forcing_fill(n, device, fp_draw, fz_draw, dt):
     """Dense fp, fz arrays of one draw, as rand_force fills them
     (src/pseudospec2D_Rot2D3C.f90:1100-1122)."""
@@ -304,6 +337,7 @@ def kmax_index(n):
 __all__ = [
     "Plan", "Params", "ForceDraw", "fftp2d_complex_to_real", "fftp2d_real_to_complex",
     "derivk2", "laplak2", "inv_laplak2", "poisson", "energy", "inerprod", "energy_device",
-    "inerprod_device", "CFL_condition", "CFL_condition_device", "rk4_step", "forcing_fill",
+    "inerprod_device", "CFL_condition", "CFL_condition_device", "rk4_step", "rk4_run",
+    "forcing_fill",
     "randu", "rand_force_draw", "rand_force", "kmax_index",
 ]
