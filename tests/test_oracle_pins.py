@@ -1,0 +1,354 @@
+"""Pins of the CPU oracle (oracle/spectral.py) against facts that do not
+restate its formulas: brute-force DFT sums, analytic derivatives and Jacobians
+of single Fourier modes, Parseval in real space, the Park-Miller known answer,
+the exact limits of the time scheme (Taylor polynomial of the linear
+propagator, backward Euler for pure dissipation) and conservation laws.
+Each is chosen so that a sign, index, transposition or normalisation slip in
+the oracle fails at least one of them."""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from conftest import random_state, rel_l2
+from oracle import spectral as O
+
+TWO_PI = 2.0 * math.pi
+
+
+def _grid_xy(n):
+    x = TWO_PI * np.arange(n) / n
+    X, Y = np.meshgrid(x, x)  # X[y][x] = x_i, Y[y][x] = y_j
+    return X, Y
+
+
+# ---------------------------------------------------------------- transforms
+def _brute_c2r(c):
+    """Reference order: complex backward sum along ky per kx column, then the
+    real c2r along x using only Re at kx = 0 and kx = n/2."""
+    n = c.shape[1]
+    j = np.arange(n)
+    ky = np.where(j < n // 2, j, j - n)
+    F = np.zeros((n // 2 + 1, n), complex)  # F[kx][y]
+    for y in range(n):
+        F[:, y] = (c * np.exp(2j * math.pi * ky[None, :] * y / n)).sum(axis=1)
+    R = np.zeros((n, n))
+    for y in range(n):
+        for x in range(n):
+            acc = F[0, y].real + F[n // 2, y].real * (-1) ** x
+            for kx in range(1, n // 2):
+                acc += 2.0 * (F[kx, y] * np.exp(2j * math.pi * kx * x / n)).real
+            R[y, x] = acc
+    return R
+
+
+def _brute_r2c(r):
+    n = r.shape[0]
+    out = np.zeros((n // 2 + 1, n), complex)
+    xs = np.arange(n)
+    for kx in range(n // 2 + 1):
+        for j in range(n):
+            ky = j if j < n // 2 else j - n
+            ph = np.exp(-2j * math.pi * (kx * xs[None, :] + ky * xs[:, None]) / n)
+            out[kx, j] = (r * ph).sum()
+    return out
+
+
+@pytest.mark.parametrize("n", [8, 16])
+def test_c2r_r2c_equal_brute_force_dft(n):
+    rng = np.random.default_rng(1)
+    c = rng.standard_normal((n // 2 + 1, n)) + 1j * rng.standard_normal((n // 2 + 1, n))
+    assert rel_l2(O.fftp2d_complex_to_real(c), _brute_c2r(c)) < 1e-13
+    r = rng.standard_normal((n, n))
+    assert rel_l2(O.fftp2d_real_to_complex(r), _brute_r2c(r)) < 1e-13
+
+
+def test_transforms_of_delta_and_cosine():
+    n = 16
+    r = np.zeros((n, n))
+    x0, y0 = 3, 5
+    r[y0, x0] = 1.0
+    c = O.fftp2d_real_to_complex(r)
+    kx = np.arange(n // 2 + 1)[:, None]
+    j = np.arange(n)
+    ky = np.where(j < n // 2, j, j - n)[None, :]
+    assert np.allclose(c, np.exp(-2j * math.pi * (kx * x0 + ky * y0) / n), atol=1e-14)
+    X, Y = _grid_xy(n)
+    a, b = 2, -3
+    c = O.fftp2d_real_to_complex(np.cos(a * X + b * Y))
+    expect = np.zeros_like(c)
+    expect[a, b % n] = n * n / 2
+    assert np.allclose(c, expect, atol=1e-11)
+    assert np.allclose(O.fftp2d_complex_to_real(c), n * n * np.cos(a * X + b * Y), atol=1e-10)
+
+
+# ---------------------------------------------------------------- operators
+@pytest.mark.parametrize("a,b", [(2, 3), (3, -1), (0, 4), (5, 0), (1, -5)])
+def test_derivatives_of_single_modes(a, b):
+    n = 32
+    X, Y = _grid_xy(n)
+    ph = 0.7
+    f = np.cos(a * X + b * Y + ph)
+    F = O.fftp2d_real_to_complex(f)
+    n2 = float(n * n)
+    dfx = O.fftp2d_complex_to_real(O.derivk2(F, 1)) / n2
+    dfy = O.fftp2d_complex_to_real(O.derivk2(F, 2)) / n2
+    assert np.allclose(dfx, -a * np.sin(a * X + b * Y + ph), atol=1e-12)
+    assert np.allclose(dfy, -b * np.sin(a * X + b * Y + ph), atol=1e-12)
+    lap = O.fftp2d_complex_to_real(O.laplak2(F)) / n2
+    assert np.allclose(lap, -(a * a + b * b) * f, atol=1e-11)
+    inv = O.fftp2d_complex_to_real(O.inv_laplak2(F)) / n2
+    assert np.allclose(inv, -f / (a * a + b * b), atol=1e-13)
+
+
+def test_derivk2_removes_modes_beyond_two_thirds():
+    n = 32
+    F = np.zeros((n // 2 + 1, n), complex)
+    F[11, 2] = 3.0 - 1.0j  # |k|^2 = 125 > (32/3)^2 = 113.8: removed
+    F[2, n - 11] = 1.0     # same shell, negative ky
+    F[10, 3] = 2.0 + 2.0j  # 109 <= 113.8: kept
+    for d in (0, 1, 2):
+        out = O.derivk2(F, d)
+        assert out[11, 2] == 0 and out[2, n - 11] == 0
+        assert out[10, 3] != 0
+    assert O.derivk2(F, 0)[10, 3] == F[10, 3]
+    assert O.derivk2(F, 1)[10, 3] == 10j * F[10, 3]
+    assert O.derivk2(F, 2)[10, 3] == 3j * F[10, 3]
+
+
+@pytest.mark.parametrize("ka,kb", [((2, 1), (1, 3)), ((0, 2), (3, -1)), ((3, -2), (-1, -2))])
+def test_poisson_equals_analytic_jacobian(ka, kb):
+    n = 32
+    X, Y = _grid_xy(n)
+    t1 = ka[0] * X + ka[1] * Y + 0.3
+    t2 = kb[0] * X + kb[1] * Y - 1.1
+    A = np.cos(t1)
+    B = np.sin(t2)
+    # J(A,B) = A_x B_y - A_y B_x
+    J = (-ka[0] * np.sin(t1)) * (kb[1] * np.cos(t2)) - (-ka[1] * np.sin(t1)) * (kb[0] * np.cos(t2))
+    c = O.poisson(O.fftp2d_real_to_complex(A), O.fftp2d_real_to_complex(B))
+    assert np.allclose(O.fftp2d_complex_to_real(c) / float(n * n), J, atol=1e-11)
+
+
+def test_poisson_galerkin_invariants():
+    n = 32
+    ps, vz = random_state(n, 3)
+    w = O.laplak2(ps)
+    j1 = O.poisson(ps, w)
+    j2 = O.poisson(ps, vz)
+    scale = math.sqrt(O.energy(j1, 0) * O.energy(ps, 0))
+    assert abs(O.inerprod(ps, j1, 0)) < 1e-12 * scale
+    assert abs(O.inerprod(w, j1, 0)) < 1e-12 * math.sqrt(O.energy(j1, 0) * O.energy(w, 0))
+    assert abs(O.inerprod(vz, j2, 0)) < 1e-12 * math.sqrt(O.energy(j2, 0) * O.energy(vz, 0))
+
+
+def test_energy_and_inerprod_by_parseval():
+    n = 32
+    ps, vz = random_state(n, 5)
+    n2 = float(n * n)
+    px = O.fftp2d_complex_to_real(O.derivk2(ps, 1)) / n2
+    py = O.fftp2d_complex_to_real(O.derivk2(ps, 2)) / n2
+    p = O.fftp2d_complex_to_real(ps) / n2
+    v = O.fftp2d_complex_to_real(vz) / n2
+    w = O.fftp2d_complex_to_real(O.laplak2(ps)) / n2
+    assert math.isclose(O.energy(ps, 1), np.mean(px * px + py * py), rel_tol=1e-12)
+    assert math.isclose(O.energy(ps, 0), np.mean(p * p), rel_tol=1e-12)
+    assert math.isclose(O.energy(ps, 2), np.mean(w * w), rel_tol=1e-12)
+    assert math.isclose(O.inerprod(ps, vz, 0), np.mean(p * v), rel_tol=1e-10, abs_tol=1e-14)
+    assert math.isclose(O.inerprod(vz, ps, 1), np.mean(px * (O.fftp2d_complex_to_real(O.derivk2(vz, 1)) / n2)
+                                                     + py * (O.fftp2d_complex_to_real(O.derivk2(vz, 2)) / n2)),
+                        rel_tol=1e-10, abs_tol=1e-14)
+
+
+def test_cfl_max_of_single_mode():
+    n = 32
+    X, Y = _grid_xy(n)
+    a, b = 3, 4
+    ps = O.fftp2d_real_to_complex(0.25 * np.cos(a * X + b * Y))
+    # |grad psi| = 0.25*5*|sin| attains 1.25 on the grid (sin = 1 at some point)
+    m = O.cfl_max_grad2(ps)
+    assert math.isclose(math.sqrt(m) / n ** 2, 1.25, rel_tol=1e-12)
+
+
+# ---------------------------------------------------------------- random numbers
+def test_park_miller_known_answer():
+    """Park & Miller (1988): the minimal standard generator started at 1
+    reaches 1043618065 after 10000 draws.  randu keeps its state XOR-masked."""
+    mask = 123459876
+    idum = 1 ^ mask
+    for _ in range(10000):
+        _, idum = O.randu(idum)
+    assert idum ^ mask == 1043618065
+
+
+def _f32(fr):
+    """round a Fraction to the nearest float32 (ties to even) exactly"""
+    import struct
+    x = float(fr)  # exact enough to find the neighbours
+    f = struct.unpack("f", struct.pack("f", x))[0]
+    cands = [f, np.nextafter(np.float32(f), np.float32(np.inf)), np.nextafter(np.float32(f), np.float32(-np.inf))]
+    best = min(cands, key=lambda c: (abs(Fraction(float(c)) - fr), int(np.float32(c).view(np.int32)) & 1))
+    return Fraction(float(best))
+
+
+def test_randu_float32_rounding_chain():
+    idum = 12345
+    for _ in range(200):
+        r, idum2 = O.randu(idum)
+        state = idum2 ^ 123459876
+        am = Fraction(1, 2 ** 31)  # REAL am = 1./2147483647 rounds to 2**-31
+        t = _f32(_f32(Fraction(state)) * am)
+        t = _f32(t - Fraction(1, 2))
+        t = _f32(t * 2)
+        assert Fraction(float(r)) == t
+        assert -1.0 <= r < 1.0
+        idum = idum2
+
+
+def test_rand_force_field_is_the_drawn_cosine():
+    n, kup, dt, fp0 = 32, 5.0, 1e-3, 0.7
+    seed = 4
+    hits = 0
+    for _ in range(30):
+        kx, ky, ph, _ = O.rand_force_draw(kup, seed)
+        fp, seed = O.rand_force(kup, kup, fp0, dt, seed, 1, n)
+        amp = math.sqrt(2 * fp0) / (math.sqrt(dt) * kup)
+        X, Y = _grid_xy(n)
+        f = O.fftp2d_complex_to_real(fp) / float(n * n)
+        if ky != 0:
+            # column |ky| (x wavenumber), full index kx (y wavenumber)
+            expect = 2 * amp * np.cos(abs(ky) * X + kx * Y + ph)
+            hits += 1
+        elif kx > 0:
+            expect = 2 * amp * np.cos(kx * Y + ph)
+            hits += 1
+        else:
+            expect = np.zeros((n, n))
+        assert np.allclose(f, expect, atol=1e-9 * max(1.0, amp))
+    assert hits > 10
+
+
+# ---------------------------------------------------------------- time scheme
+def _single_mode(n, kx, ky, cpsi, cvz):
+    ps = np.zeros((n // 2 + 1, n), complex)
+    vz = np.zeros_like(ps)
+    ps[kx, ky % n] = cpsi
+    vz[kx, ky % n] = cvz
+    return ps, vz
+
+
+@pytest.mark.parametrize("kx,ky", [(2, 3), (3, -1), (1, 1)])
+def test_inertial_wave_is_taylor_polynomial_of_propagator(kx, ky):
+    """nu = nu+ = 0, no forcing, one wavevector: both brackets vanish and the
+    4-stage scheme is exactly sum_m (dt L)^m/m! of the linear operator
+    L = [[0, 2 Omega i ky/k^2], [2 Omega i ky, 0]]; it approximates
+    exp(dt L) to O(dt^5)."""
+    n, om = 32, 1.7
+    k2 = kx * kx + ky * ky
+    L = np.array([[0, 2 * om * 1j * ky / k2], [2 * om * 1j * ky, 0]])
+    x0 = np.array([3.0 + 1.0j, -2.0 + 0.5j]) * n * n
+    p = O.Params(0, 0, 1, 2, 0, 0, 1, 2, om)
+    errs = []
+    for dt in (0.1, 0.05):
+        ps, vz = _single_mode(n, kx, ky, x0[0], x0[1])
+        a, b = O.rk_stages(ps, vz, 0 * ps, 0 * vz, dt, p)
+        got = np.array([a[kx, ky % n], b[kx, ky % n]])
+        T4 = sum(np.linalg.matrix_power(dt * L, m) / math.factorial(m) for m in range(5))
+        assert np.allclose(got, T4 @ x0, rtol=1e-12, atol=1e-9)
+        w, V = np.linalg.eig(dt * L)
+        ex = (V @ np.diag(np.exp(w)) @ np.linalg.inv(V)) @ x0
+        errs.append(np.abs(got - ex).max())
+    assert 20 < errs[0] / errs[1] < 40  # fifth-order local error
+
+
+def test_pure_dissipation_is_backward_euler():
+    n = 32
+    kx, ky = 3, -2
+    k2 = float(kx * kx + ky * ky)
+    p = O.Params(0.01, 0.3, 2, 1, 0.02, 0.1, 1, 3, 0.0)
+    ps, vz = _single_mode(n, kx, ky, 5.0 + 2.0j, -1.0 + 4.0j)
+    dt = 0.05
+    a, b = O.rk_stages(ps, vz, 0 * ps, 0 * vz, dt, p)
+    dpsi = 0.01 * k2 ** 2 + 0.3 * k2 ** -1
+    dvz = 0.02 * k2 + 0.1 * k2 ** -3
+    assert np.isclose(a[kx, ky % n], (5.0 + 2.0j) / (1 + dpsi * dt), rtol=1e-14)
+    assert np.isclose(b[kx, ky % n], (-1.0 + 4.0j) / (1 + dvz * dt), rtol=1e-14)
+
+
+def test_forcing_enters_as_dt_times_f():
+    n = 32
+    p = O.Params(0, 0, 1, 2, 0, 0, 1, 2, 0.0)
+    ps = np.zeros((n // 2 + 1, n), complex)
+    vz = np.zeros_like(ps)
+    fp = np.zeros_like(ps)
+    fz = np.zeros_like(ps)
+    fp[2, 3] = 7.0 - 1.0j
+    fz[0, 4] = 2.0 + 3.0j
+    fz[0, n - 4] = 2.0 - 3.0j
+    a, b = O.rk_stages(ps, vz, fp, fz, 0.01, p)
+    assert np.isclose(a[2, 3], 0.07 - 0.01j, rtol=1e-14)
+    assert np.isclose(b[0, 4], 0.02 + 0.03j, rtol=1e-14)
+    assert np.isclose(b[0, n - 4], 0.02 - 0.03j, rtol=1e-14)
+
+
+def _drift(n, om, dt, steps, seed, quantity):
+    ps, vz = random_state(n, seed)
+    p = O.Params(0, 0, 1, 2, 0, 0, 1, 2, om)
+    q0 = quantity(ps, vz)
+    for _ in range(steps):
+        ps, vz = O.rk_stages(ps, vz, 0 * ps, 0 * vz, dt, p)
+    return abs(quantity(ps, vz) - q0) / q0
+
+
+def test_total_energy_conserved_with_rotation():
+    """1/2 d/dt(<|grad psi|^2> + <vz^2>) = 0 without dissipation and forcing.
+    The 4-stage scheme is fourth order for linear problems only (see the
+    inertial-wave test) and second order once the brackets act, so the drift
+    over a fixed time falls as dt^2; a wrong sign in either rotation term would
+    make the continuous system itself non-conservative (no convergence)."""
+    E = lambda ps, vz: O.energy(ps, 1) + O.energy(vz, 0)
+    d1 = _drift(32, 3.0, 4e-3, 10, 7, E)
+    d2 = _drift(32, 3.0, 2e-3, 20, 7, E)
+    assert d1 < 1e-7
+    assert 3.0 < d1 / d2 < 5.0
+
+
+def test_enstrophy_conserved_without_rotation():
+    Z = lambda ps, vz: O.energy(ps, 2)
+    assert _drift(32, 0.0, 2e-3, 10, 9, Z) < 1e-8
+
+
+# ---------------------------------------------------------------- diagnostics
+def test_spectrum_sums_to_energy_and_transfers_to_zero():
+    n = 32
+    ps, vz = random_state(n, 11)
+    ek, ezk = O.spectrum(ps, vz)
+    assert math.isclose(ek.sum(), O.energy(ps, 1), rel_tol=1e-13)
+    assert math.isclose(ezk.sum(), O.energy(vz, 0) - 0.0, rel_tol=1e-13)
+    (f0, f1, f2), (x0, x1, x2) = O.transfers(ps, vz)
+    scale = O.energy(ps, 1) * math.sqrt(O.energy(ps, 2))
+    assert abs(f1.sum()) < 1e-12 * scale
+    assert abs(f0.sum()) < 1e-12 * scale * n
+    assert abs(f2.sum()) < 1e-12 * scale
+    assert np.allclose(x1[-1], f1.sum())
+
+
+def test_cond_check_columns_by_parseval():
+    n = 32
+    ps, vz = random_state(n, 13)
+    p = O.Params(1e-3, 0.1, 1, 2, 2e-3, 0.2, 1, 2, 1.5)
+    rows = O.cond_check(ps, vz, 0 * ps, 0 * vz, 0.25, p, 4.0)
+    n2 = float(n * n)
+    px = O.fftp2d_complex_to_real(O.derivk2(ps, 1)) / n2
+    py = O.fftp2d_complex_to_real(O.derivk2(ps, 2)) / n2
+    v = O.fftp2d_complex_to_real(vz) / n2
+    w = O.fftp2d_complex_to_real(O.laplak2(ps)) / n2
+    eb = rows["energy_bal"]
+    assert eb[0] == 0.25
+    assert math.isclose(eb[1], np.mean(px ** 2 + py ** 2), rel_tol=1e-12)
+    assert math.isclose(eb[2], np.mean(v ** 2), rel_tol=1e-12)
+    assert math.isclose(eb[9], 2 * 1.5 * np.mean(v * py), rel_tol=1e-10, abs_tol=1e-14)
+    assert math.isclose(rows["enstrophy_bal"][1], np.mean(w * w), rel_tol=1e-12)
+    mm = rows["max_min"]
+    assert math.isclose(mm[1], (w * n2).max(), rel_tol=1e-12) and math.isclose(mm[2], (w * n2).min(), rel_tol=1e-12)
