@@ -55,9 +55,14 @@ typedef struct r2d3c_force_draw {
   int enabled;
 } r2d3c_force_draw;
 
+/* Forcing of one step: a rand_force draw per field (iflow = 3) and/or a
+ * static dense spectral array per field (iflow = 1, src/Rot2D3C.para.f90:343-371;
+ * NULL = none), added as fp / fz in the stage update. */
 typedef struct r2d3c_forcing {
   r2d3c_force_draw psi; /* fp: kin = 1 */
   r2d3c_force_draw vz;  /* fz: kin = 0 */
+  const double *dense_psi;
+  const double *dense_vz;
 } r2d3c_forcing;
 
 #define R2D3C_DT_FIXED 0 /* dt given by the caller (no reference counterpart) */
@@ -110,6 +115,28 @@ int r2d3c_energy(r2d3c_plan *plan, const double *a, int kin, double *d_out, void
  * inerprod (src/pseudospec2D_Rot2D3C.f90:400-451). */
 int r2d3c_inerprod(r2d3c_plan *plan, const double *a, const double *b, int kin,
                    double *d_out, void *stream);
+
+/* ---- diagnostics (device results, deterministic) ----------------------- */
+
+/* The balance sums of cond_check (src/pseudospec2D_Rot2D3C.f90:455-612) in
+ * d_out[26]: Ep1 injp1 Dsp Hds Epv injz Dspv Hdsv coup Ep2 injp2 Dspw Hdsw Eph
+ * injh1 injh2 Dsph1 Hdsh1 Dsph2 Hdsh2 Efp Efpz, then max and min of
+ * c2r(laplak2(psi)) and of c2r(derivk2(vz,0)).  fp, fz: dense forcing arrays
+ * of the previous step (zeros without forcing).  Writing the four balance
+ * files from these is left to the host (one rank: the reference's min is
+ * MPI_MAX over one value, i.e. the min). */
+int r2d3c_cond_check(r2d3c_plan *plan, const r2d3c_params *p, double kup, const double *psi,
+                     const double *vz, const double *fp, const double *fz, double *d_out,
+                     void *stream);
+/* Shell spectra (src/pseudospec2D_Rot2D3C.f90:615-712): d_out[2*(n/2+1)] =
+ * Ek(k) then Ezk(k), k = 1..n/2+1. */
+int r2d3c_spectrum(r2d3c_plan *plan, const double *psi, const double *vz, double *d_out,
+                   void *stream);
+/* Shell transfers (src/pseudospec2D_Rot2D3C.f90:715-837): d_out[3*(n/2+1)] =
+ * Fl0 (enstrophy), Fl1 (2D energy), Fl2 (vz energy) per shell; the fluxes
+ * are their running sums from k = 1. */
+int r2d3c_transfers(r2d3c_plan *plan, const double *psi, const double *vz, double *d_out,
+                    void *stream);
 
 /* ---- time step -------------------------------------------------------- */
 

@@ -23,14 +23,23 @@ Pinning (checks that do not restate the formula; see tests/test_oracle_*.py):
   rand_force .............. real-space transform is the analytic cosine of the
                             drawn wavevector; amplitude from the injection rate
   rk_stages ............... linear inertial-wave limit equals the 4th-order Taylor
-                            polynomial of the exact propagator; energy (and, at
-                            Omega=0, enstrophy) conserved to O(dt^5) per step when
-                            nu = nu+ = 0
+                            polynomial of the exact propagator (5th-order local
+                            error against exp(dt L)); pure dissipation equals
+                            backward Euler; forcing enters as dt*f; energy (and,
+                            at Omega=0, enstrophy) conserved to second order in
+                            dt when nu = nu+ = 0 (the scheme is fourth order for
+                            linear problems only); a mutation test plants
+                            plausible slips and checks that these pins fail
   spectrum / transfers .... sum_k E(k) equals energy(); sum_k T(k) = 0
   cond_check .............. each column equals the Parseval real-space value
-  main-loop composition ... parity unpinned: only a run of the reference pins
-                            the statement order, and the reference cannot be
-                            built here (it needs FFTW 2.x, absent from the image)
+  reference_loop /
+  run_random_forcing ...... parity unpinned: they restate the statement order of
+                            the main loop (CFL, cond_check with the previous
+                            step's forcing, new draws, outputs, time += dt,
+                            stages; counters of a fresh start and of a restart),
+                            which only a run of the reference would pin, and the
+                            reference cannot be built here (it needs FFTW 2.x,
+                            absent from the image)
 """
 import math
 
@@ -447,3 +456,58 @@ def run_random_forcing(ps, vz, steps, cfl, p, kdn, kup, fp0, fz0, seed,
         dts.append(dt)
         ps, vz = rk_stages(ps, vz, fp, fz, dt, p)
     return ps, vz, time, dts, seed, seed1
+
+
+# --------------------------------------------------------------------------
+# the main loop with its output cadence (src/Rot2D3C.para.f90:245-250,
+# 300-308, 376-615); parity unpinned beyond the statement order read there
+# --------------------------------------------------------------------------
+def reference_loop(n, prm, stat=0, time=0.0, ps=None, vz=None):
+    """Run the reference loop for iflow = 3 (or 0) and return its events:
+    balance rows (every cstep), spectra (every sstep) and field dumps (every
+    tstep), each with the output index and time the reference writes.
+    prm: dict with the parameter.inp names.  A fresh start (stat = 0) builds
+    the initial condition itself; a restart takes ps, vz."""
+    p = Params(prm["nu"], prm["hnu"], prm["nn"], prm["mm"], prm["nuv"], prm["hnuv"],
+               prm["nnv"], prm["mmv"], prm["omega"])
+    seed = prm["seed"]
+    if stat == 0:
+        ps, vz, seed = initial_condition(n, prm["kup"], prm["u0"], prm["v0"], seed)
+        ini, timet, timec, times = 1, prm["tstep"], prm["cstep"], prm["sstep"]
+        out_idx, spec_idx = 0, 0
+    else:
+        ini = (stat - 1) * prm["tstep"]
+        timet = timec = times = 0
+        out_idx = stat
+        spec_idx = int(float(ini) / float(prm["sstep"]) + 1)
+    ev = {"balance": [], "spectrum": [], "field": []}
+    zero = np.zeros_like(ps)
+    fp, fz = zero, zero
+    seed1 = seed + 1
+    cfl = prm["CFL"]
+    if prm["iflow"] == 3:
+        dt = CFL_condition(cfl, ps, vz, p.nu, p.nn, p.nuv, p.nnv, p.omega)
+        fp, seed = rand_force(prm["kdn"], prm["kup"], prm["fp0"], dt, seed, 1, n)
+        fz, seed1 = rand_force(prm["kdn"], prm["kup"], prm["fz0"], dt, seed1, 0, n)
+    for _ in range(ini, prm["step"] + 1):
+        dt = CFL_condition(cfl, ps, vz, p.nu, p.nn, p.nuv, p.nnv, p.omega)
+        if timec == prm["cstep"]:
+            timec = 0
+            ev["balance"].append(cond_check(ps, vz, fp, fz, time, p, prm["kup"]))
+        if prm["iflow"] == 3:
+            fp, seed = rand_force(prm["kdn"], prm["kup"], prm["fp0"], dt, seed, 1, n)
+            fz, seed1 = rand_force(prm["kdn"], prm["kup"], prm["fz0"], dt, seed1, 0, n)
+        if times == prm["sstep"]:
+            times = 0
+            spec_idx += 1
+            ev["spectrum"].append((spec_idx, time, spectrum(ps, vz), transfers(ps, vz)))
+        if timet == prm["tstep"]:
+            timet = 0
+            out_idx += 1
+            ev["field"].append((out_idx, time, ps.copy(), vz.copy()))
+        timet += 1
+        times += 1
+        timec += 1
+        time = time + dt
+        ps, vz = rk_stages(ps, vz, fp, fz, dt, p)
+    return ps, vz, time, ev

@@ -30,7 +30,7 @@ class ForceDraw(ctypes.Structure):
 
 
 class Forcing(ctypes.Structure):
-    _fields_ = [("psi", ForceDraw), ("vz", ForceDraw)]
+    _fields_ = [("psi", ForceDraw), ("vz", ForceDraw), ("dense_psi", _vp), ("dense_vz", _vp)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/rot2d3c.h
@@ -48,6 +48,10 @@ SIGNATURES = {
     "r2d3c_poisson": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
     "r2d3c_energy": (_c_int, [_vp, _vp, _c_int, _vp, _vp]),
     "r2d3c_inerprod": (_c_int, [_vp, _vp, _vp, _c_int, _vp, _vp]),
+    "r2d3c_cond_check": (_c_int, [_vp, ctypes.POINTER(Params), _c_double, _vp, _vp, _vp, _vp, _vp,
+                                  _vp]),
+    "r2d3c_spectrum": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "r2d3c_transfers": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
     "r2d3c_cfl": (_c_int, [_vp, ctypes.POINTER(Params), _c_double, _vp, _vp, _vp]),
     "r2d3c_step": (_c_int, [_vp, ctypes.POINTER(Params), _vp, _vp, ctypes.POINTER(Forcing),
                             _c_int, _c_double, _vp, _vp]),

@@ -405,6 +405,196 @@ __global__ void k_force_fill(cplx *__restrict__ fp, ForceSet f, double dt, int n
 }
 
 // ---------------------------------------------------------------------------
+// diagnostics: cond_check, spectrum, transfers (src/pseudospec2D_Rot2D3C.f90:455-837)
+// ---------------------------------------------------------------------------
+// ka2^kin as the reference's sums use it; false when the mode is excluded
+// (kin < 0 skips ka2 < tiny, src/pseudospec2D_Rot2D3C.f90:377-386)
+__device__ __forceinline__ bool kpow(double ka2, int kin, double tiny, double *w) {
+  if (kin >= 0) { *w = ipow(ka2, kin); return true; }
+  if (ka2 < tiny) return false;
+  *w = 1.0 / ipow(ka2, -kin);
+  return true;
+}
+
+struct CondConst {
+  int n, nn, mm, nnv, mmv;
+  double kmax, tiny, band_lo, band_hi;  // Efp band: kup^2 <= ka2 <= 2.01f*kup^2
+};
+constexpr int kCondSums = 22;
+
+// The 22 sums of cond_check in one pass, one partial per block and sum.
+// Order: Ep1 injp1 Dsp Hds Epv injz Dspv Hdsv coup Ep2 injp2 Dspw Hdsw Eph
+// injh1 injh2 Dsph1 Hdsh1 Dsph2 Hdsh2 Efp Efpz (:484-568).
+__global__ __launch_bounds__(256) void k_cond_partial(const cplx *__restrict__ ps,
+                                                       const cplx *__restrict__ vz,
+                                                       const cplx *__restrict__ fp,
+                                                       const cplx *__restrict__ fz, CondConst c,
+                                                       double *__restrict__ partial) {
+  __shared__ double red[256];
+  double acc[kCondSums];
+#pragma unroll
+  for (int q = 0; q < kCondSums; ++q) acc[q] = 0.0;
+  const int n = c.n;
+  const size_t total = (size_t)(n / 2 + 1) * n;
+  const double inv_n4 = 1.0 / ((double)n * n * (double)n * n);
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    const double kx = kxof(i, n), ky = kyof(j, n);
+    const double ka2 = kx * kx + ky * ky;
+    const double two = (i == 0) ? 1.0 : 2.0;
+    const cplx a = ps[e], v = vz[e], f = fp[e], g = fz[e];
+    const double pp = a.x * a.x + a.y * a.y, vv = v.x * v.x + v.y * v.y;
+    const bool in = ka2 <= c.kmax;
+    double w;
+    // energy(a, kin): every mode for kin >= 0
+    auto en = [&](int q, double sq, int kin) { if (kpow(ka2, kin, c.tiny, &w)) acc[q] += two * sq * w * inv_n4; };
+    // inerprod(a, b, kin): Re(b conj a) inside the dealiasing mask
+    auto ip = [&](int q, cplx x, cplx y, int kin) {
+      if (in && kpow(ka2, kin, c.tiny, &w)) acc[q] += two * w * (y.x * x.x + y.y * x.y) * inv_n4;
+    };
+    en(0, pp, 1);
+    ip(1, a, f, 1);
+    en(2, pp, c.nn + 1);
+    en(3, pp, 1 - c.mm);
+    en(4, vv, 0);
+    ip(5, v, g, 0);
+    en(6, vv, c.nnv);
+    en(7, vv, -c.mmv);
+    ip(8, v, mk(-ky * a.y, ky * a.x), 0);  // C1 = derivk2(ps, 2)
+    en(9, pp, 2);
+    ip(10, a, f, 2);
+    en(11, pp, c.nn + 2);
+    en(12, pp, 2 - c.mm);
+    ip(13, v, a, 1);
+    ip(14, v, f, 1);
+    ip(15, a, g, 1);
+    ip(16, v, a, c.nn + 1);
+    ip(17, v, a, 1 - c.mm);
+    ip(18, a, v, c.nnv + 1);
+    ip(19, a, v, 1 - c.mmv);
+    if (ka2 <= c.band_hi && ka2 >= c.band_lo) {
+      acc[20] += two * ka2 * pp * inv_n4;
+      acc[21] += two * vv * inv_n4;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kCondSums; ++q) {
+    red[threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[q * gridDim.x + blockIdx.x] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void k_cond_finish(const double *__restrict__ partial, int np, double *__restrict__ out) {
+  const int q = threadIdx.x;
+  if (q >= kCondSums) return;
+  double s = 0.0;
+  for (int b = 0; b < np; ++b) s += partial[q * np + b];
+  out[q] = s;
+}
+
+// max and min of a real field: per-block partials, then one block
+__global__ __launch_bounds__(256) void k_minmax_partial(const double *__restrict__ r, size_t total,
+                                                         double *__restrict__ partial) {
+  __shared__ double mx[256], mn[256];
+  double a = -INFINITY, b = INFINITY;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    a = fmax(a, r[e]);
+    b = fmin(b, r[e]);
+  }
+  mx[threadIdx.x] = a;
+  mn[threadIdx.x] = b;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + st]);
+      mn[threadIdx.x] = fmin(mn[threadIdx.x], mn[threadIdx.x + st]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = mx[0];
+    partial[2 * blockIdx.x + 1] = mn[0];
+  }
+}
+__global__ void k_minmax_finish(const double *__restrict__ partial, int np, double *__restrict__ out) {
+  double a = -INFINITY, b = INFINITY;
+  for (int k = 0; k < np; ++k) {
+    a = fmax(a, partial[2 * k]);
+    b = fmin(b, partial[2 * k + 1]);
+  }
+  out[0] = a;
+  out[1] = b;
+}
+
+// Shell sums, one block per shell kmn = blockIdx.x + 1, kmn = int(sqrt(ka2) + 0.5)
+// (src/pseudospec2D_Rot2D3C.f90:652-656).  kmn is monotone in |ky| at fixed kx, so
+// each thread walks its columns over the contiguous |ky| range of the shell,
+// in a fixed order (deterministic).
+// mode 0 (spectrum): out0 = sum two*ka2*|ps|^2, out1 = sum two*|vz|^2
+// mode 1 (transfers): out0 = sum two*ka2*Re(ps conj c2), out1 = sum two*Re(ps conj c2),
+//                     out2 = -sum two*Re(vz conj c1)  (all /n^4)
+__device__ __forceinline__ int shell_of(double ka2) { return (int)(sqrt(ka2) + 0.5); }
+
+__global__ __launch_bounds__(256) void k_shells(const cplx *__restrict__ ps, const cplx *__restrict__ vz,
+                                                 const cplx *__restrict__ c1, const cplx *__restrict__ c2,
+                                                 int mode, int n, double *__restrict__ out) {
+  __shared__ double red[3][256];
+  const int shell = blockIdx.x + 1, nsh = n / 2 + 1;
+  const double inv_n4 = 1.0 / ((double)n * n * (double)n * n);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  const int imax = shell < n / 2 ? shell : n / 2;  // kmn >= kx
+  for (int i = threadIdx.x; i <= imax; i += blockDim.x) {
+    const double kx = kxof(i, n);
+    const double two = (i == 0) ? 1.0 : 2.0;
+    const double lo2 = ((double)shell - 0.5) * ((double)shell - 0.5) - kx * kx;
+    int a0 = lo2 > 0.0 ? (int)sqrt(lo2) - 1 : 0;
+    if (a0 < 0) a0 = 0;
+    for (int side = 0; side < 2; ++side) {  // ky >= 0, then ky < 0
+      int a = side == 0 ? a0 : (a0 > 1 ? a0 : 1);
+      const int amax = side == 0 ? n / 2 - 1 : n / 2;
+      while (a <= amax && shell_of(kx * kx + (double)a * a) < shell) ++a;
+      for (; a <= amax; ++a) {
+        const double ka2 = kx * kx + (double)a * a;
+        if (shell_of(ka2) != shell) break;
+        const int j = side == 0 ? a : n - a;
+        const size_t e = (size_t)i * n + j;
+        const cplx p = ps[e];
+        if (mode == 0) {
+          const cplx v = vz[e];
+          s0 += two * ka2 * (p.x * p.x + p.y * p.y) * inv_n4;
+          s1 += two * (v.x * v.x + v.y * v.y) * inv_n4;
+        } else {
+          const cplx b2 = c2[e], b1 = c1[e], v = vz[e];
+          const double re = p.x * b2.x + p.y * b2.y;
+          s0 += two * ka2 * re * inv_n4;
+          s1 += two * re * inv_n4;
+          s2 -= two * (v.x * b1.x + v.y * b1.y) * inv_n4;
+        }
+      }
+    }
+  }
+  red[0][threadIdx.x] = s0;
+  red[1][threadIdx.x] = s1;
+  red[2][threadIdx.x] = s2;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < (mode == 0 ? 2 : 3); ++q) out[q * nsh + blockIdx.x] = red[q][0];
+}
+
+// ---------------------------------------------------------------------------
 // the fused step
 // ---------------------------------------------------------------------------
 struct StepConst {
@@ -416,6 +606,7 @@ struct StepConst {
   double dt_fixed;
   DtConst dc;
   ForceSet fp, fz;
+  const cplx *fpd, *fzd;  // static dense forcing (NULL = none)
   double two_omega_n2, inv_n4, n2;
 };
 
@@ -784,7 +975,9 @@ __global__ __launch_bounds__(N / 8) void k_step_colfused(
     if (ka2 <= c.g.kmax && ka2 > c.g.tiny) {
       const double tmp2 = 1.0 / ka2;
       const double tmp3 = 1.0 + (nu * ipow(ka2, nn) + hnu * ipow(tmp2, mm)) * tmp1;
-      const cplx f = j == fj0 ? fv0 : (j == fj1 ? fv1 : mk(0.0, 0.0));
+      cplx f = j == fj0 ? fv0 : (j == fj1 ? fv1 : mk(0.0, 0.0));
+      const cplx *fd = field == 0 ? c.fpd : c.fzd;
+      if (fd) f = cadd(f, fd[(size_t)kx * N + j]);
       const cplx s0 = state[(size_t)kx * N + j];
       cplx t = field == 0 ? cscale(v[q], tmp2) : v[q];
       t = cadd(t, f);
@@ -842,6 +1035,9 @@ struct r2d3c_plan {
   cplx *mstep = nullptr;          // 6 mixed fields + 2 bracket fields (fused step)
   cplx *mfull = nullptr;          // 4 full-range mixed fields
   double *real = nullptr;         // n*n real scratch
+  double *real2 = nullptr;        // second n*n real scratch (extrema)
+  cplx *spec3 = nullptr;          // three spectral scratch fields (transfers)
+  double *dpart = nullptr;        // diagnostics partials
   unsigned long long *maxbits = nullptr;
   double *dtslot = nullptr;
   double *partial = nullptr;      // reduction partials
@@ -1067,11 +1263,14 @@ static int run_steps(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
   for (int st = 0; st < nsteps; ++st) {
     std::memset(&c.fp, 0, sizeof(c.fp));
     std::memset(&c.fz, 0, sizeof(c.fz));
+    c.fpd = c.fzd = nullptr;
     if (fo) {
       int rc = make_force_set(&fo[st].psi, N, &c.fp);
       if (rc) return rc;
       rc = make_force_set(&fo[st].vz, N, &c.fz);
       if (rc) return rc;
+      c.fpd = (const cplx *)fo[st].dense_psi;
+      c.fzd = (const cplx *)fo[st].dense_vz;
     }
     double *dt_out = d_dt ? d_dt + st : p->dtslot;
     int rc = step_seq<N>(p, c, psi, vz, s, dt_out, st == 0, st + 1 < nsteps, st > 0, st & 1);
@@ -1124,13 +1323,18 @@ int r2d3c_plan_create(r2d3c_plan **out, int n) {
       (e = hipMalloc(&p->mfull, b_mf)) != hipSuccess || (e = hipMalloc(&p->real, b_re)) != hipSuccess ||
       (e = hipMalloc(&p->maxbits, 256)) != hipSuccess || (e = hipMalloc(&p->dtslot, 256)) != hipSuccess ||
       (e = hipMalloc(&p->partial, sizeof(double) * kReduceBlocks)) != hipSuccess ||
+      (e = hipMalloc(&p->real2, b_re)) != hipSuccess ||
+      (e = hipMalloc(&p->spec3, 3 * sizeof(cplx) * (size_t)(n / 2 + 1) * n)) != hipSuccess ||
+      (e = hipMalloc(&p->dpart, sizeof(double) * kCondSums * kReduceBlocks)) != hipSuccess ||
       (e = hipMemcpy(p->tw, tw.data(), b_tw, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemset(p->mstep, 0, b_ms)) != hipSuccess || (e = hipMemset(p->mfull, 0, b_mf)) != hipSuccess) {
     int rc = hip_err(e, "plan allocation");
     r2d3c_plan_destroy(p);
     return rc;
   }
-  p->bytes = (long long)(b_tw + b_ms + b_mf + b_re + 512 + sizeof(double) * kReduceBlocks);
+  p->bytes = (long long)(b_tw + b_ms + b_mf + 2 * b_re + 512 + sizeof(double) * kReduceBlocks +
+                         3 * sizeof(cplx) * (size_t)(n / 2 + 1) * n +
+                         sizeof(double) * kCondSums * kReduceBlocks);
   *out = p;
   return 0;
 }
@@ -1145,6 +1349,9 @@ int r2d3c_plan_destroy(r2d3c_plan *p) {
   (void)hipFree(p->maxbits);
   (void)hipFree(p->dtslot);
   (void)hipFree(p->partial);
+  (void)hipFree(p->real2);
+  (void)hipFree(p->spec3);
+  (void)hipFree(p->dpart);
   delete p;
   return 0;
 }
@@ -1291,6 +1498,66 @@ int r2d3c_forcing_fill(r2d3c_plan *p, const r2d3c_forcing *fo, double dt, double
   if (fp) k_force_fill<<<launch_grid(total), 256, 0, s>>>((cplx *)fp, a, dt, p->n);
   if (fz) k_force_fill<<<launch_grid(total), 256, 0, s>>>((cplx *)fz, b, dt, p->n);
   LAUNCHCHK("forcing fill");
+  return 0;
+}
+
+int r2d3c_cond_check(r2d3c_plan *p, const r2d3c_params *q, double kup, const double *psi,
+                     const double *vz, const double *fp, const double *fz, double *d_out,
+                     void *stream) {
+  NEED(p); NEED(psi); NEED(vz); NEED(fp); NEED(fz); NEED(d_out);
+  int rc = check_params(q);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int n = p->n;
+  CondConst c;
+  c.n = n; c.nn = q->nn; c.mm = q->mm; c.nnv = q->nnv; c.mmv = q->mmv;
+  c.kmax = p->g.kmax; c.tiny = p->g.tiny;
+  c.band_lo = kup * kup;
+  c.band_hi = (double)2.01f * kup * kup;  // `(2.01)*kup*kup`: a default-REAL literal (:559)
+  k_cond_partial<<<kReduceBlocks, 256, 0, s>>>((const cplx *)psi, (const cplx *)vz, (const cplx *)fp,
+                                               (const cplx *)fz, c, p->dpart);
+  k_cond_finish<<<1, 32, 0, s>>>(p->dpart, kReduceBlocks, d_out);
+  LAUNCHCHK("cond_check sums");
+  // extrema of c2r(laplak2(ps)) (-omega*n^2) and c2r(derivk2(vz,0)) (:573-586)
+  const size_t sz = (size_t)(n / 2 + 1) * n, nn2 = (size_t)n * n;
+  cplx *t = p->spec3;
+  k_spec_op<<<launch_grid(sz), 256, 0, s>>>((const cplx *)psi, t, OP_LAP, n, p->g.kmax, p->g.tiny);
+  rc = r2d3c_c2r(p, (const double *)t, p->real2, stream);
+  if (rc) return rc;
+  k_minmax_partial<<<kReduceBlocks, 256, 0, s>>>(p->real2, nn2, p->dpart);
+  k_minmax_finish<<<1, 1, 0, s>>>(p->dpart, kReduceBlocks, d_out + kCondSums);
+  k_spec_op<<<launch_grid(sz), 256, 0, s>>>((const cplx *)vz, t, OP_MASK, n, p->g.kmax, p->g.tiny);
+  rc = r2d3c_c2r(p, (const double *)t, p->real2, stream);
+  if (rc) return rc;
+  k_minmax_partial<<<kReduceBlocks, 256, 0, s>>>(p->real2, nn2, p->dpart);
+  k_minmax_finish<<<1, 1, 0, s>>>(p->dpart, kReduceBlocks, d_out + kCondSums + 2);
+  LAUNCHCHK("cond_check extrema");
+  return 0;
+}
+
+int r2d3c_spectrum(r2d3c_plan *p, const double *psi, const double *vz, double *d_out, void *stream) {
+  NEED(p); NEED(psi); NEED(vz); NEED(d_out);
+  k_shells<<<p->n / 2 + 1, 256, 0, (hipStream_t)stream>>>((const cplx *)psi, (const cplx *)vz, nullptr,
+                                                          nullptr, 0, p->n, d_out);
+  LAUNCHCHK("spectrum");
+  return 0;
+}
+
+int r2d3c_transfers(r2d3c_plan *p, const double *psi, const double *vz, double *d_out,
+                    void *stream) {
+  NEED(p); NEED(psi); NEED(vz); NEED(d_out);
+  hipStream_t s = (hipStream_t)stream;
+  const int n = p->n;
+  const size_t sz = (size_t)(n / 2 + 1) * n;
+  cplx *c1 = p->spec3, *c2 = c1 + sz, *w = c2 + sz;
+  // c1 = laplak2(ps); c2 = poisson(ps, c1); c1 = poisson(ps, vz)  (:759-761)
+  k_spec_op<<<launch_grid(sz), 256, 0, s>>>((const cplx *)psi, w, OP_LAP, n, p->g.kmax, p->g.tiny);
+  int rc = r2d3c_poisson(p, psi, (const double *)w, (double *)c2, stream);
+  if (rc) return rc;
+  rc = r2d3c_poisson(p, psi, vz, (double *)c1, stream);
+  if (rc) return rc;
+  k_shells<<<n / 2 + 1, 256, 0, s>>>((const cplx *)psi, (const cplx *)vz, c1, c2, 1, n, d_out);
+  LAUNCHCHK("transfers");
   return 0;
 }
 

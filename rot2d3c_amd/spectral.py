@@ -224,21 +224,33 @@ class ForceDraw:
         return _lib.ForceDraw(self.kx, self.ky, self.phase, self.f0, self.kupk, 1)
 
 
-def _forcing_c(fp_draw, fz_draw):
+def _forcing_c(fp_draw, fz_draw, dense=None):
+    """fp_draw / fz_draw: ForceDraw or None; dense: optional (fp, fz) pair of
+    static spectral arrays (either may be None)."""
     f = _lib.Forcing()
     if fp_draw is not None:
         f.psi = fp_draw.c()
     if fz_draw is not None:
         f.vz = fz_draw.c()
+    if dense is not None:
+        a, b = dense
+        if a is not None:
+            _spec_n(a)
+            f.dense_psi = a.data_ptr()
+        if b is not None:
+            _spec_n(b)
+            f.dense_vz = b.data_ptr()
     return f
 
 
-def rk4_step(ps, vz, p, fp_draw=None, fz_draw=None, dt=None, cfl=None, dt_out=None):
+def rk4_step(ps, vz, p, fp_draw=None, fz_draw=None, dt=None, cfl=None, dt_out=None,
+             dense=None):
     """One time step in place (src/Rot2D3C.para.f90:399 and :545-611).
 
     Exactly one of `dt` (fixed step) and `cfl` (CFL-adaptive step from the
     pre-step state, as the reference) must be given.  The dt used is written
-    to `dt_out` (a 1-element float64 CUDA tensor) when given."""
+    to `dt_out` (a 1-element float64 CUDA tensor) when given.  dense:
+    optional (fp, fz) static forcing arrays added to the draws (iflow = 1)."""
     n = _spec_n(ps)
     if _spec_n(vz) != n:
         raise ValueError("size mismatch")
@@ -246,7 +258,7 @@ def rk4_step(ps, vz, p, fp_draw=None, fz_draw=None, dt=None, cfl=None, dt_out=No
         raise ValueError("give exactly one of dt and cfl")
     pl = Plan.get(n, ps.device)
     cp = p.c()
-    fo = _forcing_c(fp_draw, fz_draw)
+    fo = _forcing_c(fp_draw, fz_draw, dense)
     mode, val = (R2D3C_DT_FIXED, dt) if dt is not None else (R2D3C_DT_CFL, cfl)
     optr = _ptr(dt_out) if dt_out is not None else ctypes.c_void_p(0)
     check(_lib.lib().r2d3c_step(pl.handle, ctypes.byref(cp), _ptr(ps), _ptr(vz), ctypes.byref(fo),
@@ -299,6 +311,73 @@ def forcing_fill(n, device, fp_draw, fz_draw, dt):
     return fp, fz
 
 
+# --- diagnostics: src/pseudospec2D_Rot2D3C.f90:455-837 -----------------------
+COND_NAMES = ("Ep1", "injp1", "Dsp", "Hds", "Epv", "injz", "Dspv", "Hdsv", "coup", "Ep2",
+              "injp2", "Dspw", "Hdsw", "Eph", "injh1", "injh2", "Dsph1", "Hdsh1", "Dsph2",
+              "Hdsh2", "Efp", "Efpz", "maxw", "minw", "maxv", "minv")
+
+
+def cond_check_values(ps, vz, fp, fz, p, kup):
+    """The 26 device-computed quantities of cond_check as a dict of floats
+    (src/pseudospec2D_Rot2D3C.f90:484-586)."""
+    n = _spec_n(ps)
+    pl = Plan.get(n, ps.device)
+    out = torch.empty(len(COND_NAMES), dtype=torch.float64, device=ps.device)
+    cp = p.c()
+    check(_lib.lib().r2d3c_cond_check(pl.handle, ctypes.byref(cp), float(kup), _ptr(ps), _ptr(vz),
+                                      _ptr(fp), _ptr(fz), _ptr(out), _stream(ps.device)),
+          "r2d3c_cond_check")
+    return dict(zip(COND_NAMES, out.tolist()))
+
+
+def cond_check(ps, vz, fp, fz, time, p, kup):
+    """Rows of energy_bal.txt, enstrophy_bal.txt, helicity_bal.txt and
+    max_min.txt, in the reference's column order (:592-608)."""
+    v = cond_check_values(ps, vz, fp, fz, p, kup)
+    om2 = 2 * p.omega
+    return {
+        "energy_bal": [time, v["Ep1"], v["Epv"], v["injp1"], v["injz"], p.nu * v["Dsp"],
+                       p.nuv * v["Dspv"], p.hnu * v["Hds"], p.hnuv * v["Hdsv"], om2 * v["coup"],
+                       v["Efp"], v["Efpz"]],
+        "enstrophy_bal": [time, v["Ep2"], v["injp2"], p.nu * v["Dspw"], p.hnu * v["Hdsw"]],
+        "helicity_bal": [time, -v["Eph"], -v["injh1"], -v["injh2"], -p.nu * v["Dsph1"],
+                         -p.nuv * v["Dsph2"], -p.hnu * v["Hdsh1"], -p.hnuv * v["Hdsh2"]],
+        "max_min": [time, v["maxw"], v["minw"], v["maxv"], v["minv"]],
+    }
+
+
+def spectrum(ps, vz):
+    """(Ek, Ezk) numpy arrays over shells k = 1..n/2+1 (:615-712)."""
+    n = _spec_n(ps)
+    pl = Plan.get(n, ps.device)
+    out = torch.empty(2 * (n // 2 + 1), dtype=torch.float64, device=ps.device)
+    check(_lib.lib().r2d3c_spectrum(pl.handle, _ptr(ps), _ptr(vz), _ptr(out), _stream(ps.device)),
+          "r2d3c_spectrum")
+    o = out.cpu().numpy()
+    return o[: n // 2 + 1], o[n // 2 + 1:]
+
+
+def transfers(ps, vz):
+    """((Fl0, Fl1, Fl2), (fluxes)) over shells k = 1..n/2+1 (:715-837); the
+    fluxes are the running sums from k = 1 (:816-831)."""
+    import numpy as np
+    n = _spec_n(ps)
+    pl = Plan.get(n, ps.device)
+    out = torch.empty(3 * (n // 2 + 1), dtype=torch.float64, device=ps.device)
+    check(_lib.lib().r2d3c_transfers(pl.handle, _ptr(ps), _ptr(vz), _ptr(out), _stream(ps.device)),
+          "r2d3c_transfers")
+    o = out.cpu().numpy().reshape(3, n // 2 + 1)
+    fl = (o[0], o[1], o[2])
+    fx = []
+    for f in fl:  # fx = Fl(i) + fx, in order
+        acc, c = 0.0, np.empty_like(f)
+        for i, x in enumerate(f):
+            acc = x + acc
+            c[i] = acc
+        fx.append(c)
+    return fl, tuple(fx)
+
+
 # --- host random numbers: src/pseudospec2D_mod.f90:52-71 ---------------------
 def randu(idum):
     """Returns (float32 value as float, new idum)."""
@@ -339,5 +418,6 @@ __all__ = [
     "derivk2", "laplak2", "inv_laplak2", "poisson", "energy", "inerprod", "energy_device",
     "inerprod_device", "CFL_condition", "CFL_condition_device", "rk4_step", "rk4_run",
     "forcing_fill",
-    "randu", "rand_force_draw", "rand_force", "kmax_index",
+    "randu", "rand_force_draw", "rand_force", "kmax_index", "cond_check", "cond_check_values",
+    "spectrum", "transfers", "COND_NAMES",
 ]

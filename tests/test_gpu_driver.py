@@ -1,0 +1,85 @@
+"""GPU: the driver (rot2d3c_amd.driver) runs the reference's main loop and
+writes the reference's files; compare them with the oracle's loop."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import spectral as O
+
+pytestmark = pytest.mark.gpu
+
+PRM = dict(CFL=0.4, step=9, cstep=1, sstep=3, tstep=4, fp0=1.0, fz0=0.5, u0=1.0, v0=0.7,
+           kdn=4.0, kup=4.0, nu=1e-3, hnu=0.1, nn=1, mm=2, nuv=2e-3, hnuv=0.05, nnv=1, mmv=2,
+           omega=1.5, seed=5, iflow=3, prm1=1.0, prm2=1.0, dt_corr=0.0, idir="ins", odir="outs")
+
+
+def _setup(tmp_path, stat=0, time=0.0):
+    from rot2d3c_amd import io as rio
+    rio.write_parameter_inp(str(tmp_path / "parameter.inp"), PRM)
+    rio.write_status_inp(str(tmp_path / "status.inp"), stat, time)
+
+
+def test_driver_fresh_start_matches_oracle_loop(tmp_path, cuda):
+    print("driver fresh start n=32", flush=True)
+    from rot2d3c_amd import driver, io as rio
+    n = 32
+    _setup(tmp_path)
+    r = driver.Run(n, str(tmp_path), "cuda:0")
+    gps, gvz = r.run()
+    ops, ovz, otime, ev = O.reference_loop(n, PRM)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(gps.cpu().numpy(), ops) < 1e-11 and rel(gvz.cpu().numpy(), ovz) < 1e-11
+    assert math.isclose(r.time, otime, rel_tol=1e-12)
+    bal = np.loadtxt(str(tmp_path / "energy_bal.txt"))
+    assert bal.shape[0] == len(ev["balance"]) == PRM["step"]
+    for row, b in zip(bal, ev["balance"]):
+        assert np.allclose(row, b["energy_bal"], rtol=1e-10, atol=1e-13)
+    ens = np.loadtxt(str(tmp_path / "enstrophy_bal.txt"))
+    hel = np.loadtxt(str(tmp_path / "helicity_bal.txt"))
+    mm = np.loadtxt(str(tmp_path / "max_min.txt"))
+    for k, b in enumerate(ev["balance"]):
+        assert np.allclose(ens[k], b["enstrophy_bal"], rtol=1e-10, atol=1e-13)
+        assert np.allclose(hel[k], b["helicity_bal"], rtol=1e-10, atol=1e-13)
+        assert np.allclose(mm[k], b["max_min"], rtol=1e-10, atol=1e-10)
+    for idx, t, (ek, ezk), ((f0, f1, f2), (x0, x1, x2)) in ev["spectrum"]:
+        sp = np.loadtxt(str(tmp_path / "outs" / ("spectrum.%04d.txt" % idx)))
+        assert np.allclose(sp[:, 0], ek, rtol=1e-10, atol=1e-14 * ek.max())
+        fl = np.loadtxt(str(tmp_path / "outs" / ("fluxes.%04d.txt" % idx)))
+        assert np.allclose(fl[:, 1], x1, rtol=1e-9, atol=1e-11 * np.abs(x1).max())
+    ts = open(str(tmp_path / "time_spec.txt")).read().splitlines()
+    assert [l[:4] for l in ts] == ["%04d" % e[0] for e in ev["spectrum"]]
+    tf = open(str(tmp_path / "time_field.txt")).read().splitlines()
+    assert [l[:3] for l in tf] == ["%03d" % e[0] for e in ev["field"]]
+    for idx, t, ps, vz in ev["field"]:
+        assert abs(float(tf[idx - 1][3:]) - t) < 1e-6
+        R = rio.read_field(str(tmp_path / "outs"), "ps", idx, n)
+        expect = O.fftp2d_complex_to_real(ps / float(n * n))
+        assert np.allclose(R, expect, rtol=1e-10, atol=1e-12 * np.abs(expect).max())
+
+
+def test_driver_restart_reads_its_own_dumps(tmp_path, cuda):
+    """stat > 0: the fields of output `stat` are read back from idir and the
+    counters restart as in src/Rot2D3C.para.f90:300-329."""
+    print("driver restart n=32", flush=True)
+    from rot2d3c_amd import driver, io as rio
+    n = 32
+    _setup(tmp_path)
+    driver.Run(n, str(tmp_path), "cuda:0").run()
+    os.rename(str(tmp_path / "outs"), str(tmp_path / "ins"))
+    tf = open(str(tmp_path / "time_field.txt")).read().splitlines()
+    t2 = float(tf[1][3:])
+    rio.write_status_inp(str(tmp_path / "status.inp"), 2, t2)
+    for f in ("energy_bal.txt", "time_field.txt", "time_spec.txt"):
+        os.remove(str(tmp_path / f))
+    r = driver.Run(n, str(tmp_path), "cuda:0")
+    gps, gvz = r.run()
+    R1 = rio.read_field(str(tmp_path / "ins"), "ps", 2, n)
+    R2 = rio.read_field(str(tmp_path / "ins"), "vz", 2, n)
+    ps0, vz0 = O.fftp2d_real_to_complex(R1), O.fftp2d_real_to_complex(R2)
+    ops, ovz, otime, ev = O.reference_loop(n, PRM, stat=2, time=t2, ps=ps0, vz=vz0)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(gps.cpu().numpy(), ops) < 1e-11
+    tf2 = open(str(tmp_path / "time_field.txt")).read().splitlines()
+    assert [l[:3] for l in tf2] == ["%03d" % e[0] for e in ev["field"]]

@@ -196,3 +196,37 @@ def test_run_equals_repeated_steps_bitwise(n, mode, cuda):
     assert torch.equal(torch.view_as_real(a_vz), torch.view_as_real(b_vz))
     g = O.grid(n)
     assert np.all(_host(b_ps)[~((g.ka2 <= g.kmax) & (g.ka2 > g.tiny))] == 0)
+
+
+@pytest.mark.parametrize("n", [32, 256])
+def test_cond_check_matches_oracle(n, cuda):
+    print("cond_check n=%d" % n, flush=True)
+    S = _S()
+    ps, vz = random_state(n, 31)
+    kup = 5.0
+    fp, _ = O.rand_force(kup, kup, 1.0, 1e-3, 3, 1, n)
+    fz, _ = O.rand_force(kup, kup, 0.5, 1e-3, 8, 0, n)
+    p = O.Params(1e-3, 0.1, 1, 2, 2e-3, 0.2, 2, 1, 1.5)
+    sp = S.Params(p.nu, p.hnu, p.nn, p.mm, p.nuv, p.hnuv, p.nnv, p.mmv, p.omega)
+    ref = O.cond_check(ps, vz, fp, fz, 0.125, p, kup)
+    got = S.cond_check(_dev(ps, cuda), _dev(vz, cuda), _dev(fp, cuda), _dev(fz, cuda), 0.125, sp, kup)
+    for name in ref:
+        r, g = np.array(ref[name]), np.array(got[name])
+        scale = np.maximum(np.abs(r), 1e-300)
+        assert np.all(np.abs(g - r) <= 1e-12 * scale + 1e-14 * np.abs(r).max()), (name, g, r)
+
+
+@pytest.mark.parametrize("n", [32, 256, 2048])
+def test_spectrum_transfers_match_oracle(n, cuda):
+    print("spectrum/transfers n=%d" % n, flush=True)
+    S = _S()
+    ps, vz = random_state(n, 41)
+    ek, ezk = S.spectrum(_dev(ps, cuda), _dev(vz, cuda))
+    oek, oezk = O.spectrum(ps, vz)
+    assert rel_l2(ek, oek) < 1e-13 and rel_l2(ezk, oezk) < 1e-13
+    (f0, f1, f2), (x0, x1, x2) = S.transfers(_dev(ps, cuda), _dev(vz, cuda))
+    (of0, of1, of2), (ox0, ox1, ox2) = O.transfers(ps, vz)
+    for a, b in ((f0, of0), (f1, of1), (f2, of2)):
+        assert np.max(np.abs(a - b)) <= 1e-11 * np.max(np.abs(b))
+    for a, b in ((x0, ox0), (x1, ox1), (x2, ox2)):
+        assert np.max(np.abs(a - b)) <= 1e-11 * np.max(np.abs(b))
