@@ -63,6 +63,11 @@ typedef struct r2d3c_forcing {
   r2d3c_force_draw vz;  /* fz: kin = 0 */
   const double *dense_psi;
   const double *dense_vz;
+  /* iflow = 2: fp = const_inj(C1, ci_kdn, ci_kup, ci_fp0, 1) and
+   * fz = const_inj(C2, ci_kdn, ci_kup, ci_fz0, 0) at every stage
+   * (src/Rot2D3C.para.f90:565-569); the step then runs stage by stage. */
+  int ci_enabled;
+  double ci_kdn, ci_kup, ci_fp0, ci_fz0;
 } r2d3c_forcing;
 
 #define R2D3C_DT_FIXED 0 /* dt given by the caller (no reference counterpart) */
@@ -115,6 +120,18 @@ int r2d3c_energy(r2d3c_plan *plan, const double *a, int kin, double *d_out, void
  * inerprod (src/pseudospec2D_Rot2D3C.f90:400-451). */
 int r2d3c_inerprod(r2d3c_plan *plan, const double *a, const double *b, int kin,
                    double *d_out, void *stream);
+
+/* Constant-injection forcing of iflow = 2 for one rank: fp =
+ * ps/(|ps|^2+1) on the band kdn^2 <= k^2 <= kup^2 rescaled by
+ * fp0/inerprod(ps, ., kin), plus i*randu*|k|*ps with randu re-seeded to 0 on
+ * every call.  Replaces const_inj (src/pseudospec2D_Rot2D3C.f90:977-1060). */
+int r2d3c_const_inj(r2d3c_plan *plan, const double *ps, double kdn, double kup, double fp0,
+                    int kin, double *fp, void *stream);
+
+/* Copies the fp, fz of the last stage of the most recent iflow = 2 step (the
+ * forcing cond_check sees at the next iteration) into fp, fz (either may be
+ * NULL).  No reference counterpart: the reference keeps them in its arrays. */
+int r2d3c_const_inj_last(r2d3c_plan *plan, double *fp, double *fz, void *stream);
 
 /* ---- diagnostics (device results, deterministic) ----------------------- */
 

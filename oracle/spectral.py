@@ -20,6 +20,11 @@ Pinning (checks that do not restate the formula; see tests/test_oracle_*.py):
   randu ................... Park-Miller minimal-standard known answer
                             (state 1 -> 1043618065 after 10000 draws); float32
                             rounding checked against exact rationals
+  const_inj ............... injection <ps, f>_kin equals fp0 exactly (the random
+                            part is orthogonal to ps); band support; conjugate
+                            symmetry of the kx = 0 column; the per-mode random
+                            sequence (randu re-seeded to the rank) is parity
+                            unpinned beyond the loop order read in :1033-1057
   rand_force .............. real-space transform is the analytic cosine of the
                             drawn wavevector; amplitude from the injection rate
   rk_stages ............... linear inertial-wave limit equals the 4th-order Taylor
@@ -260,6 +265,62 @@ def rand_force(kdn, kup, fp0, dt, seed, kin, n):
                 fp[0, j] = val
                 fp[0, n - j] = val.conjugate()
     return fp, seed
+
+
+def const_inj(ps, kdn, kup, fp0, kin, myrank=0):
+    """Constant-injection forcing (iflow = 2, src/pseudospec2D_Rot2D3C.f90:977-1060):
+    f = ps/(|ps|^2+1) on the band kdn^2 <= ka2 <= kup^2, rescaled by
+    fp0/inerprod(ps, f, kin), plus i*randu*|k|*ps with randu re-seeded to the
+    rank on every call (:1032).  The kx = 0 column is filled for j = 2..n/2 and
+    mirrored by conjugation.  Returns fp."""
+    n = _n_of(ps)
+    g = grid(n)
+    band = (g.ka2 <= kup * kup) & (g.ka2 >= kdn * kdn)
+    fp = np.zeros_like(ps)
+    fp[1:] = np.where(band[1:], ps[1:] / (np.abs(ps[1:]) ** 2 + 1.0), 0.0)
+    for j in range(1, n // 2):
+        if band[0, j]:
+            fp[0, j] = ps[0, j] / (abs(ps[0, j]) ** 2 + 1.0)
+            fp[0, n - j] = np.conj(fp[0, j])
+    efp = inerprod(ps, fp, kin)
+    seed1 = myrank
+    out = np.zeros_like(ps)
+    for i in range(n // 2 + 1):
+        js = range(n) if i != 0 else range(1, n // 2)
+        for j in js:
+            if band[i, j]:
+                r, seed1 = randu(seed1)
+                tmp3 = float(r) * math.sqrt(g.ka2[i, j])
+                out[i, j] = fp[i, j] * fp0 / efp + 1j * tmp3 * ps[i, j]
+                if i == 0:
+                    out[0, n - j] = np.conj(out[0, j])
+    return out
+
+
+def rk_stages_const_inj(ps, vz, dt, p, kdn, kup, fp0, fz0, ord=4):
+    """rk_stages with iflow = 2: fp = const_inj(C1), fz = const_inj(C2) at the
+    top of every stage (src/Rot2D3C.para.f90:565-569)."""
+    g = grid(_n_of(ps))
+    upd = (g.ka2 <= g.kmax) & (g.ka2 > g.tiny)
+    ka2 = np.where(upd, g.ka2, 1.0)
+    c1, c2 = ps.copy(), vz.copy()
+    for o in range(ord, 0, -1):
+        fp = const_inj(c1, kdn, kup, fp0, 1)
+        fz = const_inj(c2, kdn, kup, fz0, 0)
+        c4 = laplak2(c1)
+        c5 = poisson(c1, c4)
+        c6 = poisson(c1, c2)
+        c3 = derivk2(c1, 2)
+        c4 = derivk2(c2, 2)
+        tmp1 = dt / float(o)
+        tmp2 = 1.0 / ka2
+        tmp3 = 1.0 + (p.nu * ka2 ** p.nn + p.hnu * tmp2 ** p.mm) * tmp1
+        n1 = (ps + ((-c5 + 2 * p.omega * c4) * tmp2 + fp) * tmp1) / tmp3
+        tmp3 = 1.0 + (p.nuv * ka2 ** p.nnv + p.hnuv * tmp2 ** p.mmv) * tmp1
+        n2 = (vz + (c6 + 2 * p.omega * c3 + fz) * tmp1) / tmp3
+        c1 = np.where(upd, n1, 0.0)
+        c2 = np.where(upd, n2, 0.0)
+    return derivk2(c1, 0), derivk2(c2, 0)
 
 
 # --------------------------------------------------------------------------

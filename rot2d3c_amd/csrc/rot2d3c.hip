@@ -610,6 +610,99 @@ struct StepConst {
   double two_omega_n2, inv_n4, n2;
 };
 
+// ---------------------------------------------------------------------------
+// iflow = 2: const_inj and the stage-by-stage step
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool ci_band(double ka2, double lo2, double hi2) {
+  return ka2 <= hi2 && ka2 >= lo2;
+}
+// fpre = ps/(|ps|^2+1) on the band; kx = 0 column for j = 1..n/2-1 mirrored by
+// conjugation, j = 0 and n/2 zero (src/pseudospec2D_Rot2D3C.f90:1005-1028)
+__global__ void k_ci_pre(const cplx *__restrict__ ps, cplx *__restrict__ fpre, double lo2, double hi2,
+                         int n) {
+  const size_t total = (size_t)(n / 2 + 1) * n;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    int jj = j;
+    bool conj = false, zero = false;
+    if (i == 0) {
+      if (j == 0 || j == n / 2) zero = true;
+      else if (j > n / 2) { jj = n - j; conj = true; }
+    }
+    const double kx = kxof(i, n), ky = kyof(jj, n), ka2 = kx * kx + ky * ky;
+    cplx v = mk(0.0, 0.0);
+    if (!zero && ci_band(ka2, lo2, hi2)) {
+      const cplx a = ps[(size_t)i * n + jj];
+      const double d = a.x * a.x + a.y * a.y + 1.0;
+      v = mk(a.x / d, a.y / d);
+      if (conj) v.y = -v.y;
+    }
+    fpre[e] = v;
+  }
+}
+// fp = fpre*fp0/Efp + i*(r*|k|)*ps on the band (:1033-1057); r: the randu
+// value of the mode (0 off the band), Efp read from device memory
+__global__ void k_ci_post(const cplx *__restrict__ ps, const cplx *__restrict__ fpre,
+                          const float *__restrict__ rtab, const double *__restrict__ efp, double fp0,
+                          double lo2, double hi2, cplx *__restrict__ fp, int n) {
+  const size_t total = (size_t)(n / 2 + 1) * n;
+  const double ef = *efp;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    int jj = j;
+    bool conj = false, zero = false;
+    if (i == 0) {
+      if (j == 0 || j == n / 2) zero = true;
+      else if (j > n / 2) { jj = n - j; conj = true; }
+    }
+    const double kx = kxof(i, n), ky = kyof(jj, n), ka2 = kx * kx + ky * ky;
+    cplx v = mk(0.0, 0.0);
+    if (!zero && ci_band(ka2, lo2, hi2)) {
+      const size_t ee = (size_t)i * n + jj;
+      const cplx f = fpre[ee], a = ps[ee];
+      const double tmp3 = (double)rtab[ee] * sqrt(ka2);
+      v = mk(f.x * fp0 / ef + (-tmp3 * a.y), f.y * fp0 / ef + tmp3 * a.x);
+      if (conj) v.y = -v.y;
+    }
+    fp[e] = v;
+  }
+}
+// One stage update of src/Rot2D3C.para.f90:578-599 from spectral C3..C6 and
+// dense fp, fz; dt from memory (CFL) or the argument.
+__global__ void k_update(const cplx *__restrict__ ps, const cplx *__restrict__ vz, cplx *c1, cplx *c2,
+                         const cplx *__restrict__ c3, const cplx *__restrict__ c4,
+                         const cplx *__restrict__ c5, const cplx *__restrict__ c6,
+                         const cplx *__restrict__ fp, const cplx *__restrict__ fz, StepConst c,
+                         const double *__restrict__ dtp, int o) {
+  const int n = c.g.n;
+  const size_t total = (size_t)(n / 2 + 1) * n;
+  const double dt = dtp ? *dtp : c.dt_fixed;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    const double kx = kxof(i, n), ky = kyof(j, n), ka2 = kx * kx + ky * ky;
+    cplx a = mk(0.0, 0.0), b = mk(0.0, 0.0);
+    if (ka2 <= c.g.kmax && ka2 > c.g.tiny) {
+      const double tmp1 = dt / (double)o, tmp2 = 1.0 / ka2;
+      double tmp3 = 1.0 + (c.nu * ipow(ka2, c.nn) + c.hnu * ipow(tmp2, c.mm)) * tmp1;
+      const cplx t5 = c5[e], t4 = c4[e], f = fp[e];
+      cplx t = mk((-t5.x + c.two_omega * t4.x) * tmp2 + f.x, (-t5.y + c.two_omega * t4.y) * tmp2 + f.y);
+      t = cadd(ps[e], cscale(t, tmp1));
+      a = mk(t.x / tmp3, t.y / tmp3);
+      tmp3 = 1.0 + (c.nuv * ipow(ka2, c.nnv) + c.hnuv * ipow(tmp2, c.mmv)) * tmp1;
+      const cplx t6 = c6[e], t3 = c3[e], g = fz[e];
+      t = mk(t6.x + c.two_omega * t3.x + g.x, t6.y + c.two_omega * t3.y + g.y);
+      t = cadd(vz[e], cscale(t, tmp1));
+      b = mk(t.x / tmp3, t.y / tmp3);
+    }
+    c1[e] = a;
+    c2[e] = b;
+  }
+}
+__global__ void k_set_f64(double *p, double v) { *p = v; }
+
 // Column inverse of the stage-1 state.  Logical blocks [0, nc): psi column kx
 // -> M0 = psi~, M1 = d_y psi~, M2 = w~, M3 = d_y w~ with w = -k^2 psi
 // (laplak2, src/Rot2D3C.para.f90:571); blocks [ncp, ncp+nc): vz column kx ->
@@ -1038,6 +1131,11 @@ struct r2d3c_plan {
   double *real2 = nullptr;        // second n*n real scratch (extrema)
   cplx *spec3 = nullptr;          // three spectral scratch fields (transfers)
   double *dpart = nullptr;        // diagnostics partials
+  cplx *ci = nullptr;             // iflow = 2: C1..C6, fpre, fp, fz
+  float *ci_rtab = nullptr;       // const_inj random values per mode
+  double ci_key[2] = {-1.0, -1.0};  // (kdn, kup) of ci_rtab
+  std::vector<float> ci_host;
+  double *ci_efp = nullptr;
   unsigned long long *maxbits = nullptr;
   double *dtslot = nullptr;
   double *partial = nullptr;      // reduction partials
@@ -1085,6 +1183,12 @@ static int hip_err(hipError_t e, const char *where) {
   } while (0)
 
 static constexpr int kReduceBlocks = 256;
+
+static int launch_grid(size_t total) {
+  size_t g = (total + 255) / 256;
+  return (int)(g > 2048 ? 2048 : (g == 0 ? 1 : g));
+}
+
 
 #define R2D_DISPATCH(n, FN, ...)                                   \
   switch (n) {                                                     \
@@ -1254,10 +1358,87 @@ static int step_seq(r2d3c_plan *p, StepConst &c, cplx *psi, cplx *vz, hipStream_
   return 0;
 }
 
+extern "C" float r2d3c_randu(int *idum);
+
+// randu values of the band modes in const_inj's loop order, seed re-set to the
+// rank (0) on every call (src/pseudospec2D_Rot2D3C.f90:1032-1057); uploaded
+// once per (kdn, kup).
+static int ci_table(r2d3c_plan *p, double kdn, double kup, hipStream_t s) {
+  if (p->ci_key[0] == kdn && p->ci_key[1] == kup) return 0;
+  const int n = p->n;
+  p->ci_host.assign((size_t)(n / 2 + 1) * n, 0.0f);
+  int seed = 0;
+  for (int i = 0; i <= n / 2; ++i) {
+    const double kx = i < n / 2 ? i : i - n;
+    const int j0 = i == 0 ? 1 : 0, j1 = i == 0 ? n / 2 - 1 : n - 1;
+    for (int j = j0; j <= j1; ++j) {
+      const double ky = j < n / 2 ? j : j - n, ka2 = kx * kx + ky * ky;
+      if (ka2 <= kup * kup && ka2 >= kdn * kdn) p->ci_host[(size_t)i * n + j] = r2d3c_randu(&seed);
+    }
+  }
+  HIPCHK(hipMemcpyAsync(p->ci_rtab, p->ci_host.data(), sizeof(float) * p->ci_host.size(),
+                        hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));  // ci_host may be refilled by the next key
+  p->ci_key[0] = kdn;
+  p->ci_key[1] = kup;
+  return 0;
+}
+
+static int const_inj_dev(r2d3c_plan *p, const cplx *ps, double kdn, double kup, double fp0, int kin,
+                         cplx *fpre, cplx *fp, hipStream_t s);
+
+template <int N>
+static int run_step_ci(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
+                       const r2d3c_forcing *fo, int dt_mode, double dt_or_cfl, double *d_dt,
+                       hipStream_t s) {
+  StepConst c;
+  make_step_const(p, q, dt_mode, dt_or_cfl, &c);
+  const size_t sz = (size_t)(N / 2 + 1) * N;
+  cplx *C1 = p->ci, *C2 = C1 + sz, *C3 = C2 + sz, *C4 = C3 + sz, *C5 = C4 + sz, *C6 = C5 + sz;
+  cplx *fpre = C6 + sz, *fp = fpre + sz, *fz = fp + sz;
+  double *dtp = d_dt ? d_dt : p->dtslot;
+  int rc;
+  if (dt_mode == R2D3C_DT_CFL) {
+    rc = r2d3c_cfl(p, q, dt_or_cfl, (const double *)psi, dtp, s);
+  } else {
+    k_set_f64<<<1, 1, 0, s>>>(dtp, dt_or_cfl);
+    rc = 0;
+  }
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(C1, psi, sizeof(cplx) * sz, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(C2, vz, sizeof(cplx) * sz, hipMemcpyDeviceToDevice, s));
+  const int g = launch_grid(sz);
+  for (int o = 4; o >= 1; --o) {
+    if ((rc = const_inj_dev(p, C1, fo->ci_kdn, fo->ci_kup, fo->ci_fp0, 1, fpre, fp, s))) return rc;
+    if ((rc = const_inj_dev(p, C2, fo->ci_kdn, fo->ci_kup, fo->ci_fz0, 0, fpre, fz, s))) return rc;
+    k_spec_op<<<g, 256, 0, s>>>(C1, C4, OP_LAP, N, p->g.kmax, p->g.tiny);
+    if ((rc = run_poisson<N>(p, C1, C4, C5, s))) return rc;
+    if ((rc = run_poisson<N>(p, C1, C2, C6, s))) return rc;
+    k_spec_op<<<g, 256, 0, s>>>(C1, C3, OP_DY, N, p->g.kmax, p->g.tiny);
+    k_spec_op<<<g, 256, 0, s>>>(C2, C4, OP_DY, N, p->g.kmax, p->g.tiny);
+    k_update<<<g, 256, 0, s>>>(psi, vz, C1, C2, C3, C4, C5, C6, fp, fz, c, dtp, o);
+  }
+  k_spec_op<<<g, 256, 0, s>>>(C1, psi, OP_MASK, N, p->g.kmax, p->g.tiny);
+  k_spec_op<<<g, 256, 0, s>>>(C2, vz, OP_MASK, N, p->g.kmax, p->g.tiny);
+  LAUNCHCHK("iflow=2 step");
+  return 0;
+}
+
 template <int N>
 static int run_steps(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
                      const r2d3c_forcing *fo, int nsteps, int dt_mode, double dt_or_cfl,
                      double *d_dt, hipStream_t s) {
+  if (fo && fo[0].ci_enabled) {  // iflow = 2 runs stage by stage
+    if ((fo[0].ci_kdn > fo[0].ci_kup) || fo[0].ci_kup < 0.0)
+      return set_err(R2D3C_EINVAL, "const_inj band must satisfy 0 <= kdn <= kup");
+    int rc = ci_table(p, fo[0].ci_kdn, fo[0].ci_kup, s);
+    if (rc) return rc;
+    for (int st = 0; st < nsteps; ++st) {
+      rc = run_step_ci<N>(p, q, psi, vz, &fo[st], dt_mode, dt_or_cfl, d_dt ? d_dt + st : nullptr, s);
+      if (rc) return rc;
+    }
+    return 0;
+  }
   StepConst c;
   make_step_const(p, q, dt_mode, dt_or_cfl, &c);
   for (int st = 0; st < nsteps; ++st) {
@@ -1279,10 +1460,6 @@ static int run_steps(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
   return 0;
 }
 
-static int launch_grid(size_t total) {
-  size_t g = (total + 255) / 256;
-  return (int)(g > 2048 ? 2048 : (g == 0 ? 1 : g));
-}
 
 extern "C" {
 
@@ -1326,6 +1503,9 @@ int r2d3c_plan_create(r2d3c_plan **out, int n) {
       (e = hipMalloc(&p->real2, b_re)) != hipSuccess ||
       (e = hipMalloc(&p->spec3, 3 * sizeof(cplx) * (size_t)(n / 2 + 1) * n)) != hipSuccess ||
       (e = hipMalloc(&p->dpart, sizeof(double) * kCondSums * kReduceBlocks)) != hipSuccess ||
+      (e = hipMalloc(&p->ci, 9 * sizeof(cplx) * (size_t)(n / 2 + 1) * n)) != hipSuccess ||
+      (e = hipMalloc(&p->ci_rtab, sizeof(float) * (size_t)(n / 2 + 1) * n)) != hipSuccess ||
+      (e = hipMalloc(&p->ci_efp, 256)) != hipSuccess ||
       (e = hipMemcpy(p->tw, tw.data(), b_tw, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemset(p->mstep, 0, b_ms)) != hipSuccess || (e = hipMemset(p->mfull, 0, b_mf)) != hipSuccess) {
     int rc = hip_err(e, "plan allocation");
@@ -1334,7 +1514,9 @@ int r2d3c_plan_create(r2d3c_plan **out, int n) {
   }
   p->bytes = (long long)(b_tw + b_ms + b_mf + 2 * b_re + 512 + sizeof(double) * kReduceBlocks +
                          3 * sizeof(cplx) * (size_t)(n / 2 + 1) * n +
-                         sizeof(double) * kCondSums * kReduceBlocks);
+                         sizeof(double) * kCondSums * kReduceBlocks +
+                         9 * sizeof(cplx) * (size_t)(n / 2 + 1) * n +
+                         sizeof(float) * (size_t)(n / 2 + 1) * n + 256);
   *out = p;
   return 0;
 }
@@ -1352,6 +1534,9 @@ int r2d3c_plan_destroy(r2d3c_plan *p) {
   (void)hipFree(p->real2);
   (void)hipFree(p->spec3);
   (void)hipFree(p->dpart);
+  (void)hipFree(p->ci);
+  (void)hipFree(p->ci_rtab);
+  (void)hipFree(p->ci_efp);
   delete p;
   return 0;
 }
@@ -1532,6 +1717,43 @@ int r2d3c_cond_check(r2d3c_plan *p, const r2d3c_params *q, double kup, const dou
   k_minmax_partial<<<kReduceBlocks, 256, 0, s>>>(p->real2, nn2, p->dpart);
   k_minmax_finish<<<1, 1, 0, s>>>(p->dpart, kReduceBlocks, d_out + kCondSums + 2);
   LAUNCHCHK("cond_check extrema");
+  return 0;
+}
+
+}  // extern "C"
+
+static int const_inj_dev(r2d3c_plan *p, const cplx *ps, double kdn, double kup, double fp0, int kin,
+                         cplx *fpre, cplx *fp, hipStream_t s) {
+  const int n = p->n;
+  const size_t sz = (size_t)(n / 2 + 1) * n;
+  const int g = launch_grid(sz);
+  k_ci_pre<<<g, 256, 0, s>>>(ps, fpre, kdn * kdn, kup * kup, n);
+  int rc = r2d3c_inerprod(p, (const double *)ps, (const double *)fpre, kin, p->ci_efp, s);
+  if (rc) return rc;
+  k_ci_post<<<g, 256, 0, s>>>(ps, fpre, p->ci_rtab, p->ci_efp, fp0, kdn * kdn, kup * kup, fp, n);
+  LAUNCHCHK("const_inj");
+  return 0;
+}
+
+extern "C" {
+
+int r2d3c_const_inj(r2d3c_plan *p, const double *ps, double kdn, double kup, double fp0, int kin,
+                    double *fp, void *stream) {
+  NEED(p); NEED(ps); NEED(fp);
+  if (kdn > kup || kup < 0.0) return set_err(R2D3C_EINVAL, "const_inj band must satisfy 0 <= kdn <= kup");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = ci_table(p, kdn, kup, s);
+  if (rc) return rc;
+  const size_t sz = (size_t)(p->n / 2 + 1) * p->n;
+  return const_inj_dev(p, (const cplx *)ps, kdn, kup, fp0, kin, p->ci + 6 * sz, (cplx *)fp, s);
+}
+
+int r2d3c_const_inj_last(r2d3c_plan *p, double *fp, double *fz, void *stream) {
+  NEED(p);
+  const size_t sz = (size_t)(p->n / 2 + 1) * p->n;
+  hipStream_t s = (hipStream_t)stream;
+  if (fp) HIPCHK(hipMemcpyAsync(fp, p->ci + 7 * sz, sizeof(cplx) * sz, hipMemcpyDeviceToDevice, s));
+  if (fz) HIPCHK(hipMemcpyAsync(fz, p->ci + 8 * sz, sizeof(cplx) * sz, hipMemcpyDeviceToDevice, s));
   return 0;
 }
 

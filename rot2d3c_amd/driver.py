@@ -15,8 +15,8 @@ formats (rot2d3c_amd.io), so the reference's analysis scripts read them.
 
 The grid size n is a compile-time constant of the reference
 (src/pseudospec2D_mod.f90:11); here it is the --n option.  iflow = 2
-(const_inj, which rescales the forcing at every stage from the stage's own
-state and draws its random numbers per MPI rank) is not supported.
+(const_inj) recomputes the forcing at every stage from the stage's own state;
+its random numbers are those of rank 0 (one process).
 
 Usage: python -m rot2d3c_amd.driver --n 256 [--workdir DIR] [--nprocs-out P]
 """
@@ -82,8 +82,8 @@ class Run:
         q = self.prm
         self.params = S.Params(q["nu"], q["hnu"], q["nn"], q["mm"], q["nuv"], q["hnuv"],
                                q["nnv"], q["mmv"], q["omega"])
-        if q["iflow"] not in (0, 1, 3):
-            raise NotImplementedError("iflow = %d is not supported (1 and 3 are)" % q["iflow"])
+        if q["iflow"] not in (0, 1, 2, 3):
+            raise ValueError("iflow = %d is not one of 1, 2, 3" % q["iflow"])
         self.odir = self._path(q["odir"])
         self.idir = self._path(q["idir"])
 
@@ -177,8 +177,13 @@ class Run:
             spec_idx = int(float(ini) / float(sstep) + 1)
         zero = torch.zeros_like(ps)
         dense = None
+        ci = None
         fp_draw = fz_draw = None
         fpd = fzd = zero
+        if q["iflow"] == 2:  # pre-loop const_inj pair (:372-375)
+            ci = (q["kdn"], q["kup"], q["fp0"], q["fz0"])
+            fpd = S.const_inj(ps, q["kdn"], q["kup"], q["fp0"], 1)
+            fzd = S.const_inj(vz, q["kdn"], q["kup"], q["fz0"], 0)
         if q["iflow"] == 1:
             fpd, fzd = self.static_forcing()
             dense = (fpd, fzd)
@@ -215,7 +220,14 @@ class Run:
             timet += 1
             times += 1
             timec += 1
-            S.rk4_step(ps, vz, self.params, fp_draw, fz_draw, cfl=cfl, dt_out=dt_dev, dense=dense)
+            if ci is not None:
+                # the forcing cond_check sees next is the one of the last stage
+                # (o = 1), computed from that stage's input state
+                S.rk4_step(ps, vz, self.params, cfl=cfl, dt_out=dt_dev, const_inj=ci)
+                fpd, fzd = S.const_inj_last(ps)
+            else:
+                S.rk4_step(ps, vz, self.params, fp_draw, fz_draw, cfl=cfl, dt_out=dt_dev,
+                           dense=dense)
             last_dt = float(dt_dev.item())
             if not math.isfinite(last_dt):
                 raise FloatingPointError("non-finite dt at iteration %d: the run blew up" % t)

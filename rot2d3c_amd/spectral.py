@@ -224,10 +224,14 @@ class ForceDraw:
         return _lib.ForceDraw(self.kx, self.ky, self.phase, self.f0, self.kupk, 1)
 
 
-def _forcing_c(fp_draw, fz_draw, dense=None):
+def _forcing_c(fp_draw, fz_draw, dense=None, const_inj=None):
     """fp_draw / fz_draw: ForceDraw or None; dense: optional (fp, fz) pair of
-    static spectral arrays (either may be None)."""
+    static spectral arrays (either may be None); const_inj: optional
+    (kdn, kup, fp0, fz0) of iflow = 2."""
     f = _lib.Forcing()
+    if const_inj is not None:
+        f.ci_enabled = 1
+        f.ci_kdn, f.ci_kup, f.ci_fp0, f.ci_fz0 = (float(x) for x in const_inj)
     if fp_draw is not None:
         f.psi = fp_draw.c()
     if fz_draw is not None:
@@ -244,13 +248,15 @@ def _forcing_c(fp_draw, fz_draw, dense=None):
 
 
 def rk4_step(ps, vz, p, fp_draw=None, fz_draw=None, dt=None, cfl=None, dt_out=None,
-             dense=None):
+             dense=None, const_inj=None):
     """One time step in place (src/Rot2D3C.para.f90:399 and :545-611).
 
     Exactly one of `dt` (fixed step) and `cfl` (CFL-adaptive step from the
     pre-step state, as the reference) must be given.  The dt used is written
     to `dt_out` (a 1-element float64 CUDA tensor) when given.  dense:
-    optional (fp, fz) static forcing arrays added to the draws (iflow = 1)."""
+    optional (fp, fz) static forcing arrays added to the draws (iflow = 1).
+    const_inj: (kdn, kup, fp0, fz0) for iflow = 2 (forcing recomputed from
+    the stage state at every stage)."""
     n = _spec_n(ps)
     if _spec_n(vz) != n:
         raise ValueError("size mismatch")
@@ -258,7 +264,7 @@ def rk4_step(ps, vz, p, fp_draw=None, fz_draw=None, dt=None, cfl=None, dt_out=No
         raise ValueError("give exactly one of dt and cfl")
     pl = Plan.get(n, ps.device)
     cp = p.c()
-    fo = _forcing_c(fp_draw, fz_draw, dense)
+    fo = _forcing_c(fp_draw, fz_draw, dense, const_inj)
     mode, val = (R2D3C_DT_FIXED, dt) if dt is not None else (R2D3C_DT_CFL, cfl)
     optr = _ptr(dt_out) if dt_out is not None else ctypes.c_void_p(0)
     check(_lib.lib().r2d3c_step(pl.handle, ctypes.byref(cp), _ptr(ps), _ptr(vz), ctypes.byref(fo),
@@ -308,6 +314,27 @@ def forcing_fill(n, device, fp_draw, fz_draw, dt):
     fo = _forcing_c(fp_draw, fz_draw)
     check(_lib.lib().r2d3c_forcing_fill(pl.handle, ctypes.byref(fo), float(dt), _ptr(fp), _ptr(fz),
                                         _stream(dev)), "r2d3c_forcing_fill")
+    return fp, fz
+
+
+def const_inj(ps, kdn, kup, fp0, kin, out=None):
+    """Constant-injection forcing of iflow = 2 for one rank
+    (src/pseudospec2D_Rot2D3C.f90:977-1060)."""
+    n = _spec_n(ps)
+    out = _new_spec(n, ps) if out is None else out
+    pl = Plan.get(n, ps.device)
+    check(_lib.lib().r2d3c_const_inj(pl.handle, _ptr(ps), float(kdn), float(kup), float(fp0),
+                                     int(kin), _ptr(out), _stream(ps.device)), "r2d3c_const_inj")
+    return out
+
+
+def const_inj_last(ps):
+    """(fp, fz) of the last stage of the most recent iflow = 2 step on ps's plan."""
+    n = _spec_n(ps)
+    pl = Plan.get(n, ps.device)
+    fp, fz = _new_spec(n, ps), _new_spec(n, ps)
+    check(_lib.lib().r2d3c_const_inj_last(pl.handle, _ptr(fp), _ptr(fz), _stream(ps.device)),
+          "r2d3c_const_inj_last")
     return fp, fz
 
 
@@ -419,5 +446,5 @@ __all__ = [
     "inerprod_device", "CFL_condition", "CFL_condition_device", "rk4_step", "rk4_run",
     "forcing_fill",
     "randu", "rand_force_draw", "rand_force", "kmax_index", "cond_check", "cond_check_values",
-    "spectrum", "transfers", "COND_NAMES",
+    "spectrum", "transfers", "COND_NAMES", "const_inj", "const_inj_last",
 ]

@@ -83,3 +83,60 @@ def test_driver_restart_reads_its_own_dumps(tmp_path, cuda):
     assert rel(gps.cpu().numpy(), ops) < 1e-11
     tf2 = open(str(tmp_path / "time_field.txt")).read().splitlines()
     assert [l[:3] for l in tf2] == ["%03d" % e[0] for e in ev["field"]]
+
+
+def test_driver_iflow2_matches_oracle_loop(tmp_path, cuda):
+    """iflow = 2 through the driver: the stage-by-stage step and the balance
+    rows that see each step's last-stage forcing."""
+    print("driver iflow=2 n=32", flush=True)
+    from rot2d3c_amd import driver, io as rio
+    n = 32
+    prm = dict(PRM, iflow=2, kdn=2.0, kup=5.0, step=6, sstep=100, tstep=100)
+    rio.write_parameter_inp(str(tmp_path / "parameter.inp"), prm)
+    rio.write_status_inp(str(tmp_path / "status.inp"), 0, 0.0)
+    r = driver.Run(n, str(tmp_path), "cuda:0")
+    gps, gvz = r.run()
+    # oracle loop with per-stage const_inj
+    p = O.Params(prm["nu"], prm["hnu"], prm["nn"], prm["mm"], prm["nuv"], prm["hnuv"],
+                 prm["nnv"], prm["mmv"], prm["omega"])
+    ps, vz, _ = O.initial_condition(n, prm["kup"], prm["u0"], prm["v0"], prm["seed"])
+    fp = O.const_inj(ps, prm["kdn"], prm["kup"], prm["fp0"], 1)
+    fz = O.const_inj(vz, prm["kdn"], prm["kup"], prm["fz0"], 0)
+    time, rows = 0.0, []
+    for _ in range(prm["step"]):
+        dt = O.CFL_condition(prm["CFL"], ps, vz, p.nu, p.nn, p.nuv, p.nnv, p.omega)
+        rows.append(O.cond_check(ps, vz, fp, fz, time, p, prm["kup"])["energy_bal"])
+        time += dt
+        # last-stage forcing of this step, as the balance of the next iteration sees it
+        c1, c2 = ps, vz
+        g = O.grid(n)
+        ps, vz = O.rk_stages_const_inj(ps, vz, dt, p, prm["kdn"], prm["kup"], prm["fp0"], prm["fz0"])
+        fp, fz = _last_stage_forcing(c1, c2, dt, p, prm)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(gps.cpu().numpy(), ps) < 1e-11
+    bal = np.loadtxt(str(tmp_path / "energy_bal.txt"))
+    assert np.allclose(bal, np.array(rows), rtol=1e-9, atol=1e-12)
+
+
+def _last_stage_forcing(ps, vz, dt, p, prm):
+    """fp, fz computed at the top of stage o = 1 (from that stage's input)."""
+    g = O.grid(ps.shape[1])
+    upd = (g.ka2 <= g.kmax) & (g.ka2 > g.tiny)
+    ka2 = np.where(upd, g.ka2, 1.0)
+    c1, c2 = ps.copy(), vz.copy()
+    for o in range(4, 0, -1):
+        fp = O.const_inj(c1, prm["kdn"], prm["kup"], prm["fp0"], 1)
+        fz = O.const_inj(c2, prm["kdn"], prm["kup"], prm["fz0"], 0)
+        if o == 1:
+            return fp, fz
+        c4 = O.laplak2(c1)
+        c5 = O.poisson(c1, c4)
+        c6 = O.poisson(c1, c2)
+        c3 = O.derivk2(c1, 2)
+        c4 = O.derivk2(c2, 2)
+        tmp1, tmp2 = dt / float(o), 1.0 / ka2
+        t3 = 1.0 + (p.nu * ka2 ** p.nn + p.hnu * tmp2 ** p.mm) * tmp1
+        n1 = (ps + ((-c5 + 2 * p.omega * c4) * tmp2 + fp) * tmp1) / t3
+        t3 = 1.0 + (p.nuv * ka2 ** p.nnv + p.hnuv * tmp2 ** p.mmv) * tmp1
+        n2 = (vz + (c6 + 2 * p.omega * c3 + fz) * tmp1) / t3
+        c1, c2 = np.where(upd, n1, 0.0), np.where(upd, n2, 0.0)

@@ -230,3 +230,36 @@ def test_spectrum_transfers_match_oracle(n, cuda):
         assert np.max(np.abs(a - b)) <= 1e-11 * np.max(np.abs(b))
     for a, b in ((x0, ox0), (x1, ox1), (x2, ox2)):
         assert np.max(np.abs(a - b)) <= 1e-11 * np.max(np.abs(b))
+
+
+@pytest.mark.parametrize("n", [32, 256])
+def test_const_inj_matches_oracle(n, cuda):
+    print("const_inj n=%d" % n, flush=True)
+    S = _S()
+    ps, vz = random_state(n, 51)
+    for kin, f0 in ((1, 0.7), (0, 1.1)):
+        got = _host(S.const_inj(_dev(ps, cuda), 3.0, 7.0, f0, kin))
+        ref = O.const_inj(ps, 3.0, 7.0, f0, kin)
+        assert rel_l2(got, ref) < 1e-13
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_step_iflow2_10_steps(n, cuda):
+    """iflow = 2: forcing from each stage's own state (stage-by-stage path)."""
+    print("10 iflow=2 steps n=%d" % n, flush=True)
+    import torch
+    S = _S()
+    ps, vz = random_state(n, 61)
+    p = O.Params(1e-3, 0.1, 1, 2, 2e-3, 0.05, 1, 2, 1.0)
+    sp = S.Params(p.nu, p.hnu, p.nn, p.mm, p.nuv, p.hnuv, p.nnv, p.mmv, p.omega)
+    ci = (2.0, 5.0, 0.5, 0.3)
+    ops, ovz = ps.copy(), vz.copy()
+    tps, tvz = _dev(ps, cuda), _dev(vz, cuda)
+    dto = torch.zeros(1, dtype=torch.float64, device=cuda)
+    for _ in range(10):
+        dt = O.CFL_condition(0.5, ops, ovz, p.nu, p.nn, p.nuv, p.nnv, p.omega)
+        ops, ovz = O.rk_stages_const_inj(ops, ovz, dt, p, *ci)
+        S.rk4_step(tps, tvz, sp, cfl=0.5, dt_out=dto, const_inj=ci)
+        assert abs(float(dto.item()) - dt) <= 1e-12 * dt
+    assert rel_l2(_host(tps), ops) < TOL
+    assert rel_l2(_host(tvz), ovz) < TOL

@@ -352,3 +352,25 @@ def test_cond_check_columns_by_parseval():
     assert math.isclose(rows["enstrophy_bal"][1], np.mean(w * w), rel_tol=1e-12)
     mm = rows["max_min"]
     assert math.isclose(mm[1], (w * n2).max(), rel_tol=1e-12) and math.isclose(mm[2], (w * n2).min(), rel_tol=1e-12)
+
+
+def test_const_inj_injects_fp0_on_its_band():
+    """const_inj is built so that the injection <ps, f>_kin is exactly fp0
+    (src/pseudospec2D_Rot2D3C.f90:1030,1038: f*fp0/Efp plus a part i*r*|k|*ps
+    orthogonal to ps)."""
+    n = 32
+    ps, vz = random_state(n, 17)
+    g = O.grid(n)
+    for kin, fp0 in ((1, 0.7), (0, 1.3)):
+        f = O.const_inj(ps, 3.0, 6.0, fp0, kin)
+        assert math.isclose(O.inerprod(ps, f, kin), fp0, rel_tol=1e-12)
+        band = (g.ka2 >= 9.0) & (g.ka2 <= 36.0)
+        assert np.all(f[~band] == 0)
+        assert np.all(f[0, 1:n // 2] == np.conj(f[0, n - 1:n // 2:-1]))
+        # beyond the real rescaling of ps/(|ps|^2+1), the rest is i*r*|k|*ps
+        # with r in [-1, 1): strip the deterministic part and look at r
+        det = np.where(band, ps / (np.abs(ps) ** 2 + 1.0), 0)
+        det = det * fp0 / O.inerprod(ps, det, kin)
+        m = band & (np.abs(ps) > 0)
+        r = (f[m] - det[m]) / (1j * np.sqrt(g.ka2[m]) * ps[m])
+        assert np.allclose(r.imag, 0.0, atol=1e-9) and np.all(np.abs(r.real) <= 1.0)
