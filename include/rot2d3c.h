@@ -65,9 +65,12 @@ typedef struct r2d3c_forcing {
   const double *dense_vz;
   /* iflow = 2: fp = const_inj(C1, ci_kdn, ci_kup, ci_fp0, 1) and
    * fz = const_inj(C2, ci_kdn, ci_kup, ci_fz0, 0) at every stage
-   * (src/Rot2D3C.para.f90:565-569); the step then runs stage by stage. */
+   * (src/Rot2D3C.para.f90:565-569); the step then runs stage by stage.
+   * ci_nprocs: the MPI rank count whose random numbers to reproduce
+   * (<= 1: one rank). */
   int ci_enabled;
   double ci_kdn, ci_kup, ci_fp0, ci_fz0;
+  int ci_nprocs;
 } r2d3c_forcing;
 
 #define R2D3C_DT_FIXED 0 /* dt given by the caller (no reference counterpart) */
@@ -121,12 +124,14 @@ int r2d3c_energy(r2d3c_plan *plan, const double *a, int kin, double *d_out, void
 int r2d3c_inerprod(r2d3c_plan *plan, const double *a, const double *b, int kin,
                    double *d_out, void *stream);
 
-/* Constant-injection forcing of iflow = 2 for one rank: fp =
- * ps/(|ps|^2+1) on the band kdn^2 <= k^2 <= kup^2 rescaled by
- * fp0/inerprod(ps, ., kin), plus i*randu*|k|*ps with randu re-seeded to 0 on
- * every call.  Replaces const_inj (src/pseudospec2D_Rot2D3C.f90:977-1060). */
+/* Constant-injection forcing of iflow = 2: fp = ps/(|ps|^2+1) on the band
+ * kdn^2 <= k^2 <= kup^2 rescaled by fp0/inerprod(ps, ., kin), plus
+ * i*randu*|k|*ps.  The random numbers are those of an nprocs-rank run
+ * (<= 1: one rank): each rank owns the kx columns of its slab and restarts
+ * randu at seed = rank on every call.  Replaces const_inj
+ * (src/pseudospec2D_Rot2D3C.f90:977-1060). */
 int r2d3c_const_inj(r2d3c_plan *plan, const double *ps, double kdn, double kup, double fp0,
-                    int kin, double *fp, void *stream);
+                    int kin, int nprocs, double *fp, void *stream);
 
 /* Copies the fp, fz of the last stage of the most recent iflow = 2 step (the
  * forcing cond_check sees at the next iteration) into fp, fz (either may be
@@ -139,12 +144,13 @@ int r2d3c_const_inj_last(r2d3c_plan *plan, double *fp, double *fz, void *stream)
  * d_out[26]: Ep1 injp1 Dsp Hds Epv injz Dspv Hdsv coup Ep2 injp2 Dspw Hdsw Eph
  * injh1 injh2 Dsph1 Hdsh1 Dsph2 Hdsh2 Efp Efpz, then max and min of
  * c2r(laplak2(psi)) and of c2r(derivk2(vz,0)).  fp, fz: dense forcing arrays
- * of the previous step (zeros without forcing).  Writing the four balance
- * files from these is left to the host (one rank: the reference's min is
- * MPI_MAX over one value, i.e. the min). */
+ * of the previous step (zeros without forcing).  The extrema are those of an
+ * nprocs-rank run (<= 1: one rank): each rank reduces its slab of real-space
+ * rows and both are combined with MPI_MAX (:578,586), so "min" is the largest
+ * slab minimum.  Writing the four balance files is left to the host. */
 int r2d3c_cond_check(r2d3c_plan *plan, const r2d3c_params *p, double kup, const double *psi,
-                     const double *vz, const double *fp, const double *fz, double *d_out,
-                     void *stream);
+                     const double *vz, const double *fp, const double *fz, int nprocs,
+                     double *d_out, void *stream);
 /* Shell spectra (src/pseudospec2D_Rot2D3C.f90:615-712): d_out[2*(n/2+1)] =
  * Ek(k) then Ezk(k), k = 1..n/2+1. */
 int r2d3c_spectrum(r2d3c_plan *plan, const double *psi, const double *vz, double *d_out,

@@ -22,9 +22,10 @@ Pinning (checks that do not restate the formula; see tests/test_oracle_*.py):
                             rounding checked against exact rationals
   const_inj ............... injection <ps, f>_kin equals fp0 exactly (the random
                             part is orthogonal to ps); band support; conjugate
-                            symmetry of the kx = 0 column; the per-mode random
-                            sequence (randu re-seeded to the rank) is parity
-                            unpinned beyond the loop order read in :1033-1057
+                            symmetry of the kx = 0 column; the random factors
+                            recovered from the output are, slab by slab, the
+                            known-answer Park-Miller stream from seed = rank
+                            (P = 1, 2, 4)
   rand_force .............. real-space transform is the analytic cosine of the
                             drawn wavevector; amplitude from the injection rate
   rk_stages ............... linear inertial-wave limit equals the 4th-order Taylor
@@ -36,15 +37,19 @@ Pinning (checks that do not restate the formula; see tests/test_oracle_*.py):
                             linear problems only); a mutation test plants
                             plausible slips and checks that these pins fail
   spectrum / transfers .... sum_k E(k) equals energy(); sum_k T(k) = 0
-  cond_check .............. each column equals the Parseval real-space value
+  cond_check .............. each column equals the Parseval real-space value;
+                            the energy, enstrophy and helicity budgets close
+                            (to 1e-3 with steady forcing, statistically with
+                            random forcing) as the reference's BalanceTest
+                            script checks them; max_min per slab by hand
   reference_loop /
-  run_random_forcing ...... parity unpinned: they restate the statement order of
-                            the main loop (CFL, cond_check with the previous
-                            step's forcing, new draws, outputs, time += dt,
-                            stages; counters of a fresh start and of a restart),
-                            which only a run of the reference would pin, and the
-                            reference cannot be built here (it needs FFTW 2.x,
-                            absent from the image)
+  run_random_forcing ...... the forcing lag of cond_check is pinned by the
+                            random-forcing budget closure (a mutation test shows
+                            it); the rest of the statement order (outputs,
+                            time += dt, counters of a fresh start and of a
+                            restart) is parity unpinned: only a run of the
+                            reference would pin it, and the reference cannot be
+                            built here (it needs FFTW 2.x, absent from the image)
 """
 import math
 
@@ -267,12 +272,23 @@ def rand_force(kdn, kup, fp0, dt, seed, kin, n):
     return fp, seed
 
 
-def const_inj(ps, kdn, kup, fp0, kin, myrank=0):
+def slab(n1, n2, nprocs, irank):
+    """0-based half-open [lo, hi) of rank irank's share of the 1-based range
+    [n1, n2] (the even split of `range`, src/fftp.f90:17-47)."""
+    iwork1, iwork2 = divmod(n2 - n1 + 1, nprocs)
+    ista = irank * iwork1 + n1 + min(irank, iwork2)
+    iend = ista + iwork1 - 1 + (1 if iwork2 > irank else 0)
+    return ista - 1, iend
+
+
+def const_inj(ps, kdn, kup, fp0, kin, nprocs=1):
     """Constant-injection forcing (iflow = 2, src/pseudospec2D_Rot2D3C.f90:977-1060):
     f = ps/(|ps|^2+1) on the band kdn^2 <= ka2 <= kup^2, rescaled by
-    fp0/inerprod(ps, f, kin), plus i*randu*|k|*ps with randu re-seeded to the
-    rank on every call (:1032).  The kx = 0 column is filled for j = 2..n/2 and
-    mirrored by conjugation.  Returns fp."""
+    fp0/inerprod(ps, f, kin), plus i*randu*|k|*ps.  The kx = 0 column is filled
+    for j = 2..n/2 and mirrored by conjugation.  With nprocs ranks each rank
+    owns the kx columns of its slab (src/Rot2D3C.para.f90:102) and re-seeds
+    randu with its rank on every call (:1032), drawing in its own loop order.
+    Returns fp."""
     n = _n_of(ps)
     g = grid(n)
     band = (g.ka2 <= kup * kup) & (g.ka2 >= kdn * kdn)
@@ -283,21 +299,23 @@ def const_inj(ps, kdn, kup, fp0, kin, myrank=0):
             fp[0, j] = ps[0, j] / (abs(ps[0, j]) ** 2 + 1.0)
             fp[0, n - j] = np.conj(fp[0, j])
     efp = inerprod(ps, fp, kin)
-    seed1 = myrank
     out = np.zeros_like(ps)
-    for i in range(n // 2 + 1):
-        js = range(n) if i != 0 else range(1, n // 2)
-        for j in js:
-            if band[i, j]:
-                r, seed1 = randu(seed1)
-                tmp3 = float(r) * math.sqrt(g.ka2[i, j])
-                out[i, j] = fp[i, j] * fp0 / efp + 1j * tmp3 * ps[i, j]
-                if i == 0:
-                    out[0, n - j] = np.conj(out[0, j])
+    for rank in range(nprocs):
+        seed1 = rank
+        lo, hi = slab(1, n // 2 + 1, nprocs, rank)
+        for i in range(lo, hi):
+            js = range(n) if i != 0 else range(1, n // 2)
+            for j in js:
+                if band[i, j]:
+                    r, seed1 = randu(seed1)
+                    tmp3 = float(r) * math.sqrt(g.ka2[i, j])
+                    out[i, j] = fp[i, j] * fp0 / efp + 1j * tmp3 * ps[i, j]
+                    if i == 0:
+                        out[0, n - j] = np.conj(out[0, j])
     return out
 
 
-def rk_stages_const_inj(ps, vz, dt, p, kdn, kup, fp0, fz0, ord=4):
+def rk_stages_const_inj(ps, vz, dt, p, kdn, kup, fp0, fz0, ord=4, nprocs=1):
     """rk_stages with iflow = 2: fp = const_inj(C1), fz = const_inj(C2) at the
     top of every stage (src/Rot2D3C.para.f90:565-569)."""
     g = grid(_n_of(ps))
@@ -305,8 +323,8 @@ def rk_stages_const_inj(ps, vz, dt, p, kdn, kup, fp0, fz0, ord=4):
     ka2 = np.where(upd, g.ka2, 1.0)
     c1, c2 = ps.copy(), vz.copy()
     for o in range(ord, 0, -1):
-        fp = const_inj(c1, kdn, kup, fp0, 1)
-        fz = const_inj(c2, kdn, kup, fz0, 0)
+        fp = const_inj(c1, kdn, kup, fp0, 1, nprocs)
+        fz = const_inj(c2, kdn, kup, fz0, 0, nprocs)
         c4 = laplak2(c1)
         c5 = poisson(c1, c4)
         c6 = poisson(c1, c2)
@@ -366,9 +384,21 @@ def rk_stages(ps, vz, fp, fz, dt, p, ord=4):
 _F32_201 = float(np.float32(2.01))  # `(2.01)*kup*kup` is a default-REAL literal (:559)
 
 
-def cond_check(ps, vz, fp, fz, time, p, kup):
+def _slab_extrema(r, nprocs):
+    """(max, min) of max_min.txt: each rank reduces its rows [jsta, jend]
+    (src/Rot2D3C.para.f90:103) and both are combined with MPI_MAX
+    (src/pseudospec2D_Rot2D3C.f90:578,586), so the "min" is the largest of the
+    slab minima."""
+    n = r.shape[0]
+    sl = [slab(1, n, nprocs, k) for k in range(nprocs)]
+    return (max(float(r[lo:hi].max()) for lo, hi in sl),
+            max(float(r[lo:hi].min()) for lo, hi in sl))
+
+
+def cond_check(ps, vz, fp, fz, time, p, kup, nprocs=1):
     """Rows written to energy_bal.txt, enstrophy_bal.txt, helicity_bal.txt and
-    max_min.txt (src/pseudospec2D_Rot2D3C.f90:455-612), for one rank."""
+    max_min.txt (src/pseudospec2D_Rot2D3C.f90:455-612); the sums do not depend
+    on the rank count, the extrema do (nprocs slabs of real-space rows)."""
     g = grid(_n_of(ps))
     n4 = float(g.n) ** 4
     Ep1 = energy(ps, 1)
@@ -395,10 +425,8 @@ def cond_check(ps, vz, fp, fz, time, p, kup):
     two = g.two[:, None]
     Efp = float(np.where(band, two * g.ka2 * np.abs(ps) ** 2 / n4, 0.0).sum())
     Efpz = float(np.where(band, two * np.abs(vz) ** 2 / n4, 0.0).sum())
-    r = fftp2d_complex_to_real(laplak2(ps))
-    maxw, minw = float(r.max()), float(r.min())
-    r = fftp2d_complex_to_real(derivk2(vz, 0))
-    maxv, minv = float(r.max()), float(r.min())
+    maxw, minw = _slab_extrema(fftp2d_complex_to_real(laplak2(ps)), nprocs)
+    maxv, minv = _slab_extrema(fftp2d_complex_to_real(derivk2(vz, 0)), nprocs)
     om2 = 2 * p.omega
     return {
         "energy_bal": [time, Ep1, Epv, injp1, injz, p.nu * Dsp, p.nuv * Dspv,

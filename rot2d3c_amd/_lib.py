@@ -32,7 +32,7 @@ class ForceDraw(ctypes.Structure):
 class Forcing(ctypes.Structure):
     _fields_ = [("psi", ForceDraw), ("vz", ForceDraw), ("dense_psi", _vp), ("dense_vz", _vp),
                 ("ci_enabled", _c_int), ("ci_kdn", _c_double), ("ci_kup", _c_double),
-                ("ci_fp0", _c_double), ("ci_fz0", _c_double)]
+                ("ci_fp0", _c_double), ("ci_fz0", _c_double), ("ci_nprocs", _c_int)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/rot2d3c.h
@@ -50,10 +50,11 @@ SIGNATURES = {
     "r2d3c_poisson": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
     "r2d3c_energy": (_c_int, [_vp, _vp, _c_int, _vp, _vp]),
     "r2d3c_inerprod": (_c_int, [_vp, _vp, _vp, _c_int, _vp, _vp]),
-    "r2d3c_const_inj": (_c_int, [_vp, _vp, _c_double, _c_double, _c_double, _c_int, _vp, _vp]),
+    "r2d3c_const_inj": (_c_int, [_vp, _vp, _c_double, _c_double, _c_double, _c_int, _c_int, _vp,
+                                 _vp]),
     "r2d3c_const_inj_last": (_c_int, [_vp, _vp, _vp, _vp]),
-    "r2d3c_cond_check": (_c_int, [_vp, ctypes.POINTER(Params), _c_double, _vp, _vp, _vp, _vp, _vp,
-                                  _vp]),
+    "r2d3c_cond_check": (_c_int, [_vp, ctypes.POINTER(Params), _c_double, _vp, _vp, _vp, _vp,
+                                  _c_int, _vp, _vp]),
     "r2d3c_spectrum": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
     "r2d3c_transfers": (_c_int, [_vp, _vp, _vp, _vp, _vp]),
     "r2d3c_cfl": (_c_int, [_vp, ctypes.POINTER(Params), _c_double, _vp, _vp, _vp]),

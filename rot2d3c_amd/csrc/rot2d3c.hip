@@ -534,6 +534,46 @@ __global__ void k_minmax_finish(const double *__restrict__ partial, int np, doub
   out[1] = b;
 }
 
+// Extrema of an nprocs-rank run: block r reduces the rows [lo, hi) of rank r's
+// slab (the split of `range`, src/fftp.f90:17-47, over rows 1..n); the finish
+// combines maxima and minima alike with a max, as the reference's MPI_MAX
+// does for both (src/pseudospec2D_Rot2D3C.f90:578,586).
+__global__ __launch_bounds__(256) void k_minmax_slabs(const double *__restrict__ r, int n,
+                                                       double *__restrict__ partial) {
+  __shared__ double mx[256], mn[256];
+  const int np = gridDim.x, rk = blockIdx.x, w1 = n / np, w2 = n % np;
+  const int lo = rk * w1 + min(rk, w2), hi = lo + w1 + (rk < w2 ? 1 : 0);
+  double a = -INFINITY, b = INFINITY;
+  for (size_t e = (size_t)lo * n + threadIdx.x; e < (size_t)hi * n; e += blockDim.x) {
+    a = fmax(a, r[e]);
+    b = fmin(b, r[e]);
+  }
+  mx[threadIdx.x] = a;
+  mn[threadIdx.x] = b;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + st]);
+      mn[threadIdx.x] = fmin(mn[threadIdx.x], mn[threadIdx.x + st]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * rk] = mx[0];
+    partial[2 * rk + 1] = mn[0];
+  }
+}
+__global__ void k_minmax_finish_ranks(const double *__restrict__ partial, int np,
+                                      double *__restrict__ out) {
+  double a = -INFINITY, b = -INFINITY;
+  for (int k = 0; k < np; ++k) {
+    a = fmax(a, partial[2 * k]);
+    b = fmax(b, partial[2 * k + 1]);
+  }
+  out[0] = a;
+  out[1] = b;
+}
+
 // Shell sums, one block per shell kmn = blockIdx.x + 1, kmn = int(sqrt(ka2) + 0.5)
 // (src/pseudospec2D_Rot2D3C.f90:652-656).  kmn is monotone in |ky| at fixed kx, so
 // each thread walks its columns over the contiguous |ky| range of the shell,
@@ -1133,7 +1173,7 @@ struct r2d3c_plan {
   double *dpart = nullptr;        // diagnostics partials
   cplx *ci = nullptr;             // iflow = 2: C1..C6, fpre, fp, fz
   float *ci_rtab = nullptr;       // const_inj random values per mode
-  double ci_key[2] = {-1.0, -1.0};  // (kdn, kup) of ci_rtab
+  double ci_key[3] = {-1.0, -1.0, -1.0};  // (kdn, kup, nprocs) of ci_rtab
   std::vector<float> ci_host;
   double *ci_efp = nullptr;
   unsigned long long *maxbits = nullptr;
@@ -1360,27 +1400,37 @@ static int step_seq(r2d3c_plan *p, StepConst &c, cplx *psi, cplx *vz, hipStream_
 
 extern "C" float r2d3c_randu(int *idum);
 
-// randu values of the band modes in const_inj's loop order, seed re-set to the
-// rank (0) on every call (src/pseudospec2D_Rot2D3C.f90:1032-1057); uploaded
-// once per (kdn, kup).
-static int ci_table(r2d3c_plan *p, double kdn, double kup, hipStream_t s) {
-  if (p->ci_key[0] == kdn && p->ci_key[1] == kup) return 0;
-  const int n = p->n;
-  p->ci_host.assign((size_t)(n / 2 + 1) * n, 0.0f);
-  int seed = 0;
-  for (int i = 0; i <= n / 2; ++i) {
-    const double kx = i < n / 2 ? i : i - n;
-    const int j0 = i == 0 ? 1 : 0, j1 = i == 0 ? n / 2 - 1 : n - 1;
-    for (int j = j0; j <= j1; ++j) {
-      const double ky = j < n / 2 ? j : j - n, ka2 = kx * kx + ky * ky;
-      if (ka2 <= kup * kup && ka2 >= kdn * kdn) p->ci_host[(size_t)i * n + j] = r2d3c_randu(&seed);
+// randu values of the band modes of const_inj for an nprocs-rank run: rank r
+// owns the kx columns [lo, hi) of its slab (the split of `range`,
+// src/fftp.f90:17-47) and restarts randu at seed r on every call, drawing in
+// its loop order (src/pseudospec2D_Rot2D3C.f90:1032-1057).  Uploaded once per
+// (kdn, kup, nprocs).
+static int ci_table(r2d3c_plan *p, double kdn, double kup, int nprocs, hipStream_t s) {
+  nprocs = nprocs < 1 ? 1 : nprocs;
+  const int n = p->n, ncol = n / 2 + 1;
+  if (nprocs > ncol) return set_err(R2D3C_EINVAL, "const_inj: more ranks than kx columns");
+  if (p->ci_key[0] == kdn && p->ci_key[1] == kup && p->ci_key[2] == nprocs) return 0;
+  p->ci_host.assign((size_t)ncol * n, 0.0f);
+  const int w1 = ncol / nprocs, w2 = ncol % nprocs;
+  for (int r = 0, lo = 0; r < nprocs; ++r) {
+    const int hi = lo + w1 + (r < w2 ? 1 : 0);
+    int seed = r;
+    for (int i = lo; i < hi; ++i) {
+      const double kx = i < n / 2 ? i : i - n;
+      const int j0 = i == 0 ? 1 : 0, j1 = i == 0 ? n / 2 - 1 : n - 1;
+      for (int j = j0; j <= j1; ++j) {
+        const double ky = j < n / 2 ? j : j - n, ka2 = kx * kx + ky * ky;
+        if (ka2 <= kup * kup && ka2 >= kdn * kdn) p->ci_host[(size_t)i * n + j] = r2d3c_randu(&seed);
+      }
     }
+    lo = hi;
   }
   HIPCHK(hipMemcpyAsync(p->ci_rtab, p->ci_host.data(), sizeof(float) * p->ci_host.size(),
                         hipMemcpyHostToDevice, s));
   HIPCHK(hipStreamSynchronize(s));  // ci_host may be refilled by the next key
   p->ci_key[0] = kdn;
   p->ci_key[1] = kup;
+  p->ci_key[2] = nprocs;
   return 0;
 }
 
@@ -1431,7 +1481,7 @@ static int run_steps(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
   if (fo && fo[0].ci_enabled) {  // iflow = 2 runs stage by stage
     if ((fo[0].ci_kdn > fo[0].ci_kup) || fo[0].ci_kup < 0.0)
       return set_err(R2D3C_EINVAL, "const_inj band must satisfy 0 <= kdn <= kup");
-    int rc = ci_table(p, fo[0].ci_kdn, fo[0].ci_kup, s);
+    int rc = ci_table(p, fo[0].ci_kdn, fo[0].ci_kup, fo[0].ci_nprocs, s);
     if (rc) return rc;
     for (int st = 0; st < nsteps; ++st) {
       rc = run_step_ci<N>(p, q, psi, vz, &fo[st], dt_mode, dt_or_cfl, d_dt ? d_dt + st : nullptr, s);
@@ -1687,11 +1737,14 @@ int r2d3c_forcing_fill(r2d3c_plan *p, const r2d3c_forcing *fo, double dt, double
 }
 
 int r2d3c_cond_check(r2d3c_plan *p, const r2d3c_params *q, double kup, const double *psi,
-                     const double *vz, const double *fp, const double *fz, double *d_out,
-                     void *stream) {
+                     const double *vz, const double *fp, const double *fz, int nprocs,
+                     double *d_out, void *stream) {
   NEED(p); NEED(psi); NEED(vz); NEED(fp); NEED(fz); NEED(d_out);
   int rc = check_params(q);
   if (rc) return rc;
+  nprocs = nprocs < 1 ? 1 : nprocs;
+  if (nprocs > p->n) return set_err(R2D3C_EINVAL, "cond_check: more ranks than rows");
+  if (nprocs > kReduceBlocks) return set_err(R2D3C_ENOTSUP, "cond_check: at most 256 ranks");
   hipStream_t s = (hipStream_t)stream;
   const int n = p->n;
   CondConst c;
@@ -1709,13 +1762,20 @@ int r2d3c_cond_check(r2d3c_plan *p, const r2d3c_params *q, double kup, const dou
   k_spec_op<<<launch_grid(sz), 256, 0, s>>>((const cplx *)psi, t, OP_LAP, n, p->g.kmax, p->g.tiny);
   rc = r2d3c_c2r(p, (const double *)t, p->real2, stream);
   if (rc) return rc;
-  k_minmax_partial<<<kReduceBlocks, 256, 0, s>>>(p->real2, nn2, p->dpart);
-  k_minmax_finish<<<1, 1, 0, s>>>(p->dpart, kReduceBlocks, d_out + kCondSums);
+  auto extrema = [&](double *out) {
+    if (nprocs == 1) {
+      k_minmax_partial<<<kReduceBlocks, 256, 0, s>>>(p->real2, nn2, p->dpart);
+      k_minmax_finish<<<1, 1, 0, s>>>(p->dpart, kReduceBlocks, out);
+    } else {
+      k_minmax_slabs<<<nprocs, 256, 0, s>>>(p->real2, n, p->dpart);
+      k_minmax_finish_ranks<<<1, 1, 0, s>>>(p->dpart, nprocs, out);
+    }
+  };
+  extrema(d_out + kCondSums);
   k_spec_op<<<launch_grid(sz), 256, 0, s>>>((const cplx *)vz, t, OP_MASK, n, p->g.kmax, p->g.tiny);
   rc = r2d3c_c2r(p, (const double *)t, p->real2, stream);
   if (rc) return rc;
-  k_minmax_partial<<<kReduceBlocks, 256, 0, s>>>(p->real2, nn2, p->dpart);
-  k_minmax_finish<<<1, 1, 0, s>>>(p->dpart, kReduceBlocks, d_out + kCondSums + 2);
+  extrema(d_out + kCondSums + 2);
   LAUNCHCHK("cond_check extrema");
   return 0;
 }
@@ -1738,11 +1798,11 @@ static int const_inj_dev(r2d3c_plan *p, const cplx *ps, double kdn, double kup, 
 extern "C" {
 
 int r2d3c_const_inj(r2d3c_plan *p, const double *ps, double kdn, double kup, double fp0, int kin,
-                    double *fp, void *stream) {
+                    int nprocs, double *fp, void *stream) {
   NEED(p); NEED(ps); NEED(fp);
   if (kdn > kup || kup < 0.0) return set_err(R2D3C_EINVAL, "const_inj band must satisfy 0 <= kdn <= kup");
   hipStream_t s = (hipStream_t)stream;
-  int rc = ci_table(p, kdn, kup, s);
+  int rc = ci_table(p, kdn, kup, nprocs, s);
   if (rc) return rc;
   const size_t sz = (size_t)(p->n / 2 + 1) * p->n;
   return const_inj_dev(p, (const cplx *)ps, kdn, kup, fp0, kin, p->ci + 6 * sz, (cplx *)fp, s);

@@ -15,10 +15,15 @@ formats (rot2d3c_amd.io), so the reference's analysis scripts read them.
 
 The grid size n is a compile-time constant of the reference
 (src/pseudospec2D_mod.f90:11); here it is the --n option.  iflow = 2
-(const_inj) recomputes the forcing at every stage from the stage's own state;
-its random numbers are those of rank 0 (one process).
+(const_inj) recomputes the forcing at every stage from the stage's own state.
 
-Usage: python -m rot2d3c_amd.driver --n 256 [--workdir DIR] [--nprocs-out P]
+--nprocs P reproduces what a P-rank run of the reference writes, where that
+depends on the rank count: one slab file per rank in each field dump, the
+const_inj random numbers (each rank restarts randu at seed = rank on its kx
+slab) and the max_min.txt extrema (MPI_MAX of the per-slab values, also for
+the minima).  Everything else is independent of P up to rounding.
+
+Usage: python -m rot2d3c_amd.driver --n 256 [--workdir DIR] [--nprocs P]
 """
 import argparse
 import math
@@ -71,11 +76,13 @@ def _masked_scale(a, scale, n, strict_dc=True):
 
 
 class Run:
-    def __init__(self, n, workdir=".", device="cuda:0", nprocs_out=1, log=None):
+    def __init__(self, n, workdir=".", device="cuda:0", nprocs=1, log=None):
         self.n = int(n)
         self.wd = workdir
         self.dev = torch.device(device)
-        self.nprocs = int(nprocs_out)
+        self.nprocs = int(nprocs)
+        if not 1 <= self.nprocs <= self.n // 2 + 1:
+            raise ValueError("nprocs must be in 1..n/2+1")
         self.log = log or (lambda *a: None)
         self.stat, self.time = rio.read_status_inp(os.path.join(workdir, "status.inp"))
         self.prm = rio.read_parameter_inp(os.path.join(workdir, "parameter.inp"))
@@ -157,7 +164,7 @@ class Run:
             f.write("%s%s\n" % (ext4, rio.fortran_f(self.time, 12, 6)))
 
     def write_balance(self, ps, vz, fp, fz):
-        rows = S.cond_check(ps, vz, fp, fz, self.time, self.params, self.prm["kup"])
+        rows = S.cond_check(ps, vz, fp, fz, self.time, self.params, self.prm["kup"], self.nprocs)
         for name in ("energy_bal", "enstrophy_bal", "helicity_bal", "max_min"):
             rio.append_rows(os.path.join(self.wd, name + ".txt"), [rows[name]])
         return rows
@@ -181,9 +188,9 @@ class Run:
         fp_draw = fz_draw = None
         fpd = fzd = zero
         if q["iflow"] == 2:  # pre-loop const_inj pair (:372-375)
-            ci = (q["kdn"], q["kup"], q["fp0"], q["fz0"])
-            fpd = S.const_inj(ps, q["kdn"], q["kup"], q["fp0"], 1)
-            fzd = S.const_inj(vz, q["kdn"], q["kup"], q["fz0"], 0)
+            ci = (q["kdn"], q["kup"], q["fp0"], q["fz0"], self.nprocs)
+            fpd = S.const_inj(ps, q["kdn"], q["kup"], q["fp0"], 1, self.nprocs)
+            fzd = S.const_inj(vz, q["kdn"], q["kup"], q["fz0"], 0, self.nprocs)
         if q["iflow"] == 1:
             fpd, fzd = self.static_forcing()
             dense = (fpd, fzd)
@@ -242,11 +249,12 @@ def main(argv=None):
     ap.add_argument("--n", type=int, required=True, help="grid size (power of two)")
     ap.add_argument("--workdir", default=".")
     ap.add_argument("--device", default="cuda:0")
-    ap.add_argument("--nprocs-out", type=int, default=1,
-                    help="number of per-rank slab files per field dump")
+    ap.add_argument("--nprocs", "--nprocs-out", type=int, default=1,
+                    help="MPI rank count to reproduce (slab files, const_inj "
+                         "random numbers, max_min extrema)")
     ap.add_argument("-v", "--verbose", action="store_true")
     a = ap.parse_args(argv)
-    r = Run(a.n, a.workdir, a.device, a.nprocs_out, log=print if a.verbose else None)
+    r = Run(a.n, a.workdir, a.device, a.nprocs, log=print if a.verbose else None)
     r.run()
     return 0
 

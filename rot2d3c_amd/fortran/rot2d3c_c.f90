@@ -21,6 +21,9 @@ module rot2d3c_c
   type, bind(c) :: r2d3c_forcing
     type(r2d3c_force_draw) :: psi, vz
     type(c_ptr) :: dense_psi = c_null_ptr, dense_vz = c_null_ptr
+    integer(c_int) :: ci_enabled = 0
+    real(c_double) :: ci_kdn = 0, ci_kup = 0, ci_fp0 = 0, ci_fz0 = 0
+    integer(c_int) :: ci_nprocs = 1
   end type
 
   interface

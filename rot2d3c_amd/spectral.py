@@ -227,11 +227,12 @@ class ForceDraw:
 def _forcing_c(fp_draw, fz_draw, dense=None, const_inj=None):
     """fp_draw / fz_draw: ForceDraw or None; dense: optional (fp, fz) pair of
     static spectral arrays (either may be None); const_inj: optional
-    (kdn, kup, fp0, fz0) of iflow = 2."""
+    (kdn, kup, fp0, fz0[, nprocs]) of iflow = 2."""
     f = _lib.Forcing()
     if const_inj is not None:
         f.ci_enabled = 1
-        f.ci_kdn, f.ci_kup, f.ci_fp0, f.ci_fz0 = (float(x) for x in const_inj)
+        f.ci_kdn, f.ci_kup, f.ci_fp0, f.ci_fz0 = (float(x) for x in const_inj[:4])
+        f.ci_nprocs = int(const_inj[4]) if len(const_inj) > 4 else 1
     if fp_draw is not None:
         f.psi = fp_draw.c()
     if fz_draw is not None:
@@ -255,8 +256,9 @@ def rk4_step(ps, vz, p, fp_draw=None, fz_draw=None, dt=None, cfl=None, dt_out=No
     pre-step state, as the reference) must be given.  The dt used is written
     to `dt_out` (a 1-element float64 CUDA tensor) when given.  dense:
     optional (fp, fz) static forcing arrays added to the draws (iflow = 1).
-    const_inj: (kdn, kup, fp0, fz0) for iflow = 2 (forcing recomputed from
-    the stage state at every stage)."""
+    const_inj: (kdn, kup, fp0, fz0[, nprocs]) for iflow = 2 (forcing
+    recomputed from the stage state at every stage, with the random numbers
+    of an nprocs-rank run)."""
     n = _spec_n(ps)
     if _spec_n(vz) != n:
         raise ValueError("size mismatch")
@@ -317,14 +319,16 @@ def forcing_fill(n, device, fp_draw, fz_draw, dt):
     return fp, fz
 
 
-def const_inj(ps, kdn, kup, fp0, kin, out=None):
-    """Constant-injection forcing of iflow = 2 for one rank
-    (src/pseudospec2D_Rot2D3C.f90:977-1060)."""
+def const_inj(ps, kdn, kup, fp0, kin, nprocs=1, out=None):
+    """Constant-injection forcing of iflow = 2 (src/pseudospec2D_Rot2D3C.f90:977-1060)
+    with the random numbers of an nprocs-rank run (randu restarted at seed =
+    rank on each rank's kx slab)."""
     n = _spec_n(ps)
     out = _new_spec(n, ps) if out is None else out
     pl = Plan.get(n, ps.device)
     check(_lib.lib().r2d3c_const_inj(pl.handle, _ptr(ps), float(kdn), float(kup), float(fp0),
-                                     int(kin), _ptr(out), _stream(ps.device)), "r2d3c_const_inj")
+                                     int(kin), int(nprocs), _ptr(out), _stream(ps.device)),
+          "r2d3c_const_inj")
     return out
 
 
@@ -344,23 +348,25 @@ COND_NAMES = ("Ep1", "injp1", "Dsp", "Hds", "Epv", "injz", "Dspv", "Hdsv", "coup
               "Hdsh2", "Efp", "Efpz", "maxw", "minw", "maxv", "minv")
 
 
-def cond_check_values(ps, vz, fp, fz, p, kup):
+def cond_check_values(ps, vz, fp, fz, p, kup, nprocs=1):
     """The 26 device-computed quantities of cond_check as a dict of floats
-    (src/pseudospec2D_Rot2D3C.f90:484-586)."""
+    (src/pseudospec2D_Rot2D3C.f90:484-586); the extrema are those of an
+    nprocs-rank run (MPI_MAX of per-slab values, also for the minima)."""
     n = _spec_n(ps)
     pl = Plan.get(n, ps.device)
     out = torch.empty(len(COND_NAMES), dtype=torch.float64, device=ps.device)
     cp = p.c()
     check(_lib.lib().r2d3c_cond_check(pl.handle, ctypes.byref(cp), float(kup), _ptr(ps), _ptr(vz),
-                                      _ptr(fp), _ptr(fz), _ptr(out), _stream(ps.device)),
+                                      _ptr(fp), _ptr(fz), int(nprocs), _ptr(out),
+                                      _stream(ps.device)),
           "r2d3c_cond_check")
     return dict(zip(COND_NAMES, out.tolist()))
 
 
-def cond_check(ps, vz, fp, fz, time, p, kup):
+def cond_check(ps, vz, fp, fz, time, p, kup, nprocs=1):
     """Rows of energy_bal.txt, enstrophy_bal.txt, helicity_bal.txt and
     max_min.txt, in the reference's column order (:592-608)."""
-    v = cond_check_values(ps, vz, fp, fz, p, kup)
+    v = cond_check_values(ps, vz, fp, fz, p, kup, nprocs)
     om2 = 2 * p.omega
     return {
         "energy_bal": [time, v["Ep1"], v["Epv"], v["injp1"], v["injz"], p.nu * v["Dsp"],

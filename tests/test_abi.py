@@ -59,3 +59,33 @@ def test_argument_errors_are_reported():
     assert l.r2d3c_plan_create(None, 64) == -1
     assert l.r2d3c_c2r(None, None, None, None) == -1
     assert l.r2d3c_plan_destroy(None) == 0
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """ctypes' Params, ForceDraw and Forcing have the sizes and field offsets
+    that a C compiler gives the header's structs."""
+    import ctypes
+    import shutil
+    import subprocess
+    from rot2d3c_amd import _lib
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"r2d3c_params": _lib.Params, "r2d3c_force_draw": _lib.ForceDraw,
+               "r2d3c_forcing": _lib.Forcing}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "rot2d3c.h"', "int main(void) {"]
+    for cname, ct in structs.items():
+        lines.append('printf("%%zu\\n", sizeof(%s));' % cname)
+        for fname, _ in ct._fields_:
+            lines.append('printf("%%zu\\n", offsetof(%s, %s));' % (cname, fname))
+    lines.append("return 0; }")
+    (tmp_path / "l.c").write_text("\n".join(lines))
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), "l.c", "-o", "l"], cwd=tmp_path,
+                   check=True, capture_output=True)
+    got = [int(x) for x in subprocess.run(["./l"], cwd=tmp_path, check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    expect = []
+    for ct in structs.values():
+        expect.append(ctypes.sizeof(ct))
+        expect += [getattr(ct, f).offset for f, _ in ct._fields_]
+    assert got == expect

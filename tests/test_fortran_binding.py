@@ -19,12 +19,16 @@ PROGRAM = """program t
   integer(c_int) :: s
   real :: r
   real(c_double) :: kx, ky, ph
+  type(r2d3c_forcing) :: f
+  type(r2d3c_params) :: p
+  type(r2d3c_force_draw) :: d
   s = 7
   r = r2d3c_randu(s)
   write(*,'(ES16.8,I12)') r, s
   s = 11
   call r2d3c_rand_force_draw(10.d0, s, kx, ky, ph)
   write(*,'(3ES25.16,I12)') kx, ky, ph, s
+  write(*,'(3I8)') c_sizeof(p), c_sizeof(d), c_sizeof(f)
 end program
 """
 
@@ -45,3 +49,9 @@ def test_fortran_binding_links_and_matches_oracle(tmp_path):
     kx, ky, ph, s2 = O.rand_force_draw(10.0, 11)
     assert (float(out[2]), float(out[3]), int(out[5])) == (kx, ky, s2)
     assert float(out[4]) == pytest.approx(ph, rel=1e-15)
+    # the derived types have the C structs' sizes (a short type would make
+    # r2d3c_step read past the caller's struct)
+    import ctypes
+    from rot2d3c_amd import _lib
+    sizes = [ctypes.sizeof(t) for t in (_lib.Params, _lib.ForceDraw, _lib.Forcing)]
+    assert [int(x) for x in out[6:9]] == sizes

@@ -85,48 +85,56 @@ def test_driver_restart_reads_its_own_dumps(tmp_path, cuda):
     assert [l[:3] for l in tf2] == ["%03d" % e[0] for e in ev["field"]]
 
 
-def test_driver_iflow2_matches_oracle_loop(tmp_path, cuda):
+@pytest.mark.parametrize("nprocs", [1, 2])
+def test_driver_iflow2_matches_oracle_loop(tmp_path, cuda, nprocs):
     """iflow = 2 through the driver: the stage-by-stage step and the balance
-    rows that see each step's last-stage forcing."""
-    print("driver iflow=2 n=32", flush=True)
+    rows that see each step's last-stage forcing.  nprocs = 2 reproduces a
+    2-rank run: const_inj random numbers per slab, max_min per slab."""
+    print("driver iflow=2 n=32 nprocs=%d" % nprocs, flush=True)
     from rot2d3c_amd import driver, io as rio
     n = 32
-    prm = dict(PRM, iflow=2, kdn=2.0, kup=5.0, step=6, sstep=100, tstep=100)
+    # with 2 ranks the band reaches into rank 1's kx slab (kx >= 9)
+    prm = dict(PRM, iflow=2, kdn=2.0, kup=5.0 if nprocs == 1 else 10.0, step=6, sstep=100,
+               tstep=100)
     rio.write_parameter_inp(str(tmp_path / "parameter.inp"), prm)
     rio.write_status_inp(str(tmp_path / "status.inp"), 0, 0.0)
-    r = driver.Run(n, str(tmp_path), "cuda:0")
+    r = driver.Run(n, str(tmp_path), "cuda:0", nprocs=nprocs)
     gps, gvz = r.run()
     # oracle loop with per-stage const_inj
     p = O.Params(prm["nu"], prm["hnu"], prm["nn"], prm["mm"], prm["nuv"], prm["hnuv"],
                  prm["nnv"], prm["mmv"], prm["omega"])
     ps, vz, _ = O.initial_condition(n, prm["kup"], prm["u0"], prm["v0"], prm["seed"])
-    fp = O.const_inj(ps, prm["kdn"], prm["kup"], prm["fp0"], 1)
-    fz = O.const_inj(vz, prm["kdn"], prm["kup"], prm["fz0"], 0)
-    time, rows = 0.0, []
+    fp = O.const_inj(ps, prm["kdn"], prm["kup"], prm["fp0"], 1, nprocs)
+    fz = O.const_inj(vz, prm["kdn"], prm["kup"], prm["fz0"], 0, nprocs)
+    time, rows, mm = 0.0, [], []
     for _ in range(prm["step"]):
         dt = O.CFL_condition(prm["CFL"], ps, vz, p.nu, p.nn, p.nuv, p.nnv, p.omega)
-        rows.append(O.cond_check(ps, vz, fp, fz, time, p, prm["kup"])["energy_bal"])
+        cc = O.cond_check(ps, vz, fp, fz, time, p, prm["kup"], nprocs)
+        rows.append(cc["energy_bal"])
+        mm.append(cc["max_min"])
         time += dt
         # last-stage forcing of this step, as the balance of the next iteration sees it
         c1, c2 = ps, vz
-        g = O.grid(n)
-        ps, vz = O.rk_stages_const_inj(ps, vz, dt, p, prm["kdn"], prm["kup"], prm["fp0"], prm["fz0"])
-        fp, fz = _last_stage_forcing(c1, c2, dt, p, prm)
+        ps, vz = O.rk_stages_const_inj(ps, vz, dt, p, prm["kdn"], prm["kup"], prm["fp0"], prm["fz0"],
+                                       nprocs=nprocs)
+        fp, fz = _last_stage_forcing(c1, c2, dt, p, prm, nprocs)
     rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
     assert rel(gps.cpu().numpy(), ps) < 1e-11
     bal = np.loadtxt(str(tmp_path / "energy_bal.txt"))
     assert np.allclose(bal, np.array(rows), rtol=1e-9, atol=1e-12)
+    got_mm = np.loadtxt(str(tmp_path / "max_min.txt"))
+    assert np.allclose(got_mm, np.array(mm), rtol=1e-9, atol=1e-10)
 
 
-def _last_stage_forcing(ps, vz, dt, p, prm):
+def _last_stage_forcing(ps, vz, dt, p, prm, nprocs=1):
     """fp, fz computed at the top of stage o = 1 (from that stage's input)."""
     g = O.grid(ps.shape[1])
     upd = (g.ka2 <= g.kmax) & (g.ka2 > g.tiny)
     ka2 = np.where(upd, g.ka2, 1.0)
     c1, c2 = ps.copy(), vz.copy()
     for o in range(4, 0, -1):
-        fp = O.const_inj(c1, prm["kdn"], prm["kup"], prm["fp0"], 1)
-        fz = O.const_inj(c2, prm["kdn"], prm["kup"], prm["fz0"], 0)
+        fp = O.const_inj(c1, prm["kdn"], prm["kup"], prm["fp0"], 1, nprocs)
+        fz = O.const_inj(c2, prm["kdn"], prm["kup"], prm["fz0"], 0, nprocs)
         if o == 1:
             return fp, fz
         c4 = O.laplak2(c1)
@@ -140,3 +148,27 @@ def _last_stage_forcing(ps, vz, dt, p, prm):
         t3 = 1.0 + (p.nuv * ka2 ** p.nnv + p.hnuv * tmp2 ** p.mmv) * tmp1
         n2 = (vz + (c6 + 2 * p.omega * c3 + fz) * tmp1) / t3
         c1, c2 = np.where(upd, n1, 0.0), np.where(upd, n2, 0.0)
+
+
+def test_driver_balance_files_close_their_budgets(tmp_path, cuda):
+    """The acceptance check of the reference (BalanceTest_Rot2D3C.py, restated
+    in rot2d3c_amd.balance) on the files the GPU driver writes: with random
+    forcing and Omega != 0 the total energy and helicity budgets close within
+    4 standard errors; with Omega = 0 so do the 2-D, vz and enstrophy ones.
+    (The oracle's loop gives at most 2.8 standard errors on these inputs.)"""
+    from rot2d3c_amd import balance as B, driver, io as rio
+    n = 32
+    for omega in (0.0, 1.5):
+        print("driver budgets n=32 omega=%g" % omega, flush=True)
+        d = tmp_path / ("om%g" % omega)
+        d.mkdir()
+        prm = dict(PRM, step=500, cstep=1, sstep=10 ** 6, tstep=10 ** 6, omega=omega, CFL=0.3,
+                   nu=1e-2, nuv=1e-2, fp0=1.0, fz0=1.0, kdn=1.0, kup=4.0)
+        rio.write_parameter_inp(str(d / "parameter.inp"), prm)
+        rio.write_status_inp(str(d / "status.inp"), 0, 0.0)
+        driver.Run(n, str(d), "cuda:0").run()
+        bud = B.budgets_from_files(str(d))
+        closing = ["energy", "helicity"] + (["energy_2d", "energy_z", "enstrophy"] if omega == 0 else [])
+        for k in closing:
+            m, se = bud[k]
+            assert abs(m) < 4 * se, (omega, k, m, se)

@@ -208,12 +208,14 @@ def test_cond_check_matches_oracle(n, cuda):
     fz, _ = O.rand_force(kup, kup, 0.5, 1e-3, 8, 0, n)
     p = O.Params(1e-3, 0.1, 1, 2, 2e-3, 0.2, 2, 1, 1.5)
     sp = S.Params(p.nu, p.hnu, p.nn, p.mm, p.nuv, p.hnuv, p.nnv, p.mmv, p.omega)
-    ref = O.cond_check(ps, vz, fp, fz, 0.125, p, kup)
-    got = S.cond_check(_dev(ps, cuda), _dev(vz, cuda), _dev(fp, cuda), _dev(fz, cuda), 0.125, sp, kup)
-    for name in ref:
-        r, g = np.array(ref[name]), np.array(got[name])
-        scale = np.maximum(np.abs(r), 1e-300)
-        assert np.all(np.abs(g - r) <= 1e-12 * scale + 1e-14 * np.abs(r).max()), (name, g, r)
+    for nprocs in (1, 4):  # extrema: MPI_MAX of per-slab values with 4 ranks
+        ref = O.cond_check(ps, vz, fp, fz, 0.125, p, kup, nprocs)
+        got = S.cond_check(_dev(ps, cuda), _dev(vz, cuda), _dev(fp, cuda), _dev(fz, cuda), 0.125,
+                           sp, kup, nprocs)
+        for name in ref:
+            r, g = np.array(ref[name]), np.array(got[name])
+            scale = np.maximum(np.abs(r), 1e-300)
+            assert np.all(np.abs(g - r) <= 1e-12 * scale + 1e-14 * np.abs(r).max()), (name, g, r)
 
 
 @pytest.mark.parametrize("n", [32, 256, 2048])
@@ -232,33 +234,40 @@ def test_spectrum_transfers_match_oracle(n, cuda):
         assert np.max(np.abs(a - b)) <= 1e-11 * np.max(np.abs(b))
 
 
-@pytest.mark.parametrize("n", [32, 256])
-def test_const_inj_matches_oracle(n, cuda):
-    print("const_inj n=%d" % n, flush=True)
+@pytest.mark.parametrize("n,nprocs", [(32, 1), (256, 1), (32, 2), (64, 4), (256, 8)])
+def test_const_inj_matches_oracle(n, nprocs, cuda):
+    """nprocs > 1: the random numbers of a P-rank run (seed = rank per slab)."""
+    print("const_inj n=%d nprocs=%d" % (n, nprocs), flush=True)
     S = _S()
     ps, vz = random_state(n, 51)
-    for kin, f0 in ((1, 0.7), (0, 1.1)):
-        got = _host(S.const_inj(_dev(ps, cuda), 3.0, 7.0, f0, kin))
-        ref = O.const_inj(ps, 3.0, 7.0, f0, kin)
-        assert rel_l2(got, ref) < 1e-13
+    # the wide band reaches past rank 0's kx slab, the narrow one does not
+    for kdn, kup in ((3.0, 7.0), (3.0, n / 3.0)):
+        for kin, f0 in ((1, 0.7), (0, 1.1)):
+            got = _host(S.const_inj(_dev(ps, cuda), kdn, kup, f0, kin, nprocs))
+            ref = O.const_inj(ps, kdn, kup, f0, kin, nprocs)
+            assert rel_l2(got, ref) < 1e-13
+    if nprocs > 1:  # the other ranks' random numbers differ from one rank's
+        one = O.const_inj(ps, 3.0, n / 3.0, 0.7, 1, 1)
+        assert rel_l2(one, O.const_inj(ps, 3.0, n / 3.0, 0.7, 1, nprocs)) > 1e-3
 
 
-@pytest.mark.parametrize("n", [32, 64])
-def test_step_iflow2_10_steps(n, cuda):
+@pytest.mark.parametrize("n,nprocs", [(32, 1), (64, 1), (64, 2)])
+def test_step_iflow2_10_steps(n, nprocs, cuda):
     """iflow = 2: forcing from each stage's own state (stage-by-stage path)."""
-    print("10 iflow=2 steps n=%d" % n, flush=True)
+    print("10 iflow=2 steps n=%d nprocs=%d" % (n, nprocs), flush=True)
     import torch
     S = _S()
     ps, vz = random_state(n, 61)
     p = O.Params(1e-3, 0.1, 1, 2, 2e-3, 0.05, 1, 2, 1.0)
     sp = S.Params(p.nu, p.hnu, p.nn, p.mm, p.nuv, p.hnuv, p.nnv, p.mmv, p.omega)
-    ci = (2.0, 5.0, 0.5, 0.3)
+    # with 2 ranks the band reaches into rank 1's kx slab (kx >= 17)
+    ci = (2.0, 5.0 if nprocs == 1 else 20.0, 0.5, 0.3, nprocs)
     ops, ovz = ps.copy(), vz.copy()
     tps, tvz = _dev(ps, cuda), _dev(vz, cuda)
     dto = torch.zeros(1, dtype=torch.float64, device=cuda)
     for _ in range(10):
         dt = O.CFL_condition(0.5, ops, ovz, p.nu, p.nn, p.nuv, p.nnv, p.omega)
-        ops, ovz = O.rk_stages_const_inj(ops, ovz, dt, p, *ci)
+        ops, ovz = O.rk_stages_const_inj(ops, ovz, dt, p, *ci[:4], nprocs=nprocs)
         S.rk4_step(tps, tvz, sp, cfl=0.5, dt_out=dto, const_inj=ci)
         assert abs(float(dto.item()) - dt) <= 1e-12 * dt
     assert rel_l2(_host(tps), ops) < TOL
