@@ -773,20 +773,25 @@ __global__ __launch_bounds__(N / 8) void k_step_colinv(const cplx *__restrict__ 
   const int nf = field == 0 ? 4 : 2;
   cplx *dst0 = M + (size_t)(field == 0 ? 0 : 4) * c.fs;
   for (int f = 0; f < nf; ++f) {
+    // opaque copy of the thread index: keeps the per-q index and address math
+    // from being hoisted out of this loop (it would pin ~120 VGPRs and halve
+    // the occupancy)
+    int bb = b;
+    asm volatile("" : "+v"(bb));
     cplx v[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const int ky = kyof(ilv<N>(b, q), N);
+      const int ky = kyof(ilv<N>(bb, q), N);
       const double ka2 = (double)kx * kx + (double)ky * ky;
       cplx a = s[q];
       if (f >= 2) a = mk(-ka2 * a.x, -ka2 * a.y);
       if (f & 1) a = mk(-(double)ky * a.y, (double)ky * a.x);
       v[q] = a;
     }
-    fft_dit<N, +1, 1>(v, lds, b, tw);
+    fft_dit<N, +1, 1>(v, lds, bb, tw);
     cplx *dst = dst0 + (size_t)f * c.fs;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) dst[tix(b + T * q, kx, c.g.kp)] = v[q];
+    for (int q = 0; q < 8; ++q) dst[tix(bb + T * q, kx, c.g.kp)] = v[q];
   }
 }
 
@@ -1139,20 +1144,22 @@ __global__ __launch_bounds__(N / 8) void k_step_colfused(
     const int nf = field == 0 ? 4 : 2;
     cplx *dst0 = M + (size_t)(field == 0 ? 0 : 4) * c.fs;
     for (int f = 0; f < nf; ++f) {
+      int bb = b;  // opaque thread index, as in k_step_colinv
+      asm volatile("" : "+v"(bb));
       cplx w[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const int ky = kyof(ilv<N>(b, q), N);
+        const int ky = kyof(ilv<N>(bb, q), N);
         const double ka2 = (double)kx * kx + (double)ky * ky;
         cplx a = s[q];
         if (f >= 2) a = mk(-ka2 * a.x, -ka2 * a.y);
         if (f & 1) a = mk(-(double)ky * a.y, (double)ky * a.x);
         w[q] = a;
       }
-      fft_dit<N, +1, 1>(w, lds, b, tw);
+      fft_dit<N, +1, 1>(w, lds, bb, tw);
       cplx *dst = dst0 + (size_t)f * c.fs;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) dst[tix(b + T * q, kx, c.g.kp)] = w[q];
+      for (int q = 0; q < 8; ++q) dst[tix(bb + T * q, kx, c.g.kp)] = w[q];
     }
   }
 }
