@@ -222,6 +222,47 @@ float r2d3c_randu(int *idum);
  * (src/pseudospec2D_Rot2D3C.f90:1086-1091). */
 void r2d3c_rand_force_draw(double kup, int *seed, double *kx, double *ky, double *phase);
 
+/* ---- schedules and autotuning (no reference counterpart) --------------- */
+
+/* Schedule of the persistent row kernel of r2d3c_step / r2d3c_run for n >= 256:
+ * rows per block (2, or 1 when n >= 512) and blocks per CU (1..4).  Every
+ * schedule gives bit-identical results; only the speed differs.  Default 2, 1. */
+int r2d3c_plan_set_schedule(r2d3c_plan *plan, int row_rows, int row_blocks_per_cu);
+int r2d3c_plan_get_schedule(const r2d3c_plan *plan, int *row_rows, int *row_blocks_per_cu);
+/* Times `reps` fixed-dt steps of every valid schedule on scratch state, keeps
+ * the fastest in the plan and returns its milliseconds per step (may be NULL).
+ * Allocates and frees its scratch; synchronizes the stream. */
+int r2d3c_autotune(r2d3c_plan *plan, int reps, double *ms_per_step, void *stream);
+
+/* ---- ensembles of independent runs (no reference counterpart) ---------- */
+
+/* One run of an ensemble: grid size, parameters, fixed time step and its
+ * device state (caller-owned, reference layout, advanced in place). */
+typedef struct r2d3c_member {
+  int n;
+  r2d3c_params params;
+  double dt;
+  double *psi;
+  double *vz;
+} r2d3c_member;
+typedef struct r2d3c_ensemble r2d3c_ensemble;
+
+/* Groups the runs by grid size and allocates each run's workspace.  Runs of
+ * one size advance together: one launch per kernel and stage covers all of
+ * them, and the sizes are independent branches of one captured HIP graph per
+ * time step.  Each run's result equals r2d3c_run of that run alone (fixed dt,
+ * same forcing) bit for bit. */
+int r2d3c_ensemble_create(r2d3c_ensemble **ens, const r2d3c_member *members, int count);
+int r2d3c_ensemble_destroy(r2d3c_ensemble *ens);
+/* r2d3c_plan_set_schedule for the runs of size n. */
+int r2d3c_ensemble_set_schedule(r2d3c_ensemble *ens, int n, int row_rows, int row_blocks_per_cu);
+/* Advances every run n_steps steps.  forcing: NULL, or an array of
+ * n_steps*count forcing structs, step-major with the runs in creation order,
+ * holding rand_force draws only; the table is uploaded before the steps, so the steps run without host
+ * round trips.  Asynchronous with respect to `stream`. */
+int r2d3c_ensemble_run(r2d3c_ensemble *ens, const r2d3c_forcing *forcing, int n_steps,
+                       void *stream);
+
 #ifdef __cplusplus
 }
 #endif

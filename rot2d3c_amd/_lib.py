@@ -35,6 +35,11 @@ class Forcing(ctypes.Structure):
                 ("ci_fp0", _c_double), ("ci_fz0", _c_double), ("ci_nprocs", _c_int)]
 
 
+class Member(ctypes.Structure):
+    """r2d3c_member: one run of an ensemble."""
+    _fields_ = [("n", _c_int), ("params", Params), ("dt", _c_double), ("psi", _vp), ("vz", _vp)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/rot2d3c.h
 SIGNATURES = {
     "r2d3c_plan_create": (_c_int, [ctypes.POINTER(_vp), _c_int]),
@@ -65,6 +70,13 @@ SIGNATURES = {
     "r2d3c_forcing_fill": (_c_int, [_vp, ctypes.POINTER(Forcing), _c_double, _vp, _vp, _vp]),
     "r2d3c_timing_enable": (_c_int, [_vp, _c_int]),
     "r2d3c_timing_read": (_c_int, [_vp, ctypes.POINTER(_c_double), ctypes.POINTER(ctypes.c_longlong)]),
+    "r2d3c_plan_set_schedule": (_c_int, [_vp, _c_int, _c_int]),
+    "r2d3c_plan_get_schedule": (_c_int, [_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),
+    "r2d3c_autotune": (_c_int, [_vp, _c_int, ctypes.POINTER(_c_double), _vp]),
+    "r2d3c_ensemble_create": (_c_int, [ctypes.POINTER(_vp), ctypes.POINTER(Member), _c_int]),
+    "r2d3c_ensemble_destroy": (_c_int, [_vp]),
+    "r2d3c_ensemble_set_schedule": (_c_int, [_vp, _c_int, _c_int, _c_int]),
+    "r2d3c_ensemble_run": (_c_int, [_vp, ctypes.POINTER(Forcing), _c_int, _vp]),
     "r2d3c_randu": (ctypes.c_float, [ctypes.POINTER(_c_int)]),
     "r2d3c_rand_force_draw": (None, [_c_double, ctypes.POINTER(_c_int),
                                      ctypes.POINTER(_c_double), ctypes.POINTER(_c_double),

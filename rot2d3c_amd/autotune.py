@@ -1,0 +1,89 @@
+"""Per-size schedule autotuner with a JSON cache.
+
+A schedule is the launch shape of the persistent row kernel for one grid size:
+rows per block (2, or 1 when n >= 512) and blocks per CU (1 or 2).  Schedules
+change only the thread and LDS layout, never the arithmetic, so a run gives
+bit-identical results under every schedule; the tuner only looks for the
+fastest.  r2d3c_autotune times each candidate on the device; the choice is
+cached per (GPU architecture, CU count, n) in a JSON file, by default
+~/.cache/rot2d3c_amd/schedules.json (R2D3C_SCHEDULE_CACHE overrides it).
+"""
+import ctypes
+import json
+import os
+import threading
+
+import torch
+
+from . import _lib
+from ._lib import check
+
+DEFAULT = (2, 1)
+_lock = threading.Lock()
+
+
+def cache_path():
+    return os.environ.get("R2D3C_SCHEDULE_CACHE") or os.path.join(
+        os.path.expanduser("~"), ".cache", "rot2d3c_amd", "schedules.json")
+
+
+def device_key(device):
+    pr = torch.cuda.get_device_properties(torch.device(device))
+    return "%s:%d" % (getattr(pr, "gcnArchName", pr.name), pr.multi_processor_count)
+
+
+def _load():
+    try:
+        with open(cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(table):
+    path = cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp%d" % os.getpid()
+        with open(tmp, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # a read-only home only costs a re-tune next time
+
+
+def tune(plan_handle, reps=5, device=None):
+    """Time every schedule on the plan (scratch state, `reps` steps each),
+    keep the fastest in the plan and return ((rows, blocks_per_cu), ms/step)."""
+    ms = ctypes.c_double()
+    stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+    check(_lib.lib().r2d3c_autotune(plan_handle, int(reps), ctypes.byref(ms), stream),
+          "r2d3c_autotune")
+    rw, bpc = ctypes.c_int(), ctypes.c_int()
+    check(_lib.lib().r2d3c_plan_get_schedule(plan_handle, ctypes.byref(rw), ctypes.byref(bpc)),
+          "r2d3c_plan_get_schedule")
+    return (rw.value, bpc.value), ms.value
+
+
+def schedule(n, device, reps=5, retune=False):
+    """The cached schedule of grid size n on this device, tuned on first use."""
+    from .spectral import Plan
+    device = torch.device(device)
+    key = "%s:%d" % (device_key(device), int(n))
+    with _lock:
+        table = _load()
+        if not retune and key in table:
+            return tuple(table[key]["schedule"])
+        with torch.cuda.device(device):
+            (rw, bpc), ms = tune(Plan.get(n, device).handle, reps, device)
+        table = _load()
+        table[key] = {"schedule": [rw, bpc], "ms_per_step": ms}
+        _store(table)
+        return rw, bpc
+
+
+def apply(plan_handle, n, device, reps=5):
+    """Set the cached (or freshly tuned) schedule of size n on a plan."""
+    rw, bpc = schedule(n, device, reps)
+    check(_lib.lib().r2d3c_plan_set_schedule(plan_handle, rw, bpc), "r2d3c_plan_set_schedule")
+    return rw, bpc

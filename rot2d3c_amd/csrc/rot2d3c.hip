@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/rot2d3c.h"
@@ -748,15 +749,14 @@ __global__ void k_set_f64(double *p, double v) { *p = v; }
 // (laplak2, src/Rot2D3C.para.f90:571); blocks [ncp, ncp+nc): vz column kx ->
 // M4 = vz~, M5 = d_y vz~.  All inputs dealiased as derivk2 does.
 template <int N>
-__global__ __launch_bounds__(N / 8) void k_step_colinv(const cplx *__restrict__ psi,
-                                                        const cplx *__restrict__ vz,
-                                                        cplx *__restrict__ M, StepConst c,
-                                                        unsigned long long *maxbits,
-                                                        const cplx *__restrict__ tw) {
+__device__ __forceinline__ void colinv_body(const cplx *__restrict__ psi,
+                                            const cplx *__restrict__ vz, cplx *__restrict__ M,
+                                            const StepConst &c, unsigned long long *maxbits,
+                                            const cplx *__restrict__ tw, int lb) {
   constexpr int T = N / 8;
   __shared__ cplx lds[Shape<N>::LDSN];
-  const int lb = xcd_pairs(blockIdx.x), b = threadIdx.x;
-  if (lb == 0 && b == 0) *maxbits = 0ull;
+  const int b = threadIdx.x;
+  if (maxbits && lb == 0 && b == 0) *maxbits = 0ull;
   const int ncp = (c.g.nc + 1) & ~1;
   int field, kx;
   if (lb < c.g.nc) { field = 0; kx = lb; }
@@ -795,6 +795,15 @@ __global__ __launch_bounds__(N / 8) void k_step_colinv(const cplx *__restrict__ 
   }
 }
 
+template <int N>
+__global__ __launch_bounds__(N / 8) void k_step_colinv(const cplx *__restrict__ psi,
+                                                        const cplx *__restrict__ vz,
+                                                        cplx *__restrict__ M, StepConst c,
+                                                        unsigned long long *maxbits,
+                                                        const cplx *__restrict__ tw) {
+  colinv_body<N>(psi, vz, M, c, maxbits, tw, xcd_pairs(blockIdx.x));
+}
+
 // Row kernel of one stage: for RW rows, z1 = psi_x + i psi_y,
 // z2 = w_x + i w_y, z3 = vz_x + i vz_y (x derivatives as i*kx on the mixed
 // fields), then
@@ -827,17 +836,17 @@ __device__ __forceinline__ void load_packed_step(cplx v[8], const cplx *__restri
   }
 }
 
-#ifndef R2D_ROW_MINW
-#define R2D_ROW_MINW 1
-#endif
 template <int N>
-__global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS, R2D_ROW_MINW) void k_step_row(
+__global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
     const cplx *__restrict__ M, cplx *__restrict__ Q, StepConst c, unsigned long long *maxbits,
-    const cplx *__restrict__ tw) {
+    const cplx *__restrict__ tw, size_t mstride, const double *__restrict__ om_tab) {
   constexpr int T = N / 8, RW = RowStep<N>::ROWS;
   __shared__ cplx lds[RW * Shape<N>::LDSN];
   const int r = threadIdx.x % RW, b = threadIdx.x / RW;
   const int y = xcd_group<RowStep<N>::G>(blockIdx.x) * RW + r;
+  M += blockIdx.y * mstride;
+  Q += blockIdx.y * mstride;
+  const double two_omega_n2 = om_tab ? om_tab[blockIdx.y] : c.two_omega_n2;
   const int nc = c.g.nc, kp = c.g.kp;
   const size_t fs = c.fs;
   cplx v[8];
@@ -863,8 +872,8 @@ __global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS, R2D_ROW_MINW) void k_step
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const double vx = v[q].x, vy = v[q].y;
-    const double p1 = (-j1[q] + c.two_omega_n2 * vy) * c.inv_n4;
-    const double p2 = ((px[q] * vy - py[q] * vx) + c.two_omega_n2 * py[q]) * c.inv_n4;
+    const double p1 = (-j1[q] + two_omega_n2 * vy) * c.inv_n4;
+    const double p2 = ((px[q] * vy - py[q] * vx) + two_omega_n2 * py[q]) * c.inv_n4;
     v[q] = mk(p1, p2);
   }
   fft_canon<N, -1, RW>(v, lds + r, b, tw);
@@ -874,46 +883,37 @@ __global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS, R2D_ROW_MINW) void k_step
   unpack_store<N, RW>(lds + r, b, nc, kp, Q, y, Q + fs, y);
 }
 
-// Persistent variant of the row kernel for n >= 256.  One block per CU walks
-// its y tiles; the inputs of each packed transform (a field pair of RW rows)
-// are copied straight into LDS with global_load_lds_dwordx4 while the previous
+
+// Persistent variant of the row kernel for n >= 256.  A block walks its y
+// tiles; the inputs of each packed transform (a field pair of RW rows) are
+// copied straight into LDS with global_load_lds_dwordx4 while the previous
 // transform runs, so the block never waits on HBM between its four transforms.
 // LDS: RW*(n + n/16) complex exchange slots + one staging buffer of 2*nc*RW.
-#ifndef R2D_ROW2_RW
-#define R2D_ROW2_RW 2
-#endif
-#ifndef R2D_ROW_FOURSTEP
-#define R2D_ROW_FOURSTEP 0
-#endif
-#ifndef R2D_ROW2_BPC
-#define R2D_ROW2_BPC 1
-#endif
-template <int N> struct RowV2 {
+// RWQ (rows per block, 1 or 2) and the grid size are schedule choices: they
+// change neither the arithmetic nor its order, only the thread and LDS
+// layout, so every schedule gives bit-identical results.
+template <int N, int RWQ> struct RowV2 {
   static constexpr int T = N / 8;
   static constexpr int NCMAX = N / 3 + 2;
   static constexpr int fits(int rw) {
     return ((2 * NCMAX * rw + 63) / 64 * 64 + rw * Shape<N>::LDSN) * 16 <= 160 * 1024 &&
            T * rw <= 1024;
   }
-  static constexpr int RW = (R2D_ROW2_RW == 2 && fits(2)) ? 2 : 1;
+  static constexpr int RW = (RWQ == 2 && fits(2)) ? 2 : 1;
   static constexpr int THREADS = T * RW;
   static constexpr int SBN = (2 * NCMAX * RW + 63) / 64 * 64;
   static constexpr int G = 4 / RW;  // blocks per y tile
-  // four-step transforms give each row whole waves: stage rows one after the
-  // other; otherwise rows alternate lane by lane and are staged interleaved
-  static constexpr bool ROW_MAJOR = N >= 512 && R2D_ROW_FOURSTEP;
 };
 
 __device__ __forceinline__ void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 // Stage field pair p of the RW rows (tile t, sub-block sub) into sb as
-// sb[f2*nc*RW + slot(kx, rr)] = M_{2p+f2}(kx, 4t + sub*RW + rr), with
-// slot = rr*nc + kx (ROW_MAJOR) or kx*RW + rr.  Each wave
+// sb[f2*nc*RW + kx*RW + rr] = M_{2p+f2}(kx, 4t + sub*RW + rr).  Each wave
 // instruction fills 64 consecutive slots (LDS address = wave base + lane*16).
-template <int N>
+template <int N, int RWQ>
 __device__ __forceinline__ void stage_pair(cplx *sb, const cplx *__restrict__ M, size_t fs, int p,
                                            int t, int sub, int nc, int kp) {
-  using RV = RowV2<N>;
+  using RV = RowV2<N, RWQ>;
   constexpr int RW = RV::RW, NW = RV::THREADS / 64;
   // opaque copy of the thread index: keeps the per-lane index math below from
   // being hoisted out of the caller's tile loop (it would pin ~40 VGPRs)
@@ -926,9 +926,7 @@ __device__ __forceinline__ void stage_pair(cplx *sb, const cplx *__restrict__ M,
     const int e = w0 + lane;
     if (e < total) {
       const int f2 = e >= per, rem = e - f2 * per;
-      int rr, kx;
-      if constexpr (RowV2<N>::ROW_MAJOR) { rr = rem / nc; kx = rem - rr * nc; }
-      else { kx = rem / RW; rr = rem - kx * RW; }
+      const int kx = rem / RW, rr = rem - kx * RW;
       const cplx *g = base + (size_t)f2 * fs + kx * 4 + rr;
       __builtin_amdgcn_global_load_lds((const void *)g,
                                        (__attribute__((address_space(3))) void *)(sb + w0), 16, 0,
@@ -938,9 +936,9 @@ __device__ __forceinline__ void stage_pair(cplx *sb, const cplx *__restrict__ M,
 }
 
 // z = c2r(i kx A) + i c2r(B) of row r from the staged pair (see load_packed_step)
-template <int N>
+template <int N, int RWQ>
 __device__ __forceinline__ void build_packed(cplx v[8], const cplx *sb, int r, int b, int nc) {
-  constexpr int T = N / 8, RW = RowV2<N>::RW;
+  constexpr int T = N / 8, RW = RowV2<N, RWQ>::RW;
   const int per = nc * RW;
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
@@ -949,7 +947,7 @@ __device__ __forceinline__ void build_packed(cplx v[8], const cplx *sb, int r, i
     const bool lo = k < nc, hi = k > N - nc;
     if (lo || hi) {
       const int kk = lo ? k : N - k;
-      const int o = RowV2<N>::ROW_MAJOR ? r * nc + kk : kk * RW + r;
+      const int o = kk * RW + r;
       const cplx f = sb[o], B = sb[per + o];
       const cplx A = mk(-(double)kk * f.y, (double)kk * f.x);
       z = packed_z<N>(k, A, B, hi);
@@ -958,23 +956,24 @@ __device__ __forceinline__ void build_packed(cplx v[8], const cplx *sb, int r, i
   }
 }
 
-template <int N>
-__global__ __launch_bounds__(RowV2<N>::THREADS) void k_step_row2(
+// The persistent row kernel.  blockIdx.x walks the y tiles of one run;
+// blockIdx.y selects the run of an ensemble (its mixed fields lie mstride
+// elements after the previous run's, and om_tab, when given, holds each run's
+// 2 Omega n^2), 0 for a single run.
+template <int N, int RWQ>
+__global__ __launch_bounds__((RowV2<N, RWQ>::THREADS)) void k_step_row2(
     const cplx *__restrict__ M, cplx *__restrict__ Q, StepConst c, unsigned long long *maxbits,
-    const cplx *__restrict__ tw) {
-  using RV = RowV2<N>;
+    const cplx *__restrict__ tw, size_t mstride, const double *__restrict__ om_tab) {
+  using RV = RowV2<N, RWQ>;
   constexpr int T = N / 8, RW = RV::RW, G = RV::G;
-  // n >= 512: each hardware wave serves one row (four-step transforms need a
-  // whole wave per 512-point part) and the rows' LDS areas are contiguous;
-  // below, rows interleave lane by lane and slot by slot.
-  constexpr bool WV = N >= 512 && R2D_ROW_FOURSTEP;
-  constexpr int IL = WV ? 1 : RW;
   __shared__ cplx xb[RW * Shape<N>::LDSN];
   __shared__ cplx sb[RV::SBN];
   const int tid = threadIdx.x;
-  const int r = WV ? (tid >> 6) % RW : tid % RW;
-  const int b = WV ? ((tid >> 6) / RW) * 64 + (tid & 63) : tid / RW;
-  cplx *P = WV ? xb + r * Shape<N>::LDSN : xb + r;
+  const int r = tid % RW, b = tid / RW;
+  cplx *P = xb + r;
+  M += blockIdx.y * mstride;
+  Q += blockIdx.y * mstride;
+  const double two_omega_n2 = om_tab ? om_tab[blockIdx.y] : c.two_omega_n2;
   const int nc = c.g.nc, kp = c.g.kp;
   const size_t fs = c.fs;
   // tile of iteration k: G consecutive slots of one XCD share a tile
@@ -985,17 +984,17 @@ __global__ __launch_bounds__(RowV2<N>::THREADS) void k_step_row2(
   int k = 0, t = tile_of(0);
   if (t >= NTILES) return;
   double m = 0.0;
-  stage_pair<N>(sb, M, fs, 0, t, sub, nc, kp);
+  stage_pair<N, RWQ>(sb, M, fs, 0, t, sub, nc, kp);
   wait_vm();
   __syncthreads();
   for (;;) {
     const int y = 4 * t + sub * RW + r;
     cplx v[8];
     double px[8], py[8], j1[8];
-    build_packed<N>(v, sb, r, b, nc);
+    build_packed<N, RWQ>(v, sb, r, b, nc);
     __syncthreads();
-    stage_pair<N>(sb, M, fs, 1, t, sub, nc, kp);
-    if constexpr (WV) fft_dif<N, +1, IL>(v, P, b, tw); else fft_canon<N, +1, IL>(v, P, b, tw);
+    stage_pair<N, RWQ>(sb, M, fs, 1, t, sub, nc, kp);
+    fft_canon<N, +1, RW>(v, P, b, tw);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       px[q] = v[q].x;
@@ -1004,31 +1003,31 @@ __global__ __launch_bounds__(RowV2<N>::THREADS) void k_step_row2(
     }
     wait_vm();
     __syncthreads();
-    build_packed<N>(v, sb, r, b, nc);
+    build_packed<N, RWQ>(v, sb, r, b, nc);
     __syncthreads();
-    stage_pair<N>(sb, M, fs, 2, t, sub, nc, kp);
-    if constexpr (WV) fft_dif<N, +1, IL>(v, P, b, tw); else fft_canon<N, +1, IL>(v, P, b, tw);
+    stage_pair<N, RWQ>(sb, M, fs, 2, t, sub, nc, kp);
+    fft_canon<N, +1, RW>(v, P, b, tw);
 #pragma unroll
     for (int q = 0; q < 8; ++q) j1[q] = px[q] * v[q].y - py[q] * v[q].x;
     wait_vm();
     __syncthreads();
-    build_packed<N>(v, sb, r, b, nc);
+    build_packed<N, RWQ>(v, sb, r, b, nc);
     __syncthreads();
     const int tn = tile_of(k + 1);
-    if (tn < NTILES) stage_pair<N>(sb, M, fs, 0, tn, sub, nc, kp);
-    if constexpr (WV) fft_dif<N, +1, IL>(v, P, b, tw); else fft_canon<N, +1, IL>(v, P, b, tw);
+    if (tn < NTILES) stage_pair<N, RWQ>(sb, M, fs, 0, tn, sub, nc, kp);
+    fft_canon<N, +1, RW>(v, P, b, tw);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const double vx = v[q].x, vy = v[q].y;
-      const double p1 = (-j1[q] + c.two_omega_n2 * vy) * c.inv_n4;
-      const double p2 = ((px[q] * vy - py[q] * vx) + c.two_omega_n2 * py[q]) * c.inv_n4;
+      const double p1 = (-j1[q] + two_omega_n2 * vy) * c.inv_n4;
+      const double p2 = ((px[q] * vy - py[q] * vx) + two_omega_n2 * py[q]) * c.inv_n4;
       v[q] = mk(p1, p2);
     }
-    if constexpr (WV) fft_dit<N, -1, IL>(v, P, b, tw); else fft_canon<N, -1, IL>(v, P, b, tw);
+    fft_canon<N, -1, RW>(v, P, b, tw);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) P[pad(b + T * q) * IL] = v[q];
+    for (int q = 0; q < 8; ++q) P[pad(b + T * q) * RW] = v[q];
     __syncthreads();
-    unpack_store<N, IL>(P, b, nc, kp, Q, y, Q + fs, y);
+    unpack_store<N, RW>(P, b, nc, kp, Q, y, Q + fs, y);
     if (tn >= NTILES) break;
     wait_vm();
     __syncthreads();
@@ -1059,14 +1058,15 @@ __global__ __launch_bounds__(RowV2<N>::THREADS) void k_step_row2(
 enum { FM_EMIT = 1, FM_STATE = 2, FM_DISK = 4, FM_ZERO = 8 };
 
 template <int N>
-__global__ __launch_bounds__(N / 8) void k_step_colfused(
-    const cplx *__restrict__ Q, cplx *psi, cplx *vz, cplx *__restrict__ M, StepConst c, int o,
-    int mode, const unsigned long long *mread, unsigned long long *mreset, double *dt_out,
-    const cplx *__restrict__ tw) {
+__device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *psi, cplx *vz,
+                                              cplx *__restrict__ M, const StepConst &c, int o,
+                                              int mode, const unsigned long long *mread,
+                                              unsigned long long *mreset, double *dt_out,
+                                              const cplx *__restrict__ tw, int lb) {
   auto pos = [](int b, int q) { return ilv<N>(b, q); };
   constexpr int T = N / 8;
   __shared__ cplx lds[Shape<N>::LDSN];
-  const int lb = xcd_pairs(blockIdx.x), b = threadIdx.x;
+  const int b = threadIdx.x;
   const int nc = c.g.nc, ncp = (nc + 1) & ~1;
   int field, kx;
   if (lb < nc) { field = 0; kx = lb; }
@@ -1164,6 +1164,58 @@ __global__ __launch_bounds__(N / 8) void k_step_colfused(
   }
 }
 
+template <int N>
+__global__ __launch_bounds__(N / 8) void k_step_colfused(
+    const cplx *__restrict__ Q, cplx *psi, cplx *vz, cplx *__restrict__ M, StepConst c, int o,
+    int mode, const unsigned long long *mread, unsigned long long *mreset, double *dt_out,
+    const cplx *__restrict__ tw) {
+  colfused_body<N>(Q, psi, vz, M, c, o, mode, mread, mreset, dt_out, tw, xcd_pairs(blockIdx.x));
+}
+
+// ---------------------------------------------------------------------------
+// ensembles: many independent runs of one size per launch (blockIdx.y = run)
+// ---------------------------------------------------------------------------
+// One run of an ensemble: its constants (parameters, fixed dt, geometry; the
+// forcing of each step comes from the forcing table), its state and its
+// mixed-field workspace (8 fields: M0..M5, then the bracket fields Q0, Q1).
+struct EnsMember {
+  StepConst c;
+  cplx *psi, *vz, *M;
+};
+struct EnsArgs {
+  const EnsMember *mem;   // the runs of this launch's size class
+  const ForceSet *ftab;   // [step][run][field], NULL = no forcing
+  const int *step;        // current step of the forcing table
+  int stride, base;       // runs per table step, index of this class's first run
+};
+__device__ __forceinline__ StepConst ens_const(const EnsArgs &a, int m) {
+  StepConst c = a.mem[m].c;
+  if (a.ftab) {
+    const ForceSet *f = a.ftab + ((size_t)(*a.step) * a.stride + a.base + m) * 2;
+    c.fp = f[0];
+    c.fz = f[1];
+  }
+  return c;
+}
+
+template <int N>
+__global__ __launch_bounds__(N / 8) void k_ens_colinv(EnsArgs a, const cplx *__restrict__ tw) {
+  const EnsMember &e = a.mem[blockIdx.y];
+  const StepConst c = ens_const(a, blockIdx.y);
+  colinv_body<N>(e.psi, e.vz, e.M, c, nullptr, tw, xcd_pairs(blockIdx.x));
+}
+template <int N>
+__global__ __launch_bounds__(N / 8) void k_ens_colfused(EnsArgs a, int o, int mode,
+                                                         const cplx *__restrict__ tw) {
+  const EnsMember &e = a.mem[blockIdx.y];
+  const StepConst c = ens_const(a, blockIdx.y);
+  colfused_body<N>(e.M + 6 * c.fs, e.psi, e.vz, e.M, c, o, mode, nullptr, nullptr, nullptr, tw,
+                   xcd_pairs(blockIdx.x));
+}
+// advances the forcing-table step at the end of a time step
+__global__ void k_ens_tick(int *step) { step[0] = step[0] + 1; }
+__global__ void k_set_i32(int *p, int v) { p[0] = v; }
+
 // ---------------------------------------------------------------------------
 // host side: plan, dispatch, C ABI
 // ---------------------------------------------------------------------------
@@ -1187,6 +1239,9 @@ struct r2d3c_plan {
   double *dtslot = nullptr;
   double *partial = nullptr;      // reduction partials
   long long bytes = 0;
+  // schedule of the persistent row kernel (r2d3c_plan_set_schedule /
+  // r2d3c_autotune): rows per block and blocks per CU
+  int row_rw = 2, row_bpc = 1;
   // kernel timing of r2d3c_step (r2d3c_timing_enable / r2d3c_timing_read)
   int timing = 0;
   std::vector<hipEvent_t> evpool;
@@ -1333,11 +1388,11 @@ static int make_force_set(const r2d3c_force_draw *d, int n, ForceSet *f) {
   return 0;
 }
 
-// persistent row kernel grid: one block per CU, a multiple of 8*G, no more
-// blocks than tile slots
-template <int N> static int row2_grid(int cus) {
-  constexpr int G = RowV2<N>::G, Q8 = 8 * G;
-  int g = cus * R2D_ROW2_BPC / Q8 * Q8;
+// Persistent row grid: bpc blocks per CU, a multiple of 8 G (the tile map),
+// at most one block per tile slot.
+template <int N, int RWQ> static int row2_grid(int cus, int bpc) {
+  constexpr int G = RowV2<N, RWQ>::G, Q8 = 8 * G;
+  int g = cus * bpc / Q8 * Q8;
   const int need = (N / 4) * G;
   if (g > need) g = need;
   if (g < Q8) g = Q8;
@@ -1385,12 +1440,18 @@ static int step_seq(r2d3c_plan *p, StepConst &c, cplx *psi, cplx *vz, hipStream_
   for (int o = 4; o >= 1; --o) {
     unsigned long long *macc = (o == 4 && c.cfl_mode) ? mb : nullptr;
     ev_pair(p, 1, &ea, &eb);
-    if constexpr (RowV2<N>::THREADS >= 64) {
-      hipExtLaunchKernelGGL(k_step_row2<N>, dim3(row2_grid<N>(p->cus)), dim3(RowV2<N>::THREADS), 0,
-                            s, ea, eb, 0, (const cplx *)M, Q, c, macc, tw);
+    if constexpr (RowV2<N, 2>::THREADS >= 64) {
+      if (p->row_rw == 1 && RowV2<N, 1>::THREADS >= 64)
+        hipExtLaunchKernelGGL(k_step_row2<N, 1>, dim3(row2_grid<N, 1>(p->cus, p->row_bpc)),
+                              dim3(RowV2<N, 1>::THREADS), 0, s, ea, eb, 0, (const cplx *)M, Q, c,
+                              macc, tw, (size_t)0, (const double *)nullptr);
+      else
+        hipExtLaunchKernelGGL(k_step_row2<N, 2>, dim3(row2_grid<N, 2>(p->cus, p->row_bpc)),
+                              dim3(RowV2<N, 2>::THREADS), 0, s, ea, eb, 0, (const cplx *)M, Q, c,
+                              macc, tw, (size_t)0, (const double *)nullptr);
     } else {
       hipExtLaunchKernelGGL(k_step_row<N>, dim3(N / RW), dim3(N / 8 * RW), 0, s, ea, eb, 0,
-                            (const cplx *)M, Q, c, macc, tw);
+                            (const cplx *)M, Q, c, macc, tw, (size_t)0, (const double *)nullptr);
     }
     int mode = FM_EMIT;
     if (o == 1) mode = FM_STATE | (masked ? FM_DISK : FM_ZERO) | (chain ? FM_EMIT : 0);
@@ -1872,6 +1933,389 @@ void r2d3c_rand_force_draw(double kup, int *seed, double *kx, double *ky, double
   *kx = std::floor(kup * std::cos(theta) + 0.5);
   *ky = std::floor(kup * std::sin(theta) + 0.5);
   *phase = ph;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// schedules and their autotuner
+// ---------------------------------------------------------------------------
+static bool schedule_valid(int n, int rw, int bpc) {
+  if (bpc < 1 || bpc > 4 || (rw != 1 && rw != 2)) return false;
+  // one row per block needs n/8 >= 64 threads (whole waves)
+  return rw == 2 || n >= 512;
+}
+
+template <int N>
+static int tune_steps(r2d3c_plan *p, cplx *psi, cplx *vz, int reps, hipStream_t s, float *ms) {
+  r2d3c_params q{1e-3, 0.1, 1e-3, 0.1, 1.0, 1, 2, 1, 2};
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  int rc = run_steps<N>(p, &q, psi, vz, nullptr, 2, R2D3C_DT_FIXED, 1e-6, nullptr, s);
+  if (!rc) {
+    HIPCHK(hipEventRecord(a, s));
+    rc = run_steps<N>(p, &q, psi, vz, nullptr, reps, R2D3C_DT_FIXED, 1e-6, nullptr, s);
+    HIPCHK(hipEventRecord(b, s));
+    HIPCHK(hipEventSynchronize(b));
+    HIPCHK(hipEventElapsedTime(ms, a, b));
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc;
+}
+
+extern "C" {
+
+int r2d3c_plan_set_schedule(r2d3c_plan *p, int row_rows, int row_blocks_per_cu) {
+  NEED(p);
+  if (!schedule_valid(p->n, row_rows, row_blocks_per_cu))
+    return set_err(R2D3C_EINVAL, "schedule: rows per block 1 (n >= 512) or 2, 1..4 blocks per CU");
+  p->row_rw = row_rows;
+  p->row_bpc = row_blocks_per_cu;
+  return 0;
+}
+
+int r2d3c_plan_get_schedule(const r2d3c_plan *p, int *row_rows, int *row_blocks_per_cu) {
+  NEED(p); NEED(row_rows); NEED(row_blocks_per_cu);
+  *row_rows = p->row_rw;
+  *row_blocks_per_cu = p->row_bpc;
+  return 0;
+}
+
+int r2d3c_autotune(r2d3c_plan *p, int reps, double *ms_per_step, void *stream) {
+  NEED(p);
+  if (reps < 1) return set_err(R2D3C_EINVAL, "reps must be >= 1");
+  hipStream_t s = (hipStream_t)stream;
+  const int n = p->n;
+  const size_t sz = (size_t)(n / 2 + 1) * n * sizeof(cplx);
+  cplx *psi = nullptr, *vz = nullptr;
+  HIPCHK(hipMalloc(&psi, sz));
+  hipError_t e = hipMalloc(&vz, sz);
+  if (e != hipSuccess) { (void)hipFree(psi); return hip_err(e, "hipMalloc"); }
+  // the timings do not depend on the values: a zero state is enough
+  int rc = 0;
+  if ((e = hipMemsetAsync(psi, 0, sz, s)) != hipSuccess || (e = hipMemsetAsync(vz, 0, sz, s)) != hipSuccess)
+    rc = hip_err(e, "hipMemsetAsync");
+  const int cand[4][2] = {{2, 1}, {2, 2}, {1, 1}, {1, 2}};
+  int best = -1;
+  float best_ms = 0.f;
+  const int rw0 = p->row_rw, bpc0 = p->row_bpc;
+  for (int k = 0; k < 4 && !rc; ++k) {
+    if (!schedule_valid(n, cand[k][0], cand[k][1])) continue;
+    if (n < 256 && k > 0) break;  // no persistent row kernel below 256: nothing to tune
+    p->row_rw = cand[k][0];
+    p->row_bpc = cand[k][1];
+    float ms = 0.f;
+    auto tune = [&](auto tag) -> int {
+      constexpr int NN = decltype(tag)::value;
+      return tune_steps<NN>(p, psi, vz, reps, s, &ms);
+    };
+    switch (n) {
+      case 16: rc = tune(std::integral_constant<int, 16>()); break;
+      case 32: rc = tune(std::integral_constant<int, 32>()); break;
+      case 64: rc = tune(std::integral_constant<int, 64>()); break;
+      case 128: rc = tune(std::integral_constant<int, 128>()); break;
+      case 256: rc = tune(std::integral_constant<int, 256>()); break;
+      case 512: rc = tune(std::integral_constant<int, 512>()); break;
+      case 1024: rc = tune(std::integral_constant<int, 1024>()); break;
+      case 2048: rc = tune(std::integral_constant<int, 2048>()); break;
+      case 4096: rc = tune(std::integral_constant<int, 4096>()); break;
+      default: rc = set_err(R2D3C_ENOTSUP, "unsupported grid size");
+    }
+    if (!rc && (best < 0 || ms < best_ms)) { best = k; best_ms = ms; }
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(psi);
+  (void)hipFree(vz);
+  if (rc || best < 0) {
+    p->row_rw = rw0;
+    p->row_bpc = bpc0;
+    return rc ? rc : set_err(R2D3C_EINVAL, "no schedule could be timed");
+  }
+  p->row_rw = cand[best][0];
+  p->row_bpc = cand[best][1];
+  if (ms_per_step) *ms_per_step = best_ms / reps;
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// ensembles of independent runs
+// ---------------------------------------------------------------------------
+struct EnsClass {
+  int n = 0, first = 0, count = 0;  // runs [first, first + count) of the ensemble, all of size n
+  r2d3c_plan *plan = nullptr;       // geometry, twiddles and schedule of size n
+  EnsMember *d_mem = nullptr;
+  cplx *work = nullptr;             // 8 mixed fields per run, run i at 8 fs i
+  double *d_om = nullptr;           // 2 Omega n^2 per run (row kernel)
+  StepConst c0{};                   // geometry shared by the class (row kernel)
+  hipStream_t st = nullptr;         // branch stream of this class in the step graph
+  hipEvent_t done = nullptr;
+};
+
+struct r2d3c_ensemble {
+  int count = 0;
+  std::vector<EnsClass> cls;
+  std::vector<int> order;           // run index in class order -> caller's index
+  std::vector<r2d3c_member> mem;    // caller's runs
+  hipStream_t es = nullptr;         // private stream: launches and graph capture
+  hipEvent_t ev_in = nullptr, ev_out = nullptr, fork = nullptr;
+  ForceSet *d_ftab = nullptr;
+  size_t ftab_cap = 0;              // ForceSet entries allocated
+  std::vector<ForceSet> h_ftab;
+  int *d_step = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const ForceSet *graph_ftab = nullptr;
+  bool graph_valid = false;
+};
+
+template <int N>
+static void ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, hipStream_t s,
+                           bool first, bool masked, bool chain) {
+  const r2d3c_plan *p = k.plan;
+  const int nc = p->g.nc, ncp = (nc + 1) & ~1;
+  const cplx *tw = p->tw;
+  EnsArgs a{k.d_mem, ftab, e->d_step, e->count, k.first};
+  if (first) k_ens_colinv<N><<<dim3(round16(ncp + nc), k.count), N / 8, 0, s>>>(a, tw);
+  for (int o = 4; o >= 1; --o) {
+    // one row launch covers the class: the runs' workspaces are 8 fs apart
+    const size_t ms = 8 * p->fs;
+    const cplx *M0 = k.work;
+    cplx *Q0 = k.work + 6 * p->fs;
+    if constexpr (RowV2<N, 2>::THREADS >= 64) {
+      if (p->row_rw == 1 && RowV2<N, 1>::THREADS >= 64)
+        k_step_row2<N, 1><<<dim3(row2_grid<N, 1>(p->cus, p->row_bpc), k.count),
+                            RowV2<N, 1>::THREADS, 0, s>>>(M0, Q0, k.c0, nullptr, tw, ms, k.d_om);
+      else
+        k_step_row2<N, 2><<<dim3(row2_grid<N, 2>(p->cus, p->row_bpc), k.count),
+                            RowV2<N, 2>::THREADS, 0, s>>>(M0, Q0, k.c0, nullptr, tw, ms, k.d_om);
+    } else {
+      constexpr int RW = RowStep<N>::ROWS;
+      k_step_row<N><<<dim3(N / RW, k.count), N / 8 * RW, 0, s>>>(M0, Q0, k.c0, nullptr, tw, ms,
+                                                                 k.d_om);
+    }
+    int mode = FM_EMIT;
+    if (o == 1) mode = FM_STATE | (masked ? FM_DISK : FM_ZERO) | (chain ? FM_EMIT : 0);
+    const int nblk = round16(2 * ncp + ((mode & FM_ZERO) ? (N / 2 + 1 - nc) : 0));
+    k_ens_colfused<N><<<dim3(nblk, k.count), N / 8, 0, s>>>(a, o, mode, tw);
+  }
+}
+
+static int ens_dispatch(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, hipStream_t s,
+                        bool first, bool masked, bool chain) {
+  switch (k.n) {
+    case 16: ens_class_step<16>(e, k, ftab, s, first, masked, chain); break;
+    case 32: ens_class_step<32>(e, k, ftab, s, first, masked, chain); break;
+    case 64: ens_class_step<64>(e, k, ftab, s, first, masked, chain); break;
+    case 128: ens_class_step<128>(e, k, ftab, s, first, masked, chain); break;
+    case 256: ens_class_step<256>(e, k, ftab, s, first, masked, chain); break;
+    case 512: ens_class_step<512>(e, k, ftab, s, first, masked, chain); break;
+    case 1024: ens_class_step<1024>(e, k, ftab, s, first, masked, chain); break;
+    case 2048: ens_class_step<2048>(e, k, ftab, s, first, masked, chain); break;
+    case 4096: ens_class_step<4096>(e, k, ftab, s, first, masked, chain); break;
+    default: return set_err(R2D3C_ENOTSUP, "unsupported grid size");
+  }
+  LAUNCHCHK("ensemble step");
+  return 0;
+}
+
+// Captures one steady time step of every class (their launches are independent
+// branches of the graph) followed by the forcing-table tick.
+static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab) {
+  if (e->exec) { (void)hipGraphExecDestroy(e->exec); e->exec = nullptr; }
+  if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
+  e->graph_valid = false;
+  HIPCHK(hipStreamBeginCapture(e->es, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  hipError_t he = hipEventRecord(e->fork, e->es);
+  for (auto &k : e->cls) {
+    if (he != hipSuccess || rc) break;
+    he = hipStreamWaitEvent(k.st, e->fork, 0);
+    if (he == hipSuccess) rc = ens_dispatch(e, k, ftab, k.st, false, true, true);
+    if (he == hipSuccess && !rc) he = hipEventRecord(k.done, k.st);
+    if (he == hipSuccess && !rc) he = hipStreamWaitEvent(e->es, k.done, 0);
+  }
+  if (he == hipSuccess && !rc) {
+    k_ens_tick<<<1, 1, 0, e->es>>>(e->d_step);
+    he = hipGetLastError();
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t ee = hipStreamEndCapture(e->es, &g);
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (he != hipSuccess) { if (g) (void)hipGraphDestroy(g); return hip_err(he, "ensemble capture"); }
+  if (ee != hipSuccess) return hip_err(ee, "hipStreamEndCapture");
+  e->graph = g;
+  HIPCHK(hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0));
+  e->graph_ftab = ftab;
+  e->graph_valid = true;
+  return 0;
+}
+
+static void ens_free(r2d3c_ensemble *e) {
+  if (!e) return;
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  for (auto &k : e->cls) {
+    if (k.d_mem) (void)hipFree(k.d_mem);
+    if (k.work) (void)hipFree(k.work);
+    if (k.d_om) (void)hipFree(k.d_om);
+    if (k.st) (void)hipStreamDestroy(k.st);
+    if (k.done) (void)hipEventDestroy(k.done);
+    if (k.plan) r2d3c_plan_destroy(k.plan);
+  }
+  if (e->d_ftab) (void)hipFree(e->d_ftab);
+  if (e->d_step) (void)hipFree(e->d_step);
+  if (e->ev_in) (void)hipEventDestroy(e->ev_in);
+  if (e->ev_out) (void)hipEventDestroy(e->ev_out);
+  if (e->fork) (void)hipEventDestroy(e->fork);
+  if (e->es) (void)hipStreamDestroy(e->es);
+  delete e;
+}
+
+extern "C" {
+
+int r2d3c_ensemble_create(r2d3c_ensemble **out, const r2d3c_member *members, int count) {
+  if (!out) return set_err(R2D3C_EINVAL, "ensemble pointer is NULL");
+  *out = nullptr;
+  NEED(members);
+  if (count < 1) return set_err(R2D3C_EINVAL, "an ensemble needs at least one run");
+  for (int m = 0; m < count; ++m) {
+    const r2d3c_member &r = members[m];
+    if (r.n < 16 || r.n > 4096 || (r.n & (r.n - 1)))
+      return set_err(R2D3C_ENOTSUP, "run sizes must be powers of two in [16, 4096]");
+    if (!r.psi || !r.vz || r.psi == r.vz) return set_err(R2D3C_EINVAL, "run state pointers");
+    if (!(r.dt > 0.0)) return set_err(R2D3C_EINVAL, "ensembles step with a fixed dt > 0");
+    int rc = check_params(&r.params);
+    if (rc) return rc;
+  }
+  r2d3c_ensemble *e = new r2d3c_ensemble();
+  e->count = count;
+  e->mem.assign(members, members + count);
+  int rc = 0;
+  hipError_t he = hipSuccess;
+  // classes by size, in the order sizes first appear; runs keep their order
+  std::vector<int> sizes;
+  for (int m = 0; m < count; ++m) {
+    bool seen = false;
+    for (int z : sizes) seen |= z == members[m].n;
+    if (!seen) sizes.push_back(members[m].n);
+  }
+  for (int z : sizes) {
+    EnsClass k;
+    k.n = z;
+    k.first = (int)e->order.size();
+    for (int m = 0; m < count; ++m)
+      if (members[m].n == z) e->order.push_back(m);
+    k.count = (int)e->order.size() - k.first;
+    e->cls.push_back(k);
+  }
+  for (auto &k : e->cls) {
+    if ((rc = r2d3c_plan_create(&k.plan, k.n))) break;
+    const size_t fs = k.plan->fs;
+    if ((he = hipMalloc(&k.work, sizeof(cplx) * 8 * fs * k.count)) != hipSuccess) break;
+    if ((he = hipMalloc(&k.d_mem, sizeof(EnsMember) * k.count)) != hipSuccess) break;
+    std::vector<EnsMember> hm(k.count);
+    for (int i = 0; i < k.count; ++i) {
+      const r2d3c_member &r = members[e->order[k.first + i]];
+      make_step_const(k.plan, &r.params, R2D3C_DT_FIXED, r.dt, &hm[i].c);
+      hm[i].psi = (cplx *)r.psi;
+      hm[i].vz = (cplx *)r.vz;
+      hm[i].M = k.work + (size_t)i * 8 * fs;
+    }
+    if ((he = hipMemcpy(k.d_mem, hm.data(), sizeof(EnsMember) * k.count, hipMemcpyHostToDevice)) != hipSuccess) break;
+    std::vector<double> om(k.count);
+    for (int i = 0; i < k.count; ++i) om[i] = hm[i].c.two_omega_n2;
+    k.c0 = hm[0].c;
+    if ((he = hipMalloc(&k.d_om, sizeof(double) * k.count)) != hipSuccess) break;
+    if ((he = hipMemcpy(k.d_om, om.data(), sizeof(double) * k.count, hipMemcpyHostToDevice)) != hipSuccess) break;
+    if ((he = hipStreamCreateWithFlags(&k.st, hipStreamNonBlocking)) != hipSuccess) break;
+    if ((he = hipEventCreateWithFlags(&k.done, hipEventDisableTiming)) != hipSuccess) break;
+  }
+  if (!rc && he == hipSuccess) he = hipStreamCreateWithFlags(&e->es, hipStreamNonBlocking);
+  if (!rc && he == hipSuccess) he = hipEventCreateWithFlags(&e->ev_in, hipEventDisableTiming);
+  if (!rc && he == hipSuccess) he = hipEventCreateWithFlags(&e->ev_out, hipEventDisableTiming);
+  if (!rc && he == hipSuccess) he = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
+  if (!rc && he == hipSuccess) he = hipMalloc(&e->d_step, sizeof(int));
+  if (!rc && he != hipSuccess) rc = hip_err(he, "ensemble set-up");
+  if (rc) { ens_free(e); return rc; }
+  *out = e;
+  return 0;
+}
+
+int r2d3c_ensemble_destroy(r2d3c_ensemble *e) {
+  if (e && e->es) (void)hipStreamSynchronize(e->es);
+  ens_free(e);
+  return 0;
+}
+
+int r2d3c_ensemble_set_schedule(r2d3c_ensemble *e, int n, int row_rows, int row_blocks_per_cu) {
+  NEED(e);
+  for (auto &k : e->cls)
+    if (k.n == n) {
+      int rc = r2d3c_plan_set_schedule(k.plan, row_rows, row_blocks_per_cu);
+      if (!rc) e->graph_valid = false;
+      return rc;
+    }
+  return set_err(R2D3C_EINVAL, "no run of that size in the ensemble");
+}
+
+int r2d3c_ensemble_run(r2d3c_ensemble *e, const r2d3c_forcing *forcing, int n_steps,
+                       void *stream) {
+  NEED(e);
+  if (n_steps < 0) return set_err(R2D3C_EINVAL, "n_steps must be >= 0");
+  if (n_steps == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const ForceSet *ftab = nullptr;
+  if (forcing) {
+    const size_t need = (size_t)n_steps * e->count * 2;
+    e->h_ftab.assign(need, ForceSet{});
+    for (int st = 0; st < n_steps; ++st)
+      for (int j = 0; j < e->count; ++j) {
+        const int m = e->order[j];
+        const r2d3c_forcing &f = forcing[(size_t)st * e->count + m];
+        if (f.dense_psi || f.dense_vz || f.ci_enabled)
+          return set_err(R2D3C_ENOTSUP, "ensembles take rand_force draws only");
+        ForceSet *dst = &e->h_ftab[((size_t)st * e->count + j) * 2];
+        int rc = make_force_set(&f.psi, e->mem[m].n, dst);
+        if (!rc) rc = make_force_set(&f.vz, e->mem[m].n, dst + 1);
+        if (rc) return rc;
+      }
+    HIPCHK(hipStreamSynchronize(e->es));  // the previous run may still read the table
+    if (need > e->ftab_cap) {
+      if (e->d_ftab) HIPCHK(hipFree(e->d_ftab));
+      e->d_ftab = nullptr;
+      e->ftab_cap = 0;
+      HIPCHK(hipMalloc(&e->d_ftab, sizeof(ForceSet) * need));
+      e->ftab_cap = need;
+    }
+    HIPCHK(hipMemcpyAsync(e->d_ftab, e->h_ftab.data(), sizeof(ForceSet) * need,
+                          hipMemcpyHostToDevice, e->es));
+    ftab = e->d_ftab;
+  }
+  HIPCHK(hipEventRecord(e->ev_in, s));
+  HIPCHK(hipStreamWaitEvent(e->es, e->ev_in, 0));
+  k_set_i32<<<1, 1, 0, e->es>>>(e->d_step, 0);
+  // the first step reads the state's own columns and zeroes what lies
+  // outside the update mask, as the first step of r2d3c_run does
+  for (auto &k : e->cls) {
+    int rc = ens_dispatch(e, k, ftab, e->es, true, false, n_steps > 1);
+    if (rc) return rc;
+  }
+  if (n_steps > 1) {
+    k_ens_tick<<<1, 1, 0, e->es>>>(e->d_step);
+    LAUNCHCHK("ensemble tick");
+    if (!e->graph_valid || e->graph_ftab != ftab) {
+      int rc = ens_capture(e, ftab);
+      if (rc) return rc;
+    }
+    for (int st = 1; st < n_steps; ++st) HIPCHK(hipGraphLaunch(e->exec, e->es));
+  }
+  HIPCHK(hipEventRecord(e->ev_out, e->es));
+  HIPCHK(hipStreamWaitEvent(s, e->ev_out, 0));
+  return 0;
 }
 
 }  // extern "C"

@@ -1,0 +1,137 @@
+"""Ensembles of independent runs on one GPU (no reference counterpart: the
+reference runs one simulation per MPI job).
+
+Runs of one grid size advance together: every kernel of the step covers all
+of them in one launch (the run is blockIdx.y), and each size class is an
+independent branch of one HIP graph per time step, captured once and replayed.
+The forcing of every step is drawn on the host before the steps start
+(rand_force's wavevectors and phases do not depend on the flow) and uploaded
+as one table, so the steps need no host round trip.  Each run equals the same
+run advanced alone by rk4_run with its fixed dt, bit for bit.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib, autotune
+from . import spectral as S
+from ._lib import check
+
+
+class Member:
+    """One run: grid size n, parameters (spectral.Params), fixed dt and its
+    device state ps, vz (complex128, (n/2+1, n), advanced in place)."""
+
+    def __init__(self, ps, vz, params, dt):
+        self.n = S._spec_n(ps)
+        if S._spec_n(vz) != self.n or ps.device != vz.device:
+            raise ValueError("psi and vz of a run must match")
+        self.ps, self.vz, self.params, self.dt = ps, vz, params, float(dt)
+
+    def c(self):
+        return _lib.Member(self.n, self.params.c(), self.dt, S._ptr(self.ps), S._ptr(self.vz))
+
+
+class Ensemble:
+    def __init__(self, members, schedules="auto"):
+        """members: list of Member on one device.  schedules: "auto" (cached
+        autotuned schedule per size), None (defaults) or {n: (rows, bpc)}."""
+        if not members:
+            raise ValueError("empty ensemble")
+        self.members = list(members)
+        self.device = self.members[0].ps.device
+        arr = (_lib.Member * len(self.members))(*[m.c() for m in self.members])
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(_lib.lib().r2d3c_ensemble_create(ctypes.byref(h), arr, len(arr)),
+                  "r2d3c_ensemble_create")
+        self.handle = h
+        self.sizes = sorted({m.n for m in self.members})
+        for n in self.sizes:
+            if schedules == "auto":
+                rw, bpc = autotune.schedule(n, self.device)
+            elif schedules:
+                rw, bpc = schedules.get(n, autotune.DEFAULT)
+            else:
+                continue
+            check(_lib.lib().r2d3c_ensemble_set_schedule(self.handle, n, rw, bpc),
+                  "r2d3c_ensemble_set_schedule")
+
+    def run(self, n_steps, forcing=None):
+        """Advance every run n_steps steps on the current stream.  forcing:
+        None or a list over steps of a list over runs of (fp_draw, fz_draw)
+        (spectral.ForceDraw or None each)."""
+        fo = None
+        if forcing is not None:
+            if len(forcing) != n_steps or any(len(f) != len(self.members) for f in forcing):
+                raise ValueError("forcing must be n_steps lists of one (fp, fz) pair per run")
+            fo = (_lib.Forcing * (n_steps * len(self.members)))()
+            for st, row in enumerate(forcing):
+                for m, pair in enumerate(row):
+                    fp, fz = pair if pair is not None else (None, None)
+                    fo[st * len(self.members) + m] = S._forcing_c(fp, fz)
+        with torch.cuda.device(self.device):
+            check(_lib.lib().r2d3c_ensemble_run(self.handle, fo, int(n_steps),
+                                                S._stream(self.device)), "r2d3c_ensemble_run")
+
+    def close(self):
+        if self.handle:
+            with torch.cuda.device(self.device):
+                _lib.lib().r2d3c_ensemble_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# --- the batched workload: 24 x 256^2 + 12 x 512^2 + 4 x 1024^2 --------------
+MIX = ((256, 24), (512, 12), (1024, 4))
+
+
+def mixed_ensemble_spec(mix=MIX, omega_seed=1234):
+    """Per-run settings of the batched workload: run r (in size order) has
+    seed r, Omega ~ U[0, 10] from numpy's default_rng(omega_seed),
+    nu = nuv = 1e-3 * 256/n, hnu = hnuv = 0.1 (nn = nnv = 1, mm = mmv = 2),
+    forcing shell kf = n/32 for psi and vz with injection rate 1.0, and the
+    fixed dt = 1e-3 * 256/n (the CFL number k_c u dt stays ~0.25 for u ~ 3)."""
+    rng = np.random.default_rng(omega_seed)
+    out, r = [], 0
+    for n, count in mix:
+        for _ in range(count):
+            nu = 1e-3 * 256.0 / n
+            out.append(dict(n=n, seed=r, omega=float(rng.uniform(0.0, 10.0)), nu=nu,
+                            hnu=0.1, kf=n / 32.0, f0=1.0, dt=1e-3 * 256.0 / n))
+            r += 1
+    return out
+
+
+def build(spec, device, schedules="auto"):
+    """Allocate the runs of `spec` (random-phase initial conditions seeded by
+    each run's seed) and return (Ensemble, per-run forcing draw functions)."""
+    from .ic import random_phase_ic
+    members = []
+    for r in spec:
+        ps, vz = random_phase_ic(r["n"], r["seed"])
+        p = S.Params(r["nu"], r["hnu"], 1, 2, r["nu"], r["hnu"], 1, 2, r["omega"])
+        members.append(Member(torch.from_numpy(ps).to(device), torch.from_numpy(vz).to(device),
+                              p, r["dt"]))
+    return Ensemble(members, schedules)
+
+
+def forcing_table(spec, n_steps):
+    """rand_force draws of every run for n_steps steps (forcing seed = run seed)."""
+    from .ic import forcing_schedule
+    per_run = []
+    for r in spec:
+        sch = forcing_schedule(r["kf"], r["f0"], r["f0"], r["seed"], n_steps, S.rand_force_draw)
+        per_run.append([(S.ForceDraw(*a), S.ForceDraw(*b)) for a, b in sch])
+    return [[per_run[m][st] for m in range(len(spec))] for st in range(n_steps)]
+
+
+def points(spec):
+    return sum(r["n"] * r["n"] for r in spec)

@@ -72,7 +72,7 @@ def test_struct_layouts_match_the_header(tmp_path):
     if cc is None:
         pytest.skip("no C compiler")
     structs = {"r2d3c_params": _lib.Params, "r2d3c_force_draw": _lib.ForceDraw,
-               "r2d3c_forcing": _lib.Forcing}
+               "r2d3c_forcing": _lib.Forcing, "r2d3c_member": _lib.Member}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "rot2d3c.h"', "int main(void) {"]
     for cname, ct in structs.items():
         lines.append('printf("%%zu\\n", sizeof(%s));' % cname)

@@ -1,0 +1,136 @@
+"""GPU: ensembles of independent runs (rot2d3c_amd.ensemble) and row-kernel
+schedules.  Every run of an ensemble must equal the same run advanced alone
+(r2d3c_run, fixed dt, same forcing) bit for bit, whatever the schedule."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import random_state, rel_l2
+from oracle import spectral as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _spec(sizes, seed=3):
+    rng = np.random.default_rng(seed)
+    out = []
+    for r, n in enumerate(sizes):
+        out.append(dict(n=n, seed=10 + r, omega=float(rng.uniform(0, 10)), nu=float(rng.uniform(1e-3, 3e-3)),
+                        hnu=float(rng.uniform(0.05, 0.2)), kf=max(2.0, n / 16.0), f0=1.0,
+                        dt=float(rng.uniform(0.5, 1.0)) * 1e-3 * 64.0 / n))
+    return out
+
+
+def _members(spec, cuda):
+    from rot2d3c_amd import ensemble as E, spectral as S
+    ms, host = [], []
+    for r in spec:
+        ps, vz = random_state(r["n"], r["seed"])
+        host.append((ps, vz))
+        p = S.Params(r["nu"], r["hnu"], 1, 2, r["nu"] * 1.5, r["hnu"] * 0.5, 1, 2, r["omega"])
+        ms.append(E.Member(torch.from_numpy(ps).to(cuda), torch.from_numpy(vz).to(cuda), p, r["dt"]))
+    return ms, host
+
+
+def _solo(m, host, draws, nsteps, cuda):
+    from rot2d3c_amd import spectral as S
+    ps = torch.from_numpy(host[0]).to(cuda)
+    vz = torch.from_numpy(host[1]).to(cuda)
+    S.rk4_run(ps, vz, m.params, draws, nsteps, dt=m.dt)
+    return ps, vz
+
+
+@pytest.mark.parametrize("nsteps", [1, 6])
+def test_ensemble_runs_equal_solo_runs_bitwise(nsteps, cuda):
+    """Mixed sizes (three classes, interleaved), per-run Omega, nu, dt and
+    forcing; one step (no graph) and several (captured step graph)."""
+    print("ensemble vs solo, %d steps" % nsteps, flush=True)
+    from rot2d3c_amd import ensemble as E
+    spec = _spec([32, 64, 32, 256, 64, 32])
+    ms, host = _members(spec, cuda)
+    ens = E.Ensemble(ms, schedules=None)
+    table = E.forcing_table(spec, nsteps)
+    ens.run(nsteps, table)
+    torch.cuda.synchronize()
+    for r, (m, h) in enumerate(zip(ms, host)):
+        draws = [table[st][r] for st in range(nsteps)]
+        ps, vz = _solo(m, h, draws, nsteps, cuda)
+        assert torch.equal(m.ps, ps) and torch.equal(m.vz, vz), "run %d (n=%d)" % (r, m.n)
+    ens.close()
+
+
+def test_ensemble_second_run_call_and_no_forcing(cuda):
+    """A second run() reuses the captured graph with a new forcing table; a
+    third runs without forcing (a new graph): still equal to solo runs."""
+    print("ensemble repeated calls", flush=True)
+    from rot2d3c_amd import ensemble as E
+    spec = _spec([64, 128, 64], seed=5)
+    ms, host = _members(spec, cuda)
+    ens = E.Ensemble(ms, schedules=None)
+    t1 = E.forcing_table(spec, 4)
+    t2 = E.forcing_table([dict(r, seed=r["seed"] + 100) for r in spec], 3)
+    ens.run(4, t1)
+    ens.run(3, t2)
+    ens.run(2, None)
+    torch.cuda.synchronize()
+    for r, (m, h) in enumerate(zip(ms, host)):
+        ps, vz = _solo(m, h, [t1[st][r] for st in range(4)], 4, cuda)
+        from rot2d3c_amd import spectral as S
+        S.rk4_run(ps, vz, m.params, [t2[st][r] for st in range(3)], 3, dt=m.dt)
+        S.rk4_run(ps, vz, m.params, None, 2, dt=m.dt)
+        torch.cuda.synchronize()
+        assert torch.equal(m.ps, ps) and torch.equal(m.vz, vz), "run %d" % r
+    ens.close()
+
+
+def test_ensemble_run_matches_oracle(cuda):
+    """One ensemble run against the oracle's loop (fixed dt, rand_force)."""
+    print("ensemble vs oracle", flush=True)
+    from rot2d3c_amd import ensemble as E
+    spec = _spec([32, 64], seed=9)
+    ms, host = _members(spec, cuda)
+    ens = E.Ensemble(ms, schedules=None)
+    nsteps = 8
+    table = E.forcing_table(spec, nsteps)
+    ens.run(nsteps, table)
+    torch.cuda.synchronize()
+    for r, (m, (ps, vz)) in enumerate(zip(ms, host)):
+        q = m.params
+        p = O.Params(q.nu, q.hnu, q.nn, q.mm, q.nuv, q.hnuv, q.nnv, q.mmv, q.omega)
+        # the oracle's iflow = 3 loop draws the same sequence from the run's seed
+        ps, vz, *_ = O.run_random_forcing(ps, vz, nsteps, 0.0, p, 1.0, spec[r]["kf"], spec[r]["f0"],
+                                          spec[r]["f0"], spec[r]["seed"], fixed_dt=m.dt)
+        assert rel_l2(m.ps.cpu().numpy(), ps) < 1e-12 and rel_l2(m.vz.cpu().numpy(), vz) < 1e-12
+    ens.close()
+
+
+@pytest.mark.parametrize("n", [512, 1024])
+def test_every_schedule_gives_identical_results(n, cuda):
+    """Rows per block and blocks per CU change only the launch shape."""
+    print("schedules n=%d" % n, flush=True)
+    from rot2d3c_amd import spectral as S, _lib
+    from rot2d3c_amd._lib import check
+    ps0, vz0 = random_state(n, 77)
+    p = S.Params(1e-3, 0.1, 1, 2, 1e-3, 0.1, 1, 2, 2.0)
+    pl = S.Plan.get(n, cuda)
+    outs = []
+    for rw, bpc in ((2, 1), (2, 2), (1, 1), (1, 2)):
+        check(_lib.lib().r2d3c_plan_set_schedule(pl.handle, rw, bpc), "set_schedule")
+        ps = torch.from_numpy(ps0).to(cuda)
+        vz = torch.from_numpy(vz0).to(cuda)
+        S.rk4_run(ps, vz, p, None, 3, dt=2e-4)
+        torch.cuda.synchronize()
+        outs.append((ps, vz))
+    check(_lib.lib().r2d3c_plan_set_schedule(pl.handle, 2, 1), "set_schedule")
+    for ps, vz in outs[1:]:
+        assert torch.equal(ps, outs[0][0]) and torch.equal(vz, outs[0][1])
+
+
+def test_autotune_picks_a_valid_schedule(cuda, tmp_path, monkeypatch):
+    print("autotune n=512", flush=True)
+    monkeypatch.setenv("R2D3C_SCHEDULE_CACHE", str(tmp_path / "s.json"))
+    from rot2d3c_amd import autotune
+    rw, bpc = autotune.schedule(512, cuda, reps=2)
+    assert rw in (1, 2) and bpc in (1, 2)
+    assert autotune.schedule(512, cuda) == (rw, bpc)  # from the cache
+    assert (tmp_path / "s.json").exists()
