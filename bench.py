@@ -78,6 +78,38 @@ def cpu_baseline(n, steps, p, kf, f0, seed, ps0, vz0, dt):
     return time.perf_counter() - t0
 
 
+def ensemble_bench(steps, dev, tuned, warmup=10):
+    """The batched workload: 24 x 256^2 + 12 x 512^2 + 4 x 1024^2 independent
+    runs (per-run seed, Omega, nu, forcing shell, fixed dt; see
+    rot2d3c_amd.ensemble.mixed_ensemble_spec) advanced together, one captured
+    HIP graph per step.  Returns the throughput summed over the runs."""
+    import torch
+    from rot2d3c_amd import ensemble as E
+    spec = E.mixed_ensemble_spec()
+    ens = E.build(spec, dev, schedules="auto" if tuned else None)
+    table = E.forcing_table(spec, warmup + steps)
+    ens.run(warmup, table[:warmup])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.run(steps, table[warmup:])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    finite = all(bool(torch.isfinite(torch.view_as_real(m.ps)).all()) for m in ens.members)
+    ens.close()
+    pts = E.points(spec)
+    alg = sum(algorithmic_bytes(r["n"], steps)[1] for r in spec)
+    b76 = sum(algorithmic_bytes(r["n"], steps)[2]["B76S"] for r in spec)
+    ms = 1e3 * wall / steps
+    return {"workload": "24 x 256^2 + 12 x 512^2 + 4 x 1024^2 FP64 runs, own seed, Omega~U[0,10], "
+                        "nu=1e-3*256/n, kf=n/32, fixed dt=1e-3*256/n",
+            "runs": len(spec), "points": pts, "steps": steps, "ms_per_step": round(ms, 4),
+            "value": round(pts * steps / wall / 1e6, 2), "unit": "Mpts·RK4-steps/s",
+            "algorithmic_bytes_per_step": alg,
+            "frac_of_peak": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+            "model_76S_frac_of_peak": round(b76 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+            "finite": finite}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +120,11 @@ def main():
                     help="steps of the oracle timed on the host (0 disables)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 512 MiB device copy (PMC byte calibration)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default row schedule instead of the cached autotuned one")
+    ap.add_argument("--ensemble-steps", type=int, default=500,
+                    help="steps of the batched ensemble (24 x 256^2 + 12 x 512^2 + 4 x 1024^2); "
+                         "0 disables")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -112,6 +149,10 @@ def main():
     ps = torch.from_numpy(ps0).to(dev)
     vz = torch.from_numpy(vz0).to(dev)
     plan = S.Plan.get(n, dev)
+    schedule = None
+    if not args.no_autotune:
+        from rot2d3c_amd import autotune
+        schedule = autotune.apply(plan.handle, n, dev)
 
     def run(k0, k):
         # r2d3c_run: k chained steps (bitwise equal to k single steps)
@@ -176,6 +217,12 @@ def main():
                "sample": "%d steps of the same %d^2 workload through the numpy oracle "
                          "(pocketfft, single thread): %.1f s" % (args.cpu_steps, n, el)}
 
+    ens = None
+    if args.ensemble_steps > 0:
+        del ps, vz
+        torch.cuda.empty_cache()
+        ens = ensemble_bench(args.ensemble_steps, dev, not args.no_autotune)
+
     out = {
         "metric": "Mpts·RK4-steps/s at 2048² FP64 (%HBM peak); closure-vs-DNS speedup at 5% err",
         "value": round(value, 2),
@@ -191,7 +238,8 @@ def main():
         "data": "synthetic (seeded random-phase initial condition, seeded random forcing)",
         "config": {"workload": "Rot2D3C RK4 step, %d^2 FP64, dt=2e-4 fixed, nu=5e-5, nu+=0.5, "
                                "Omega=5, forcing kf=40 (psi and vz), injection 1.0" % n,
-                   "grid": n, "dt": dt, "parallelism": "single GPU"},
+                   "grid": n, "dt": dt, "parallelism": "single GPU",
+                   "row_schedule": list(schedule) if schedule else [2, 1]},
         "roofline": roof,
         "cpu_baseline": cpu,
         "step_hbm": {"algorithmic_bytes_per_step": step_bytes,
@@ -203,6 +251,7 @@ def main():
         "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kern.items()},
         "finite": finite,
+        "ensemble": ens,
     }
     print(json.dumps(out), flush=True)
 

@@ -229,8 +229,8 @@ void r2d3c_rand_force_draw(double kup, int *seed, double *kx, double *ky, double
  * schedule gives bit-identical results; only the speed differs.  Default 2, 1. */
 int r2d3c_plan_set_schedule(r2d3c_plan *plan, int row_rows, int row_blocks_per_cu);
 int r2d3c_plan_get_schedule(const r2d3c_plan *plan, int *row_rows, int *row_blocks_per_cu);
-/* Times `reps` fixed-dt steps of every valid schedule on scratch state, keeps
- * the fastest in the plan and returns its milliseconds per step (may be NULL).
+/* Times every valid schedule on scratch state (best of three trials of
+ * `reps` fixed-dt steps each), keeps the fastest in the plan and returns its milliseconds per step (may be NULL).
  * Allocates and frees its scratch; synchronizes the stream. */
 int r2d3c_autotune(r2d3c_plan *plan, int reps, double *ms_per_step, void *stream);
 

@@ -52,8 +52,9 @@ def _store(table):
         pass  # a read-only home only costs a re-tune next time
 
 
-def tune(plan_handle, reps=5, device=None):
-    """Time every schedule on the plan (scratch state, `reps` steps each),
+def tune(plan_handle, reps=20, device=None):
+    """Time every schedule on the plan (scratch state, best of three trials
+    of `reps` steps each),
     keep the fastest in the plan and return ((rows, blocks_per_cu), ms/step)."""
     ms = ctypes.c_double()
     stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
@@ -65,7 +66,7 @@ def tune(plan_handle, reps=5, device=None):
     return (rw.value, bpc.value), ms.value
 
 
-def schedule(n, device, reps=5, retune=False):
+def schedule(n, device, reps=20, retune=False):
     """The cached schedule of grid size n on this device, tuned on first use."""
     from .spectral import Plan
     device = torch.device(device)
@@ -82,7 +83,7 @@ def schedule(n, device, reps=5, retune=False):
         return rw, bpc
 
 
-def apply(plan_handle, n, device, reps=5):
+def apply(plan_handle, n, device, reps=20):
     """Set the cached (or freshly tuned) schedule of size n on a plan."""
     rw, bpc = schedule(n, device, reps)
     check(_lib.lib().r2d3c_plan_set_schedule(plan_handle, rw, bpc), "r2d3c_plan_set_schedule")
