@@ -263,6 +263,54 @@ int r2d3c_ensemble_set_schedule(r2d3c_ensemble *ens, int n, int row_rows, int ro
 int r2d3c_ensemble_run(r2d3c_ensemble *ens, const r2d3c_forcing *forcing, int n_steps,
                        void *stream);
 
+/* ---- learned closure (no reference counterpart) ------------------------ */
+
+/* Coarse-graining: box filter of width `width` (transfer function
+ * sinc(kx w/2) sinc(ky w/2); 0 = none) and truncation of a spectral field of
+ * plan `fine` to the dealiased modes of plan `coarse` (k^2 <= (nc/3)^2),
+ * coefficients rescaled to the coarse grid's unnormalised convention. */
+int r2d3c_filter(r2d3c_plan *fine, r2d3c_plan *coarse, const double *in, double *out,
+                 double width, void *stream);
+/* Network inputs on plan p's grid: fp32 [n*n][3] = (laplacian of psi, vz, psi)
+ * in real space (physical units) times scale[3] (host array). */
+int r2d3c_closure_inputs(r2d3c_plan *plan, const double *psi, const double *vz,
+                         const double *scale, float *out, void *stream);
+/* Training targets: fp32 [n*n][2] = real-space (f5 - c5, f6 - c6) restricted to
+ * the update mask, times scale[2]; f5, f6 the coarse-grained fine brackets
+ * poisson(psi, laplak2(psi)) and poisson(psi, vz), c5, c6 the same brackets of
+ * the coarse-grained fields. */
+int r2d3c_closure_targets(r2d3c_plan *plan, const double *f5, const double *c5, const double *f6,
+                          const double *c6, const double *scale, float *out, void *stream);
+/* Dense forcing of the closure from a prediction fp32 [n*n][2] (times
+ * scale[2]): fp = -r2c(pred0)/k^2 and fz = r2c(pred1) on the update mask (the
+ * subgrid parts of -C5/k^2 and C6 in the stage update). */
+int r2d3c_closure_forcing(r2d3c_plan *plan, const float *pred, const double *scale, double *fp,
+                          double *fz, void *stream);
+
+/* The closure network: 6 periodic 3x3 conv layers, 3 -> 32 -> ... -> 32 -> 2
+ * channels, GELU between layers, BF16 weights and activations, FP32
+ * accumulation (MFMA), on batches of up to `batch` images of height x width
+ * (width a multiple of 64).  Parameters: fp32, layer by layer W[9][co][ci] then
+ * b[co] with ci, co padded to 16/32 (r2d3c_cnn_layer_offsets; the padding stays
+ * zero).  Images: fp32 [batch][height][width][channels]. */
+typedef struct r2d3c_cnn r2d3c_cnn;
+int r2d3c_cnn_param_count(void);
+int r2d3c_cnn_layer_offsets(int *w_offsets, int *b_offsets, int *cin, int *cout);
+int r2d3c_cnn_create(r2d3c_cnn **net, int batch, int height, int width);
+int r2d3c_cnn_destroy(r2d3c_cnn *net);
+/* Copies device fp32 parameters in (and resets the optimiser) or out. */
+int r2d3c_cnn_set_params(r2d3c_cnn *net, const float *d_params, void *stream);
+int r2d3c_cnn_get_params(r2d3c_cnn *net, float *d_params, void *stream);
+/* out [batch][h][w][2] = network(in [batch][h][w][3]). */
+int r2d3c_cnn_forward(r2d3c_cnn *net, const float *d_in, int batch, float *d_out, void *stream);
+/* Forward, MSE loss (mean over pixels and both output channels, in d_loss),
+ * backward; the parameter gradient goes to d_grad when it is not NULL. */
+int r2d3c_cnn_gradients(r2d3c_cnn *net, const float *d_in, const float *d_target, int batch,
+                        float *d_grad, double *d_loss, void *stream);
+/* r2d3c_cnn_gradients followed by one Adam step (beta 0.9, 0.999, eps 1e-8). */
+int r2d3c_cnn_train_step(r2d3c_cnn *net, const float *d_in, const float *d_target, int batch,
+                         float lr, double *d_loss, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,20 @@ SIGNATURES = {
     "r2d3c_ensemble_destroy": (_c_int, [_vp]),
     "r2d3c_ensemble_set_schedule": (_c_int, [_vp, _c_int, _c_int, _c_int]),
     "r2d3c_ensemble_run": (_c_int, [_vp, ctypes.POINTER(Forcing), _c_int, _vp]),
+    "r2d3c_filter": (_c_int, [_vp, _vp, _vp, _vp, _c_double, _vp]),
+    "r2d3c_closure_inputs": (_c_int, [_vp, _vp, _vp, ctypes.POINTER(_c_double), _vp, _vp]),
+    "r2d3c_closure_targets": (_c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_c_double), _vp,
+                                       _vp]),
+    "r2d3c_closure_forcing": (_c_int, [_vp, _vp, ctypes.POINTER(_c_double), _vp, _vp, _vp]),
+    "r2d3c_cnn_param_count": (_c_int, []),
+    "r2d3c_cnn_layer_offsets": (_c_int, [ctypes.POINTER(_c_int)] * 4),
+    "r2d3c_cnn_create": (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int]),
+    "r2d3c_cnn_destroy": (_c_int, [_vp]),
+    "r2d3c_cnn_set_params": (_c_int, [_vp, _vp, _vp]),
+    "r2d3c_cnn_get_params": (_c_int, [_vp, _vp, _vp]),
+    "r2d3c_cnn_forward": (_c_int, [_vp, _vp, _c_int, _vp, _vp]),
+    "r2d3c_cnn_gradients": (_c_int, [_vp, _vp, _vp, _c_int, _vp, _vp, _vp]),
+    "r2d3c_cnn_train_step": (_c_int, [_vp, _vp, _vp, _c_int, ctypes.c_float, _vp, _vp]),
     "r2d3c_randu": (ctypes.c_float, [ctypes.POINTER(_c_int)]),
     "r2d3c_rand_force_draw": (None, [_c_double, ctypes.POINTER(_c_int),
                                      ctypes.POINTER(_c_double), ctypes.POINTER(_c_double),

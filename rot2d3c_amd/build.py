@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librot2d3c.so")
-SOURCES = [os.path.join(CSRC, "rot2d3c.hip")]
+SOURCES = [os.path.join(CSRC, "rot2d3c.hip"), os.path.join(CSRC, "closure.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -1271,6 +1271,8 @@ static int set_err(int code, const std::string &msg) {
   g_err = msg;
   return code;
 }
+// for the library's other translation units (closure.hip)
+int r2d3c_internal_set_error(int code, const char *msg) { return set_err(code, msg); }
 static int hip_err(hipError_t e, const char *where) {
   return set_err(-(R2D3C_EHIP_BASE + (int)e), std::string(where) + ": " + hipGetErrorString(e));
 }
@@ -2325,6 +2327,147 @@ int r2d3c_ensemble_run(r2d3c_ensemble *e, const r2d3c_forcing *forcing, int n_st
   }
   HIPCHK(hipEventRecord(e->ev_out, e->es));
   HIPCHK(hipStreamWaitEvent(s, e->ev_out, 0));
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// learned closure: coarse-graining and the closure's inputs and forcing
+// ---------------------------------------------------------------------------
+// Box filter of width h (transfer function sinc(kx h/2) sinc(ky h/2), 1 when
+// h = 0) and truncation of a fine spectral field to the coarse grid's dealiased
+// modes (k^2 <= (nc/3)^2); coefficients rescaled from nf^2 to nc^2 (the
+// unnormalised convention).
+__device__ __forceinline__ double box_tf(double k, double h) {
+  const double a = 0.5 * k * h;
+  return a == 0.0 ? 1.0 : sin(a) / a;
+}
+__global__ void k_filter(const cplx *__restrict__ fine, int nf, cplx *__restrict__ coarse, int nc,
+                         double kmax, double h, double scale) {
+  const size_t total = (size_t)(nc / 2 + 1) * nc;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / nc), j = (int)(e % nc);
+    const int kx = kxof(i, nc), ky = kyof(j, nc);
+    const double ka2 = (double)kx * kx + (double)ky * ky;
+    cplx v = mk(0.0, 0.0);
+    if (ka2 <= kmax && kx >= 0) {
+      const int jf = ky >= 0 ? ky : ky + nf;
+      const cplx a = fine[(size_t)kx * nf + jf];
+      const double g = scale * box_tf((double)kx, h) * box_tf((double)ky, h);
+      v = mk(a.x * g, a.y * g);
+    }
+    coarse[e] = v;
+  }
+}
+
+// closure spectra: mode 0 out = -(a - b)/k^2, mode 1 out = a - b, both only
+// inside the update mask tiny < k^2 <= kmax (b may be NULL)
+__global__ void k_closure_spec(const cplx *__restrict__ a, const cplx *__restrict__ b,
+                               cplx *__restrict__ out, int n, double kmax, double tiny, int mode) {
+  const size_t total = (size_t)(n / 2 + 1) * n;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    const int kx = kxof(i, n), ky = kyof(j, n);
+    const double ka2 = (double)kx * kx + (double)ky * ky;
+    cplx v = mk(0.0, 0.0);
+    if (ka2 <= kmax && ka2 > tiny) {
+      v = a[e];
+      if (b) v = mk(v.x - b[e].x, v.y - b[e].y);
+      if (mode == 0) v = mk(-v.x / ka2, -v.y / ka2);
+    }
+    out[e] = v;
+  }
+}
+
+// channel ch of a [n*n][nch] fp32 array <-> an n*n real field, times scale
+__global__ void k_chan_pack(const double *__restrict__ r, size_t nn, double scale, int ch, int nch,
+                            float *__restrict__ out) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn;
+       e += (size_t)gridDim.x * blockDim.x)
+    out[e * nch + ch] = (float)(r[e] * scale);
+}
+__global__ void k_chan_unpack(const float *__restrict__ in, size_t nn, double scale, int ch, int nch,
+                              double *__restrict__ r) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn;
+       e += (size_t)gridDim.x * blockDim.x)
+    r[e] = (double)in[e * nch + ch] * scale;
+}
+
+extern "C" {
+
+int r2d3c_filter(r2d3c_plan *fine, r2d3c_plan *coarse, const double *in, double *out, double width,
+                 void *stream) {
+  NEED(fine); NEED(coarse); NEED(in); NEED(out);
+  const int nf = fine->n, nc = coarse->n;
+  if (nc > nf) return set_err(R2D3C_EINVAL, "filter: the coarse grid must not be finer");
+  if (width < 0.0) return set_err(R2D3C_EINVAL, "filter: width must be >= 0");
+  const double sc = ((double)nc * nc) / ((double)nf * nf);
+  const size_t total = (size_t)(nc / 2 + 1) * nc;
+  k_filter<<<launch_grid(total), 256, 0, (hipStream_t)stream>>>((const cplx *)in, nf, (cplx *)out,
+                                                                  nc, coarse->g.kmax, width, sc);
+  LAUNCHCHK("filter");
+  return 0;
+}
+
+int r2d3c_closure_inputs(r2d3c_plan *p, const double *psi, const double *vz, const double *scale,
+                         float *out, void *stream) {
+  NEED(p); NEED(psi); NEED(vz); NEED(scale); NEED(out);
+  hipStream_t s = (hipStream_t)stream;
+  const int n = p->n;
+  const size_t sz = (size_t)(n / 2 + 1) * n, nn = (size_t)n * n;
+  const double inv = 1.0 / ((double)n * n);
+  cplx *t = p->spec3;
+  // channel 0: laplacian of psi, 1: vz, 2: psi (real space, physical units)
+  k_spec_op<<<launch_grid(sz), 256, 0, s>>>((const cplx *)psi, t, OP_LAP, n, p->g.kmax, p->g.tiny);
+  const double *src[3] = {(const double *)t, vz, psi};
+  for (int ch = 0; ch < 3; ++ch) {
+    int rc = r2d3c_c2r(p, src[ch], p->real, stream);
+    if (rc) return rc;
+    k_chan_pack<<<launch_grid(nn), 256, 0, s>>>(p->real, nn, scale[ch] * inv, ch, 3, out);
+  }
+  LAUNCHCHK("closure inputs");
+  return 0;
+}
+
+int r2d3c_closure_targets(r2d3c_plan *p, const double *f5, const double *c5, const double *f6,
+                          const double *c6, const double *scale, float *out, void *stream) {
+  NEED(p); NEED(f5); NEED(c5); NEED(f6); NEED(c6); NEED(scale); NEED(out);
+  hipStream_t s = (hipStream_t)stream;
+  const int n = p->n;
+  const size_t sz = (size_t)(n / 2 + 1) * n, nn = (size_t)n * n;
+  const double inv = 1.0 / ((double)n * n);
+  cplx *t = p->spec3;
+  const double *a[2] = {f5, f6}, *b[2] = {c5, c6};
+  for (int ch = 0; ch < 2; ++ch) {
+    k_closure_spec<<<launch_grid(sz), 256, 0, s>>>((const cplx *)a[ch], (const cplx *)b[ch], t, n,
+                                                    p->g.kmax, p->g.tiny, 1);
+    int rc = r2d3c_c2r(p, (const double *)t, p->real, stream);
+    if (rc) return rc;
+    k_chan_pack<<<launch_grid(nn), 256, 0, s>>>(p->real, nn, scale[ch] * inv, ch, 2, out);
+  }
+  LAUNCHCHK("closure targets");
+  return 0;
+}
+
+int r2d3c_closure_forcing(r2d3c_plan *p, const float *pred, const double *scale, double *fp,
+                          double *fz, void *stream) {
+  NEED(p); NEED(pred); NEED(scale); NEED(fp); NEED(fz);
+  hipStream_t s = (hipStream_t)stream;
+  const int n = p->n;
+  const size_t sz = (size_t)(n / 2 + 1) * n, nn = (size_t)n * n;
+  cplx *t = p->spec3;
+  double *dst[2] = {fp, fz};
+  for (int ch = 0; ch < 2; ++ch) {
+    k_chan_unpack<<<launch_grid(nn), 256, 0, s>>>(pred, nn, scale[ch], ch, 2, p->real);
+    int rc = r2d3c_r2c(p, p->real, (double *)t, stream);
+    if (rc) return rc;
+    k_closure_spec<<<launch_grid(sz), 256, 0, s>>>(t, nullptr, (cplx *)dst[ch], n, p->g.kmax,
+                                                    p->g.tiny, ch == 0 ? 0 : 1);
+  }
+  LAUNCHCHK("closure forcing");
   return 0;
 }
 

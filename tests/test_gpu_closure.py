@@ -1,0 +1,160 @@
+"""GPU: the learned closure's kernels (rot2d3c_amd/csrc/closure.hip and the
+closure helpers of rot2d3c.hip) against the numpy oracles."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import random_state, rel_l2
+from oracle import cnn as C
+from oracle import spectral as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(cuda, B, H, W, params):
+    from rot2d3c_amd import closure as L
+    return L.Net(B, H, W, cuda, params=params.astype(np.float32))
+
+
+def test_param_layout_matches_oracle(cuda):
+    from rot2d3c_amd import closure as L
+    wo, bo, ci, co = L.layer_layout()
+    offs, total = C.offsets()
+    assert [o[0] for o in offs] == wo and [o[1] for o in offs] == bo
+    assert ci == C.CI and co == C.CO
+    from rot2d3c_amd import _lib
+    assert _lib.lib().r2d3c_cnn_param_count() == total
+
+
+def test_cnn_forward_matches_oracle(cuda):
+    """BF16 MFMA forward against the oracle with the same bf16 rounding points;
+    the remaining differences are FP32 vs FP64 accumulation."""
+    print("cnn forward", flush=True)
+    rng = np.random.default_rng(3)
+    B, H, W = 2, 16, 64
+    p = C.init_params(rng)
+    x = rng.standard_normal((B, H, W, 3))
+    net = _net(cuda, B, H, W, p)
+    got = net.forward(torch.from_numpy(x.astype(np.float32)).to(cuda)).cpu().numpy()
+    ref, _ = C.forward(p, x.astype(np.float32).astype(np.float64), emulate=True)
+    assert rel_l2(got, ref) < 5e-3
+    exact, _ = C.forward(p, x, emulate=False)
+    assert rel_l2(got, exact) < 5e-2  # bf16 network vs the exact one
+    net.close()
+
+
+def test_cnn_gradients_match_oracle(cuda):
+    """Loss and every layer's weight and bias gradient of the GPU backward
+    pass against the oracle's (same bf16 rounding points)."""
+    print("cnn gradients", flush=True)
+    rng = np.random.default_rng(4)
+    B, H, W = 2, 8, 64
+    p = C.init_params(rng)
+    x = rng.standard_normal((B, H, W, 3)).astype(np.float32)
+    y = rng.standard_normal((B, H, W, 2)).astype(np.float32)
+    net = _net(cuda, B, H, W, p)
+    loss, g = net.gradients(torch.from_numpy(x).to(cuda), torch.from_numpy(y).to(cuda))
+    g = g.cpu().numpy().astype(np.float64)
+    rloss, rg = C.loss_and_grad(p, x.astype(np.float64), y.astype(np.float64), emulate=True)
+    assert abs(loss - rloss) <= 1e-4 * rloss
+    offs, _ = C.offsets()
+    for l, (wo, bo) in enumerate(offs):
+        assert rel_l2(g[wo:bo], rg[wo:bo]) < 2e-2, ("W", l)
+        assert rel_l2(g[bo:bo + C.CO[l]], rg[bo:bo + C.CO[l]]) < 2e-2, ("b", l)
+    # the padding's gradient is exactly zero
+    W0 = g[offs[0][0]:offs[0][1]].reshape(9, 32, 16)
+    assert np.all(W0[:, :, 3:] == 0)
+    net.close()
+
+
+def test_cnn_training_fits_a_teacher(cuda):
+    """Adam on the GPU fits the outputs of a random teacher network."""
+    print("cnn training", flush=True)
+    rng = np.random.default_rng(5)
+    B, H, W = 4, 16, 64
+    teacher = C.init_params(rng, scale=0.5)
+    x = rng.standard_normal((B, H, W, 3))
+    y, _ = C.forward(teacher, x, emulate=False)
+    net = _net(cuda, B, H, W, C.init_params(np.random.default_rng(6)))
+    xt = torch.from_numpy(x.astype(np.float32)).to(cuda)
+    yt = torch.from_numpy(y.astype(np.float32)).to(cuda)
+    l0, _ = net.gradients(xt, yt)
+    for _ in range(150):
+        net.train_step(xt, yt, 2e-3)
+    l1, _ = net.gradients(xt, yt)
+    assert l1 < 0.2 * l0, (l0, l1)
+    net.close()
+
+
+def test_filter_of_a_single_mode(cuda):
+    """G(cos(kx x + ky y)) on the coarse grid is sinc(kx h/2) sinc(ky h/2)
+    cos(kx x + ky y) for a mode inside the coarse dealiased disc and 0 outside."""
+    print("filter", flush=True)
+    from rot2d3c_amd import closure as L
+    nf, nc = 64, 16
+    h = 2 * math.pi / nc
+    xf = 2 * math.pi * np.arange(nf) / nf
+    xc = 2 * math.pi * np.arange(nc) / nc
+    sinc = lambda a: 1.0 if a == 0 else math.sin(a) / a
+    for kx, ky, inside in ((3, -2, True), (0, 4, True), (6, 1, False)):
+        R = np.cos(kx * xf[None, :] + ky * xf[:, None])
+        a = torch.from_numpy(O.fftp2d_real_to_complex(R)).to(cuda)
+        out = L.coarse_grain(a, nc, h).cpu().numpy()
+        got = O.fftp2d_complex_to_real(out) / float(nc * nc)
+        expect = (sinc(kx * h / 2) * sinc(ky * h / 2) if inside else 0.0) * \
+            np.cos(kx * xc[None, :] + ky * xc[:, None])
+        assert np.max(np.abs(got - expect)) < 1e-12, (kx, ky)
+
+
+def test_closure_inputs_targets_and_forcing(cuda):
+    print("closure inputs/targets/forcing", flush=True)
+    from rot2d3c_amd import closure as L
+    from rot2d3c_amd import spectral as S
+    n = 32
+    ps, vz = random_state(n, 7)
+    g = O.grid(n)
+    n2 = float(n * n)
+    inp = L.network_inputs(torch.from_numpy(ps).to(cuda), torch.from_numpy(vz).to(cuda),
+                           (2.0, 3.0, 5.0)).cpu().numpy()
+    ref = [O.fftp2d_complex_to_real(O.laplak2(ps)) / n2 * 2.0, O.fftp2d_complex_to_real(vz) / n2 * 3.0,
+           O.fftp2d_complex_to_real(ps) / n2 * 5.0]
+    for ch in range(3):
+        assert rel_l2(inp[..., ch], ref[ch]) < 1e-6
+    rng = np.random.default_rng(8)
+    f5, c5, f6, c6 = [random_state(n, 20 + k)[0] * rng.uniform(0.5, 2) for k in range(4)]
+    upd = (g.ka2 <= g.kmax) & (g.ka2 > g.tiny)
+    pl = S.Plan.get(n, cuda)
+    dev = [torch.from_numpy(a).to(cuda) for a in (f5, c5, f6, c6)]
+    tgt = torch.empty((n, n, 2), dtype=torch.float32, device=cuda)
+    from rot2d3c_amd import _lib
+    _lib.check(_lib.lib().r2d3c_closure_targets(pl.handle, *[S._ptr(t) for t in dev],
+                                                L._dvec((1.0, 1.0)), L._fptr(tgt), S._stream(cuda)),
+               "targets")
+    t5 = np.where(upd, f5 - c5, 0)
+    t6 = np.where(upd, f6 - c6, 0)
+    got = tgt.cpu().numpy()
+    assert rel_l2(got[..., 0], O.fftp2d_complex_to_real(t5) / n2) < 1e-6
+    assert rel_l2(got[..., 1], O.fftp2d_complex_to_real(t6) / n2) < 1e-6
+    fp, fz = L.closure_forcing(tgt, n, (1.0, 1.0))
+    ka2 = np.where(upd, g.ka2, 1.0)
+    assert rel_l2(fp.cpu().numpy(), np.where(upd, -t5 / ka2, 0)) < 1e-6
+    assert rel_l2(fz.cpu().numpy(), t6) < 1e-6
+
+
+def test_closure_run_without_network_is_the_plain_solver(cuda):
+    print("closure run fallback", flush=True)
+    from rot2d3c_amd import closure as L
+    from rot2d3c_amd import spectral as S
+    n = 64
+    ps, vz = random_state(n, 9)
+    p = S.Params(1e-3, 0.1, 1, 2, 1e-3, 0.1, 1, 2, 1.0)
+    a = (torch.from_numpy(ps).to(cuda), torch.from_numpy(vz).to(cuda))
+    b = (a[0].clone(), a[1].clone())
+    run = L.ClosureRun(a[0], a[1], p)
+    for _ in range(3):
+        run.step(dt=1e-3)
+        S.rk4_step(b[0], b[1], p, dt=1e-3)
+    torch.cuda.synchronize()
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
