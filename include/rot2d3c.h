@@ -303,6 +303,13 @@ int r2d3c_cnn_set_params(r2d3c_cnn *net, const float *d_params, void *stream);
 int r2d3c_cnn_get_params(r2d3c_cnn *net, float *d_params, void *stream);
 /* out [batch][h][w][2] = network(in [batch][h][w][3]). */
 int r2d3c_cnn_forward(r2d3c_cnn *net, const float *d_in, int batch, float *d_out, void *stream);
+/* The network's own input and output buffers, for callers that fill and read
+ * them in place: input bf16 [batch][h][w][16] (channels 0..2 used, 3..15
+ * zero) and the last layer's fp32 [batch][h][w][16] (channels 0, 1 are the
+ * prediction); r2d3c_cnn_forward_buffers runs the layers on them. */
+int r2d3c_cnn_buffers(r2d3c_cnn *net, void **input_bf16, const float **output_f32);
+int r2d3c_cnn_shape(const r2d3c_cnn *net, int *batch, int *height, int *width);
+int r2d3c_cnn_forward_buffers(r2d3c_cnn *net, int batch, void *stream);
 /* Forward, MSE loss (mean over pixels and both output channels, in d_loss),
  * backward; the parameter gradient goes to d_grad when it is not NULL. */
 int r2d3c_cnn_gradients(r2d3c_cnn *net, const float *d_in, const float *d_target, int batch,
@@ -310,6 +317,26 @@ int r2d3c_cnn_gradients(r2d3c_cnn *net, const float *d_in, const float *d_target
 /* r2d3c_cnn_gradients followed by one Adam step (beta 0.9, 0.999, eps 1e-8). */
 int r2d3c_cnn_train_step(r2d3c_cnn *net, const float *d_in, const float *d_target, int batch,
                          float lr, double *d_loss, void *stream);
+
+/* The closure's input and forcing steps fused around a network's own
+ * buffers (see r2d3c_cnn_buffers): r2d3c_closure_net_inputs writes the
+ * network input of (psi, vz) (as r2d3c_closure_inputs, rounded to bf16);
+ * after r2d3c_cnn_forward_buffers, r2d3c_closure_net_forcing turns the
+ * prediction into fp, fz (as r2d3c_closure_forcing).  n >= 64. */
+int r2d3c_closure_net_inputs(r2d3c_plan *plan, const double *psi, const double *vz,
+                             const double *scale, r2d3c_cnn *net, void *stream);
+int r2d3c_closure_net_forcing(r2d3c_plan *plan, r2d3c_cnn *net, const double *scale, double *fp,
+                              double *fz, void *stream);
+
+/* Makes run `run` of an ensemble a closure run: each of its steps starts with
+ * r2d3c_closure_net_inputs(scale_in) -> r2d3c_cnn_forward_buffers ->
+ * r2d3c_closure_net_forcing(scale_out) on the step's state, and the stages add that forcing (held for
+ * the four stages), all inside the step's graph.  net = NULL removes the
+ * closure (the plain solver).  The net must take 1 image of n x n, serve
+ * no other run (its buffers hold this run's activations) and stay alive while
+ * the ensemble uses it. */
+int r2d3c_ensemble_set_closure(r2d3c_ensemble *ens, int run, r2d3c_cnn *net,
+                               const double *scale_in, const double *scale_out);
 
 #ifdef __cplusplus
 }

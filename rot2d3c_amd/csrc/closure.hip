@@ -61,62 +61,99 @@ __device__ __forceinline__ float gelu_grad(float z) {
 
 enum { CONV_GELU = 0, CONV_LINEAR = 1, CONV_BWD = 2 };
 
-// One wave computes 16 consecutive pixels of one row for all CO channels:
-// D[p][co] = sum_t sum_ci X[p + d(t)][ci] Wt[t][co][ci], K = 9 CI as 9*CI/16
-// MFMA steps.  Operand layout of v_mfma_f32_16x16x16_bf16 (lane l, m = l%16,
-// q = l/16): A[m][4q..4q+3] = the pixel m's channels, B[4q..4q+3][m] = weights
-// of output channel m; D[4q+i][m] = pixel 4q+i, channel m.
+// A block of kConvWaves waves computes kConvWaves * kTilesPerWave tiles of 16
+// consecutive pixels of one row for all CO channels: D[p][co] = sum_t sum_ci X[p + d(t)][ci]
+// Wt[t][co][ci], K = 9 CI as 9*CI/16 MFMA steps.  The block first stages the
+// layer's weights in LDS (rows padded by 4 to spread the banks); each weight
+// fragment is then read once per wave and used for its kTilesPerWave tiles.  Operand
+// layout of v_mfma_f32_16x16x16_bf16 (lane l, m = l%16, q = l/16):
+// A[m][4q..4q+3] = pixel m's channels, B[4q..4q+3][m] = weights of output
+// channel m; D[4q+i][m] = pixel 4q+i, channel m.
 //   CONV_GELU / CONV_LINEAR: Z = D + bias (stored when Zout), Y = bf16(gelu(Z) or Z)
 //   CONV_BWD: Y = bf16(D * gelu'(Zprev))  (the next lower layer's dZ)
+#ifndef R2D_CONV_TPW
+#define R2D_CONV_TPW 1
+#endif
+#ifndef R2D_CONV_WPB
+#define R2D_CONV_WPB 16
+#endif
+constexpr int kTilesPerWave = R2D_CONV_TPW, kConvWaves = R2D_CONV_WPB;
 template <int CI, int CO, int MODE>
-__global__ __launch_bounds__(256) void k_conv3x3(const bf16 *__restrict__ X,
+__global__ __launch_bounds__(64 * kConvWaves) void k_conv3x3(const bf16 *__restrict__ X,
                                                   const bf16 *__restrict__ Wt,
                                                   const float *__restrict__ bias,
                                                   const float *__restrict__ Zprev,
                                                   float *__restrict__ Zout, bf16 *__restrict__ Y,
                                                   int B, int H, int W) {
-  constexpr int CI16 = CI / 16, CO16 = CO / 16;
+  constexpr int CI16 = CI / 16, CO16 = CO / 16, WS = CI + 4, TW = kTilesPerWave;
+  __shared__ bf16 ws[9 * CO * WS];
+  for (int e = threadIdx.x; e < 9 * CO * (CI / 4); e += 64 * kConvWaves) {
+    const int row = e / (CI / 4), c4 = e - row * (CI / 4);
+    *(uint2 *)(ws + row * WS + 4 * c4) = *(const uint2 *)(Wt + (size_t)row * CI + 4 * c4);
+  }
+  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tpr = W / 16;
-  const long tile = (long)blockIdx.x * 4 + wave;
-  if (tile >= (long)B * H * tpr) return;  // whole wave
-  const int b = (int)(tile / ((long)H * tpr));
-  const int rem = (int)(tile - (long)b * H * tpr);
-  const int y = rem / tpr, x0 = (rem - y * tpr) * 16;
   const int m = lane & 15, q = lane >> 4;
-  f32x4 acc[CO16];
+  const int tpr = W / 16;
+  const long ntiles = (long)B * H * tpr;
+  int tb[TW], ty[TW], tx[TW];
+  bool ok[TW];
 #pragma unroll
-  for (int n = 0; n < CO16; ++n) acc[n] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int i = 0; i < TW; ++i) {
+    const long tile = ((long)blockIdx.x * kConvWaves + wave) * TW + i;
+    ok[i] = tile < ntiles;
+    const long tt = ok[i] ? tile : 0;
+    tb[i] = (int)(tt / ((long)H * tpr));
+    const int rem = (int)(tt - (long)tb[i] * H * tpr);
+    ty[i] = rem / tpr;
+    tx[i] = (rem - ty[i] * tpr) * 16;
+  }
+  f32x4 acc[TW][CO16];
+#pragma unroll
+  for (int i = 0; i < TW; ++i)
+#pragma unroll
+    for (int n = 0; n < CO16; ++n) acc[i][n] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int t = 0; t < 9; ++t) {
     const int dy = t / 3 - 1, dx = t % 3 - 1;
-    int yy = y + dy, xx = x0 + m + dx;
-    yy += yy < 0 ? H : (yy >= H ? -H : 0);
-    xx += xx < 0 ? W : (xx >= W ? -W : 0);
-    const bf16 *xp = X + ((size_t)((size_t)b * H + yy) * W + xx) * CI + 4 * q;
 #pragma unroll
     for (int c = 0; c < CI16; ++c) {
-      const s16x4 a = *(const s16x4 *)(xp + 16 * c);
+      s16x4 a[TW];
+#pragma unroll
+      for (int i = 0; i < TW; ++i) {
+        int yy = ty[i] + dy, xx = tx[i] + m + dx;
+        yy += yy < 0 ? H : (yy >= H ? -H : 0);
+        xx += xx < 0 ? W : (xx >= W ? -W : 0);
+        a[i] = ok[i] ? *(const s16x4 *)(X + ((size_t)((size_t)tb[i] * H + yy) * W + xx) * CI +
+                                        16 * c + 4 * q)
+                     : s16x4{0, 0, 0, 0};
+      }
 #pragma unroll
       for (int n = 0; n < CO16; ++n) {
-        const s16x4 w = *(const s16x4 *)(Wt + ((size_t)t * CO + 16 * n + m) * CI + 16 * c + 4 * q);
-        acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w, acc[n], 0, 0, 0);
+        const s16x4 w = *(const s16x4 *)(ws + (t * CO + 16 * n + m) * WS + 16 * c + 4 * q);
+#pragma unroll
+        for (int i = 0; i < TW; ++i)
+          acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], w, acc[i][n], 0, 0, 0);
       }
     }
   }
 #pragma unroll
-  for (int n = 0; n < CO16; ++n) {
-    const int co = 16 * n + m;
+  for (int i = 0; i < TW; ++i) {
+    if (!ok[i]) continue;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const size_t o = ((size_t)((size_t)b * H + y) * W + x0 + 4 * q + i) * CO + co;
-      const float v = acc[n][i];
-      if constexpr (MODE == CONV_BWD) {
-        Y[o] = f2bf(v * gelu_grad(Zprev[o]));
-      } else {
-        const float z = v + bias[co];
-        if (Zout) Zout[o] = z;
-        Y[o] = f2bf(MODE == CONV_GELU ? gelu(z) : z);
+    for (int n = 0; n < CO16; ++n) {
+      const int co = 16 * n + m;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const size_t o = ((size_t)((size_t)tb[i] * H + ty[i]) * W + tx[i] + 4 * q + r) * CO + co;
+        const float v = acc[i][n][r];
+        if constexpr (MODE == CONV_BWD) {
+          Y[o] = f2bf(v * gelu_grad(Zprev[o]));
+        } else {
+          const float z = v + bias[co];
+          if (Zout) Zout[o] = z;
+          Y[o] = f2bf(MODE == CONV_GELU ? gelu(z) : z);
+        }
       }
     }
   }
@@ -361,8 +398,8 @@ int emit(r2d3c_cnn *c, hipStream_t s) {
 template <int CI, int CO, int MODE>
 void conv(const r2d3c_cnn *c, int l, const bf16 *X, const bf16 *Wt, const float *Zprev, float *Zout,
           bf16 *Y, int B, hipStream_t s) {
-  const long tiles = (long)B * c->H * (c->W / 16);
-  k_conv3x3<CI, CO, MODE><<<(unsigned)((tiles + 3) / 4), 256, 0, s>>>(
+  const long tiles = (long)B * c->H * (c->W / 16), per = kConvWaves * kTilesPerWave;
+  k_conv3x3<CI, CO, MODE><<<(unsigned)((tiles + per - 1) / per), 64 * kConvWaves, 0, s>>>(
       X, Wt, MODE == CONV_BWD ? nullptr : c->params + poff(l) + wsize(l), Zprev, Zout, Y, B, c->H, c->W);
 }
 
@@ -518,6 +555,28 @@ int r2d3c_cnn_get_params(r2d3c_cnn *c, float *d_params, void *stream) {
   CHK(hipMemcpyAsync(d_params, c->params, sizeof(float) * kParams, hipMemcpyDeviceToDevice,
                      (hipStream_t)stream));
   return 0;
+}
+
+int r2d3c_cnn_shape(const r2d3c_cnn *c, int *batch, int *height, int *width) {
+  if (!c) return err(R2D3C_EINVAL, "cnn: NULL argument");
+  if (batch) *batch = c->capB;
+  if (height) *height = c->H;
+  if (width) *width = c->W;
+  return 0;
+}
+
+int r2d3c_cnn_buffers(r2d3c_cnn *c, void **input_bf16, const float **output_f32) {
+  if (!c) return err(R2D3C_EINVAL, "cnn: NULL argument");
+  if (input_bf16) *input_bf16 = c->act[0];
+  if (output_f32) *output_f32 = c->z[5];
+  return 0;
+}
+
+int r2d3c_cnn_forward_buffers(r2d3c_cnn *c, int batch, void *stream) {
+  if (!c) return err(R2D3C_EINVAL, "cnn: NULL argument");
+  int rc = check_batch(c, batch);
+  if (rc) return rc;
+  return forward(c, batch, (hipStream_t)stream);
 }
 
 int r2d3c_cnn_forward(r2d3c_cnn *c, const float *d_in, int batch, float *d_out, void *stream) {

@@ -2047,11 +2047,32 @@ int r2d3c_autotune(r2d3c_plan *p, int reps, double *ms_per_step, void *stream) {
 
 }  // extern "C"
 
+// the network must hold at least one n x n image for the fused kernels
+static int check_net(const r2d3c_plan *p, const r2d3c_cnn *net) {
+  int b = 0, h = 0, w = 0;
+  int rc = r2d3c_cnn_shape(net, &b, &h, &w);
+  if (rc) return rc;
+  if (b < 1 || h != p->n || w != p->n)
+    return set_err(R2D3C_EINVAL, "closure: the network must take n x n images of the plan's grid");
+  return 0;
+}
+
 // ---------------------------------------------------------------------------
 // ensembles of independent runs
 // ---------------------------------------------------------------------------
+// A run whose step starts with the learned closure: inputs -> network ->
+// dense forcing (fp, fz), which the fused step adds like iflow = 1 forcing.
+struct EnsClosure {
+  int member = 0;                   // index within its class
+  r2d3c_cnn *net = nullptr;
+  double sin[3] = {1, 1, 1}, sout[2] = {1, 1};
+  const double *psi = nullptr, *vz = nullptr;
+  cplx *fp = nullptr, *fz = nullptr;
+};
+
 struct EnsClass {
   int n = 0, first = 0, count = 0;  // runs [first, first + count) of the ensemble, all of size n
+  std::vector<EnsClosure> closures;
   r2d3c_plan *plan = nullptr;       // geometry, twiddles and schedule of size n
   EnsMember *d_mem = nullptr;
   cplx *work = nullptr;             // 8 mixed fields per run, run i at 8 fs i
@@ -2079,8 +2100,14 @@ struct r2d3c_ensemble {
 };
 
 template <int N>
-static void ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, hipStream_t s,
-                           bool first, bool masked, bool chain) {
+static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, hipStream_t s,
+                          bool first, bool masked, bool chain) {
+  for (auto &cl : k.closures) {  // the closure forcing of this step, from the step's state
+    int rc = r2d3c_closure_net_inputs(k.plan, cl.psi, cl.vz, cl.sin, cl.net, s);
+    if (!rc) rc = r2d3c_cnn_forward_buffers(cl.net, 1, s);
+    if (!rc) rc = r2d3c_closure_net_forcing(k.plan, cl.net, cl.sout, (double *)cl.fp, (double *)cl.fz, s);
+    if (rc) return rc;
+  }
   const r2d3c_plan *p = k.plan;
   const int nc = p->g.nc, ncp = (nc + 1) & ~1;
   const cplx *tw = p->tw;
@@ -2108,22 +2135,25 @@ static void ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab,
     const int nblk = round16(2 * ncp + ((mode & FM_ZERO) ? (N / 2 + 1 - nc) : 0));
     k_ens_colfused<N><<<dim3(nblk, k.count), N / 8, 0, s>>>(a, o, mode, tw);
   }
+  return 0;
 }
 
 static int ens_dispatch(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, hipStream_t s,
                         bool first, bool masked, bool chain) {
+  int rc;
   switch (k.n) {
-    case 16: ens_class_step<16>(e, k, ftab, s, first, masked, chain); break;
-    case 32: ens_class_step<32>(e, k, ftab, s, first, masked, chain); break;
-    case 64: ens_class_step<64>(e, k, ftab, s, first, masked, chain); break;
-    case 128: ens_class_step<128>(e, k, ftab, s, first, masked, chain); break;
-    case 256: ens_class_step<256>(e, k, ftab, s, first, masked, chain); break;
-    case 512: ens_class_step<512>(e, k, ftab, s, first, masked, chain); break;
-    case 1024: ens_class_step<1024>(e, k, ftab, s, first, masked, chain); break;
-    case 2048: ens_class_step<2048>(e, k, ftab, s, first, masked, chain); break;
-    case 4096: ens_class_step<4096>(e, k, ftab, s, first, masked, chain); break;
+    case 16: rc = ens_class_step<16>(e, k, ftab, s, first, masked, chain); break;
+    case 32: rc = ens_class_step<32>(e, k, ftab, s, first, masked, chain); break;
+    case 64: rc = ens_class_step<64>(e, k, ftab, s, first, masked, chain); break;
+    case 128: rc = ens_class_step<128>(e, k, ftab, s, first, masked, chain); break;
+    case 256: rc = ens_class_step<256>(e, k, ftab, s, first, masked, chain); break;
+    case 512: rc = ens_class_step<512>(e, k, ftab, s, first, masked, chain); break;
+    case 1024: rc = ens_class_step<1024>(e, k, ftab, s, first, masked, chain); break;
+    case 2048: rc = ens_class_step<2048>(e, k, ftab, s, first, masked, chain); break;
+    case 4096: rc = ens_class_step<4096>(e, k, ftab, s, first, masked, chain); break;
     default: return set_err(R2D3C_ENOTSUP, "unsupported grid size");
   }
+  if (rc) return rc;
   LAUNCHCHK("ensemble step");
   return 0;
 }
@@ -2171,6 +2201,10 @@ static void ens_free(r2d3c_ensemble *e) {
   if (e->exec) (void)hipGraphExecDestroy(e->exec);
   if (e->graph) (void)hipGraphDestroy(e->graph);
   for (auto &k : e->cls) {
+    for (auto &cl : k.closures) {
+      (void)hipFree(cl.fp);
+      (void)hipFree(cl.fz);
+    }
     if (k.d_mem) (void)hipFree(k.d_mem);
     if (k.work) (void)hipFree(k.work);
     if (k.d_om) (void)hipFree(k.d_om);
@@ -2272,6 +2306,58 @@ int r2d3c_ensemble_set_schedule(r2d3c_ensemble *e, int n, int row_rows, int row_
       return rc;
     }
   return set_err(R2D3C_EINVAL, "no run of that size in the ensemble");
+}
+
+int r2d3c_ensemble_set_closure(r2d3c_ensemble *e, int run, r2d3c_cnn *net, const double *scale_in,
+                               const double *scale_out) {
+  NEED(e);
+  if (run < 0 || run >= e->count) return set_err(R2D3C_EINVAL, "no such run");
+  int j = 0;
+  while (e->order[j] != run) ++j;
+  EnsClass *kp = nullptr;
+  for (auto &k : e->cls)
+    if (j >= k.first && j < k.first + k.count) kp = &k;
+  EnsClass &k = *kp;
+  const int i = j - k.first, n = k.n;
+  HIPCHK(hipStreamSynchronize(e->es));
+  // drop an existing closure of this run
+  for (size_t c = 0; c < k.closures.size(); ++c)
+    if (k.closures[c].member == i) {
+      EnsClosure &cl = k.closures[c];
+      (void)hipFree(cl.fp);
+      (void)hipFree(cl.fz);
+      k.closures.erase(k.closures.begin() + c);
+      break;
+    }
+  EnsMember m;
+  HIPCHK(hipMemcpy(&m, k.d_mem + i, sizeof(EnsMember), hipMemcpyDeviceToHost));
+  m.c.fpd = m.c.fzd = nullptr;
+  if (net) {
+    NEED(scale_in); NEED(scale_out);
+    int rc = check_net(k.plan, net);
+    if (rc) return rc;
+    EnsClosure cl;
+    cl.member = i;
+    cl.net = net;
+    for (int c = 0; c < 3; ++c) cl.sin[c] = scale_in[c];
+    for (int c = 0; c < 2; ++c) cl.sout[c] = scale_out[c];
+    cl.psi = (const double *)m.psi;
+    cl.vz = (const double *)m.vz;
+    const size_t sz = (size_t)(n / 2 + 1) * n;
+    hipError_t he = hipMalloc(&cl.fp, sizeof(cplx) * sz);
+    if (he == hipSuccess) he = hipMalloc(&cl.fz, sizeof(cplx) * sz);
+    if (he != hipSuccess) {
+      (void)hipFree(cl.fp);
+      (void)hipFree(cl.fz);
+      return hip_err(he, "closure buffers");
+    }
+    m.c.fpd = cl.fp;
+    m.c.fzd = cl.fz;
+    k.closures.push_back(cl);
+  }
+  HIPCHK(hipMemcpy(k.d_mem + i, &m, sizeof(EnsMember), hipMemcpyHostToDevice));
+  e->graph_valid = false;
+  return 0;
 }
 
 int r2d3c_ensemble_run(r2d3c_ensemble *e, const r2d3c_forcing *forcing, int n_steps,
@@ -2469,6 +2555,178 @@ int r2d3c_closure_forcing(r2d3c_plan *p, const float *pred, const double *scale,
   }
   LAUNCHCHK("closure forcing");
   return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// fused closure input and output (the closure inside an ensemble step)
+// ---------------------------------------------------------------------------
+// float -> bfloat16, round to nearest even (the network's input format)
+__device__ __forceinline__ unsigned short to_bf16(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+// column inverses of the three input fields: y = 0 laplak2(psi), 1 vz, 2 psi
+template <int N>
+__global__ __launch_bounds__(N / 8) void k_ci_colinv(const cplx *__restrict__ psi,
+                                                      const cplx *__restrict__ vz,
+                                                      cplx *__restrict__ mix, Geo g,
+                                                      const cplx *__restrict__ tw) {
+  constexpr int T = N / 8;
+  __shared__ cplx lds[Shape<N>::LDSN];
+  const int i = blockIdx.x, f = blockIdx.y, b = threadIdx.x;
+  const int kx = kxof(i, N), op = f == 0 ? OP_LAP : OP_RAW;
+  const cplx *src = f == 1 ? vz : psi;
+  cplx v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int j = b + T * q;
+    v[q] = apply_op(op, src[(size_t)i * N + j], kx, kyof(j, N), g.kmax, g.tiny);
+  }
+  fft_canon<N, +1, 1>(v, lds, b, tw);
+  cplx *dst = mix + (size_t)f * ((size_t)N * g.kpf);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) dst[tix(b + T * q, i, g.kpf)] = v[q];
+}
+
+// Rows y, y+1 of the three fields to the network input bf16 [y][x][16]:
+// channel c = real-space field c times s[c] (s includes 1/n^2), 3..15 zero.
+// Three packed transforms: (f0, f1) of row y, (f0, f1) of row y+1, f2 of both.
+template <int N>
+__global__ __launch_bounds__(N / 8) void k_ci_row(const cplx *__restrict__ mix, Geo g,
+                                                   const cplx *__restrict__ tw, double s0, double s1,
+                                                   double s2, unsigned short *__restrict__ out) {
+  constexpr int T = N / 8;
+  __shared__ cplx lds[Shape<N>::LDSN];
+  const int b = threadIdx.x, y = 2 * blockIdx.x;
+  const size_t fs = (size_t)N * g.kpf;
+  auto load = [&](cplx v[8], const cplx *fa, int ya, const cplx *fb, int yb) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int k = b + T * q;
+      const bool mir = k > N / 2;
+      const int kk = mir ? N - k : k;
+      v[q] = packed_z<N>(k, fa[tix(ya, kk, g.kpf)], fb[tix(yb, kk, g.kpf)], mir);
+    }
+  };
+  cplx a[8], c[8], v[8];
+  load(a, mix, y, mix + fs, y);
+  fft_canon<N, +1, 1>(a, lds, b, tw);
+  load(c, mix, y + 1, mix + fs, y + 1);
+  fft_canon<N, +1, 1>(c, lds, b, tw);
+  load(v, mix + 2 * fs, y, mix + 2 * fs, y + 1);
+  fft_canon<N, +1, 1>(v, lds, b, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int x = b + T * q;
+    const cplx *zr[2] = {&a[q], &c[q]};
+    for (int r = 0; r < 2; ++r) {
+      const unsigned h0 = to_bf16((float)(zr[r]->x * s0)), h1 = to_bf16((float)(zr[r]->y * s1));
+      const unsigned h2 = to_bf16((float)((r == 0 ? v[q].x : v[q].y) * s2));
+      uint4 *px = (uint4 *)(out + ((size_t)(y + r) * N + x) * 16);
+      px[0] = make_uint4(h0 | (h1 << 16), h2, 0u, 0u);
+      px[1] = make_uint4(0u, 0u, 0u, 0u);
+    }
+  }
+}
+
+// Rows y, y+1 of the prediction (channels 0, 1 of fp32 [y][x][16], times
+// s0, s1) forward transformed as packed real pairs into mixed fields 0, 1.
+template <int N>
+__global__ __launch_bounds__(N / 8) void k_cf_row(const float *__restrict__ z, Geo g,
+                                                   const cplx *__restrict__ tw, double s0, double s1,
+                                                   cplx *__restrict__ mix) {
+  constexpr int T = N / 8;
+  __shared__ cplx lds[Shape<N>::LDSN];
+  const int b = threadIdx.x, y = 2 * blockIdx.x;
+  const size_t fs = (size_t)N * g.kpf;
+  for (int ch = 0; ch < 2; ++ch) {
+    const double sc = ch == 0 ? s0 : s1;
+    cplx v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int x = b + T * q;
+      v[q] = mk((double)z[((size_t)y * N + x) * 16 + ch] * sc,
+                (double)z[((size_t)(y + 1) * N + x) * 16 + ch] * sc);
+    }
+    fft_canon<N, -1, 1>(v, lds, b, tw);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) lds[pad(b + T * q)] = v[q];
+    __syncthreads();
+    unpack_store<N, 1>(lds, b, g.ncf, g.kpf, mix + ch * fs, y, mix + ch * fs, y + 1);
+    __syncthreads();
+  }
+}
+
+// column forwards of the two predicted fields with the closure epilogue:
+// y = 0: fp = -tau5/k^2, y = 1: fz = tau6, both on the update mask only
+template <int N>
+__global__ __launch_bounds__(N / 8) void k_cf_col(const cplx *__restrict__ mix, Geo g,
+                                                   const cplx *__restrict__ tw,
+                                                   cplx *__restrict__ fp, cplx *__restrict__ fz) {
+  constexpr int T = N / 8;
+  __shared__ cplx lds[Shape<N>::LDSN];
+  const int i = blockIdx.x, f = blockIdx.y, b = threadIdx.x;
+  const cplx *src = mix + (size_t)f * ((size_t)N * g.kpf);
+  cplx v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[q] = src[tix(b + T * q, i, g.kpf)];
+  fft_canon<N, -1, 1>(v, lds, b, tw);
+  cplx *dst = f == 0 ? fp : fz;
+  const int kx = kxof(i, N);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int j = b + T * q, ky = kyof(j, N);
+    const double ka2 = (double)kx * kx + (double)ky * ky;
+    cplx o = mk(0.0, 0.0);
+    if (ka2 <= g.kmax && ka2 > g.tiny) o = f == 0 ? mk(-v[q].x / ka2, -v[q].y / ka2) : v[q];
+    dst[(size_t)i * N + j] = o;
+  }
+}
+
+template <int N>
+static int run_closure_io(r2d3c_plan *p, int dir, const cplx *psi, const cplx *vz, const double *sc,
+                          unsigned short *in_bf16, const float *z, cplx *fp, cplx *fz,
+                          hipStream_t s) {
+  if (dir == 0) {
+    k_ci_colinv<N><<<dim3(N / 2 + 1, 3), N / 8, 0, s>>>(psi, vz, p->mfull, p->g, p->tw);
+    const double inv = 1.0 / ((double)N * N);
+    k_ci_row<N><<<N / 2, N / 8, 0, s>>>(p->mfull, p->g, p->tw, sc[0] * inv, sc[1] * inv,
+                                          sc[2] * inv, in_bf16);
+  } else {
+    k_cf_row<N><<<N / 2, N / 8, 0, s>>>(z, p->g, p->tw, sc[0], sc[1], p->mfull);
+    k_cf_col<N><<<dim3(N / 2 + 1, 2), N / 8, 0, s>>>(p->mfull, p->g, p->tw, fp, fz);
+  }
+  LAUNCHCHK("closure io");
+  return 0;
+}
+
+extern "C" {
+
+int r2d3c_closure_net_inputs(r2d3c_plan *p, const double *psi, const double *vz,
+                             const double *scale, r2d3c_cnn *net, void *stream) {
+  NEED(p); NEED(psi); NEED(vz); NEED(scale); NEED(net);
+  void *in = nullptr;
+  int rc = check_net(p, net);
+  if (!rc) rc = r2d3c_cnn_buffers(net, &in, nullptr);
+  if (rc) return rc;
+  R2D_DISPATCH(p->n, run_closure_io, p, 0, (const cplx *)psi, (const cplx *)vz, scale,
+               (unsigned short *)in, nullptr, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int r2d3c_closure_net_forcing(r2d3c_plan *p, r2d3c_cnn *net, const double *scale, double *fp,
+                              double *fz, void *stream) {
+  NEED(p); NEED(net); NEED(scale); NEED(fp); NEED(fz);
+  const float *z = nullptr;
+  int rc = check_net(p, net);
+  if (!rc) rc = r2d3c_cnn_buffers(net, nullptr, &z);
+  if (rc) return rc;
+  R2D_DISPATCH(p->n, run_closure_io, p, 1, nullptr, nullptr, scale, nullptr, z, (cplx *)fp,
+               (cplx *)fz, (hipStream_t)stream);
 }
 
 }  // extern "C"

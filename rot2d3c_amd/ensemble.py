@@ -59,19 +59,44 @@ class Ensemble:
             check(_lib.lib().r2d3c_ensemble_set_schedule(self.handle, n, rw, bpc),
                   "r2d3c_ensemble_set_schedule")
 
+    def set_closure(self, run, net=None, scale_in=None, scale_out=None):
+        """Make run `run` a closure run with network `net` (closure.Net for one
+        n x n image, used by this run only), or the plain solver (net=None)."""
+        self._nets = getattr(self, "_nets", {})
+        d = lambda v: (ctypes.c_double * len(v))(*[float(x) for x in v])
+        with torch.cuda.device(self.device):
+            check(_lib.lib().r2d3c_ensemble_set_closure(
+                self.handle, int(run), net.handle if net is not None else None,
+                d(scale_in) if net is not None else None, d(scale_out) if net is not None else None),
+                "r2d3c_ensemble_set_closure")
+        self._nets[run] = net  # keep the network alive while the ensemble uses it
+
+    def forcing_array(self, forcing):
+        """The C forcing table of `forcing` (see run), reusable across calls."""
+        n_steps = len(forcing)
+        if any(len(f) != len(self.members) for f in forcing):
+            raise ValueError("forcing must be lists of one (fp, fz) pair per run")
+        fo = (_lib.Forcing * (n_steps * len(self.members)))()
+        for st, row in enumerate(forcing):
+            for m, pair in enumerate(row):
+                fp, fz = pair if pair is not None else (None, None)
+                fo[st * len(self.members) + m] = S._forcing_c(fp, fz)
+        return fo
+
     def run(self, n_steps, forcing=None):
         """Advance every run n_steps steps on the current stream.  forcing:
-        None or a list over steps of a list over runs of (fp_draw, fz_draw)
-        (spectral.ForceDraw or None each)."""
+        None, a list over steps of a list over runs of (fp_draw, fz_draw)
+        (spectral.ForceDraw or None each), or an array from forcing_array."""
         fo = None
         if forcing is not None:
-            if len(forcing) != n_steps or any(len(f) != len(self.members) for f in forcing):
-                raise ValueError("forcing must be n_steps lists of one (fp, fz) pair per run")
-            fo = (_lib.Forcing * (n_steps * len(self.members)))()
-            for st, row in enumerate(forcing):
-                for m, pair in enumerate(row):
-                    fp, fz = pair if pair is not None else (None, None)
-                    fo[st * len(self.members) + m] = S._forcing_c(fp, fz)
+            if isinstance(forcing, ctypes.Array):
+                if len(forcing) < n_steps * len(self.members):
+                    raise ValueError("forcing table shorter than n_steps")
+                fo = forcing
+            else:
+                if len(forcing) != n_steps:
+                    raise ValueError("forcing must hold one list per step")
+                fo = self.forcing_array(forcing)
         with torch.cuda.device(self.device):
             check(_lib.lib().r2d3c_ensemble_run(self.handle, fo, int(n_steps),
                                                 S._stream(self.device)), "r2d3c_ensemble_run")
@@ -112,7 +137,7 @@ def mixed_ensemble_spec(mix=MIX, omega_seed=1234):
 
 def build(spec, device, schedules="auto"):
     """Allocate the runs of `spec` (random-phase initial conditions seeded by
-    each run's seed) and return (Ensemble, per-run forcing draw functions)."""
+    each run's seed) and return their Ensemble."""
     from .ic import random_phase_ic
     members = []
     for r in spec:

@@ -158,3 +158,76 @@ def test_closure_run_without_network_is_the_plain_solver(cuda):
         S.rk4_step(b[0], b[1], p, dt=1e-3)
     torch.cuda.synchronize()
     assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+
+
+def _fused_forcing(n, ps, vz, net, sin, sout, cuda):
+    from rot2d3c_amd import _lib, closure as L, spectral as S
+    pl = S.Plan.get(n, cuda)
+    fp = torch.empty_like(ps)
+    fz = torch.empty_like(ps)
+    lib = _lib.lib()
+    _lib.check(lib.r2d3c_closure_net_inputs(pl.handle, S._ptr(ps), S._ptr(vz), L._dvec(sin),
+                                            net.handle, S._stream(cuda)), "net inputs")
+    _lib.check(lib.r2d3c_cnn_forward_buffers(net.handle, 1, S._stream(cuda)), "forward")
+    _lib.check(lib.r2d3c_closure_net_forcing(pl.handle, net.handle, L._dvec(sout), S._ptr(fp),
+                                             S._ptr(fz), S._stream(cuda)), "net forcing")
+    return fp, fz
+
+
+def test_fused_closure_io_matches_the_separate_steps(cuda):
+    """closure_net_inputs -> forward_buffers -> closure_net_forcing equals
+    closure_inputs -> cnn_forward -> closure_forcing up to rare bf16 rounding
+    flips of the inputs (the two paths pack the row transforms differently)."""
+    print("fused closure io", flush=True)
+    from rot2d3c_amd import closure as L
+    n = 64
+    ps, vz = random_state(n, 31)
+    ps, vz = torch.from_numpy(ps).to(cuda), torch.from_numpy(vz).to(cuda)
+    net = L.Net(1, n, n, cuda, seed=3)
+    sin, sout = (0.02, 0.5, 3.0), (7.0, 0.3)
+    pred = net.forward(L.network_inputs(ps, vz, sin)[None].contiguous())
+    fp0, fz0 = L.closure_forcing(pred[0], n, sout)
+    fp1, fz1 = _fused_forcing(n, ps, vz, net, sin, sout, cuda)
+    assert rel_l2(fp1.cpu().numpy(), fp0.cpu().numpy()) < 1e-2
+    assert rel_l2(fz1.cpu().numpy(), fz0.cpu().numpy()) < 1e-2
+    # with the same network input the output path is the same arithmetic
+    fp2, fz2 = L.closure_forcing(torch.from_numpy(np.zeros((n, n, 2), np.float32)).to(cuda), n, sout)
+    assert float(fp2.abs().max()) == 0.0 and float(fz2.abs().max()) == 0.0
+    net.close()
+
+
+def test_ensemble_closure_run_tracks_the_stepwise_closure_run(cuda):
+    """A closure run inside the ensemble's step graph against ClosureRun
+    (separate kernels, same network and forcing): equal up to the inputs'
+    bf16 rounding flips; a network of the wrong size is refused."""
+    print("ensemble closure run", flush=True)
+    from rot2d3c_amd import closure as L, ensemble as E, spectral as S
+    from rot2d3c_amd._lib import R2D3CError
+    n = 64
+    ps0, vz0 = random_state(n, 33)
+    p = S.Params(1e-3, 0.1, 1, 2, 1e-3, 0.1, 1, 2, 2.0)
+    net_a = L.Net(1, n, n, cuda, seed=4)
+    net_b = L.Net(1, n, n, cuda, seed=4)
+    sin, sout = (0.02, 0.5, 3.0), (2.0, 0.2)
+    ps = torch.from_numpy(ps0).to(cuda)
+    vz = torch.from_numpy(vz0).to(cuda)
+    ens = E.Ensemble([E.Member(ps, vz, p, 1e-3)], schedules=None)
+    with pytest.raises(R2D3CError):
+        ens.set_closure(0, L.Net(1, 128, 128, cuda), sin, sout)
+    ens.set_closure(0, net_a, sin, sout)
+    spec = [dict(kf=6.0, f0=1.0, seed=5)]
+    table = E.forcing_table(spec, 5)
+    ens.run(5, table)
+    b = (torch.from_numpy(ps0).to(cuda), torch.from_numpy(vz0).to(cuda))
+    run = L.ClosureRun(b[0], b[1], p, net_b, sin, sout, guard_every=0)
+    for st in range(5):
+        run.step(*table[st][0], dt=1e-3)
+    torch.cuda.synchronize()
+    assert rel_l2(ps.cpu().numpy(), b[0].cpu().numpy()) < 1e-6
+    assert rel_l2(vz.cpu().numpy(), b[1].cpu().numpy()) < 1e-6
+    # the closure changes the run: the plain solver differs
+    c = (torch.from_numpy(ps0).to(cuda), torch.from_numpy(vz0).to(cuda))
+    S.rk4_run(c[0], c[1], p, [table[st][0] for st in range(5)], 5, dt=1e-3)
+    torch.cuda.synchronize()
+    assert rel_l2(c[0].cpu().numpy(), b[0].cpu().numpy()) > 1e-9
+    ens.close()
