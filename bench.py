@@ -78,6 +78,50 @@ def cpu_baseline(n, steps, p, kf, f0, seed, ps0, vz0, dt):
     return time.perf_counter() - t0
 
 
+def closure_bench(dns_ms_per_step, dns_dt, dev, steps=500, warmup=20):
+    """GPU-seconds per unit simulated time of the 256^2 run with the learned
+    closure (the committed trained network, rot2d3c_amd/data/closure_256.npz;
+    one-run ensemble, closure inside the step graph, fixed dt = 8 x the DNS dt:
+    the DNS's CFL number) against the 2048^2 DNS timed above.  The spectrum
+    and flux errors are those of the committed evaluation
+    (rot2d3c_amd/data/closure_256_eval.json, made by rot2d3c_amd.closure_study)."""
+    import torch
+    from rot2d3c_amd import closure as L, ensemble as E, ic, spectral as S
+    path = os.path.join(ROOT, "rot2d3c_amd", "data", "closure_256.npz")
+    z = np.load(path)
+    n, dt = int(z["nc"]), dns_dt * int(z["nf"]) / int(z["nc"])
+    ps, vz = ic.random_phase_ic(n, seed=7)
+    p = S.Params(5e-5, 0.5, 1, 2, 5e-5, 0.5, 1, 2, 5.0)
+    ens = E.Ensemble([E.Member(torch.from_numpy(ps).to(dev), torch.from_numpy(vz).to(dev), p, dt)],
+                     schedules="auto")
+    net = L.Net(1, n, n, dev, params=z["params"])
+    ens.set_closure(0, net, z["scale_in"], z["scale_out"], float(z["kmin"]))
+    spec = [dict(kf=40.0, f0=1.0, seed=3)]
+    warm = ens.forcing_array(E.forcing_table(spec, warmup))
+    table = ens.forcing_array(E.forcing_table(spec, steps))
+    ens.run(warmup, warm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.run(steps, table)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    ens.close()
+    les = ms * 1e-3 / dt
+    dns = dns_ms_per_step * 1e-3 / dns_dt
+    with open(os.path.join(ROOT, "rot2d3c_amd", "data", "closure_256_eval.json")) as f:
+        ev = json.load(f)
+    return {"workload": "256^2 run with the learned closure (BF16 MFMA CNN) vs 2048^2 DNS",
+            "speedup_vs_dns": round(dns / les, 2), "les_ms_per_step": round(ms, 4), "les_dt": dt,
+            "les_gpu_s_per_time": round(les, 4), "dns_gpu_s_per_time": round(dns, 4),
+            "spectrum_rel_err_mean": round(ev["closure"]["spectrum_rel_err_mean"], 4),
+            "spectrum_rel_err_max": round(ev["closure"]["spectrum_rel_err_max"], 4),
+            "flux_err": round(ev["closure"]["flux_err"], 4),
+            "plain_spectrum_rel_err_mean": round(ev["plain"]["spectrum_rel_err_mean"], 4),
+            "shells_within_5pct": "%d/%d" % (ev["shells_within_5pct"], ev["shells"]),
+            "errors_from": "rot2d3c_amd/data/closure_256_eval.json (k <= 85, 10 time units, "
+                           "4 forcing realisations vs the coarse-grained 2048^2 DNS)"}
+
+
 def ensemble_bench(steps, dev, tuned, warmup=10):
     """The batched workload: 24 x 256^2 + 12 x 512^2 + 4 x 1024^2 independent
     runs (per-run seed, Omega, nu, forcing shell, fixed dt; see
@@ -122,6 +166,8 @@ def main():
                     help="also run one 512 MiB device copy (PMC byte calibration)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default row schedule instead of the cached autotuned one")
+    ap.add_argument("--no-closure", action="store_true",
+                    help="skip the closure-vs-DNS timing")
     ap.add_argument("--ensemble-steps", type=int, default=500,
                     help="steps of the batched ensemble (24 x 256^2 + 12 x 512^2 + 4 x 1024^2); "
                          "0 disables")
@@ -217,6 +263,10 @@ def main():
                "sample": "%d steps of the same %d^2 workload through the numpy oracle "
                          "(pocketfft, single thread): %.1f s" % (args.cpu_steps, n, el)}
 
+    clo = None
+    if not args.no_closure:
+        clo = closure_bench(ms_per_step, dt, dev)
+
     ens = None
     if args.ensemble_steps > 0:
         del ps, vz
@@ -252,6 +302,7 @@ def main():
                     for k, v in kern.items()},
         "finite": finite,
         "ensemble": ens,
+        "closure": clo,
     }
     print(json.dumps(out), flush=True)
 

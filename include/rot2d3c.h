@@ -282,10 +282,10 @@ int r2d3c_closure_inputs(r2d3c_plan *plan, const double *psi, const double *vz,
 int r2d3c_closure_targets(r2d3c_plan *plan, const double *f5, const double *c5, const double *f6,
                           const double *c6, const double *scale, float *out, void *stream);
 /* Dense forcing of the closure from a prediction fp32 [n*n][2] (times
- * scale[2]): fp = -r2c(pred0)/k^2 and fz = r2c(pred1) on the update mask (the
- * subgrid parts of -C5/k^2 and C6 in the stage update). */
-int r2d3c_closure_forcing(r2d3c_plan *plan, const float *pred, const double *scale, double *fp,
-                          double *fz, void *stream);
+ * scale[2]): fp = -r2c(pred0)/k^2 and fz = r2c(pred1) on the update mask and
+ * |k| >= kmin (the subgrid parts of -C5/k^2 and C6 in the stage update). */
+int r2d3c_closure_forcing(r2d3c_plan *plan, const float *pred, const double *scale, double kmin,
+                          double *fp, double *fz, void *stream);
 
 /* The closure network: 6 periodic 3x3 conv layers, 3 -> 32 -> ... -> 32 -> 2
  * channels, GELU between layers, BF16 weights and activations, FP32
@@ -325,18 +325,18 @@ int r2d3c_cnn_train_step(r2d3c_cnn *net, const float *d_in, const float *d_targe
  * prediction into fp, fz (as r2d3c_closure_forcing).  n >= 64. */
 int r2d3c_closure_net_inputs(r2d3c_plan *plan, const double *psi, const double *vz,
                              const double *scale, r2d3c_cnn *net, void *stream);
-int r2d3c_closure_net_forcing(r2d3c_plan *plan, r2d3c_cnn *net, const double *scale, double *fp,
-                              double *fz, void *stream);
+int r2d3c_closure_net_forcing(r2d3c_plan *plan, r2d3c_cnn *net, const double *scale,
+                              double kmin, double *fp, double *fz, void *stream);
 
 /* Makes run `run` of an ensemble a closure run: each of its steps starts with
  * r2d3c_closure_net_inputs(scale_in) -> r2d3c_cnn_forward_buffers ->
- * r2d3c_closure_net_forcing(scale_out) on the step's state, and the stages add that forcing (held for
+ * r2d3c_closure_net_forcing(scale_out, kmin) on the step's state, and the stages add that forcing (held for
  * the four stages), all inside the step's graph.  net = NULL removes the
  * closure (the plain solver).  The net must take 1 image of n x n, serve
  * no other run (its buffers hold this run's activations) and stay alive while
  * the ensemble uses it. */
 int r2d3c_ensemble_set_closure(r2d3c_ensemble *ens, int run, r2d3c_cnn *net,
-                               const double *scale_in, const double *scale_out);
+                               const double *scale_in, const double *scale_out, double kmin);
 
 #ifdef __cplusplus
 }

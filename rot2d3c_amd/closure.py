@@ -180,14 +180,16 @@ def snapshot(ps_f, vz_f, n_coarse, width=None):
     return ps_c, vz_c, network_inputs(ps_c, vz_c), tgt
 
 
-def closure_forcing(pred, n, scale, fp=None, fz=None):
-    """Dense (fp, fz) of a prediction [n, n, 2] times scale (the de-normalisation)."""
+def closure_forcing(pred, n, scale, fp=None, fz=None, kmin=0.0):
+    """Dense (fp, fz) of a prediction [n, n, 2] times scale (the de-normalisation),
+    on the modes |k| >= kmin."""
     dev = pred.device
     fp = torch.empty((n // 2 + 1, n), dtype=torch.complex128, device=dev) if fp is None else fp
     fz = torch.empty_like(fp) if fz is None else fz
     pl = S.Plan.get(n, dev)
-    check(_lib_().r2d3c_closure_forcing(pl.handle, _fptr(pred), _dvec(scale), S._ptr(fp), S._ptr(fz),
-                                        S._stream(dev)), "r2d3c_closure_forcing")
+    check(_lib_().r2d3c_closure_forcing(pl.handle, _fptr(pred), _dvec(scale), float(kmin),
+                                        S._ptr(fp), S._ptr(fz), S._stream(dev)),
+          "r2d3c_closure_forcing")
     return fp, fz
 
 
@@ -244,11 +246,12 @@ class ClosureRun:
     With net=None (or after a fallback) it is the plain coarse solver."""
 
     def __init__(self, ps, vz, params, net=None, scale_in=None, scale_out=None, guard_every=50,
-                 energy_bound=None):
+                 energy_bound=None, kmin=0.0):
         self.ps, self.vz, self.p = ps, vz, params
         self.n = S._spec_n(ps)
         self.net = net
         self.scale_in, self.scale_out = scale_in, scale_out
+        self.kmin = kmin
         self.active = net is not None
         self.guard_every = guard_every
         self.energy_bound = energy_bound
@@ -266,7 +269,7 @@ class ClosureRun:
         if self.active:
             network_inputs(self.ps, self.vz, self.scale_in, out=self._inp[0])
             self.net.forward(self._inp, out=self._pred)
-            closure_forcing(self._pred[0], self.n, self.scale_out, self._fp, self._fz)
+            closure_forcing(self._pred[0], self.n, self.scale_out, self._fp, self._fz, self.kmin)
             dense = (self._fp, self._fz)
         S.rk4_step(self.ps, self.vz, self.p, fp_draw, fz_draw, dt=dt, cfl=cfl, dt_out=dt_out,
                    dense=dense)

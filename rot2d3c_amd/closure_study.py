@@ -4,16 +4,17 @@
    random forcing on kf = 40 for psi and vz at injection 1.0, dt = 2e-4 fixed)
    from a seeded random-phase start: spin-up, then a training window with
    `snapshots` coarse-grained snapshots (inputs and subgrid targets, see
-   rot2d3c_amd.closure), then an evaluation window whose coarse-grained
-   spectra and fluxes are averaged in time.
+   rot2d3c_amd.closure), then a tuning window and an evaluation window whose
+   coarse-grained spectra and fluxes are averaged in time.
 2. Training of the closure network on the GPU (90% / 10% split).
 3. Coarse runs (default 256^2, same parameters, fixed dt = dt nf/nc: the
-   DNS's CFL number) over the evaluation window, started from the
-   coarse-grained DNS state at its start:
-   the plain coarse solver and the closure run.  Both are judged by their
-   time-averaged energy spectrum and flux against the coarse-grained DNS for
-   k <= n_coarse/3, and timed in GPU-seconds per unit simulated time against
-   the DNS.
+   DNS's CFL number), each as several forcing realisations averaged,
+   started from the coarse-grained DNS state at the start of a window.  On
+   the tuning window each candidate low-k cutoff of the closure forcing is
+   scored; on the evaluation window the plain coarse solver and the closure
+   run with the chosen cutoff are judged by their time-averaged energy
+   spectrum and flux against the coarse-grained DNS for k <= n_coarse/3, and
+   timed in GPU-seconds per unit simulated time against the DNS.
 
 Usage: python -m rot2d3c_amd.closure_study --out DIR [--nf 2048 --nc 256 ...]
 Writes DIR/closure_params.npz (parameters and normalisation) and
@@ -80,7 +81,7 @@ def errors(les, dns, kmax):
             "flux_err": float(flux), "per_k": [float(x) for x in rel]}
 
 
-def timed_steps(ps0, vz0, p, dt, forcing, steps, net=None, sin=None, sout=None):
+def timed_steps(ps0, vz0, p, dt, forcing, steps, net=None, sin=None, sout=None, kmin=0.0):
     """Seconds per step of a one-run ensemble (each step one replay of the
     captured step graph, the closure inside it when net is given), timed over
     one run() call of `steps` steps after a warm-up call; host-side table
@@ -89,7 +90,7 @@ def timed_steps(ps0, vz0, p, dt, forcing, steps, net=None, sin=None, sout=None):
     ps, vz = ps0.clone(), vz0.clone()
     ens = E.Ensemble([E.Member(ps, vz, p, dt)], schedules="auto")
     if net is not None:
-        ens.set_closure(0, net, sin, sout)
+        ens.set_closure(0, net, sin, sout, kmin)
     warm = ens.forcing_array([[pair] for pair in forcing.take(20)])
     table = ens.forcing_array([[pair] for pair in forcing.take(steps)])
     ens.run(20, warm)
@@ -102,49 +103,78 @@ def timed_steps(ps0, vz0, p, dt, forcing, steps, net=None, sin=None, sout=None):
     return sec
 
 
-def coarse_run(ps0, vz0, p, t_end, dt, forcing, net=None, sin=None, sout=None,
-               samples=200, energy_bound=None, chunk=50):
-    """Advance a copy of (ps0, vz0) to simulated time t_end with fixed dt as a
-    one-run ensemble, the closure inside each step's graph when net is given.
-    Every `chunk` steps the energy is checked; a closure run that stops being
-    finite or exceeds energy_bound restarts the chunk from its last good state
-    as the plain solver (the fallback).  Returns (time-averaged statistics,
-    steps, finite, fallbacks)."""
+def coarse_runs(ps0, vz0, p, t_end, dt, seeds, kf, f0, params=None, sin=None, sout=None,
+                kmin=0.0, samples=200, energy_bound=None, chunk=50):
+    """len(seeds) coarse runs from (ps0, vz0), one forcing realisation each,
+    advanced together as one ensemble to simulated time t_end with fixed dt;
+    with `params` every run carries its own closure network (forcing on
+    |k| >= kmin) inside its step graph.  Every `chunk` steps each run's energy
+    is checked: a closure run that stops being finite or exceeds energy_bound
+    restarts the chunk from its last good state as the plain solver (the
+    fallback).  Returns (statistics averaged over time and runs, steps,
+    finite, fallbacks)."""
     from . import ensemble as E
-    ps, vz = ps0.clone(), vz0.clone()
-    ens = E.Ensemble([E.Member(ps, vz, p, dt)], schedules="auto")
-    if net is not None:
-        ens.set_closure(0, net, sin, sout)
+    R = len(seeds)
+    runs = [(ps0.clone(), vz0.clone()) for _ in range(R)]
+    ens = E.Ensemble([E.Member(a, b, p, dt) for a, b in runs], schedules="auto")
+    nets = []
+    if params is not None:
+        for r in range(R):
+            nets.append(L.Net(1, S._spec_n(ps0), S._spec_n(ps0), ps0.device, params=params))
+            ens.set_closure(r, nets[-1], sin, sout, kmin)
+    forcings = [Forcing(kf, f0, sd) for sd in seeds]
     nsteps = int(round(t_end / dt))
     every = max(1, nsteps // samples)
     acc, nsamp, steps, finite, fallbacks = None, 0, 0, True, 0
-    good = (ps.clone(), vz.clone())
-    closure_on = net is not None
+    good = [(a.clone(), b.clone()) for a, b in runs]
+    on = [params is not None] * R
     while steps < nsteps:
         c = min(chunk, nsteps - steps)
-        ens.run(c, [[pair] for pair in forcing.take(c)])
-        e = S.energy(ps, 1) + S.energy(vz, 0)
-        bad = not math.isfinite(e) or (energy_bound is not None and e > energy_bound)
-        if bad and closure_on:
-            ps.copy_(good[0])
-            vz.copy_(good[1])
-            ens.set_closure(0, None)
-            closure_on = False
-            fallbacks += 1
-            continue
-        if not math.isfinite(e):
-            finite = False
+        draws = [f.take(c) for f in forcings]
+        ens.run(c, [[draws[r][k] for r in range(R)] for k in range(c)])
+        redo = False
+        for r, (a, b) in enumerate(runs):
+            e = S.energy(a, 1) + S.energy(b, 0)
+            bad = not math.isfinite(e) or (energy_bound is not None and e > energy_bound)
+            if bad and on[r]:
+                on[r] = False
+                ens.set_closure(r, None)
+                fallbacks += 1
+                redo = True
+            elif not math.isfinite(e):
+                finite = False
+        if not finite:
             break
-        good[0].copy_(ps)
-        good[1].copy_(vz)
+        if redo:  # restart the chunk from the last good states
+            for r, (a, b) in enumerate(runs):
+                a.copy_(good[r][0])
+                b.copy_(good[r][1])
+            continue
+        for r, (a, b) in enumerate(runs):
+            good[r][0].copy_(a)
+            good[r][1].copy_(b)
         steps += c
         if steps // every > (steps - c) // every:
-            s = spectra(ps, vz)
-            acc = [np.array(x) for x in s] if acc is None else [u + v for u, v in zip(acc, s)]
-            nsamp += 1
+            for a, b in runs:
+                s_ = spectra(a, b)
+                acc = [np.array(x) for x in s_] if acc is None else [u + v for u, v in zip(acc, s_)]
+                nsamp += 1
     ens.close()
     stats = None if acc is None else [u / nsamp for u in acc]
     return stats, steps, finite, fallbacks
+
+
+def dns_window(advance, ps, vz, nc, steps, samples=200):
+    """Advance the DNS `steps` steps, averaging the coarse-grained spectra and
+    fluxes over `samples` instants."""
+    acc, nsamp = None, 0
+    samp = max(1, steps // samples)
+    for _ in range(0, steps, samp):
+        advance(samp)
+        s_ = spectra(L.coarse_grain(ps, nc), L.coarse_grain(vz, nc))
+        acc = [np.array(x) for x in s_] if acc is None else [u + v for u, v in zip(acc, s_)]
+        nsamp += 1
+    return [u / nsamp for u in acc]
 
 
 def main(argv=None):
@@ -156,7 +186,13 @@ def main(argv=None):
     ap.add_argument("--spinup", type=int, default=15000)
     ap.add_argument("--train-steps", type=int, default=25000)
     ap.add_argument("--snapshots", type=int, default=1000)
-    ap.add_argument("--eval-steps", type=int, default=10000)
+    ap.add_argument("--tune-steps", type=int, default=25000,
+                    help="DNS steps of the window that picks the closure's low-k cutoff")
+    ap.add_argument("--eval-steps", type=int, default=50000)
+    ap.add_argument("--kmin", type=float, nargs="+", default=[0.0, 4.0, 8.0, 16.0],
+                    help="candidate low-k cutoffs of the closure forcing")
+    ap.add_argument("--realisations", type=int, default=4,
+                    help="forcing realisations of each coarse run (averaged)")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -243,48 +279,62 @@ def main(argv=None):
     np.savez(os.path.join(a.out, "closure_params.npz"), params=params, scale_in=np.array(scale_in),
              scale_out=np.array(sy), nf=nf, nc=nc)
 
-    # 4. evaluation window: coarse-grained DNS statistics
-    ps_c0 = L.coarse_grain(ps, nc)
-    vz_c0 = L.coarse_grain(vz, nc)
-    acc, nsamp = None, 0
-    samp = max(1, a.eval_steps // 200)
-    for k in range(0, a.eval_steps, samp):
-        advance(samp)
-        s = spectra(L.coarse_grain(ps, nc), L.coarse_grain(vz, nc))
-        acc = [np.array(x) for x in s] if acc is None else [u + v for u, v in zip(acc, s)]
-        nsamp += 1
-    dns_stats = [u / nsamp for u in acc]
+    # 4. tuning window: choose the closure's low-k cutoff on data the final
+    # evaluation does not see
+    kmax = nc // 3
+    dtc = a.dt * nf / nc if a.dt_coarse is None else a.dt_coarse
+    seeds = [a.seed + 100 + r for r in range(a.realisations)]
+    ps_c0, vz_c0 = L.coarse_grain(ps, nc), L.coarse_grain(vz, nc)
+    e0 = S.energy(ps_c0, 1) + S.energy(vz_c0, 0)
+    dns_tune = dns_window(advance, ps, vz, nc, a.tune_steps)
+    Tt = a.tune_steps * a.dt
+    out["tune"] = {}
+    for km in a.kmin:
+        stats, steps, finite, fb = coarse_runs(ps_c0, vz_c0, p, Tt, dtc, seeds, kf, f0, params,
+                                               scale_in, sy, km, energy_bound=20.0 * e0)
+        r = {"finite": finite, "fallbacks": fb}
+        if finite and stats is not None:
+            r.update(errors(stats, dns_tune, kmax))
+        out["tune"][str(km)] = r
+        log("tune kmin %g: spectrum err max %s mean %s, fallbacks %d" % (
+            km, r.get("spectrum_rel_err_max"), r.get("spectrum_rel_err_mean"), fb))
+    kbest = min(a.kmin, key=lambda km: out["tune"][str(km)].get("spectrum_rel_err_mean", 1e300))
+    out["kmin"] = kbest
+
+    # 5. evaluation window: coarse-grained DNS statistics, then the coarse runs
+    ps_c0, vz_c0 = L.coarse_grain(ps, nc), L.coarse_grain(vz, nc)
+    e0 = S.energy(ps_c0, 1) + S.energy(vz_c0, 0)
+    dns_stats = dns_window(advance, ps, vz, nc, a.eval_steps)
     T = a.eval_steps * a.dt
-    log("dns evaluation window done (%d samples over t = %.3f)" % (nsamp, T))
+    log("dns evaluation window done (t = %.3f); closure kmin %g" % (T, kbest))
     del ps, vz
     torch.cuda.empty_cache()
-
-    # 5. coarse runs over the same window
-    kmax = nc // 3
-    infer = L.Net(1, nc, nc, dev, params=params)
-    e0 = S.energy(ps_c0, 1) + S.energy(vz_c0, 0)
-    dtc = a.dt * nf / nc if a.dt_coarse is None else a.dt_coarse
-    for name, net in (("plain", None), ("closure", infer)):
-        stats, steps, finite, fb = coarse_run(
-            ps_c0, vz_c0, p, T, dtc, Forcing(kf, f0, a.seed + 7), net, scale_in, sy,
-            energy_bound=20.0 * e0)
+    for name, prm in (("plain", None), ("closure", params)):
+        stats, steps, finite, fb = coarse_runs(ps_c0, vz_c0, p, T, dtc, seeds, kf, f0, prm, scale_in,
+                                               sy, kbest, energy_bound=20.0 * e0)
+        net = L.Net(1, nc, nc, dev, params=params) if prm is not None else None
         sec = timed_steps(ps_c0, vz_c0, p, dtc, Forcing(kf, f0, a.seed + 11), 500, net, scale_in,
-                          sy)
+                          sy, kbest)
         r = {"finite": finite, "steps": steps, "dt": dtc, "sim_time": steps * dtc,
-             "ms_per_step": sec * 1e3, "gpu_s_per_time": sec / dtc, "fallbacks": fb}
+             "realisations": len(seeds), "ms_per_step": sec * 1e3, "gpu_s_per_time": sec / dtc,
+             "fallbacks": fb}
         if finite and stats is not None:
             r.update(errors(stats, dns_stats, kmax))
             r["spectrum"] = [float(x) for x in (stats[0] + stats[1])[:kmax]]
         out["coarse"][name] = r
-        log("%s: finite %s, %d steps, t %.3f, %.3f GPU-s/time, spectrum err max %s mean %s" % (
-            name, finite, steps, r["sim_time"], r["gpu_s_per_time"], r.get("spectrum_rel_err_max"),
-            r.get("spectrum_rel_err_mean")))
+        log("%s: finite %s, %d steps, t %.3f, %.3f GPU-s/time, spectrum err max %s mean %s "
+            "k<=40 max %s, flux err %s" % (
+                name, finite, steps, r["sim_time"], r["gpu_s_per_time"],
+                r.get("spectrum_rel_err_max"), r.get("spectrum_rel_err_mean"),
+                r.get("spectrum_rel_err_k_le_40_max"), r.get("flux_err")))
     out["dns"]["spectrum"] = [float(x) for x in (dns_stats[0] + dns_stats[1])[:kmax]]
     c = out["coarse"]["closure"]
     out["speedup_gpu_s_per_time"] = out["dns"]["gpu_s_per_time"] / c["gpu_s_per_time"]
     out["wall_seconds"] = time.perf_counter() - t0
     with open(os.path.join(a.out, "closure_study.json"), "w") as f:
         json.dump(out, f, indent=1)
+    np.savez(os.path.join(a.out, "closure_params.npz"), params=params, scale_in=np.array(scale_in),
+             scale_out=np.array(sy), kmin=kbest, nf=nf, nc=nc)
     log("speedup %.1fx; wrote %s" % (out["speedup_gpu_s_per_time"], a.out))
     return 0
 

@@ -2065,7 +2065,7 @@ static int check_net(const r2d3c_plan *p, const r2d3c_cnn *net) {
 struct EnsClosure {
   int member = 0;                   // index within its class
   r2d3c_cnn *net = nullptr;
-  double sin[3] = {1, 1, 1}, sout[2] = {1, 1};
+  double sin[3] = {1, 1, 1}, sout[2] = {1, 1}, kmin = 0.0;
   const double *psi = nullptr, *vz = nullptr;
   cplx *fp = nullptr, *fz = nullptr;
 };
@@ -2105,7 +2105,9 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
   for (auto &cl : k.closures) {  // the closure forcing of this step, from the step's state
     int rc = r2d3c_closure_net_inputs(k.plan, cl.psi, cl.vz, cl.sin, cl.net, s);
     if (!rc) rc = r2d3c_cnn_forward_buffers(cl.net, 1, s);
-    if (!rc) rc = r2d3c_closure_net_forcing(k.plan, cl.net, cl.sout, (double *)cl.fp, (double *)cl.fz, s);
+    if (!rc)
+      rc = r2d3c_closure_net_forcing(k.plan, cl.net, cl.sout, cl.kmin, (double *)cl.fp,
+                                     (double *)cl.fz, s);
     if (rc) return rc;
   }
   const r2d3c_plan *p = k.plan;
@@ -2309,7 +2311,7 @@ int r2d3c_ensemble_set_schedule(r2d3c_ensemble *e, int n, int row_rows, int row_
 }
 
 int r2d3c_ensemble_set_closure(r2d3c_ensemble *e, int run, r2d3c_cnn *net, const double *scale_in,
-                               const double *scale_out) {
+                               const double *scale_out, double kmin) {
   NEED(e);
   if (run < 0 || run >= e->count) return set_err(R2D3C_EINVAL, "no such run");
   int j = 0;
@@ -2341,6 +2343,7 @@ int r2d3c_ensemble_set_closure(r2d3c_ensemble *e, int run, r2d3c_cnn *net, const
     cl.net = net;
     for (int c = 0; c < 3; ++c) cl.sin[c] = scale_in[c];
     for (int c = 0; c < 2; ++c) cl.sout[c] = scale_out[c];
+    cl.kmin = kmin;
     cl.psi = (const double *)m.psi;
     cl.vz = (const double *)m.vz;
     const size_t sz = (size_t)(n / 2 + 1) * n;
@@ -2449,9 +2452,10 @@ __global__ void k_filter(const cplx *__restrict__ fine, int nf, cplx *__restrict
 }
 
 // closure spectra: mode 0 out = -(a - b)/k^2, mode 1 out = a - b, both only
-// inside the update mask tiny < k^2 <= kmax (b may be NULL)
+// inside the update mask tiny < k^2 <= kmax and for k^2 >= kmin2 (b may be NULL)
 __global__ void k_closure_spec(const cplx *__restrict__ a, const cplx *__restrict__ b,
-                               cplx *__restrict__ out, int n, double kmax, double tiny, int mode) {
+                               cplx *__restrict__ out, int n, double kmax, double tiny, int mode,
+                               double kmin2) {
   const size_t total = (size_t)(n / 2 + 1) * n;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (size_t)gridDim.x * blockDim.x) {
@@ -2459,7 +2463,7 @@ __global__ void k_closure_spec(const cplx *__restrict__ a, const cplx *__restric
     const int kx = kxof(i, n), ky = kyof(j, n);
     const double ka2 = (double)kx * kx + (double)ky * ky;
     cplx v = mk(0.0, 0.0);
-    if (ka2 <= kmax && ka2 > tiny) {
+    if (ka2 <= kmax && ka2 > tiny && ka2 >= kmin2) {
       v = a[e];
       if (b) v = mk(v.x - b[e].x, v.y - b[e].y);
       if (mode == 0) v = mk(-v.x / ka2, -v.y / ka2);
@@ -2529,7 +2533,7 @@ int r2d3c_closure_targets(r2d3c_plan *p, const double *f5, const double *c5, con
   const double *a[2] = {f5, f6}, *b[2] = {c5, c6};
   for (int ch = 0; ch < 2; ++ch) {
     k_closure_spec<<<launch_grid(sz), 256, 0, s>>>((const cplx *)a[ch], (const cplx *)b[ch], t, n,
-                                                    p->g.kmax, p->g.tiny, 1);
+                                                    p->g.kmax, p->g.tiny, 1, 0.0);
     int rc = r2d3c_c2r(p, (const double *)t, p->real, stream);
     if (rc) return rc;
     k_chan_pack<<<launch_grid(nn), 256, 0, s>>>(p->real, nn, scale[ch] * inv, ch, 2, out);
@@ -2538,8 +2542,8 @@ int r2d3c_closure_targets(r2d3c_plan *p, const double *f5, const double *c5, con
   return 0;
 }
 
-int r2d3c_closure_forcing(r2d3c_plan *p, const float *pred, const double *scale, double *fp,
-                          double *fz, void *stream) {
+int r2d3c_closure_forcing(r2d3c_plan *p, const float *pred, const double *scale, double kmin,
+                          double *fp, double *fz, void *stream) {
   NEED(p); NEED(pred); NEED(scale); NEED(fp); NEED(fz);
   hipStream_t s = (hipStream_t)stream;
   const int n = p->n;
@@ -2551,7 +2555,7 @@ int r2d3c_closure_forcing(r2d3c_plan *p, const float *pred, const double *scale,
     int rc = r2d3c_r2c(p, p->real, (double *)t, stream);
     if (rc) return rc;
     k_closure_spec<<<launch_grid(sz), 256, 0, s>>>(t, nullptr, (cplx *)dst[ch], n, p->g.kmax,
-                                                    p->g.tiny, ch == 0 ? 0 : 1);
+                                                    p->g.tiny, ch == 0 ? 0 : 1, kmin * kmin);
   }
   LAUNCHCHK("closure forcing");
   return 0;
@@ -2663,10 +2667,11 @@ __global__ __launch_bounds__(N / 8) void k_cf_row(const float *__restrict__ z, G
 }
 
 // column forwards of the two predicted fields with the closure epilogue:
-// y = 0: fp = -tau5/k^2, y = 1: fz = tau6, both on the update mask only
+// y = 0: fp = -tau5/k^2, y = 1: fz = tau6, both on the update mask and
+// k^2 >= kmin2 only
 template <int N>
 __global__ __launch_bounds__(N / 8) void k_cf_col(const cplx *__restrict__ mix, Geo g,
-                                                   const cplx *__restrict__ tw,
+                                                   const cplx *__restrict__ tw, double kmin2,
                                                    cplx *__restrict__ fp, cplx *__restrict__ fz) {
   constexpr int T = N / 8;
   __shared__ cplx lds[Shape<N>::LDSN];
@@ -2683,14 +2688,15 @@ __global__ __launch_bounds__(N / 8) void k_cf_col(const cplx *__restrict__ mix, 
     const int j = b + T * q, ky = kyof(j, N);
     const double ka2 = (double)kx * kx + (double)ky * ky;
     cplx o = mk(0.0, 0.0);
-    if (ka2 <= g.kmax && ka2 > g.tiny) o = f == 0 ? mk(-v[q].x / ka2, -v[q].y / ka2) : v[q];
+    if (ka2 <= g.kmax && ka2 > g.tiny && ka2 >= kmin2)
+      o = f == 0 ? mk(-v[q].x / ka2, -v[q].y / ka2) : v[q];
     dst[(size_t)i * N + j] = o;
   }
 }
 
 template <int N>
 static int run_closure_io(r2d3c_plan *p, int dir, const cplx *psi, const cplx *vz, const double *sc,
-                          unsigned short *in_bf16, const float *z, cplx *fp, cplx *fz,
+                          double kmin, unsigned short *in_bf16, const float *z, cplx *fp, cplx *fz,
                           hipStream_t s) {
   if (dir == 0) {
     k_ci_colinv<N><<<dim3(N / 2 + 1, 3), N / 8, 0, s>>>(psi, vz, p->mfull, p->g, p->tw);
@@ -2699,7 +2705,7 @@ static int run_closure_io(r2d3c_plan *p, int dir, const cplx *psi, const cplx *v
                                           sc[2] * inv, in_bf16);
   } else {
     k_cf_row<N><<<N / 2, N / 8, 0, s>>>(z, p->g, p->tw, sc[0], sc[1], p->mfull);
-    k_cf_col<N><<<dim3(N / 2 + 1, 2), N / 8, 0, s>>>(p->mfull, p->g, p->tw, fp, fz);
+    k_cf_col<N><<<dim3(N / 2 + 1, 2), N / 8, 0, s>>>(p->mfull, p->g, p->tw, kmin * kmin, fp, fz);
   }
   LAUNCHCHK("closure io");
   return 0;
@@ -2714,18 +2720,18 @@ int r2d3c_closure_net_inputs(r2d3c_plan *p, const double *psi, const double *vz,
   int rc = check_net(p, net);
   if (!rc) rc = r2d3c_cnn_buffers(net, &in, nullptr);
   if (rc) return rc;
-  R2D_DISPATCH(p->n, run_closure_io, p, 0, (const cplx *)psi, (const cplx *)vz, scale,
+  R2D_DISPATCH(p->n, run_closure_io, p, 0, (const cplx *)psi, (const cplx *)vz, scale, 0.0,
                (unsigned short *)in, nullptr, nullptr, nullptr, (hipStream_t)stream);
 }
 
-int r2d3c_closure_net_forcing(r2d3c_plan *p, r2d3c_cnn *net, const double *scale, double *fp,
-                              double *fz, void *stream) {
+int r2d3c_closure_net_forcing(r2d3c_plan *p, r2d3c_cnn *net, const double *scale, double kmin,
+                              double *fp, double *fz, void *stream) {
   NEED(p); NEED(net); NEED(scale); NEED(fp); NEED(fz);
   const float *z = nullptr;
   int rc = check_net(p, net);
   if (!rc) rc = r2d3c_cnn_buffers(net, nullptr, &z);
   if (rc) return rc;
-  R2D_DISPATCH(p->n, run_closure_io, p, 1, nullptr, nullptr, scale, nullptr, z, (cplx *)fp,
+  R2D_DISPATCH(p->n, run_closure_io, p, 1, nullptr, nullptr, scale, kmin, nullptr, z, (cplx *)fp,
                (cplx *)fz, (hipStream_t)stream);
 }
 

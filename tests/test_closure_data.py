@@ -1,0 +1,39 @@
+"""The committed trained closure network and its evaluation record are well
+formed: parameter count and layout of the library, padding exactly zero,
+finite values, and a record whose numbers are consistent."""
+import json
+import os
+
+import numpy as np
+
+from conftest import ROOT
+from oracle import cnn as C
+
+DATA = os.path.join(ROOT, "rot2d3c_amd", "data")
+
+
+def test_trained_parameters_have_the_library_layout():
+    z = np.load(os.path.join(DATA, "closure_256.npz"))
+    p = z["params"]
+    offs, total = C.offsets()
+    assert p.dtype == np.float32 and p.shape == (total,)
+    assert np.all(np.isfinite(p))
+    W0 = p[offs[0][0]:offs[0][1]].reshape(9, 32, 16)
+    W5 = p[offs[5][0]:offs[5][1]].reshape(9, 16, 32)
+    assert np.all(W0[:, :, 3:] == 0) and np.all(W5[:, 2:, :] == 0)
+    assert np.all(p[offs[5][1] + 2:offs[5][1] + 16] == 0)
+    assert z["scale_in"].shape == (3,) and z["scale_out"].shape == (2,)
+    assert np.all(z["scale_in"] > 0) and np.all(z["scale_out"] > 0)
+    assert int(z["nf"]) == 2048 and int(z["nc"]) == 256
+
+
+def test_evaluation_record_is_consistent():
+    ev = json.load(open(os.path.join(DATA, "closure_256_eval.json")))
+    rc = np.array(ev["ratio_closure_to_dns"])
+    assert len(rc) == ev["shells"] == 85
+    assert np.isclose(np.abs(rc - 1).max(), ev["closure"]["spectrum_rel_err_max"], atol=1e-3)
+    assert np.isclose(np.abs(rc - 1).mean(), ev["closure"]["spectrum_rel_err_mean"], atol=1e-3)
+    assert ev["shells_within_5pct"] == int(np.sum(np.abs(rc - 1) <= 0.05))
+    # the closure run is better than the plain coarse run everywhere it matters
+    assert ev["closure"]["spectrum_rel_err_mean"] < ev["plain"]["spectrum_rel_err_mean"]
+    assert ev["closure"]["flux_err"] < ev["plain"]["flux_err"]

@@ -141,6 +141,11 @@ def test_closure_inputs_targets_and_forcing(cuda):
     ka2 = np.where(upd, g.ka2, 1.0)
     assert rel_l2(fp.cpu().numpy(), np.where(upd, -t5 / ka2, 0)) < 1e-6
     assert rel_l2(fz.cpu().numpy(), t6) < 1e-6
+    # a low-k cutoff removes exactly the modes with |k| < kmin
+    fp, fz = L.closure_forcing(tgt, n, (1.0, 1.0), kmin=4.0)
+    hi = upd & (g.ka2 >= 16.0)
+    assert rel_l2(fp.cpu().numpy(), np.where(hi, -t5 / ka2, 0)) < 1e-6
+    assert np.all(fz.cpu().numpy()[~hi] == 0)
 
 
 def test_closure_run_without_network_is_the_plain_solver(cuda):
@@ -169,7 +174,7 @@ def _fused_forcing(n, ps, vz, net, sin, sout, cuda):
     _lib.check(lib.r2d3c_closure_net_inputs(pl.handle, S._ptr(ps), S._ptr(vz), L._dvec(sin),
                                             net.handle, S._stream(cuda)), "net inputs")
     _lib.check(lib.r2d3c_cnn_forward_buffers(net.handle, 1, S._stream(cuda)), "forward")
-    _lib.check(lib.r2d3c_closure_net_forcing(pl.handle, net.handle, L._dvec(sout), S._ptr(fp),
+    _lib.check(lib.r2d3c_closure_net_forcing(pl.handle, net.handle, L._dvec(sout), 0.0, S._ptr(fp),
                                              S._ptr(fz), S._stream(cuda)), "net forcing")
     return fp, fz
 
