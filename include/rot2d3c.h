@@ -81,7 +81,9 @@ typedef struct r2d3c_forcing {
 /* Creates the plan of an n x n grid on the current HIP device: twiddle tables
  * and the workspace of the fused step.  Replaces fftp2d_create_plan
  * (src/fftp2D.f90:30-63) and the wavenumber set-up (src/Rot2D3C.para.f90:213-228).
- * n: power of two, 16 <= n <= 4096. */
+ * n: power of two, 16 <= n <= 4096, or 16384.  At 16384 the step runs stage
+ * by stage from full 2D transforms (no fused kernels, iflow = 2, ensembles or
+ * closure); the plan then holds about 41 GB. */
 int r2d3c_plan_create(r2d3c_plan **plan, int n);
 /* Replaces fftp2d_destroy_plan (src/fftp2D.f90:66-86). */
 int r2d3c_plan_destroy(r2d3c_plan *plan);
@@ -276,11 +278,12 @@ int r2d3c_filter(r2d3c_plan *fine, r2d3c_plan *coarse, const double *in, double 
 int r2d3c_closure_inputs(r2d3c_plan *plan, const double *psi, const double *vz,
                          const double *scale, float *out, void *stream);
 /* Training targets: fp32 [n*n][2] = real-space (f5 - c5, f6 - c6) restricted to
- * the update mask, times scale[2]; f5, f6 the coarse-grained fine brackets
- * poisson(psi, laplak2(psi)) and poisson(psi, vz), c5, c6 the same brackets of
- * the coarse-grained fields. */
+ * the update mask and |k| >= kmin, times scale[2]; f5, f6 the coarse-grained
+ * fine brackets poisson(psi, laplak2(psi)) and poisson(psi, vz), c5, c6 the
+ * same brackets of the coarse-grained fields. */
 int r2d3c_closure_targets(r2d3c_plan *plan, const double *f5, const double *c5, const double *f6,
-                          const double *c6, const double *scale, float *out, void *stream);
+                          const double *c6, const double *scale, double kmin, float *out,
+                          void *stream);
 /* Dense forcing of the closure from a prediction fp32 [n*n][2] (times
  * scale[2]): fp = -r2c(pred0)/k^2 and fz = r2c(pred1) on the update mask and
  * |k| >= kmin (the subgrid parts of -C5/k^2 and C6 in the stage update). */

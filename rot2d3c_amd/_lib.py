@@ -81,8 +81,8 @@ SIGNATURES = {
                                             ctypes.POINTER(_c_double), _c_double]),
     "r2d3c_filter": (_c_int, [_vp, _vp, _vp, _vp, _c_double, _vp]),
     "r2d3c_closure_inputs": (_c_int, [_vp, _vp, _vp, ctypes.POINTER(_c_double), _vp, _vp]),
-    "r2d3c_closure_targets": (_c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_c_double), _vp,
-                                       _vp]),
+    "r2d3c_closure_targets": (_c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_c_double),
+                                       _c_double, _vp, _vp]),
     "r2d3c_closure_forcing": (_c_int, [_vp, _vp, ctypes.POINTER(_c_double), _c_double, _vp, _vp,
                                        _vp]),
     "r2d3c_cnn_param_count": (_c_int, []),

@@ -164,8 +164,9 @@ def network_inputs(ps, vz, scale=(1.0, 1.0, 1.0), out=None):
     return out
 
 
-def snapshot(ps_f, vz_f, n_coarse, width=None):
-    """(coarse psi, coarse vz, inputs [n,n,3], targets [n,n,2]) of one fine state."""
+def snapshot(ps_f, vz_f, n_coarse, width=None, kmin=0.0):
+    """(coarse psi, coarse vz, inputs [n,n,3], targets [n,n,2]) of one fine
+    state; the targets keep the modes |k| >= kmin."""
     ps_c = coarse_grain(ps_f, n_coarse, width)
     vz_c = coarse_grain(vz_f, n_coarse, width)
     f5 = coarse_grain(S.poisson(ps_f, S.laplak2(ps_f)), n_coarse, width)
@@ -175,7 +176,8 @@ def snapshot(ps_f, vz_f, n_coarse, width=None):
     tgt = torch.empty((n_coarse, n_coarse, 2), dtype=torch.float32, device=ps_f.device)
     pl = S.Plan.get(n_coarse, ps_f.device)
     check(_lib_().r2d3c_closure_targets(pl.handle, S._ptr(f5), S._ptr(c5), S._ptr(f6), S._ptr(c6),
-                                        _dvec((1.0, 1.0)), _fptr(tgt), S._stream(ps_f.device)),
+                                        _dvec((1.0, 1.0)), float(kmin), _fptr(tgt),
+                                        S._stream(ps_f.device)),
           "r2d3c_closure_targets")
     return ps_c, vz_c, network_inputs(ps_c, vz_c), tgt
 

@@ -191,6 +191,8 @@ def main(argv=None):
     ap.add_argument("--eval-steps", type=int, default=50000)
     ap.add_argument("--kmin", type=float, nargs="+", default=[0.0, 4.0, 8.0, 16.0],
                     help="candidate low-k cutoffs of the closure forcing")
+    ap.add_argument("--target-kmin", type=float, default=0.0,
+                    help="train on the targets' modes |k| >= target_kmin only")
     ap.add_argument("--realisations", type=int, default=4,
                     help="forcing realisations of each coarse run (averaged)")
     ap.add_argument("--epochs", type=int, default=30)
@@ -244,7 +246,7 @@ def main(argv=None):
     Y = torch.empty((a.snapshots, nc, nc, 2), dtype=torch.float32, device=dev)
     for k in range(a.snapshots):
         advance(every)
-        _, _, x, y = L.snapshot(ps, vz, nc)
+        _, _, x, y = L.snapshot(ps, vz, nc, kmin=a.target_kmin)
         X[k].copy_(x)
         Y[k].copy_(y)
         if k % 100 == 0:

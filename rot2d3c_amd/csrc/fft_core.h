@@ -103,10 +103,11 @@ template <int R, int S> __device__ __forceinline__ void dftR(cplx *x) {
 // Unnormalised DFT of length N, sign S (-1 forward, +1 inverse), of the
 // canonical register set v of thread b (0 <= b < N/8).  lds points at this
 // transform's LDS area; consecutive slots are IL entries apart, so IL
-// transforms can share one interleaved LDS array.  All threads of the block
+// transforms can share one interleaved LDS array.  tw holds TS*N entries
+// (exp(-2 pi i t / (TS N))): a longer transform's table serves its halves.  All threads of the block
 // must call this together (it contains block barriers); the LDS area is free
 // again on return.
-template <int N, int S, int IL>
+template <int N, int S, int IL, int TS = 1>
 __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
                                           const cplx *__restrict__ tw) {
   using F = Shape<N>;
@@ -127,7 +128,7 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
     const int st = (N / (8 * L)) * j;
 #if R2D_TW_DERIVE
     {  // three table loads; the other powers by products (error <= 2 ulp more)
-      const cplx w1 = twid<S>(tw, st), w2 = twid<S>(tw, 2 * st), w4 = twid<S>(tw, 4 * st);
+      const cplx w1 = twid<S>(tw, TS * (st)), w2 = twid<S>(tw, TS * (2 * st)), w4 = twid<S>(tw, TS * (4 * st));
       const cplx w3 = cmul(w1, w2);
       x[1] = cmul(x[1], w1); x[2] = cmul(x[2], w2); x[3] = cmul(x[3], w3);
       x[4] = cmul(x[4], w4); x[5] = cmul(x[5], cmul(w1, w4));
@@ -135,7 +136,7 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
     }
 #else
 #pragma unroll
-    for (int m = 1; m < 8; ++m) x[m] = cmul(x[m], twid<S>(tw, st * m));
+    for (int m = 1; m < 8; ++m) x[m] = cmul(x[m], twid<S>(tw, TS * (st * m)));
 #endif
     dft8<S>(x);
     const int base = (b - j) * 8 + j;
@@ -154,12 +155,12 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
     for (int m = 0; m < RL; ++m) x[m] = lds[pad(j + m * (N / RL)) * IL];
 #if R2D_TW_DERIVE
     if constexpr (RL == 2) {
-      x[1] = cmul(x[1], twid<S>(tw, j));
+      x[1] = cmul(x[1], twid<S>(tw, TS * (j)));
     } else if constexpr (RL == 4) {
-      const cplx w1 = twid<S>(tw, j), w2 = twid<S>(tw, 2 * j);
+      const cplx w1 = twid<S>(tw, TS * (j)), w2 = twid<S>(tw, TS * (2 * j));
       x[1] = cmul(x[1], w1); x[2] = cmul(x[2], w2); x[3] = cmul(x[3], cmul(w1, w2));
     } else {
-      const cplx w1 = twid<S>(tw, j), w2 = twid<S>(tw, 2 * j), w4 = twid<S>(tw, 4 * j);
+      const cplx w1 = twid<S>(tw, TS * (j)), w2 = twid<S>(tw, TS * (2 * j)), w4 = twid<S>(tw, TS * (4 * j));
       const cplx w3 = cmul(w1, w2);
       x[1] = cmul(x[1], w1); x[2] = cmul(x[2], w2); x[3] = cmul(x[3], w3);
       x[4] = cmul(x[4], w4); x[5] = cmul(x[5], cmul(w1, w4));
@@ -167,7 +168,7 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
     }
 #else
 #pragma unroll
-    for (int m = 1; m < RL; ++m) x[m] = cmul(x[m], twid<S>(tw, j * m));
+    for (int m = 1; m < RL; ++m) x[m] = cmul(x[m], twid<S>(tw, TS * (j * m)));
 #endif
     dftR<RL, S>(x);
 #pragma unroll

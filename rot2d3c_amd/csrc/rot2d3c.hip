@@ -1229,6 +1229,7 @@ struct r2d3c_plan {
   cplx *mfull = nullptr;          // 4 full-range mixed fields
   double *real = nullptr;         // n*n real scratch
   double *real2 = nullptr;        // second n*n real scratch (extrema)
+  double *real3 = nullptr;        // third n*n real scratch (n = 16384 brackets)
   cplx *spec3 = nullptr;          // three spectral scratch fields (transfers)
   double *dpart = nullptr;        // diagnostics partials
   cplx *ci = nullptr;             // iflow = 2: C1..C6, fpre, fp, fz
@@ -1582,6 +1583,17 @@ static int run_steps(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
 }
 
 
+// n = 16384 (definitions at the end of this file)
+constexpr int kBigN = 16384, kBigH = 8192, kBigT = 1024;
+static int big_c2r(r2d3c_plan *p, const cplx *in, int op, double *out, hipStream_t s);
+static int big_r2c(r2d3c_plan *p, const double *in, cplx *out, hipStream_t s);
+static int big_poisson(r2d3c_plan *p, const cplx *a, const cplx *b, cplx *c, hipStream_t s);
+static int big_cfl(r2d3c_plan *p, const r2d3c_params *q, double cfl, const cplx *psi, double *d_dt,
+                   hipStream_t s);
+static int big_step(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
+                    const r2d3c_forcing *fo, int dt_mode, double dt_or_cfl, double *d_dt,
+                    hipStream_t s);
+
 extern "C" {
 
 const char *r2d3c_last_error(void) { return g_err.c_str(); }
@@ -1589,7 +1601,8 @@ const char *r2d3c_last_error(void) { return g_err.c_str(); }
 int r2d3c_plan_create(r2d3c_plan **out, int n) {
   if (!out) return set_err(R2D3C_EINVAL, "plan pointer is NULL");
   *out = nullptr;
-  if (n < 16 || n > 4096 || (n & (n - 1))) return set_err(R2D3C_ENOTSUP, "n must be a power of two in [16, 4096]");
+  if (n < 16 || (n > 4096 && n != 16384) || (n & (n - 1)))
+    return set_err(R2D3C_ENOTSUP, "n must be a power of two in [16, 4096], or 16384");
   r2d3c_plan *p = new r2d3c_plan();
   p->n = n;
   HIPCHK(hipGetDevice(&p->device));
@@ -1614,7 +1627,10 @@ int r2d3c_plan_create(r2d3c_plan **out, int n) {
     const long double a = twopi * (long double)t / (long double)n;
     tw[t] = make_double2((double)cosl(a), (double)(-sinl(a)));
   }
-  size_t b_tw = sizeof(cplx) * n, b_ms = sizeof(cplx) * 8 * p->fs, b_mf = sizeof(cplx) * 4 * p->fsf;
+  // n = 16384 steps stage by stage: no fused-step workspace, a third real field
+  const bool big = n == 16384;
+  size_t b_tw = sizeof(cplx) * n, b_ms = big ? 256 : sizeof(cplx) * 8 * p->fs,
+         b_mf = sizeof(cplx) * 4 * p->fsf;
   size_t b_re = sizeof(double) * (size_t)n * n;
   hipError_t e;
   if ((e = hipMalloc(&p->tw, b_tw)) != hipSuccess || (e = hipMalloc(&p->mstep, b_ms)) != hipSuccess ||
@@ -1627,13 +1643,14 @@ int r2d3c_plan_create(r2d3c_plan **out, int n) {
       (e = hipMalloc(&p->ci, 9 * sizeof(cplx) * (size_t)(n / 2 + 1) * n)) != hipSuccess ||
       (e = hipMalloc(&p->ci_rtab, sizeof(float) * (size_t)(n / 2 + 1) * n)) != hipSuccess ||
       (e = hipMalloc(&p->ci_efp, 256)) != hipSuccess ||
+      (big && (e = hipMalloc(&p->real3, b_re)) != hipSuccess) ||
       (e = hipMemcpy(p->tw, tw.data(), b_tw, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemset(p->mstep, 0, b_ms)) != hipSuccess || (e = hipMemset(p->mfull, 0, b_mf)) != hipSuccess) {
     int rc = hip_err(e, "plan allocation");
     r2d3c_plan_destroy(p);
     return rc;
   }
-  p->bytes = (long long)(b_tw + b_ms + b_mf + 2 * b_re + 512 + sizeof(double) * kReduceBlocks +
+  p->bytes = (long long)(b_tw + b_ms + b_mf + (big ? 3 : 2) * b_re + 512 + sizeof(double) * kReduceBlocks +
                          3 * sizeof(cplx) * (size_t)(n / 2 + 1) * n +
                          sizeof(double) * kCondSums * kReduceBlocks +
                          9 * sizeof(cplx) * (size_t)(n / 2 + 1) * n +
@@ -1649,6 +1666,7 @@ int r2d3c_plan_destroy(r2d3c_plan *p) {
   (void)hipFree(p->mstep);
   (void)hipFree(p->mfull);
   (void)hipFree(p->real);
+  if (p->real3) (void)hipFree(p->real3);
   (void)hipFree(p->maxbits);
   (void)hipFree(p->dtslot);
   (void)hipFree(p->partial);
@@ -1693,11 +1711,13 @@ int r2d3c_plan_size(const r2d3c_plan *p) { return p ? p->n : 0; }
 
 int r2d3c_c2r(r2d3c_plan *p, const double *in, double *out, void *stream) {
   NEED(p); NEED(in); NEED(out);
+  if (p->n == kBigN) return big_c2r(p, (const cplx *)in, OP_RAW, out, (hipStream_t)stream);
   R2D_DISPATCH(p->n, run_c2r, p, (const cplx *)in, out, (hipStream_t)stream);
 }
 
 int r2d3c_r2c(r2d3c_plan *p, const double *in, double *out, void *stream) {
   NEED(p); NEED(in); NEED(out);
+  if (p->n == kBigN) return big_r2c(p, in, (cplx *)out, (hipStream_t)stream);
   R2D_DISPATCH(p->n, run_r2c, p, in, (cplx *)out, (hipStream_t)stream);
 }
 
@@ -1722,6 +1742,8 @@ int r2d3c_inv_laplak2(r2d3c_plan *p, const double *a, double *b, void *stream) {
 
 int r2d3c_poisson(r2d3c_plan *p, const double *a, const double *b, double *c, void *stream) {
   NEED(p); NEED(a); NEED(b); NEED(c);
+  if (p->n == kBigN)
+    return big_poisson(p, (const cplx *)a, (const cplx *)b, (cplx *)c, (hipStream_t)stream);
   R2D_DISPATCH(p->n, run_poisson, p, (const cplx *)a, (const cplx *)b, (cplx *)c,
                (hipStream_t)stream);
 }
@@ -1759,6 +1781,7 @@ int r2d3c_cfl(r2d3c_plan *p, const r2d3c_params *q, double cfl, const double *ps
   NEED(p); NEED(psi); NEED(d_dt);
   int rc = check_params(q);
   if (rc) return rc;
+  if (p->n == kBigN) return big_cfl(p, q, cfl, (const cplx *)psi, d_dt, (hipStream_t)stream);
   R2D_DISPATCH(p->n, run_cfl, p, q, cfl, (const cplx *)psi, d_dt, (hipStream_t)stream);
 }
 
@@ -1771,6 +1794,7 @@ int r2d3c_step(r2d3c_plan *p, const r2d3c_params *q, double *psi, double *vz,
   if (dt_mode != R2D3C_DT_CFL && dt_mode != R2D3C_DT_FIXED)
     return set_err(R2D3C_EINVAL, "dt_mode must be R2D3C_DT_CFL or R2D3C_DT_FIXED");
   if (psi == vz) return set_err(R2D3C_EINVAL, "psi and vz must not alias");
+  if (p->n == kBigN) return r2d3c_run(p, q, psi, vz, forcing, 1, dt_mode, dt_or_cfl, d_dt, stream);
   R2D_DISPATCH(p->n, run_steps, p, q, (cplx *)psi, (cplx *)vz, forcing, 1, dt_mode, dt_or_cfl,
                d_dt, (hipStream_t)stream);
 }
@@ -1786,6 +1810,16 @@ int r2d3c_run(r2d3c_plan *p, const r2d3c_params *q, double *psi, double *vz,
   if (psi == vz) return set_err(R2D3C_EINVAL, "psi and vz must not alias");
   if (n_steps < 0) return set_err(R2D3C_EINVAL, "n_steps must be >= 0");
   if (n_steps == 0) return 0;
+  if (p->n == kBigN) {
+    for (int st = 0; st < n_steps; ++st) {
+      const r2d3c_forcing *f = forcing ? forcing + st : nullptr;
+      if (f && f->ci_enabled) return set_err(R2D3C_ENOTSUP, "iflow = 2 is not supported at n = 16384");
+      rc = big_step(p, q, (cplx *)psi, (cplx *)vz, f, dt_mode, dt_or_cfl, d_dt ? d_dt + st : nullptr,
+                    (hipStream_t)stream);
+      if (rc) return rc;
+    }
+    return 0;
+  }
   R2D_DISPATCH(p->n, run_steps, p, q, (cplx *)psi, (cplx *)vz, forcing, n_steps, dt_mode,
                dt_or_cfl, d_dt, (hipStream_t)stream);
 }
@@ -2523,7 +2557,8 @@ int r2d3c_closure_inputs(r2d3c_plan *p, const double *psi, const double *vz, con
 }
 
 int r2d3c_closure_targets(r2d3c_plan *p, const double *f5, const double *c5, const double *f6,
-                          const double *c6, const double *scale, float *out, void *stream) {
+                          const double *c6, const double *scale, double kmin, float *out,
+                          void *stream) {
   NEED(p); NEED(f5); NEED(c5); NEED(f6); NEED(c6); NEED(scale); NEED(out);
   hipStream_t s = (hipStream_t)stream;
   const int n = p->n;
@@ -2533,7 +2568,7 @@ int r2d3c_closure_targets(r2d3c_plan *p, const double *f5, const double *c5, con
   const double *a[2] = {f5, f6}, *b[2] = {c5, c6};
   for (int ch = 0; ch < 2; ++ch) {
     k_closure_spec<<<launch_grid(sz), 256, 0, s>>>((const cplx *)a[ch], (const cplx *)b[ch], t, n,
-                                                    p->g.kmax, p->g.tiny, 1, 0.0);
+                                                    p->g.kmax, p->g.tiny, 1, kmin * kmin);
     int rc = r2d3c_c2r(p, (const double *)t, p->real, stream);
     if (rc) return rc;
     k_chan_pack<<<launch_grid(nn), 256, 0, s>>>(p->real, nn, scale[ch] * inv, ch, 2, out);
@@ -2736,3 +2771,270 @@ int r2d3c_closure_net_forcing(r2d3c_plan *p, r2d3c_cnn *net, const double *scale
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// n = 16384.  A length-16384 transform exceeds one block's LDS and threads, so
+// it is done as two 8192-point halves (the even and the odd samples) in one
+// block of 1024 threads, 16 points per thread, then a radix-2 combine in
+// registers.  The fused step's row kernel would need more registers than
+// 1024 threads have, so the step runs stage by stage from full 2D transforms
+// through HBM (poisson by three real fields), like the iflow = 2 step.
+// ---------------------------------------------------------------------------
+
+// In: ev[q] = x[2(b + 1024 q)], od[q] = x[2(b + 1024 q) + 1].
+// Out: ev[q] = X[b + 1024 q], od[q] = X[b + 1024 q + 8192] (sign S).
+template <int S>
+__device__ __forceinline__ void fft16k(cplx ev[8], cplx od[8], cplx *lds, int b,
+                                       const cplx *__restrict__ tw) {
+  // opaque copies of the thread index keep each half's index math from being
+  // hoisted over the other half (at 1024 threads it would spill)
+  int b1 = b, b2 = b;
+  asm volatile("" : "+v"(b1));
+  fft_canon<kBigH, S, 1, 2>(ev, lds, b1, tw);
+  asm volatile("" : "+v"(b2));
+  fft_canon<kBigH, S, 1, 2>(od, lds, b2, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const cplx t = cmul(od[q], twid<S>(tw, b + kBigT * q)), e = ev[q];
+    ev[q] = cadd(e, t);
+    od[q] = csub(e, t);
+  }
+}
+
+// column inverse of S[i][:] (with op, times scale) -> M(i, y)
+__global__ __launch_bounds__(kBigT) void k_big_col_inv(const cplx *__restrict__ spec,
+                                                        cplx *__restrict__ mix, int op,
+                                                        double scale, Geo g,
+                                                        const cplx *__restrict__ tw) {
+  constexpr int N = kBigN;
+  __shared__ cplx lds[Shape<kBigH>::LDSN];
+  const int i = blockIdx.x, b = threadIdx.x, kx = kxof(i, N);
+  cplx ev[8], od[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int j = 2 * (b + kBigT * q);
+    ev[q] = cscale(apply_op(op, spec[(size_t)i * N + j], kx, kyof(j, N), g.kmax, g.tiny), scale);
+    od[q] = cscale(apply_op(op, spec[(size_t)i * N + j + 1], kx, kyof(j + 1, N), g.kmax, g.tiny),
+                   scale);
+  }
+  fft16k<+1>(ev, od, lds, b, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int y = b + kBigT * q;
+    mix[tix(y, i, g.kpf)] = ev[q];
+    mix[tix(y + kBigH, i, g.kpf)] = od[q];
+  }
+}
+
+// rows y, y+1 of M (half spectra) -> R[y][:], R[y+1][:] as one packed transform
+__global__ __launch_bounds__(kBigT) void k_big_row_c2r(const cplx *__restrict__ mix,
+                                                        double *__restrict__ out, Geo g,
+                                                        const cplx *__restrict__ tw) {
+  constexpr int N = kBigN;
+  __shared__ cplx lds[Shape<kBigH>::LDSN];
+  const int b = threadIdx.x, y = 2 * blockIdx.x;
+  auto z = [&](int k) {
+    const bool mir = k > N / 2;
+    const int kk = mir ? N - k : k;
+    return packed_z<N>(k, mix[tix(y, kk, g.kpf)], mix[tix(y + 1, kk, g.kpf)], mir);
+  };
+  cplx ev[8], od[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int k = 2 * (b + kBigT * q);
+    ev[q] = z(k);
+    od[q] = z(k + 1);
+  }
+  fft16k<+1>(ev, od, lds, b, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int x = b + kBigT * q;
+    out[(size_t)y * N + x] = ev[q].x;
+    out[(size_t)(y + 1) * N + x] = ev[q].y;
+    out[(size_t)y * N + x + kBigH] = od[q].x;
+    out[(size_t)(y + 1) * N + x + kBigH] = od[q].y;
+  }
+}
+
+// rows y, y+1 of R as one packed forward transform: Z[y/2][k], k = 0..N-1
+__global__ __launch_bounds__(kBigT) void k_big_row_r2c(const double *__restrict__ in,
+                                                        cplx *__restrict__ zrows,
+                                                        const cplx *__restrict__ tw) {
+  constexpr int N = kBigN;
+  __shared__ cplx lds[Shape<kBigH>::LDSN];
+  const int b = threadIdx.x, y = 2 * blockIdx.x;
+  cplx ev[8], od[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int x = 2 * (b + kBigT * q);
+    ev[q] = mk(in[(size_t)y * N + x], in[(size_t)(y + 1) * N + x]);
+    od[q] = mk(in[(size_t)y * N + x + 1], in[(size_t)(y + 1) * N + x + 1]);
+  }
+  fft16k<-1>(ev, od, lds, b, tw);
+  cplx *zr = zrows + (size_t)blockIdx.x * N;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    zr[b + kBigT * q] = ev[q];
+    zr[b + kBigT * q + kBigH] = od[q];
+  }
+}
+
+// Column i of the half spectra, unpacked from the packed row transforms
+// (A = (Z[k] + conj Z[-k])/2 of row 2p, B = (Z[k] - conj Z[-k])/(2i) of row
+// 2p+1), forward transformed along y: S[i][:]
+__global__ __launch_bounds__(kBigT) void k_big_col_fwd(const cplx *__restrict__ zrows,
+                                                        cplx *__restrict__ spec,
+                                                        const cplx *__restrict__ tw) {
+  constexpr int N = kBigN;
+  __shared__ cplx lds[Shape<kBigH>::LDSN];
+  const int i = blockIdx.x, b = threadIdx.x;
+  cplx ev[8], od[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const cplx *zr = zrows + (size_t)(b + kBigT * q) * N;
+    const cplx w = zr[i], wm = zr[(N - i) & (N - 1)];
+    ev[q] = mk(0.5 * (w.x + wm.x), 0.5 * (w.y - wm.y));
+    od[q] = mk(0.5 * (w.y + wm.y), -0.5 * (w.x - wm.x));
+  }
+  fft16k<-1>(ev, od, lds, b, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    spec[(size_t)i * N + b + kBigT * q] = ev[q];
+    spec[(size_t)i * N + b + kBigT * q + kBigH] = od[q];
+  }
+}
+
+// a = a * b;  a = (a - b * c) / n4;  max of a^2 + b^2 (bit pattern, atomicMax)
+__global__ void k_real_mul(double *__restrict__ a, const double *__restrict__ b, size_t nn) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn;
+       e += (size_t)gridDim.x * blockDim.x)
+    a[e] = a[e] * b[e];
+}
+__global__ void k_real_bracket(double *__restrict__ a, const double *__restrict__ b,
+                               const double *__restrict__ c, double n4, size_t nn) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn;
+       e += (size_t)gridDim.x * blockDim.x)
+    a[e] = (a[e] - b[e] * c[e]) / n4;
+}
+__global__ __launch_bounds__(256) void k_real_gradmax(const double *__restrict__ a,
+                                                       const double *__restrict__ b, size_t nn,
+                                                       unsigned long long *maxbits) {
+  double m = 0.0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn;
+       e += (size_t)gridDim.x * blockDim.x)
+    m = fmax(m, a[e] * a[e] + b[e] * b[e]);
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(m));
+}
+
+// The stage update of src/Rot2D3C.para.f90:578-599 with the step's forcing:
+// rand_force draws (amplitude from the device dt) and/or dense arrays.
+__global__ void k_update_forced(const cplx *__restrict__ ps, const cplx *__restrict__ vz, cplx *c1,
+                                cplx *c2, const cplx *__restrict__ c3, const cplx *__restrict__ c4,
+                                const cplx *__restrict__ c5, const cplx *__restrict__ c6,
+                                StepConst c, const double *__restrict__ dtp, int o) {
+  const int n = c.g.n;
+  const size_t total = (size_t)(n / 2 + 1) * n;
+  const double dt = dtp ? *dtp : c.dt_fixed;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    const double kx = kxof(i, n), ky = kyof(j, n), ka2 = kx * kx + ky * ky;
+    cplx a = mk(0.0, 0.0), b = mk(0.0, 0.0);
+    if (ka2 <= c.g.kmax && ka2 > c.g.tiny) {
+      cplx f = mk(0.0, 0.0), g = mk(0.0, 0.0);
+      for (int q = 0; q < c.fp.nent; ++q)
+        if (c.fp.e[q].i == i && c.fp.e[q].j == j) f = force_value(c.fp, q, dt, c.n2);
+      for (int q = 0; q < c.fz.nent; ++q)
+        if (c.fz.e[q].i == i && c.fz.e[q].j == j) g = force_value(c.fz, q, dt, c.n2);
+      if (c.fpd) f = cadd(f, c.fpd[e]);
+      if (c.fzd) g = cadd(g, c.fzd[e]);
+      const double tmp1 = dt / (double)o, tmp2 = 1.0 / ka2;
+      double tmp3 = 1.0 + (c.nu * ipow(ka2, c.nn) + c.hnu * ipow(tmp2, c.mm)) * tmp1;
+      const cplx t5 = c5[e], t4 = c4[e];
+      cplx t = mk((-t5.x + c.two_omega * t4.x) * tmp2 + f.x, (-t5.y + c.two_omega * t4.y) * tmp2 + f.y);
+      t = cadd(ps[e], cscale(t, tmp1));
+      a = mk(t.x / tmp3, t.y / tmp3);
+      tmp3 = 1.0 + (c.nuv * ipow(ka2, c.nnv) + c.hnuv * ipow(tmp2, c.mmv)) * tmp1;
+      const cplx t6 = c6[e], t3 = c3[e];
+      t = mk(t6.x + c.two_omega * t3.x + g.x, t6.y + c.two_omega * t3.y + g.y);
+      t = cadd(vz[e], cscale(t, tmp1));
+      b = mk(t.x / tmp3, t.y / tmp3);
+    }
+    c1[e] = a;
+    c2[e] = b;
+  }
+}
+
+static int big_c2r(r2d3c_plan *p, const cplx *in, int op, double *out, hipStream_t s) {
+  k_big_col_inv<<<kBigN / 2 + 1, kBigT, 0, s>>>(in, p->mfull, op, 1.0, p->g, p->tw);
+  k_big_row_c2r<<<kBigN / 2, kBigT, 0, s>>>(p->mfull, out, p->g, p->tw);
+  LAUNCHCHK("c2r 16384");
+  return 0;
+}
+static int big_r2c(r2d3c_plan *p, const double *in, cplx *out, hipStream_t s) {
+  k_big_row_r2c<<<kBigN / 2, kBigT, 0, s>>>(in, p->mfull, p->tw);
+  k_big_col_fwd<<<kBigN / 2 + 1, kBigT, 0, s>>>(p->mfull, out, p->tw);
+  LAUNCHCHK("r2c 16384");
+  return 0;
+}
+// c = r2c{(c2r(d_x a) c2r(d_y b) - c2r(d_y a) c2r(d_x b))/n^4} (poisson, :169-226)
+static int big_poisson(r2d3c_plan *p, const cplx *a, const cplx *b, cplx *c, hipStream_t s) {
+  const size_t nn = (size_t)kBigN * kBigN;
+  const double n4 = (double)kBigN * kBigN * (double)kBigN * kBigN;
+  int rc;
+  if ((rc = big_c2r(p, a, OP_DX, p->real, s)) || (rc = big_c2r(p, b, OP_DY, p->real2, s))) return rc;
+  k_real_mul<<<launch_grid(nn), 256, 0, s>>>(p->real, p->real2, nn);
+  if ((rc = big_c2r(p, a, OP_DY, p->real2, s)) || (rc = big_c2r(p, b, OP_DX, p->real3, s))) return rc;
+  k_real_bracket<<<launch_grid(nn), 256, 0, s>>>(p->real, p->real2, p->real3, n4, nn);
+  LAUNCHCHK("poisson 16384");
+  return big_r2c(p, p->real, c, s);
+}
+static int big_cfl(r2d3c_plan *p, const r2d3c_params *q, double cfl, const cplx *psi, double *d_dt,
+                   hipStream_t s) {
+  const size_t nn = (size_t)kBigN * kBigN;
+  k_set_u64<<<1, 1, 0, s>>>(p->maxbits, 0ull);
+  int rc;
+  if ((rc = big_c2r(p, psi, OP_DX, p->real, s)) || (rc = big_c2r(p, psi, OP_DY, p->real2, s))) return rc;
+  k_real_gradmax<<<kReduceBlocks, 256, 0, s>>>(p->real, p->real2, nn, p->maxbits);
+  k_dt_from_max<<<1, 1, 0, s>>>(p->maxbits, make_dt_const(kBigN, q, cfl), d_dt);
+  LAUNCHCHK("cfl 16384");
+  return 0;
+}
+// one time step, stage by stage (src/Rot2D3C.para.f90:399,545-611)
+static int big_step(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
+                    const r2d3c_forcing *fo, int dt_mode, double dt_or_cfl, double *d_dt,
+                    hipStream_t s) {
+  StepConst c;
+  make_step_const(p, q, dt_mode, dt_or_cfl, &c);
+  if (fo) {
+    int rc = make_force_set(&fo->psi, kBigN, &c.fp);
+    if (!rc) rc = make_force_set(&fo->vz, kBigN, &c.fz);
+    if (rc) return rc;
+    c.fpd = (const cplx *)fo->dense_psi;
+    c.fzd = (const cplx *)fo->dense_vz;
+  }
+  const size_t sz = (size_t)(kBigN / 2 + 1) * kBigN;
+  cplx *C1 = p->ci, *C2 = C1 + sz, *C3 = C2 + sz, *C4 = C3 + sz, *C5 = C4 + sz, *C6 = C5 + sz;
+  double *dtp = d_dt ? d_dt : p->dtslot;
+  int rc;
+  if (dt_mode == R2D3C_DT_CFL) {
+    if ((rc = big_cfl(p, q, dt_or_cfl, psi, dtp, s))) return rc;
+  } else {
+    k_set_f64<<<1, 1, 0, s>>>(dtp, dt_or_cfl);
+  }
+  HIPCHK(hipMemcpyAsync(C1, psi, sizeof(cplx) * sz, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(C2, vz, sizeof(cplx) * sz, hipMemcpyDeviceToDevice, s));
+  const int g = launch_grid(sz);
+  for (int o = 4; o >= 1; --o) {
+    k_spec_op<<<g, 256, 0, s>>>(C1, C4, OP_LAP, kBigN, p->g.kmax, p->g.tiny);
+    if ((rc = big_poisson(p, C1, C4, C5, s)) || (rc = big_poisson(p, C1, C2, C6, s))) return rc;
+    k_spec_op<<<g, 256, 0, s>>>(C1, C3, OP_DY, kBigN, p->g.kmax, p->g.tiny);
+    k_spec_op<<<g, 256, 0, s>>>(C2, C4, OP_DY, kBigN, p->g.kmax, p->g.tiny);
+    k_update_forced<<<g, 256, 0, s>>>(psi, vz, C1, C2, C3, C4, C5, C6, c, dtp, o);
+  }
+  k_spec_op<<<g, 256, 0, s>>>(C1, psi, OP_MASK, kBigN, p->g.kmax, p->g.tiny);
+  k_spec_op<<<g, 256, 0, s>>>(C2, vz, OP_MASK, kBigN, p->g.kmax, p->g.tiny);
+  LAUNCHCHK("step 16384");
+  return 0;
+}

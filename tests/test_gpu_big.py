@@ -1,0 +1,100 @@
+"""GPU: the n = 16384 path (two 8192-point halves per transform, stage-by-stage
+step).  The oracle cannot run at this size, so the checks are properties that
+do not depend on it: transforms of a few Fourier modes against their closed
+form, the r2c/c2r round trip, single-mode Jacobians, and a step from a
+band-limited state against the same step at 512^2 (whose path is checked
+against the oracle elsewhere), rescaled by the grid factor."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+N = 16384
+
+
+def _modes_field(n, modes, device):
+    """Spectral array (reference layout, n^2 x coefficients) of
+    sum_m amp_m cos(kx x + ky y + ph_m), kx >= 0."""
+    a = torch.zeros((n // 2 + 1, n), dtype=torch.complex128, device=device)
+    for kx, ky, amp, ph in modes:
+        c = 0.5 * amp * complex(math.cos(ph), math.sin(ph)) * n * n
+        j = ky % n
+        if kx == 0:
+            a[0, j] += c
+            a[0, (-ky) % n] += c.conjugate()
+        else:
+            a[kx, j] += c
+    return a
+
+
+MODES = [(3, -2, 1.0, 0.3), (0, 5, 0.5, 1.1), (17, 40, 0.25, -0.7), (1000, -2000, 0.1, 2.0)]
+
+
+def test_c2r_of_modes_and_round_trip(cuda):
+    print("16384 c2r/r2c", flush=True)
+    from rot2d3c_amd import spectral as S
+    a = _modes_field(N, MODES, cuda)
+    r = S.fftp2d_complex_to_real(a)
+    rng = np.random.default_rng(0)
+    ys, xs = rng.integers(0, N, 400), rng.integers(0, N, 400)
+    got = r[torch.from_numpy(ys).to(cuda), torch.from_numpy(xs).to(cuda)].cpu().numpy()
+    x, y = 2 * np.pi * xs / N, 2 * np.pi * ys / N
+    expect = sum(amp * np.cos(kx * x + ky * y + ph) for kx, ky, amp, ph in MODES) * N * N
+    assert np.max(np.abs(got - expect)) <= 1e-9 * N * N
+    back = S.fftp2d_real_to_complex(r)
+    err = float((back - a * (N * N)).abs().max())
+    assert err <= 1e-9 * N * N * N * N
+    del r, back
+
+
+def test_poisson_of_two_modes(cuda):
+    """poisson(a, b) of single modes a = cos(k.x), b = cos(l.x) is the
+    analytic Jacobian a_x b_y - a_y b_x, times n^2 (spectral convention)."""
+    print("16384 poisson", flush=True)
+    from rot2d3c_amd import spectral as S
+    (k1, k2), (l1, l2) = (3, -2), (1, 4)
+    a = _modes_field(N, [(k1, k2, 1.0, 0.0)], cuda)
+    b = _modes_field(N, [(l1, l2, 1.0, 0.0)], cuda)
+    c = S.poisson(a, b)
+    r = S.fftp2d_complex_to_real(c) / float(N * N)
+    rng = np.random.default_rng(1)
+    ys, xs = rng.integers(0, N, 400), rng.integers(0, N, 400)
+    got = r[torch.from_numpy(ys).to(cuda), torch.from_numpy(xs).to(cuda)].cpu().numpy()
+    x, y = 2 * np.pi * xs / N, 2 * np.pi * ys / N
+    sa, sb = np.sin(k1 * x + k2 * y), np.sin(l1 * x + l2 * y)
+    expect = (k1 * l2 - k2 * l1) * sa * sb
+    assert np.max(np.abs(got - expect)) < 1e-10
+    del a, b, c, r
+
+
+def test_step_matches_512_for_band_limited_state(cuda):
+    """Two fixed-dt steps with rotation, dissipation and forcing from a state
+    on |k| <= 12: the 16384 path against the 512 fused path (both dealias far
+    above the band the steps reach)."""
+    print("16384 step vs 512", flush=True)
+    from rot2d3c_amd import spectral as S
+    rng = np.random.default_rng(2)
+    # amplitudes 1e-2: the brackets still move the result by ~1e-4, while the
+    # harmonics the 8 stages push past the 512 grid's cutoff stay ~1e-13
+    modes_p = [(int(rng.integers(0, 9)), int(rng.integers(-8, 9)), float(rng.uniform(0.2, 1)) * 1e-2,
+                float(rng.uniform(0, 6))) for _ in range(6)]
+    modes_v = [(int(rng.integers(0, 9)), int(rng.integers(-8, 9)), float(rng.uniform(0.2, 1)) * 1e-2,
+                float(rng.uniform(0, 6))) for _ in range(6)]
+    p = S.Params(1e-4, 0.1, 1, 2, 2e-4, 0.05, 1, 2, 3.0)
+    draws = [(S.ForceDraw(6, -5, 0.4, 1.0, 8.0), S.ForceDraw(3, 7, 1.3, 0.5, 1.0)),
+             (S.ForceDraw(-7, 2, 2.2, 1.0, 8.0), S.ForceDraw(0, 8, 0.1, 0.5, 1.0))]
+    out = {}
+    for n in (512, N):
+        ps = _modes_field(n, [m for m in modes_p if m[0] * m[0] + m[1] * m[1] > 0], cuda)
+        vz = _modes_field(n, [m for m in modes_v if m[0] * m[0] + m[1] * m[1] > 0], cuda)
+        S.rk4_run(ps, vz, p, draws, 2, dt=1e-3)
+        torch.cuda.synchronize()
+        k = 64  # compare the modes |kx|, |ky| < 64
+        ps_low = torch.cat([ps[:k, :k], ps[:k, n - k:]], dim=1) / float(n * n)
+        vz_low = torch.cat([vz[:k, :k], vz[:k, n - k:]], dim=1) / float(n * n)
+        out[n] = (ps_low.cpu().numpy(), vz_low.cpu().numpy())
+        del ps, vz
+    for a, b in zip(out[512], out[N]):
+        assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-10

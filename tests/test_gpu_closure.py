@@ -130,7 +130,8 @@ def test_closure_inputs_targets_and_forcing(cuda):
     tgt = torch.empty((n, n, 2), dtype=torch.float32, device=cuda)
     from rot2d3c_amd import _lib
     _lib.check(_lib.lib().r2d3c_closure_targets(pl.handle, *[S._ptr(t) for t in dev],
-                                                L._dvec((1.0, 1.0)), L._fptr(tgt), S._stream(cuda)),
+                                                L._dvec((1.0, 1.0)), 0.0, L._fptr(tgt),
+                                                S._stream(cuda)),
                "targets")
     t5 = np.where(upd, f5 - c5, 0)
     t6 = np.where(upd, f6 - c6, 0)
