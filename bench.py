@@ -95,7 +95,12 @@ def closure_bench(dns_ms_per_step, dns_dt, dev, steps=500, warmup=20):
     ens = E.Ensemble([E.Member(torch.from_numpy(ps).to(dev), torch.from_numpy(vz).to(dev), p, dt)],
                      schedules="auto")
     net = L.Net(1, n, n, dev, params=z["params"])
-    ens.set_closure(0, net, z["scale_in"], z["scale_out"], float(z["kmin"]))
+    if "out_clamp" in z:
+        net.set_output_clamp(float(z["out_clamp"]))
+    if "sign_clip" in z:
+        net.set_sign_clip(float(z["sign_clip"]))
+    ens.set_closure(0, net, z["scale_in"], z["scale_out"], float(z["kmin"]),
+                    float(z["kclip"]) if "kclip" in z else 0.0)
     spec = [dict(kf=40.0, f0=1.0, seed=3)]
     warm = ens.forcing_array(E.forcing_table(spec, warmup))
     table = ens.forcing_array(E.forcing_table(spec, steps))
@@ -120,6 +125,40 @@ def closure_bench(dns_ms_per_step, dns_dt, dev, steps=500, warmup=20):
             "shells_within_5pct": "%d/%d" % (ev["shells_within_5pct"], ev["shells"]),
             "errors_from": "rot2d3c_amd/data/closure_256_eval.json (k <= 85, 10 time units, "
                            "4 forcing realisations vs the coarse-grained 2048^2 DNS)"}
+
+
+def large_bench(steps, dev, warmup=2):
+    """The large single run: 16384^2 FP64, fixed dt = 2e-5, nu = nuv = 5e-6,
+    hnu = hnuv = 1.0, Omega = 10, forcing kf = 256 (psi and vz, injection
+    1.0), random-phase start.  At this size the step runs stage by stage from
+    full 2D transforms (two 8192-point halves per transform)."""
+    import torch
+    from rot2d3c_amd import ic, spectral as S
+    n, dt = 16384, 2e-5
+    ps0, vz0 = ic.random_phase_ic(n, seed=1234)
+    ps = torch.from_numpy(ps0).to(dev)
+    vz = torch.from_numpy(vz0).to(dev)
+    del ps0, vz0
+    p = S.Params(5e-6, 1.0, 1, 2, 5e-6, 1.0, 1, 2, 10.0)
+    sched = ic.forcing_schedule(256.0, 1.0, 1.0, 0, warmup + steps, S.rand_force_draw)
+    draws = [(S.ForceDraw(*a), S.ForceDraw(*b)) for a, b in sched]
+    S.rk4_run(ps, vz, p, draws[:warmup], warmup, dt=dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    S.rk4_run(ps, vz, p, draws[warmup:], steps, dt=dt)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    finite = bool(torch.isfinite(torch.view_as_real(ps)).all())
+    ms = wall * 1e3 / steps
+    b76 = 76 * 16 * n * (n // 2 + 1)
+    S.Plan.release(n, dev)
+    del ps, vz
+    torch.cuda.empty_cache()
+    return {"workload": "16384^2 FP64, dt=2e-5, nu=5e-6, nu+=1.0, Omega=10, kf=256, stage-by-stage step",
+            "steps": steps, "ms_per_step": round(ms, 3),
+            "value": round(n * n * steps / wall / 1e6, 2), "unit": "Mpts·RK4-steps/s",
+            "model_76S_frac_of_peak": round(b76 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+            "finite": finite}
 
 
 def ensemble_bench(steps, dev, tuned, warmup=10):
@@ -166,6 +205,8 @@ def main():
                     help="also run one 512 MiB device copy (PMC byte calibration)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default row schedule instead of the cached autotuned one")
+    ap.add_argument("--large-steps", type=int, default=200,
+                    help="steps of the 16384^2 run (0 disables)")
     ap.add_argument("--no-closure", action="store_true",
                     help="skip the closure-vs-DNS timing")
     ap.add_argument("--ensemble-steps", type=int, default=500,
@@ -267,6 +308,10 @@ def main():
     if not args.no_closure:
         clo = closure_bench(ms_per_step, dt, dev)
 
+    big = None
+    if args.large_steps > 0:
+        big = large_bench(args.large_steps, dev)
+
     ens = None
     if args.ensemble_steps > 0:
         del ps, vz
@@ -303,6 +348,7 @@ def main():
         "finite": finite,
         "ensemble": ens,
         "closure": clo,
+        "large": big,
     }
     print(json.dumps(out), flush=True)
 

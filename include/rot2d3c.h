@@ -313,6 +313,18 @@ int r2d3c_cnn_forward(r2d3c_cnn *net, const float *d_in, int batch, float *d_out
 int r2d3c_cnn_buffers(r2d3c_cnn *net, void **input_bf16, const float **output_f32);
 int r2d3c_cnn_shape(const r2d3c_cnn *net, int *batch, int *height, int *width);
 int r2d3c_cnn_forward_buffers(r2d3c_cnn *net, int batch, void *stream);
+/* Inference calls (r2d3c_cnn_forward, r2d3c_cnn_forward_buffers) saturate
+ * the outputs to [-clamp, clamp] when clamp > 0 (0, the default: none), which
+ * bounds the network's response to inputs far outside its training data;
+ * gradients and training steps use the unsaturated network. */
+int r2d3c_cnn_set_output_clamp(r2d3c_cnn *net, double clamp);
+/* Inference sign clipping of strength s in [0, 1] (0, the default: off):
+ * output channels 0 and 1 are multiplied by 1 - s wherever they have the sign
+ * of input channels 0 and 1 (after the clamp); at s = 1 the prediction can
+ * only damp those input fields pointwise.  The closure uses it to limit local
+ * production of vorticity^2 and vz^2.  Gradients and training steps are
+ * unaffected. */
+int r2d3c_cnn_set_sign_clip(r2d3c_cnn *net, double strength);
 /* Forward, MSE loss (mean over pixels and both output channels, in d_loss),
  * backward; the parameter gradient goes to d_grad when it is not NULL. */
 int r2d3c_cnn_gradients(r2d3c_cnn *net, const float *d_in, const float *d_target, int batch,
@@ -330,16 +342,25 @@ int r2d3c_closure_net_inputs(r2d3c_plan *plan, const double *psi, const double *
                              const double *scale, r2d3c_cnn *net, void *stream);
 int r2d3c_closure_net_forcing(r2d3c_plan *plan, r2d3c_cnn *net, const double *scale,
                               double kmin, double *fp, double *fz, void *stream);
+/* Backscatter clipping of a closure forcing, in place, on the modes
+ * |k| >= kclip (kclip > 0): at each such mode where fp injects energy into psi
+ * (Re(conj(psi) fp) > 0) its component along psi is removed,
+ * fp -= Re(conj(psi) fp)/|psi|^2 psi, and likewise fz against vz, so that no
+ * such mode gains energy from the closure. */
+int r2d3c_closure_clip(r2d3c_plan *plan, const double *psi, const double *vz, double kclip,
+                       double *fp, double *fz, void *stream);
 
 /* Makes run `run` of an ensemble a closure run: each of its steps starts with
  * r2d3c_closure_net_inputs(scale_in) -> r2d3c_cnn_forward_buffers ->
- * r2d3c_closure_net_forcing(scale_out, kmin) on the step's state, and the stages add that forcing (held for
+ * r2d3c_closure_net_forcing(scale_out, kmin) (then r2d3c_closure_clip(kclip)
+ * if kclip > 0) on the step's state, and the stages add that forcing (held for
  * the four stages), all inside the step's graph.  net = NULL removes the
  * closure (the plain solver).  The net must take 1 image of n x n, serve
  * no other run (its buffers hold this run's activations) and stay alive while
  * the ensemble uses it. */
 int r2d3c_ensemble_set_closure(r2d3c_ensemble *ens, int run, r2d3c_cnn *net,
-                               const double *scale_in, const double *scale_out, double kmin);
+                               const double *scale_in, const double *scale_out, double kmin,
+                               double kclip);
 
 #ifdef __cplusplus
 }

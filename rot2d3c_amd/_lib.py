@@ -78,7 +78,7 @@ SIGNATURES = {
     "r2d3c_ensemble_set_schedule": (_c_int, [_vp, _c_int, _c_int, _c_int]),
     "r2d3c_ensemble_run": (_c_int, [_vp, ctypes.POINTER(Forcing), _c_int, _vp]),
     "r2d3c_ensemble_set_closure": (_c_int, [_vp, _c_int, _vp, ctypes.POINTER(_c_double),
-                                            ctypes.POINTER(_c_double), _c_double]),
+                                            ctypes.POINTER(_c_double), _c_double, _c_double]),
     "r2d3c_filter": (_c_int, [_vp, _vp, _vp, _vp, _c_double, _vp]),
     "r2d3c_closure_inputs": (_c_int, [_vp, _vp, _vp, ctypes.POINTER(_c_double), _vp, _vp]),
     "r2d3c_closure_targets": (_c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_c_double),
@@ -98,9 +98,12 @@ SIGNATURES = {
     "r2d3c_cnn_shape": (_c_int, [_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),
                                  ctypes.POINTER(_c_int)]),
     "r2d3c_cnn_forward_buffers": (_c_int, [_vp, _c_int, _vp]),
+    "r2d3c_cnn_set_output_clamp": (_c_int, [_vp, _c_double]),
+    "r2d3c_cnn_set_sign_clip": (_c_int, [_vp, _c_double]),
     "r2d3c_closure_net_inputs": (_c_int, [_vp, _vp, _vp, ctypes.POINTER(_c_double), _vp, _vp]),
     "r2d3c_closure_net_forcing": (_c_int, [_vp, _vp, ctypes.POINTER(_c_double), _c_double, _vp,
                                            _vp, _vp]),
+    "r2d3c_closure_clip": (_c_int, [_vp, _vp, _vp, _c_double, _vp, _vp, _vp]),
     "r2d3c_randu": (ctypes.c_float, [ctypes.POINTER(_c_int)]),
     "r2d3c_rand_force_draw": (None, [_c_double, ctypes.POINTER(_c_int),
                                      ctypes.POINTER(_c_double), ctypes.POINTER(_c_double),

@@ -114,6 +114,19 @@ class Net:
                                         self._stream()), "r2d3c_cnn_forward")
         return out
 
+    def set_output_clamp(self, clamp):
+        """Saturate inference outputs to [-clamp, clamp] (0: none); training
+        and gradients are unaffected."""
+        check(_lib_().r2d3c_cnn_set_output_clamp(self.handle, float(clamp)),
+              "r2d3c_cnn_set_output_clamp")
+
+    def set_sign_clip(self, strength):
+        """Inference only: scale output channel c (c = 0, 1) by 1 - strength
+        wherever it has the sign of input channel c (strength 1: the
+        prediction only damps those fields pointwise; 0: off)."""
+        check(_lib_().r2d3c_cnn_set_sign_clip(self.handle, float(strength)),
+              "r2d3c_cnn_set_sign_clip")
+
     def gradients(self, x, y):
         self._check(x, 3)
         self._check(y, 2)
@@ -154,6 +167,27 @@ def coarse_grain(a_fine, n_coarse, width=None):
     return out
 
 
+def box_gain(kx, ky, n_coarse, width=None):
+    """The coarse-graining's transfer function at wavevector (kx, ky):
+    sinc(kx w/2) sinc(ky w/2), w the filter width (one coarse cell)."""
+    w = 2.0 * math.pi / n_coarse if width is None else width
+    g = 1.0
+    for k in (kx, ky):
+        a = 0.5 * float(k) * w
+        g *= 1.0 if a == 0.0 else math.sin(a) / a
+    return g
+
+
+def filtered_draw(draw, n_coarse, width=None):
+    """The rand_force draw `draw` (spectral.ForceDraw) coarse-grained: the
+    filtered equation is forced by G f, whose amplitude is G times the draw's,
+    i.e. injection rate f0 G^2 (the wavevector and phase are unchanged)."""
+    if draw is None:
+        return None
+    g = box_gain(draw.kx, draw.ky, n_coarse, width)
+    return S.ForceDraw(draw.kx, draw.ky, draw.phase, draw.f0 * g * g, draw.kupk)
+
+
 def network_inputs(ps, vz, scale=(1.0, 1.0, 1.0), out=None):
     """fp32 [n, n, 3]: real-space (laplacian of psi, vz, psi) times scale."""
     n = S._spec_n(ps)
@@ -192,6 +226,21 @@ def closure_forcing(pred, n, scale, fp=None, fz=None, kmin=0.0):
     check(_lib_().r2d3c_closure_forcing(pl.handle, _fptr(pred), _dvec(scale), float(kmin),
                                         S._ptr(fp), S._ptr(fz), S._stream(dev)),
           "r2d3c_closure_forcing")
+    return fp, fz
+
+
+def clip_forcing(ps, vz, fp, fz, kclip=1.0):
+    """Backscatter clipping of a closure forcing (fp, fz), in place, on the
+    modes |k| >= kclip (default: all): where it injects energy
+    (Re(conj(psi) fp) > 0, Re(conj(vz) fz) > 0) its component along the
+    mode's coefficient is removed."""
+    n = S._spec_n(ps)
+    for t in (vz, fp, fz):
+        if S._spec_n(t) != n:
+            raise ValueError("clip_forcing: arrays of one grid size expected")
+    pl = S.Plan.get(n, ps.device)
+    check(_lib_().r2d3c_closure_clip(pl.handle, S._ptr(ps), S._ptr(vz), float(kclip), S._ptr(fp),
+                                     S._ptr(fz), S._stream(ps.device)), "r2d3c_closure_clip")
     return fp, fz
 
 
@@ -245,15 +294,17 @@ class ClosureRun:
 
     scale_in / scale_out: the per-channel normalisation of the network's
     inputs and outputs (1/std of the training inputs, std of the targets).
-    With net=None (or after a fallback) it is the plain coarse solver."""
+    With net=None (or after a fallback) it is the plain coarse solver; with
+    kclip > 0 the closure forcing is backscatter-clipped on |k| >= kclip
+    (clip_forcing)."""
 
     def __init__(self, ps, vz, params, net=None, scale_in=None, scale_out=None, guard_every=50,
-                 energy_bound=None, kmin=0.0):
+                 energy_bound=None, kmin=0.0, kclip=0.0):
         self.ps, self.vz, self.p = ps, vz, params
         self.n = S._spec_n(ps)
         self.net = net
         self.scale_in, self.scale_out = scale_in, scale_out
-        self.kmin = kmin
+        self.kmin, self.kclip = kmin, float(kclip)
         self.active = net is not None
         self.guard_every = guard_every
         self.energy_bound = energy_bound
@@ -272,6 +323,8 @@ class ClosureRun:
             network_inputs(self.ps, self.vz, self.scale_in, out=self._inp[0])
             self.net.forward(self._inp, out=self._pred)
             closure_forcing(self._pred[0], self.n, self.scale_out, self._fp, self._fz, self.kmin)
+            if self.kclip > 0.0:
+                clip_forcing(self.ps, self.vz, self._fp, self._fz, self.kclip)
             dense = (self._fp, self._fz)
         S.rk4_step(self.ps, self.vz, self.p, fp_draw, fz_draw, dt=dt, cfl=cfl, dt_out=dt_out,
                    dense=dense)

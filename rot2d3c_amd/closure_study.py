@@ -17,8 +17,9 @@
    timed in GPU-seconds per unit simulated time against the DNS.
 
 Usage: python -m rot2d3c_amd.closure_study --out DIR [--nf 2048 --nc 256 ...]
-Writes DIR/closure_params.npz (parameters and normalisation) and
-DIR/closure_study.json, printing a progress line per phase.
+Writes DIR/closure_params.npz (parameters and normalisation),
+DIR/closure_study.json (everything) and DIR/closure_eval.json (the compact
+record), printing a progress line per phase.
 """
 import argparse
 import json
@@ -40,8 +41,10 @@ class Forcing:
     seeds, then each step draws psi (seed) and vz (seed+1) forcing
     (src/Rot2D3C.para.f90:377-380,413-415)."""
 
-    def __init__(self, kf, f0, seed):
-        self.kf, self.f0 = kf, f0
+    def __init__(self, kf, f0, seed, n_coarse=None):
+        """n_coarse: coarse-grain every draw for a coarse run of that size
+        (closure.filtered_draw), so the coarse run is forced by G f."""
+        self.kf, self.f0, self.nc = kf, f0, n_coarse
         self.s0, self.s1 = seed, seed + 1
         self._draw()
 
@@ -50,6 +53,8 @@ class Forcing:
         a = S.ForceDraw(kx, ky, ph, self.f0, self.kf)
         kx, ky, ph, self.s1 = S.rand_force_draw(self.kf, self.s1)
         b = S.ForceDraw(kx, ky, ph, self.f0, 1.0)
+        if self.nc is not None:
+            a, b = L.filtered_draw(a, self.nc), L.filtered_draw(b, self.nc)
         return a, b
 
     def take(self, k):
@@ -81,7 +86,8 @@ def errors(les, dns, kmax):
             "flux_err": float(flux), "per_k": [float(x) for x in rel]}
 
 
-def timed_steps(ps0, vz0, p, dt, forcing, steps, net=None, sin=None, sout=None, kmin=0.0):
+def timed_steps(ps0, vz0, p, dt, forcing, steps, net=None, sin=None, sout=None, kmin=0.0,
+                kclip=0.0, clamp=0.0, sign_clip=0.0):
     """Seconds per step of a one-run ensemble (each step one replay of the
     captured step graph, the closure inside it when net is given), timed over
     one run() call of `steps` steps after a warm-up call; host-side table
@@ -90,7 +96,9 @@ def timed_steps(ps0, vz0, p, dt, forcing, steps, net=None, sin=None, sout=None, 
     ps, vz = ps0.clone(), vz0.clone()
     ens = E.Ensemble([E.Member(ps, vz, p, dt)], schedules="auto")
     if net is not None:
-        ens.set_closure(0, net, sin, sout, kmin)
+        net.set_output_clamp(clamp)
+        net.set_sign_clip(sign_clip)
+        ens.set_closure(0, net, sin, sout, kmin, kclip)
     warm = ens.forcing_array([[pair] for pair in forcing.take(20)])
     table = ens.forcing_array([[pair] for pair in forcing.take(steps)])
     ens.run(20, warm)
@@ -104,14 +112,19 @@ def timed_steps(ps0, vz0, p, dt, forcing, steps, net=None, sin=None, sout=None, 
 
 
 def coarse_runs(ps0, vz0, p, t_end, dt, seeds, kf, f0, params=None, sin=None, sout=None,
-                kmin=0.0, samples=200, energy_bound=None, chunk=50):
-    """len(seeds) coarse runs from (ps0, vz0), one forcing realisation each,
+                kmin=0.0, samples=200, energy_bound=None, chunk=50, kclip=0.0,
+                enstrophy_bound=None, trace=None, clamp=0.0, cfl_bound=None, sign_clip=0.0):
+    """len(seeds) coarse runs from (ps0, vz0), one coarse-grained forcing
+    realisation each (the filtered equation's forcing G f),
     advanced together as one ensemble to simulated time t_end with fixed dt;
     with `params` every run carries its own closure network (forcing on
     |k| >= kmin) inside its step graph.  Every `chunk` steps each run's energy
-    is checked: a closure run that stops being finite or exceeds energy_bound
-    restarts the chunk from its last good state as the plain solver (the
-    fallback).  Returns (statistics averaged over time and runs, steps,
+    and enstrophy are checked: a closure run that stops being finite or
+    exceeds energy_bound or enstrophy_bound, or whose fixed dt exceeds
+    cfl_bound times the CFL condition's dt, restarts the chunk from its last
+    good state as the plain solver (the fallback).  The networks' outputs
+    are saturated at +-clamp (0: none) and sign-clipped with strength
+    sign_clip (closure.Net.set_sign_clip).  Returns (statistics averaged over time and runs, steps,
     finite, fallbacks)."""
     from . import ensemble as E
     R = len(seeds)
@@ -121,11 +134,13 @@ def coarse_runs(ps0, vz0, p, t_end, dt, seeds, kf, f0, params=None, sin=None, so
     if params is not None:
         for r in range(R):
             nets.append(L.Net(1, S._spec_n(ps0), S._spec_n(ps0), ps0.device, params=params))
-            ens.set_closure(r, nets[-1], sin, sout, kmin)
-    forcings = [Forcing(kf, f0, sd) for sd in seeds]
+            nets[-1].set_output_clamp(clamp)
+            nets[-1].set_sign_clip(sign_clip)
+            ens.set_closure(r, nets[-1], sin, sout, kmin, kclip)
+    forcings = [Forcing(kf, f0, sd, S._spec_n(ps0)) for sd in seeds]
     nsteps = int(round(t_end / dt))
     every = max(1, nsteps // samples)
-    acc, nsamp, steps, finite, fallbacks = None, 0, 0, True, 0
+    acc, nsamp, steps, finite, fallbacks, cfl_max = None, 0, 0, True, 0, 0.0
     good = [(a.clone(), b.clone()) for a, b in runs]
     on = [params is not None] * R
     while steps < nsteps:
@@ -135,7 +150,18 @@ def coarse_runs(ps0, vz0, p, t_end, dt, seeds, kf, f0, params=None, sin=None, so
         redo = False
         for r, (a, b) in enumerate(runs):
             e = S.energy(a, 1) + S.energy(b, 0)
-            bad = not math.isfinite(e) or (energy_bound is not None and e > energy_bound)
+            z = S.energy(a, 2) + S.energy(b, 1)
+            cfl = (dt / S.CFL_condition(1.0, a, b, p.nu, p.nn, p.nuv, p.nnv, p.omega)
+                   if math.isfinite(e) else float("nan"))
+            bad = (not math.isfinite(e) or not math.isfinite(z)
+                   or (energy_bound is not None and e > energy_bound)
+                   or (enstrophy_bound is not None and z > enstrophy_bound)
+                   or (cfl_bound is not None and cfl > cfl_bound))
+            if on[r] and math.isfinite(cfl):
+                cfl_max = max(cfl_max, cfl)
+            if trace is not None:
+                trace("step %d run %d closure %d energy %.6g enstrophy %.6g cfl %.3f%s" % (
+                    steps + c, r, on[r], e, z, cfl, " BAD" if bad else ""))
             if bad and on[r]:
                 on[r] = False
                 ens.set_closure(r, None)
@@ -161,6 +187,7 @@ def coarse_runs(ps0, vz0, p, t_end, dt, seeds, kf, f0, params=None, sin=None, so
                 nsamp += 1
     ens.close()
     stats = None if acc is None else [u / nsamp for u in acc]
+    coarse_runs.cfl_max = cfl_max  # largest CFL number of a closure run at a check
     return stats, steps, finite, fallbacks
 
 
@@ -177,6 +204,86 @@ def dns_window(advance, ps, vz, nc, steps, samples=200):
     return [u / nsamp for u in acc]
 
 
+def train_window(a, advance, ps, vz, out, log):
+    """Training window of the DNS: coarse-grained snapshots, then training on
+    the GPU.  Returns (params, scale_in, scale_out)."""
+    dev, nc = ps.device, a.nc
+    every = max(1, a.train_steps // a.snapshots)
+    X = torch.empty((a.snapshots, nc, nc, 3), dtype=torch.float32, device=dev)
+    Y = torch.empty((a.snapshots, nc, nc, 2), dtype=torch.float32, device=dev)
+    for k in range(a.snapshots):
+        advance(every)
+        _, _, x, y = L.snapshot(ps, vz, nc, kmin=a.target_kmin)
+        X[k].copy_(x)
+        Y[k].copy_(y)
+        if k % 100 == 0:
+            log("snapshot %d/%d" % (k, a.snapshots))
+    torch.cuda.synchronize()
+    sx, sy = L.channel_std(X), L.channel_std(Y)
+    if a.no_psi_input:  # the network sees laplacian(psi) and vz only
+        sx[2] = math.inf
+    for c in range(3):
+        X[..., c] *= 1.0 / sx[c]
+    for c in range(2):
+        Y[..., c] *= 1.0 / sy[c]
+    nval = max(a.batch, a.snapshots // 10)
+    Xtr, Ytr = X[:-nval].contiguous(), Y[:-nval].contiguous()
+    Xva, Yva = X[-nval:].contiguous(), Y[-nval:].contiguous()
+
+    net = L.Net(a.batch, nc, nc, dev, seed=a.seed)
+    t2 = time.perf_counter()
+    hist = L.train(net, Xtr, Ytr, a.epochs, lr=a.lr, lr_end=a.lr / 10, seed=a.seed, log=log)
+    torch.cuda.synchronize()
+    out["train"]["seconds"] = time.perf_counter() - t2
+    out["train"]["loss"] = hist
+    out["train"]["val_loss"] = L.evaluate(net, Xva, Yva)
+    out["train"]["val_loss_zero_model"] = float((Yva.double() ** 2).mean().item())
+    log("train loss %.4f val %.4f (zero model %.4f), %.1f s" % (
+        hist[-1], out["train"]["val_loss"], out["train"]["val_loss_zero_model"],
+        out["train"]["seconds"]))
+    params = net.params().cpu().numpy()
+    net.close()
+    del X, Y, Xtr, Ytr
+    torch.cuda.empty_cache()
+    scale_in = [1.0 / v for v in sx]
+    return params, scale_in, sy
+
+
+def compact_record(out, command):
+    """The evaluation record kept beside the trained network: DNS cost,
+    training losses, the tuning scores, both coarse runs' errors and costs,
+    the speedup and the per-shell spectrum ratios to the coarse-grained DNS."""
+    keep = ("spectrum_rel_err_max", "spectrum_rel_err_mean", "spectrum_rel_err_k_le_40_max",
+            "flux_err", "gpu_s_per_time", "ms_per_step", "dt", "realisations", "fallbacks", "cfl_max")
+    dns = np.array(out["dns"]["spectrum"])
+    rec = {"command": command,
+           "dns": {"n": out["config"]["nf"], "ms_per_step": out["dns"]["ms_per_step"],
+                   "gpu_s_per_time": out["dns"]["gpu_s_per_time"]},
+           "train": {"val_loss": out["train"]["val_loss"],
+                     "val_loss_zero_model": out["train"]["val_loss_zero_model"],
+                     "seconds": out["train"]["seconds"],
+                     "epochs": None if out["config"]["params"] else out["config"]["epochs"]},
+           "kmin": out["kmin"], "gain": out["gain"], "kclip": out["kclip"], "clamp": out["clamp"],
+           "sign_clip": out["sign_clip"],
+           "dns_cfl": out["dns"]["cfl"], "cfl_guard": out["config"]["cfl_guard"],
+           "tune": {k: {m: v[m] for m in ("spectrum_rel_err_max", "spectrum_rel_err_mean",
+                                           "flux_err") if m in v}
+                    for k, v in out["tune"].items()},
+           "evaluation_window": out["config"]["eval_steps"] * out["config"]["dt"],
+           "coarse_forcing": "coarse-grained draws (injection f0 G(k)^2)"}
+    for name in ("plain", "closure"):
+        rec[name] = {m: out["coarse"][name][m] for m in keep if m in out["coarse"][name]}
+    rec["speedup_gpu_s_per_time"] = out["speedup_gpu_s_per_time"]
+    rec["shells"] = int(dns.size)
+    for name in ("closure", "plain"):
+        sp = out["coarse"][name].get("spectrum")
+        ratio = None if sp is None else np.array(sp) / dns
+        rec["ratio_%s_to_dns" % name] = None if ratio is None else [float(x) for x in ratio]
+        if name == "closure":
+            rec["shells_within_5pct"] = 0 if ratio is None else int(np.sum(np.abs(ratio - 1) <= 0.05))
+    return rec
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--out", required=True)
@@ -191,6 +298,31 @@ def main(argv=None):
     ap.add_argument("--eval-steps", type=int, default=50000)
     ap.add_argument("--kmin", type=float, nargs="+", default=[0.0, 4.0, 8.0, 16.0],
                     help="candidate low-k cutoffs of the closure forcing")
+    ap.add_argument("--gain", type=float, nargs="+", default=[1.0],
+                    help="candidate gains of the closure forcing (tuned with the cutoff)")
+    ap.add_argument("--kclip", type=float, nargs="+", default=[0.0],
+                    help="candidate wavenumbers from which the closure forcing is "
+                         "backscatter-clipped (closure.clip_forcing; 0 = none), tuned with "
+                         "the cutoff and gain"),
+    ap.add_argument("--guard", type=float, default=4.0,
+                    help="a closure run falls back to the plain solver when its enstrophy "
+                         "exceeds this multiple of the window's initial (coarse-grained) one")
+    ap.add_argument("--clamp", type=float, nargs="+", default=[0.0],
+                    help="candidate output saturations of the network (in units of the "
+                         "targets' std; 0 = none), tuned with the cutoff, gain and kclip")
+    ap.add_argument("--sign-clip", type=float, nargs="+", default=[0.0],
+                    help="candidate strengths in [0, 1] of the network's sign clipping "
+                         "(closure.Net.set_sign_clip), tuned with the others")
+    ap.add_argument("--cfl-guard", type=float, default=2.0,
+                    help="a closure run falls back when the CFL number of its fixed dt "
+                         "exceeds this multiple of the DNS's")
+    ap.add_argument("--no-psi-input", action="store_true",
+                    help="zero the network's psi input channel (inputs: laplacian of psi, vz)")
+    ap.add_argument("--trace", action="store_true",
+                    help="log every run's energy and enstrophy at every check")
+    ap.add_argument("--params", default=None,
+                    help="skip the training: take the network and its normalisation from this "
+                         "closure_params.npz (the DNS still runs to reach the same windows)")
     ap.add_argument("--target-kmin", type=float, default=0.0,
                     help="train on the targets' modes |k| >= target_kmin only")
     ap.add_argument("--realisations", type=int, default=4,
@@ -238,48 +370,25 @@ def main(argv=None):
     dns_ms = (time.perf_counter() - t1) * 1e3 / 1000
     out["dns"]["ms_per_step"] = dns_ms
     out["dns"]["gpu_s_per_time"] = dns_ms * 1e-3 / a.dt
-    log("dns %.4f ms/step, %.3f GPU-s per unit time" % (dns_ms, out["dns"]["gpu_s_per_time"]))
+    out["dns"]["cfl"] = a.dt / S.CFL_condition(1.0, ps, vz, p.nu, p.nn, p.nuv, p.nnv, p.omega)
+    log("dns %.4f ms/step, %.3f GPU-s per unit time, CFL number %.3f" % (
+        dns_ms, out["dns"]["gpu_s_per_time"], out["dns"]["cfl"]))
 
     # 2. training window: snapshots
     every = max(1, a.train_steps // a.snapshots)
-    X = torch.empty((a.snapshots, nc, nc, 3), dtype=torch.float32, device=dev)
-    Y = torch.empty((a.snapshots, nc, nc, 2), dtype=torch.float32, device=dev)
-    for k in range(a.snapshots):
-        advance(every)
-        _, _, x, y = L.snapshot(ps, vz, nc, kmin=a.target_kmin)
-        X[k].copy_(x)
-        Y[k].copy_(y)
-        if k % 100 == 0:
-            log("snapshot %d/%d" % (k, a.snapshots))
-    torch.cuda.synchronize()
-    sx, sy = L.channel_std(X), L.channel_std(Y)
-    for c in range(3):
-        X[..., c] *= 1.0 / sx[c]
-    for c in range(2):
-        Y[..., c] *= 1.0 / sy[c]
-    nval = max(a.batch, a.snapshots // 10)
-    Xtr, Ytr = X[:-nval].contiguous(), Y[:-nval].contiguous()
-    Xva, Yva = X[-nval:].contiguous(), Y[-nval:].contiguous()
-
-    # 3. training
-    net = L.Net(a.batch, nc, nc, dev, seed=a.seed)
-    t2 = time.perf_counter()
-    hist = L.train(net, Xtr, Ytr, a.epochs, lr=a.lr, lr_end=a.lr / 10, seed=a.seed, log=log)
-    torch.cuda.synchronize()
-    out["train"]["seconds"] = time.perf_counter() - t2
-    out["train"]["loss"] = hist
-    out["train"]["val_loss"] = L.evaluate(net, Xva, Yva)
-    out["train"]["val_loss_zero_model"] = float((Yva.double() ** 2).mean().item())
-    log("train loss %.4f val %.4f (zero model %.4f), %.1f s" % (
-        hist[-1], out["train"]["val_loss"], out["train"]["val_loss_zero_model"],
-        out["train"]["seconds"]))
-    params = net.params().cpu().numpy()
-    net.close()
-    del X, Y, Xtr, Ytr
-    torch.cuda.empty_cache()
-    scale_in = [1.0 / v for v in sx]
+    if a.params:
+        advance(every * a.snapshots)
+        z = np.load(a.params)
+        params = z["params"]
+        scale_in, sy = [float(v) for v in z["scale_in"]], [float(v) for v in z["scale_out"]]
+        out["train"] = {"params": a.params}
+        for k in ("val_loss", "val_loss_zero_model", "seconds"):
+            out["train"][k] = float(z[k]) if k in z else None
+    else:
+        params, scale_in, sy = train_window(a, advance, ps, vz, out, log)
     np.savez(os.path.join(a.out, "closure_params.npz"), params=params, scale_in=np.array(scale_in),
-             scale_out=np.array(sy), nf=nf, nc=nc)
+             scale_out=np.array(sy), nf=nf, nc=nc, **{k: out["train"][k] for k in (
+                 "val_loss", "val_loss_zero_model", "seconds") if out["train"].get(k) is not None})
 
     # 4. tuning window: choose the closure's low-k cutoff on data the final
     # evaluation does not see
@@ -288,38 +397,58 @@ def main(argv=None):
     seeds = [a.seed + 100 + r for r in range(a.realisations)]
     ps_c0, vz_c0 = L.coarse_grain(ps, nc), L.coarse_grain(vz, nc)
     e0 = S.energy(ps_c0, 1) + S.energy(vz_c0, 0)
+    z0 = S.energy(ps_c0, 2) + S.energy(vz_c0, 1)
     dns_tune = dns_window(advance, ps, vz, nc, a.tune_steps)
     Tt = a.tune_steps * a.dt
     out["tune"] = {}
-    for km in a.kmin:
+    cands = [dict(kmin=km, gain=g, kclip=kc, clamp=cl, sign_clip=sc)
+             for km in a.kmin for g in a.gain for kc in a.kclip for cl in a.clamp
+             for sc in a.sign_clip]
+    key = lambda c: "kmin %g gain %g kclip %g clamp %g sign_clip %g" % (
+        c["kmin"], c["gain"], c["kclip"], c["clamp"], c["sign_clip"])
+    cfl_bound = a.cfl_guard * out["dns"]["cfl"]
+    for c in cands:
         stats, steps, finite, fb = coarse_runs(ps_c0, vz_c0, p, Tt, dtc, seeds, kf, f0, params,
-                                               scale_in, sy, km, energy_bound=20.0 * e0)
-        r = {"finite": finite, "fallbacks": fb}
+                                               scale_in, [c["gain"] * v for v in sy], c["kmin"],
+                                               energy_bound=20.0 * e0, kclip=c["kclip"],
+                                               enstrophy_bound=a.guard * z0, clamp=c["clamp"],
+                                               cfl_bound=cfl_bound, sign_clip=c["sign_clip"])
+        r = {"finite": finite, "fallbacks": fb, "cfl_max": coarse_runs.cfl_max}
         if finite and stats is not None:
             r.update(errors(stats, dns_tune, kmax))
-        out["tune"][str(km)] = r
-        log("tune kmin %g: spectrum err max %s mean %s, fallbacks %d" % (
-            km, r.get("spectrum_rel_err_max"), r.get("spectrum_rel_err_mean"), fb))
-    kbest = min(a.kmin, key=lambda km: out["tune"][str(km)].get("spectrum_rel_err_mean", 1e300))
-    out["kmin"] = kbest
+        out["tune"][key(c)] = r
+        log("tune %s: spectrum err max %s mean %s, fallbacks %d, max CFL %.3f" % (
+            key(c), r.get("spectrum_rel_err_max"), r.get("spectrum_rel_err_mean"), fb,
+            r["cfl_max"]))
+    best = min(cands, key=lambda c: out["tune"][key(c)].get("spectrum_rel_err_mean", 1e300)
+               + (1e300 if out["tune"][key(c)]["fallbacks"] else 0.0))
+    out.update(best)
+    kbest, gbest, kcbest, clbest, scbest = (best[k] for k in ("kmin", "gain", "kclip", "clamp",
+                                                              "sign_clip"))
+    sy = [gbest * v for v in sy]
 
     # 5. evaluation window: coarse-grained DNS statistics, then the coarse runs
     ps_c0, vz_c0 = L.coarse_grain(ps, nc), L.coarse_grain(vz, nc)
     e0 = S.energy(ps_c0, 1) + S.energy(vz_c0, 0)
+    z0 = S.energy(ps_c0, 2) + S.energy(vz_c0, 1)
     dns_stats = dns_window(advance, ps, vz, nc, a.eval_steps)
     T = a.eval_steps * a.dt
-    log("dns evaluation window done (t = %.3f); closure kmin %g" % (T, kbest))
+    log("dns evaluation window done (t = %.3f); closure %s" % (T, key(best)))
     del ps, vz
     torch.cuda.empty_cache()
     for name, prm in (("plain", None), ("closure", params)):
         stats, steps, finite, fb = coarse_runs(ps_c0, vz_c0, p, T, dtc, seeds, kf, f0, prm, scale_in,
-                                               sy, kbest, energy_bound=20.0 * e0)
+                                               sy, kbest, energy_bound=20.0 * e0, kclip=kcbest,
+                                               enstrophy_bound=a.guard * z0,
+                                               trace=log if a.trace and prm is not None else None,
+                                               clamp=clbest, cfl_bound=cfl_bound,
+                                               sign_clip=scbest)
         net = L.Net(1, nc, nc, dev, params=params) if prm is not None else None
-        sec = timed_steps(ps_c0, vz_c0, p, dtc, Forcing(kf, f0, a.seed + 11), 500, net, scale_in,
-                          sy, kbest)
+        sec = timed_steps(ps_c0, vz_c0, p, dtc, Forcing(kf, f0, a.seed + 11, nc), 500, net, scale_in,
+                          sy, kbest, kcbest, clbest, scbest)
         r = {"finite": finite, "steps": steps, "dt": dtc, "sim_time": steps * dtc,
              "realisations": len(seeds), "ms_per_step": sec * 1e3, "gpu_s_per_time": sec / dtc,
-             "fallbacks": fb}
+             "fallbacks": fb, "cfl_max": coarse_runs.cfl_max if prm is not None else None}
         if finite and stats is not None:
             r.update(errors(stats, dns_stats, kmax))
             r["spectrum"] = [float(x) for x in (stats[0] + stats[1])[:kmax]]
@@ -336,7 +465,13 @@ def main(argv=None):
     with open(os.path.join(a.out, "closure_study.json"), "w") as f:
         json.dump(out, f, indent=1)
     np.savez(os.path.join(a.out, "closure_params.npz"), params=params, scale_in=np.array(scale_in),
-             scale_out=np.array(sy), kmin=kbest, nf=nf, nc=nc)
+             scale_out=np.array(sy), kmin=kbest, gain=gbest, kclip=kcbest,
+             out_clamp=clbest, sign_clip=scbest, nf=nf, nc=nc, **{k: out["train"][k] for k in (
+                 "val_loss", "val_loss_zero_model", "seconds") if out["train"].get(k) is not None})
+    cmd = "python -m rot2d3c_amd.closure_study " + " ".join(
+        sys.argv[1:] if argv is None else list(argv))
+    with open(os.path.join(a.out, "closure_eval.json"), "w") as f:
+        json.dump(compact_record(out, cmd), f, indent=1)
     log("speedup %.1fx; wrote %s" % (out["speedup_gpu_s_per_time"], a.out))
     return 0
 

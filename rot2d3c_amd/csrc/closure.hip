@@ -84,7 +84,8 @@ __global__ __launch_bounds__(64 * kConvWaves) void k_conv3x3(const bf16 *__restr
                                                   const float *__restrict__ bias,
                                                   const float *__restrict__ Zprev,
                                                   float *__restrict__ Zout, bf16 *__restrict__ Y,
-                                                  int B, int H, int W) {
+                                                  int B, int H, int W, float clamp,
+                                                  const bf16 *__restrict__ X0, float keep) {
   constexpr int CI16 = CI / 16, CO16 = CO / 16, WS = CI + 4, TW = kTilesPerWave;
   __shared__ bf16 ws[9 * CO * WS];
   for (int e = threadIdx.x; e < 9 * CO * (CI / 4); e += 64 * kConvWaves) {
@@ -150,7 +151,14 @@ __global__ __launch_bounds__(64 * kConvWaves) void k_conv3x3(const bf16 *__restr
         if constexpr (MODE == CONV_BWD) {
           Y[o] = f2bf(v * gelu_grad(Zprev[o]));
         } else {
-          const float z = v + bias[co];
+          float z = v + bias[co];
+          if constexpr (MODE == CONV_LINEAR) {
+            if (clamp > 0.f) z = fminf(fmaxf(z, -clamp), clamp);
+            // sign clipping: where output channel co < 2 shares the sign of
+            // input channel co (X0 is the network input, 16 channels) it is
+            // scaled by keep = 1 - strength
+            if (X0 && co < 2 && z * bf2f(X0[o / CO * 16 + co]) > 0.f) z *= keep;
+          }
           if (Zout) Zout[o] = z;
           Y[o] = f2bf(MODE == CONV_GELU ? gelu(z) : z);
         }
@@ -358,6 +366,8 @@ struct r2d3c_cnn {
   int wblocks = 0;
   long step = 0;
   bool weights_ready = false;
+  float out_clamp = 0.f;        // inference output saturation (0: none)
+  float sign_clip = 0.f;        // inference sign clipping strength of outputs 0, 1
 };
 
 namespace {
@@ -397,18 +407,24 @@ int emit(r2d3c_cnn *c, hipStream_t s) {
 
 template <int CI, int CO, int MODE>
 void conv(const r2d3c_cnn *c, int l, const bf16 *X, const bf16 *Wt, const float *Zprev, float *Zout,
-          bf16 *Y, int B, hipStream_t s) {
+          bf16 *Y, int B, hipStream_t s, float clamp = 0.f, const bf16 *X0 = nullptr,
+          float keep = 1.f) {
   const long tiles = (long)B * c->H * (c->W / 16), per = kConvWaves * kTilesPerWave;
   k_conv3x3<CI, CO, MODE><<<(unsigned)((tiles + per - 1) / per), 64 * kConvWaves, 0, s>>>(
-      X, Wt, MODE == CONV_BWD ? nullptr : c->params + poff(l) + wsize(l), Zprev, Zout, Y, B, c->H, c->W);
+      X, Wt, MODE == CONV_BWD ? nullptr : c->params + poff(l) + wsize(l), Zprev, Zout, Y, B, c->H,
+      c->W, clamp, X0, keep);
 }
 
-int forward(r2d3c_cnn *c, int B, hipStream_t s) {
+// inference options: clamp > 0 saturates the outputs to [-clamp, clamp]; with
+// sign_clip > 0 the output channels 0, 1 are scaled by 1 - sign_clip where
+// they share the sign of input channels 0, 1
+int forward(r2d3c_cnn *c, int B, hipStream_t s, float clamp = 0.f, float sign_clip = 0.f) {
   const bf16 *wf = c->wf;
   conv<16, 32, CONV_GELU>(c, 0, c->act[0], wf + woff(0), nullptr, c->z[0], c->act[1], B, s);
   for (int l = 1; l < 5; ++l)
     conv<32, 32, CONV_GELU>(c, l, c->act[l], wf + woff(l), nullptr, c->z[l], c->act[l + 1], B, s);
-  conv<32, 16, CONV_LINEAR>(c, 5, c->act[5], wf + woff(5), nullptr, c->z[5], c->act[6], B, s);
+  conv<32, 16, CONV_LINEAR>(c, 5, c->act[5], wf + woff(5), nullptr, c->z[5], c->act[6], B, s,
+                            clamp, sign_clip > 0.f ? c->act[0] : nullptr, 1.f - sign_clip);
   KCHK("cnn forward");
   return 0;
 }
@@ -576,7 +592,21 @@ int r2d3c_cnn_forward_buffers(r2d3c_cnn *c, int batch, void *stream) {
   if (!c) return err(R2D3C_EINVAL, "cnn: NULL argument");
   int rc = check_batch(c, batch);
   if (rc) return rc;
-  return forward(c, batch, (hipStream_t)stream);
+  return forward(c, batch, (hipStream_t)stream, c->out_clamp, c->sign_clip);
+}
+
+int r2d3c_cnn_set_sign_clip(r2d3c_cnn *c, double strength) {
+  if (!c) return err(R2D3C_EINVAL, "cnn: NULL argument");
+  if (!(strength >= 0.0 && strength <= 1.0)) return err(R2D3C_EINVAL, "cnn: strength in [0, 1]");
+  c->sign_clip = (float)strength;
+  return 0;
+}
+
+int r2d3c_cnn_set_output_clamp(r2d3c_cnn *c, double clamp) {
+  if (!c) return err(R2D3C_EINVAL, "cnn: NULL argument");
+  if (!(clamp >= 0.0)) return err(R2D3C_EINVAL, "cnn: clamp must be >= 0");
+  c->out_clamp = (float)clamp;
+  return 0;
 }
 
 int r2d3c_cnn_forward(r2d3c_cnn *c, const float *d_in, int batch, float *d_out, void *stream) {
@@ -584,7 +614,8 @@ int r2d3c_cnn_forward(r2d3c_cnn *c, const float *d_in, int batch, float *d_out, 
   int rc = check_batch(c, batch);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
-  if ((rc = pack(c, d_in, batch, s)) || (rc = forward(c, batch, s))) return rc;
+  if ((rc = pack(c, d_in, batch, s)) || (rc = forward(c, batch, s, c->out_clamp, c->sign_clip)))
+    return rc;
   const long P = (long)batch * c->H * c->W;
   k_unpack_output<<<(unsigned)((P * 2 + 255) / 256), 256, 0, s>>>(c->z[5], P, d_out);
   KCHK("cnn output");

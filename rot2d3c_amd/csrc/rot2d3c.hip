@@ -2100,6 +2100,7 @@ struct EnsClosure {
   int member = 0;                   // index within its class
   r2d3c_cnn *net = nullptr;
   double sin[3] = {1, 1, 1}, sout[2] = {1, 1}, kmin = 0.0;
+  double kclip = 0.0;               // backscatter clipping on |k| >= kclip (0: none)
   const double *psi = nullptr, *vz = nullptr;
   cplx *fp = nullptr, *fz = nullptr;
 };
@@ -2142,6 +2143,8 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
     if (!rc)
       rc = r2d3c_closure_net_forcing(k.plan, cl.net, cl.sout, cl.kmin, (double *)cl.fp,
                                      (double *)cl.fz, s);
+    if (!rc && cl.kclip > 0.0)
+      rc = r2d3c_closure_clip(k.plan, cl.psi, cl.vz, cl.kclip, (double *)cl.fp, (double *)cl.fz, s);
     if (rc) return rc;
   }
   const r2d3c_plan *p = k.plan;
@@ -2345,7 +2348,7 @@ int r2d3c_ensemble_set_schedule(r2d3c_ensemble *e, int n, int row_rows, int row_
 }
 
 int r2d3c_ensemble_set_closure(r2d3c_ensemble *e, int run, r2d3c_cnn *net, const double *scale_in,
-                               const double *scale_out, double kmin) {
+                               const double *scale_out, double kmin, double kclip) {
   NEED(e);
   if (run < 0 || run >= e->count) return set_err(R2D3C_EINVAL, "no such run");
   int j = 0;
@@ -2378,6 +2381,7 @@ int r2d3c_ensemble_set_closure(r2d3c_ensemble *e, int run, r2d3c_cnn *net, const
     for (int c = 0; c < 3; ++c) cl.sin[c] = scale_in[c];
     for (int c = 0; c < 2; ++c) cl.sout[c] = scale_out[c];
     cl.kmin = kmin;
+    cl.kclip = kclip > 0.0 ? kclip : 0.0;
     cl.psi = (const double *)m.psi;
     cl.vz = (const double *)m.vz;
     const size_t sz = (size_t)(n / 2 + 1) * n;
@@ -2729,6 +2733,29 @@ __global__ __launch_bounds__(N / 8) void k_cf_col(const cplx *__restrict__ mix, 
   }
 }
 
+// Backscatter clipping on the modes |k| >= kclip: where the closure forcing
+// injects energy (Re(conj(psi) fp) > 0 for psi, whose energy is k^2 |psi|^2;
+// Re(conj(vz) fz) > 0 for vz) the injecting component along the mode's own
+// coefficient is removed, fp -= Re(conj(psi) fp) / |psi|^2 psi (same for
+// fz), leaving the energy-neutral part.
+__global__ void k_closure_clip(const cplx *__restrict__ psi, const cplx *__restrict__ vz,
+                               cplx *__restrict__ fp, cplx *__restrict__ fz, int n,
+                               double kclip2) {
+  const size_t total = (size_t)(n / 2 + 1) * n;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    const double kx = kxof(i, n), ky = kyof(j, n);
+    if (kx * kx + ky * ky < kclip2) continue;
+    const cplx a = psi[e], f = fp[e];
+    const double ta = a.x * f.x + a.y * f.y, na = a.x * a.x + a.y * a.y;
+    if (ta > 0.0 && na > 0.0) fp[e] = mk(f.x - ta / na * a.x, f.y - ta / na * a.y);
+    const cplx b = vz[e], g = fz[e];
+    const double tb = b.x * g.x + b.y * g.y, nb = b.x * b.x + b.y * b.y;
+    if (tb > 0.0 && nb > 0.0) fz[e] = mk(g.x - tb / nb * b.x, g.y - tb / nb * b.y);
+  }
+}
+
 template <int N>
 static int run_closure_io(r2d3c_plan *p, int dir, const cplx *psi, const cplx *vz, const double *sc,
                           double kmin, unsigned short *in_bf16, const float *z, cplx *fp, cplx *fz,
@@ -2768,6 +2795,17 @@ int r2d3c_closure_net_forcing(r2d3c_plan *p, r2d3c_cnn *net, const double *scale
   if (rc) return rc;
   R2D_DISPATCH(p->n, run_closure_io, p, 1, nullptr, nullptr, scale, kmin, nullptr, z, (cplx *)fp,
                (cplx *)fz, (hipStream_t)stream);
+}
+
+int r2d3c_closure_clip(r2d3c_plan *p, const double *psi, const double *vz, double kclip,
+                       double *fp, double *fz, void *stream) {
+  NEED(p); NEED(psi); NEED(vz); NEED(fp); NEED(fz);
+  if (!(kclip > 0.0)) return set_err(R2D3C_EINVAL, "kclip must be > 0");
+  const size_t total = (size_t)(p->n / 2 + 1) * p->n;
+  k_closure_clip<<<launch_grid(total), 256, 0, (hipStream_t)stream>>>(
+      (const cplx *)psi, (const cplx *)vz, (cplx *)fp, (cplx *)fz, p->n, kclip * kclip);
+  LAUNCHCHK("closure clip");
+  return 0;
 }
 
 }  // extern "C"

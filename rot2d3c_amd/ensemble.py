@@ -59,17 +59,18 @@ class Ensemble:
             check(_lib.lib().r2d3c_ensemble_set_schedule(self.handle, n, rw, bpc),
                   "r2d3c_ensemble_set_schedule")
 
-    def set_closure(self, run, net=None, scale_in=None, scale_out=None, kmin=0.0):
+    def set_closure(self, run, net=None, scale_in=None, scale_out=None, kmin=0.0, kclip=0.0):
         """Make run `run` a closure run with network `net` (closure.Net for one
-        n x n image, used by this run only; its forcing acts on |k| >= kmin),
-        or the plain solver (net=None)."""
+        n x n image, used by this run only; its forcing acts on |k| >= kmin,
+        backscatter-clipped on |k| >= kclip if kclip > 0), or the plain solver
+        (net=None)."""
         self._nets = getattr(self, "_nets", {})
         d = lambda v: (ctypes.c_double * len(v))(*[float(x) for x in v])
         with torch.cuda.device(self.device):
             check(_lib.lib().r2d3c_ensemble_set_closure(
                 self.handle, int(run), net.handle if net is not None else None,
                 d(scale_in) if net is not None else None, d(scale_out) if net is not None else None,
-                float(kmin)),
+                float(kmin), float(kclip)),
                 "r2d3c_ensemble_set_closure")
         self._nets[run] = net  # keep the network alive while the ensemble uses it
 

@@ -56,6 +56,18 @@ class Plan:
                 p = _plans[key] = Plan(n, device)
             return p
 
+    @staticmethod
+    def release(n, device):
+        """Destroy the cached plan of grid size n on device, if any (the
+        n = 16384 plan holds ~41 GB of workspace)."""
+        device = torch.device(device)
+        if device.index is None:
+            device = torch.device("cuda", torch.cuda.current_device())
+        with _plans_lock:
+            p = _plans.pop((device.index, int(n)), None)
+        if p is not None:
+            p.close()
+
 
 def _stream(device):
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)

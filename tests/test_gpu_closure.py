@@ -237,3 +237,114 @@ def test_ensemble_closure_run_tracks_the_stepwise_closure_run(cuda):
     torch.cuda.synchronize()
     assert rel_l2(c[0].cpu().numpy(), b[0].cpu().numpy()) > 1e-9
     ens.close()
+
+
+def test_backscatter_clip_properties(cuda):
+    """closure.clip_forcing on random fields: modes that drained energy are
+    untouched bit for bit; at the others the energy transfer becomes zero,
+    the energy-neutral part Im(conj(psi) f) is kept and the removed part is
+    parallel to the mode's coefficient; zero coefficients are left alone."""
+    print("backscatter clip", flush=True)
+    from rot2d3c_amd import closure as L
+    n = 64
+    rng = np.random.default_rng(41)
+    shape = (n // 2 + 1, n)
+    arr = [rng.standard_normal(shape) + 1j * rng.standard_normal(shape) for _ in range(4)]
+    arr[0][3, :7] = 0.0  # modes with psi = 0
+    arr[0][0, 0] = arr[1][0, 0] = 0.0  # the mean mode (|k| = 0 < kclip) is outside the clipping
+    ps, vz, fp, fz = [torch.from_numpy(a).to(cuda) for a in arr]
+    # on |k| >= 10 only: every mode below is untouched
+    fp10, fz10 = fp.clone(), fz.clone()
+    L.clip_forcing(ps, vz, fp10, fz10, kclip=10.0)
+    kx = np.arange(n // 2 + 1)[:, None]
+    ky = np.fft.fftfreq(n, 1.0 / n)[None, :]
+    low = kx ** 2 + ky ** 2 < 100.0
+    L.clip_forcing(ps, vz, fp, fz)
+    torch.cuda.synchronize()
+    for f10, f, f0 in ((fp10, fp, arr[2]), (fz10, fz, arr[3])):
+        f10 = f10.cpu().numpy()
+        assert np.array_equal(f10[low], f0[low]) and np.array_equal(f10[~low], f.cpu().numpy()[~low])
+    for a, f0, f1 in ((arr[0], arr[2], fp.cpu().numpy()), (arr[1], arr[3], fz.cpu().numpy())):
+        t0 = (np.conj(a) * f0).real
+        t1 = (np.conj(a) * f1).real
+        drain = t0 <= 0
+        assert 0.3 < drain.mean() < 0.7
+        assert np.array_equal(f1[drain], f0[drain])
+        scale = np.abs(a) * np.abs(f0)
+        assert np.all(np.abs(t1[~drain]) <= 1e-13 * scale[~drain])
+        assert np.allclose((np.conj(a) * f1).imag, (np.conj(a) * f0).imag, rtol=1e-13, atol=1e-13)
+        d = f0 - f1
+        assert np.all(np.abs((np.conj(a) * d).imag) <= 1e-13 * scale + 1e-300)
+    assert np.array_equal(fp.cpu().numpy()[3, :7], arr[2][3, :7])
+
+
+def test_ensemble_clipped_closure_tracks_the_stepwise_run(cuda):
+    """The clipped closure inside the ensemble's step graph against
+    ClosureRun(kclip=10), and clipping changes the run."""
+    print("ensemble clipped closure run", flush=True)
+    from rot2d3c_amd import closure as L, ensemble as E, spectral as S
+    n = 64
+    ps0, vz0 = random_state(n, 35)
+    p = S.Params(1e-3, 0.1, 1, 2, 1e-3, 0.1, 1, 2, 2.0)
+    sin, sout = (0.02, 0.5, 3.0), (2.0, 0.2)
+    nets = [L.Net(1, n, n, cuda, seed=6) for _ in range(3)]
+    spec = [dict(kf=6.0, f0=1.0, seed=7)]
+    table = E.forcing_table(spec, 5)
+    ps = torch.from_numpy(ps0).to(cuda)
+    vz = torch.from_numpy(vz0).to(cuda)
+    ens = E.Ensemble([E.Member(ps, vz, p, 1e-3)], schedules=None)
+    ens.set_closure(0, nets[0], sin, sout, 0.0, kclip=10.0)
+    ens.run(5, table)
+    runs = []
+    for net, kclip in ((nets[1], 10.0), (nets[2], 0.0)):
+        b = (torch.from_numpy(ps0).to(cuda), torch.from_numpy(vz0).to(cuda))
+        run = L.ClosureRun(b[0], b[1], p, net, sin, sout, guard_every=0, kclip=kclip)
+        for st in range(5):
+            run.step(*table[st][0], dt=1e-3)
+        runs.append(b)
+    torch.cuda.synchronize()
+    assert rel_l2(ps.cpu().numpy(), runs[0][0].cpu().numpy()) < 1e-6
+    assert rel_l2(vz.cpu().numpy(), runs[0][1].cpu().numpy()) < 1e-6
+    assert rel_l2(runs[1][0].cpu().numpy(), runs[0][0].cpu().numpy()) > 1e-9
+    ens.close()
+
+
+def test_output_clamp_saturates_inference_only(cuda):
+    """With an output clamp the forward pass returns exactly the clipped
+    unclamped outputs (the saturation acts on the same fp32 values); with
+    sign clipping exactly the outputs zeroed where they share the input's
+    sign; gradients are those of the plain network."""
+    print("cnn output clamp", flush=True)
+    from rot2d3c_amd import closure as L
+    B, H, W = 2, 8, 64
+    rng = np.random.default_rng(43)
+    net = _net(cuda, B, H, W, C.init_params(rng, scale=2.0))
+    x = torch.from_numpy(rng.standard_normal((B, H, W, 3)).astype(np.float32)).to(cuda)
+    y = torch.from_numpy(rng.standard_normal((B, H, W, 2)).astype(np.float32)).to(cuda)
+    y0 = net.forward(x).cpu().numpy()
+    l0, g0 = net.gradients(x, y)
+    c = float(np.quantile(np.abs(y0), 0.7))
+    net.set_output_clamp(c)
+    y1 = net.forward(x).cpu().numpy()
+    assert np.mean(np.abs(y0) > c) > 0.2
+    assert np.array_equal(y1, np.clip(y0, -c, c))
+    l1, g1 = net.gradients(x, y)
+    assert l1 == l0 and torch.equal(g1, g0)
+    net.set_output_clamp(0.0)
+    assert np.array_equal(net.forward(x).cpu().numpy(), y0)
+    # sign clipping: channel c zeroed where it has the sign of input channel c
+    same = y0 * x.cpu().numpy()[..., :2] > 0
+    assert 0.3 < same.mean() < 0.7
+    for st in (1.0, 0.25):
+        net.set_sign_clip(st)
+        y2 = net.forward(x).cpu().numpy()
+        keep = np.float32(1.0 - st)
+        assert np.array_equal(y2, np.where(same, y0 * keep, y0).astype(np.float32))
+    l2, g2 = net.gradients(x, y)
+    assert l2 == l0 and torch.equal(g2, g0)
+    net.set_sign_clip(0.0)
+    with pytest.raises(Exception):
+        net.set_sign_clip(1.5)
+    with pytest.raises(Exception):
+        net.set_output_clamp(-1.0)
+    net.close()
