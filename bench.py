@@ -82,7 +82,8 @@ def closure_bench(dns_ms_per_step, dns_dt, dev, steps=500, warmup=20):
     """GPU-seconds per unit simulated time of the 256^2 run with the learned
     closure (the committed trained network, rot2d3c_amd/data/closure_256.npz;
     one-run ensemble, closure inside the step graph, fixed dt = 8 x the DNS dt:
-    the DNS's CFL number) against the 2048^2 DNS timed above.  The spectrum
+    the DNS's CFL number; chunks of 50 steps, each followed by the run-time
+    stabiliser's CFL check) against the 2048^2 DNS timed above.  The spectrum
     and flux errors are those of the committed evaluation
     (rot2d3c_amd/data/closure_256_eval.json, made by rot2d3c_amd.closure_study)."""
     import torch
@@ -103,11 +104,23 @@ def closure_bench(dns_ms_per_step, dns_dt, dev, steps=500, warmup=20):
                     float(z["kclip"]) if "kclip" in z else 0.0)
     spec = [dict(kf=40.0, f0=1.0, seed=3)]
     warm = ens.forcing_array(E.forcing_table(spec, warmup))
-    table = ens.forcing_array(E.forcing_table(spec, steps))
+    # chunks of 50 steps with the run-time stabiliser's CFL check after each,
+    # as the evaluated closure runs
+    chunk = 50
+    steps = max(chunk, steps // chunk * chunk)
+    full = E.forcing_table(spec, steps)
+    tables = [ens.forcing_array(full[i:i + chunk]) for i in range(0, steps, chunk)]
+    sm = L.SafeMode(*[float(v) for v in z["safe_cfl"]], float(z["sign_clip"])) \
+        if "safe_cfl" in z else None
     ens.run(warmup, warm)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ens.run(steps, table)
+    for tab in tables:
+        ens.run(chunk, tab)
+        if sm is not None and sm.check(0, L.cfl_number(ens.members[0].ps, ens.members[0].vz, p, dt)):
+            sm.apply(net, sm.active[0])
+            ens.set_closure(0, net, z["scale_in"], z["scale_out"], float(z["kmin"]),
+                            float(z["kclip"]) if "kclip" in z else 0.0)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
     ens.close()
@@ -123,6 +136,8 @@ def closure_bench(dns_ms_per_step, dns_dt, dev, steps=500, warmup=20):
             "flux_err": round(ev["closure"]["flux_err"], 4),
             "plain_spectrum_rel_err_mean": round(ev["plain"]["spectrum_rel_err_mean"], 4),
             "shells_within_5pct": "%d/%d" % (ev["shells_within_5pct"], ev["shells"]),
+            "stabiliser_time_fraction": ev["closure"].get("safe_fraction"),
+            "fallbacks": ev["closure"].get("fallbacks"),
             "errors_from": "rot2d3c_amd/data/closure_256_eval.json (k <= 85, 10 time units, "
                            "4 forcing realisations vs the coarse-grained 2048^2 DNS)"}
 

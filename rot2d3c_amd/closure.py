@@ -288,6 +288,55 @@ def evaluate(net, X, Y):
     return tot / max(cnt, 1)
 
 
+# --- run-time stabiliser ------------------------------------------------------
+def cfl_number(ps, vz, p, dt):
+    """CFL number of a fixed step dt: dt over the CFL condition's step at CFL
+    number 1 (the reference's formula; NaN for a non-finite state)."""
+    d1 = S.CFL_condition(1.0, ps, vz, p.nu, p.nn, p.nuv, p.nnv, p.omega)
+    return dt / d1 if math.isfinite(d1) and d1 > 0.0 else float("nan")
+
+
+class SafeMode:
+    """Run-time stabiliser of closure runs with a fixed step.
+
+    A learned closure can feed a compact vortex until the fixed step's CFL
+    number grows without bound, although energy and enstrophy stay normal.
+    Checked every few steps (check()), a closure run whose CFL number exceeds
+    cfl_on continues with its network fully sign-clipped (it can only damp
+    vorticity and vz pointwise, Net.set_sign_clip(1)) until the CFL number
+    falls below cfl_off, then returns to the network's own setting.
+
+    apply(net, flag) sets a network's mode; for a run inside an ensemble the
+    caller re-registers the network (Ensemble.set_closure) so the step graph
+    picks the change up."""
+
+    def __init__(self, cfl_on, cfl_off, base_sign_clip=0.0):
+        if not cfl_off <= cfl_on:
+            raise ValueError("SafeMode: cfl_off must not exceed cfl_on")
+        self.cfl_on, self.cfl_off, self.base = float(cfl_on), float(cfl_off), float(base_sign_clip)
+        self.active = {}
+        self.checks = 0
+        self.active_checks = 0
+
+    def apply(self, net, flag):
+        net.set_sign_clip(1.0 if flag else self.base)
+
+    def check(self, run, cfl):
+        """Record run's CFL number; returns True when its mode must change
+        (then self.active[run] is the new mode)."""
+        was = self.active.get(run, False)
+        self.checks += 1
+        self.active_checks += was
+        now = cfl > self.cfl_on if not was else not cfl < self.cfl_off
+        self.active[run] = now
+        return now != was
+
+    @property
+    def fraction(self):
+        """Fraction of checks at which a run was in the damping mode."""
+        return self.active_checks / max(self.checks, 1)
+
+
 # --- the coarse run with the closure -----------------------------------------
 class ClosureRun:
     """Coarse run of psi, vz (advanced in place) with the learned closure.

@@ -113,7 +113,8 @@ def timed_steps(ps0, vz0, p, dt, forcing, steps, net=None, sin=None, sout=None, 
 
 def coarse_runs(ps0, vz0, p, t_end, dt, seeds, kf, f0, params=None, sin=None, sout=None,
                 kmin=0.0, samples=200, energy_bound=None, chunk=50, kclip=0.0,
-                enstrophy_bound=None, trace=None, clamp=0.0, cfl_bound=None, sign_clip=0.0):
+                enstrophy_bound=None, trace=None, clamp=0.0, cfl_bound=None, sign_clip=0.0,
+                safe=None):
     """len(seeds) coarse runs from (ps0, vz0), one coarse-grained forcing
     realisation each (the filtered equation's forcing G f),
     advanced together as one ensemble to simulated time t_end with fixed dt;
@@ -124,8 +125,12 @@ def coarse_runs(ps0, vz0, p, t_end, dt, seeds, kf, f0, params=None, sin=None, so
     cfl_bound times the CFL condition's dt, restarts the chunk from its last
     good state as the plain solver (the fallback).  The networks' outputs
     are saturated at +-clamp (0: none) and sign-clipped with strength
-    sign_clip (closure.Net.set_sign_clip).  Returns (statistics averaged over time and runs, steps,
-    finite, fallbacks)."""
+    sign_clip (closure.Net.set_sign_clip).  safe = (cfl_on, cfl_off): a
+    closure run whose fixed dt's CFL number exceeds cfl_on continues with the
+    fully sign-clipped (damping only) network until it falls below cfl_off
+    (checked every chunk; coarse_runs.safe_fraction is the fraction of
+    run-chunks spent so).  Returns (statistics averaged over time and runs,
+    steps, finite, fallbacks)."""
     from . import ensemble as E
     R = len(seeds)
     runs = [(ps0.clone(), vz0.clone()) for _ in range(R)]
@@ -143,25 +148,29 @@ def coarse_runs(ps0, vz0, p, t_end, dt, seeds, kf, f0, params=None, sin=None, so
     acc, nsamp, steps, finite, fallbacks, cfl_max = None, 0, 0, True, 0, 0.0
     good = [(a.clone(), b.clone()) for a, b in runs]
     on = [params is not None] * R
+    sm = None if safe is None else L.SafeMode(safe[0], safe[1], sign_clip)
+
     while steps < nsteps:
         c = min(chunk, nsteps - steps)
         draws = [f.take(c) for f in forcings]
         ens.run(c, [[draws[r][k] for r in range(R)] for k in range(c)])
         redo = False
+        cfls = [float("nan")] * R
         for r, (a, b) in enumerate(runs):
             e = S.energy(a, 1) + S.energy(b, 0)
             z = S.energy(a, 2) + S.energy(b, 1)
-            cfl = (dt / S.CFL_condition(1.0, a, b, p.nu, p.nn, p.nuv, p.nnv, p.omega)
-                   if math.isfinite(e) else float("nan"))
+            cfl = L.cfl_number(a, b, p, dt) if math.isfinite(e) else float("nan")
             bad = (not math.isfinite(e) or not math.isfinite(z)
                    or (energy_bound is not None and e > energy_bound)
                    or (enstrophy_bound is not None and z > enstrophy_bound)
                    or (cfl_bound is not None and cfl > cfl_bound))
+            cfls[r] = cfl
             if on[r] and math.isfinite(cfl):
                 cfl_max = max(cfl_max, cfl)
             if trace is not None:
-                trace("step %d run %d closure %d energy %.6g enstrophy %.6g cfl %.3f%s" % (
-                    steps + c, r, on[r], e, z, cfl, " BAD" if bad else ""))
+                trace("step %d run %d closure %d%s energy %.6g enstrophy %.6g cfl %.3f%s" % (
+                    steps + c, r, on[r], " safe" if sm and sm.active.get(r) else "", e, z, cfl,
+                    " BAD" if bad else ""))
             if bad and on[r]:
                 on[r] = False
                 ens.set_closure(r, None)
@@ -179,6 +188,9 @@ def coarse_runs(ps0, vz0, p, t_end, dt, seeds, kf, f0, params=None, sin=None, so
         for r, (a, b) in enumerate(runs):
             good[r][0].copy_(a)
             good[r][1].copy_(b)
+            if on[r] and sm is not None and sm.check(r, cfls[r]):
+                sm.apply(nets[r], sm.active[r])
+                ens.set_closure(r, nets[r], sin, sout, kmin, kclip)  # recaptures the step graph
         steps += c
         if steps // every > (steps - c) // every:
             for a, b in runs:
@@ -188,6 +200,7 @@ def coarse_runs(ps0, vz0, p, t_end, dt, seeds, kf, f0, params=None, sin=None, so
     ens.close()
     stats = None if acc is None else [u / nsamp for u in acc]
     coarse_runs.cfl_max = cfl_max  # largest CFL number of a closure run at a check
+    coarse_runs.safe_fraction = 0.0 if sm is None else sm.fraction
     return stats, steps, finite, fallbacks
 
 
@@ -254,7 +267,8 @@ def compact_record(out, command):
     training losses, the tuning scores, both coarse runs' errors and costs,
     the speedup and the per-shell spectrum ratios to the coarse-grained DNS."""
     keep = ("spectrum_rel_err_max", "spectrum_rel_err_mean", "spectrum_rel_err_k_le_40_max",
-            "flux_err", "gpu_s_per_time", "ms_per_step", "dt", "realisations", "fallbacks", "cfl_max")
+            "flux_err", "gpu_s_per_time", "ms_per_step", "dt", "realisations", "fallbacks", "cfl_max",
+            "safe_fraction")
     dns = np.array(out["dns"]["spectrum"])
     rec = {"command": command,
            "dns": {"n": out["config"]["nf"], "ms_per_step": out["dns"]["ms_per_step"],
@@ -266,6 +280,7 @@ def compact_record(out, command):
            "kmin": out["kmin"], "gain": out["gain"], "kclip": out["kclip"], "clamp": out["clamp"],
            "sign_clip": out["sign_clip"],
            "dns_cfl": out["dns"]["cfl"], "cfl_guard": out["config"]["cfl_guard"],
+           "safe_cfl": out["config"]["safe_cfl"],
            "tune": {k: {m: v[m] for m in ("spectrum_rel_err_max", "spectrum_rel_err_mean",
                                            "flux_err") if m in v}
                     for k, v in out["tune"].items()},
@@ -313,6 +328,10 @@ def main(argv=None):
     ap.add_argument("--sign-clip", type=float, nargs="+", default=[0.0],
                     help="candidate strengths in [0, 1] of the network's sign clipping "
                          "(closure.Net.set_sign_clip), tuned with the others")
+    ap.add_argument("--safe-cfl", type=float, nargs=2, default=None, metavar=("ON", "OFF"),
+                    help="switch a closure run to the fully sign-clipped network while the "
+                         "CFL number of its fixed dt is above ON times the DNS's, back below "
+                         "OFF times (default: never)")
     ap.add_argument("--cfl-guard", type=float, default=2.0,
                     help="a closure run falls back when the CFL number of its fixed dt "
                          "exceeds this multiple of the DNS's")
@@ -370,7 +389,7 @@ def main(argv=None):
     dns_ms = (time.perf_counter() - t1) * 1e3 / 1000
     out["dns"]["ms_per_step"] = dns_ms
     out["dns"]["gpu_s_per_time"] = dns_ms * 1e-3 / a.dt
-    out["dns"]["cfl"] = a.dt / S.CFL_condition(1.0, ps, vz, p.nu, p.nn, p.nuv, p.nnv, p.omega)
+    out["dns"]["cfl"] = L.cfl_number(ps, vz, p, a.dt)
     log("dns %.4f ms/step, %.3f GPU-s per unit time, CFL number %.3f" % (
         dns_ms, out["dns"]["gpu_s_per_time"], out["dns"]["cfl"]))
 
@@ -407,19 +426,22 @@ def main(argv=None):
     key = lambda c: "kmin %g gain %g kclip %g clamp %g sign_clip %g" % (
         c["kmin"], c["gain"], c["kclip"], c["clamp"], c["sign_clip"])
     cfl_bound = a.cfl_guard * out["dns"]["cfl"]
+    safe = None if a.safe_cfl is None else tuple(v * out["dns"]["cfl"] for v in a.safe_cfl)
     for c in cands:
         stats, steps, finite, fb = coarse_runs(ps_c0, vz_c0, p, Tt, dtc, seeds, kf, f0, params,
                                                scale_in, [c["gain"] * v for v in sy], c["kmin"],
                                                energy_bound=20.0 * e0, kclip=c["kclip"],
                                                enstrophy_bound=a.guard * z0, clamp=c["clamp"],
-                                               cfl_bound=cfl_bound, sign_clip=c["sign_clip"])
-        r = {"finite": finite, "fallbacks": fb, "cfl_max": coarse_runs.cfl_max}
+                                               cfl_bound=cfl_bound, sign_clip=c["sign_clip"],
+                                               safe=safe)
+        r = {"finite": finite, "fallbacks": fb, "cfl_max": coarse_runs.cfl_max,
+             "safe_fraction": coarse_runs.safe_fraction}
         if finite and stats is not None:
             r.update(errors(stats, dns_tune, kmax))
         out["tune"][key(c)] = r
-        log("tune %s: spectrum err max %s mean %s, fallbacks %d, max CFL %.3f" % (
+        log("tune %s: spectrum err max %s mean %s, fallbacks %d, max CFL %.3f, safe %.3f" % (
             key(c), r.get("spectrum_rel_err_max"), r.get("spectrum_rel_err_mean"), fb,
-            r["cfl_max"]))
+            r["cfl_max"], r["safe_fraction"]))
     best = min(cands, key=lambda c: out["tune"][key(c)].get("spectrum_rel_err_mean", 1e300)
                + (1e300 if out["tune"][key(c)]["fallbacks"] else 0.0))
     out.update(best)
@@ -442,13 +464,14 @@ def main(argv=None):
                                                enstrophy_bound=a.guard * z0,
                                                trace=log if a.trace and prm is not None else None,
                                                clamp=clbest, cfl_bound=cfl_bound,
-                                               sign_clip=scbest)
+                                               sign_clip=scbest, safe=safe)
         net = L.Net(1, nc, nc, dev, params=params) if prm is not None else None
         sec = timed_steps(ps_c0, vz_c0, p, dtc, Forcing(kf, f0, a.seed + 11, nc), 500, net, scale_in,
                           sy, kbest, kcbest, clbest, scbest)
         r = {"finite": finite, "steps": steps, "dt": dtc, "sim_time": steps * dtc,
              "realisations": len(seeds), "ms_per_step": sec * 1e3, "gpu_s_per_time": sec / dtc,
-             "fallbacks": fb, "cfl_max": coarse_runs.cfl_max if prm is not None else None}
+             "fallbacks": fb, "cfl_max": coarse_runs.cfl_max if prm is not None else None,
+             "safe_fraction": coarse_runs.safe_fraction if prm is not None else None}
         if finite and stats is not None:
             r.update(errors(stats, dns_stats, kmax))
             r["spectrum"] = [float(x) for x in (stats[0] + stats[1])[:kmax]]
@@ -466,7 +489,8 @@ def main(argv=None):
         json.dump(out, f, indent=1)
     np.savez(os.path.join(a.out, "closure_params.npz"), params=params, scale_in=np.array(scale_in),
              scale_out=np.array(sy), kmin=kbest, gain=gbest, kclip=kcbest,
-             out_clamp=clbest, sign_clip=scbest, nf=nf, nc=nc, **{k: out["train"][k] for k in (
+             out_clamp=clbest, sign_clip=scbest, nf=nf, nc=nc,
+             **({} if safe is None else {"safe_cfl": np.array(safe)}), **{k: out["train"][k] for k in (
                  "val_loss", "val_loss_zero_model", "seconds") if out["train"].get(k) is not None})
     cmd = "python -m rot2d3c_amd.closure_study " + " ".join(
         sys.argv[1:] if argv is None else list(argv))

@@ -25,6 +25,10 @@ def test_trained_parameters_have_the_library_layout():
     assert z["scale_in"].shape == (3,) and z["scale_out"].shape == (2,)
     assert np.all(z["scale_in"] > 0) and np.all(z["scale_out"] > 0)
     assert int(z["nf"]) == 2048 and int(z["nc"]) == 256
+    # inference settings chosen on the tuning window
+    assert float(z["out_clamp"]) >= 0.0 and 0.0 <= float(z["sign_clip"]) <= 1.0
+    on, off = (float(v) for v in z["safe_cfl"])
+    assert 0.0 < off <= on
 
 
 def test_evaluation_record_is_consistent():
@@ -37,3 +41,8 @@ def test_evaluation_record_is_consistent():
     # the closure run is better than the plain coarse run everywhere it matters
     assert ev["closure"]["spectrum_rel_err_mean"] < ev["plain"]["spectrum_rel_err_mean"]
     assert ev["closure"]["flux_err"] < ev["plain"]["flux_err"]
+    # the recorded evaluation: stable (no fallback to the plain solver) and
+    # within 5% on average over the shells k <= 85
+    assert ev["closure"]["fallbacks"] == 0
+    assert 0.0 <= ev["closure"]["safe_fraction"] < 0.5
+    assert ev["closure"]["spectrum_rel_err_mean"] <= 0.05
