@@ -146,7 +146,8 @@ def large_bench(steps, dev, warmup=2):
     """The large single run: 16384^2 FP64, fixed dt = 2e-5, nu = nuv = 5e-6,
     hnu = hnuv = 1.0, Omega = 10, forcing kf = 256 (psi and vz, injection
     1.0), random-phase start.  At this size the step runs stage by stage from
-    full 2D transforms (two 8192-point halves per transform)."""
+    2D transforms of the dealiased columns (two 8192-point halves per
+    transform)."""
     import torch
     from rot2d3c_amd import ic, spectral as S
     n, dt = 16384, 2e-5
@@ -170,6 +171,7 @@ def large_bench(steps, dev, warmup=2):
     del ps, vz
     torch.cuda.empty_cache()
     return {"workload": "16384^2 FP64, dt=2e-5, nu=5e-6, nu+=1.0, Omega=10, kf=256, stage-by-stage step",
+            "kernels": "profiles/step16384_kernels.md",
             "steps": steps, "ms_per_step": round(ms, 3),
             "value": round(n * n * steps / wall / 1e6, 2), "unit": "Mpts·RK4-steps/s",
             "model_76S_frac_of_peak": round(b76 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
@@ -227,6 +229,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=500,
                     help="steps of the batched ensemble (24 x 256^2 + 12 x 512^2 + 4 x 1024^2); "
                          "0 disables")
+    ap.add_argument("--large-only", action="store_true",
+                    help="run only the 16384^2 section and print its object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -234,6 +238,11 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.large_only:
+        dev = torch.device("cuda:0")
+        torch.cuda.set_device(dev)
+        print(json.dumps({"large": large_bench(max(args.large_steps, 1), dev)}), flush=True)
+        return
     from rot2d3c_amd import _lib
     from rot2d3c_amd import ic, spectral as S
 

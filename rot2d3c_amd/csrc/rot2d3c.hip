@@ -68,7 +68,8 @@ __device__ __forceinline__ int xcd_pairs(int hb) {
 __host__ __forceinline__ int round16(int x) { return (x + 15) & ~15; }
 
 // spectral operators applied on load (derivk2 / laplak2 / inv_laplak2)
-enum { OP_RAW = 0, OP_DX = 1, OP_DY = 2, OP_MASK = 3, OP_LAP = 4, OP_INVLAP = 5 };
+enum { OP_RAW = 0, OP_DX = 1, OP_DY = 2, OP_MASK = 3, OP_LAP = 4, OP_INVLAP = 5, OP_LDX = 6,
+       OP_LDY = 7 };
 
 struct Geo {
   int n;
@@ -86,6 +87,15 @@ __device__ __forceinline__ cplx apply_op(int op, cplx a, int kx, int ky, double 
     case OP_MASK: return ka2 <= kmax ? a : mk(0.0, 0.0);
     case OP_LAP: return mk(-ka2 * a.x, -ka2 * a.y);
     case OP_INVLAP: return ka2 > tiny ? mk(-a.x / ka2, -a.y / ka2) : mk(0.0, 0.0);
+    // derivatives of the laplacian: OP_DX / OP_DY applied to OP_LAP's value
+    case OP_LDX: {
+      const cplx l = mk(-ka2 * a.x, -ka2 * a.y);
+      return ka2 <= kmax ? mk(-(double)kx * l.y, (double)kx * l.x) : mk(0.0, 0.0);
+    }
+    case OP_LDY: {
+      const cplx l = mk(-ka2 * a.x, -ka2 * a.y);
+      return ka2 <= kmax ? mk(-(double)ky * l.y, (double)ky * l.x) : mk(0.0, 0.0);
+    }
     default: return a;
   }
 }
@@ -1585,7 +1595,8 @@ static int run_steps(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
 
 // n = 16384 (definitions at the end of this file)
 constexpr int kBigN = 16384, kBigH = 8192, kBigT = 1024;
-static int big_c2r(r2d3c_plan *p, const cplx *in, int op, double *out, hipStream_t s);
+static int big_c2r(r2d3c_plan *p, const cplx *in, int op, double *out, hipStream_t s,
+                   int ncols = kBigN / 2 + 1);
 static int big_r2c(r2d3c_plan *p, const double *in, cplx *out, hipStream_t s);
 static int big_poisson(r2d3c_plan *p, const cplx *a, const cplx *b, cplx *c, hipStream_t s);
 static int big_cfl(r2d3c_plan *p, const r2d3c_params *q, double cfl, const cplx *psi, double *d_dt,
@@ -2864,16 +2875,23 @@ __global__ __launch_bounds__(kBigT) void k_big_col_inv(const cplx *__restrict__ 
   }
 }
 
-// rows y, y+1 of M (half spectra) -> R[y][:], R[y+1][:] as one packed transform
+// rows y, y+1 of M (half spectra) -> R[y][:], R[y+1][:] as one packed
+// transform; columns kx >= ncols are taken as zero (not read)
 __global__ __launch_bounds__(kBigT) void k_big_row_c2r(const cplx *__restrict__ mix,
                                                         double *__restrict__ out, Geo g,
-                                                        const cplx *__restrict__ tw) {
+                                                        const cplx *__restrict__ tw, int ncols) {
   constexpr int N = kBigN;
   __shared__ cplx lds[Shape<kBigH>::LDSN];
-  const int b = threadIdx.x, y = 2 * blockIdx.x;
+  // Consecutive workgroups go to the 8 XCDs in turn: the two row pairs of one
+  // 4-row tile (which share every 64-byte chunk of the tiled layout) are
+  // given to consecutive workgroups of one XCD, so the second finds the
+  // chunks in that XCD's L2.
+  const int xcd = blockIdx.x & 7, l = blockIdx.x >> 3;
+  const int b = threadIdx.x, y = 2 * ((((l >> 1) << 3) + xcd) * 2 + (l & 1));
   auto z = [&](int k) {
     const bool mir = k > N / 2;
     const int kk = mir ? N - k : k;
+    if (kk >= ncols) return mk(0.0, 0.0);
     return packed_z<N>(k, mix[tix(y, kk, g.kpf)], mix[tix(y + 1, kk, g.kpf)], mir);
   };
   cplx ev[8], od[8];
@@ -2917,15 +2935,53 @@ __global__ __launch_bounds__(kBigT) void k_big_row_r2c(const double *__restrict_
   }
 }
 
+// rows y, y+1 of the bracket J = (a b - c d) / n4 as one packed forward
+// transform: Z[y/2][k], k = 0..N-1 (the product a b is rounded first, as when
+// it is stored)
+__global__ __launch_bounds__(kBigT) void k_big_row_r2c_jac(const double *__restrict__ a,
+                                                            const double *__restrict__ b,
+                                                            const double *__restrict__ c,
+                                                            const double *__restrict__ d,
+                                                            double n4, cplx *__restrict__ zrows,
+                                                            const cplx *__restrict__ tw) {
+  constexpr int N = kBigN;
+  __shared__ cplx lds[Shape<kBigH>::LDSN];
+  const int t = threadIdx.x, y = 2 * blockIdx.x;
+  cplx ev[8], od[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int x = 2 * (t + kBigT * q);
+    const size_t e0 = (size_t)y * N + x, e1 = e0 + N;
+    const double2 a0 = *(const double2 *)(a + e0), a1 = *(const double2 *)(a + e1);
+    const double2 b0 = *(const double2 *)(b + e0), b1 = *(const double2 *)(b + e1);
+    const double2 c0 = *(const double2 *)(c + e0), c1 = *(const double2 *)(c + e1);
+    const double2 d0 = *(const double2 *)(d + e0), d1 = *(const double2 *)(d + e1);
+    ev[q] = mk((__dmul_rn(a0.x, b0.x) - c0.x * d0.x) / n4, (__dmul_rn(a1.x, b1.x) - c1.x * d1.x) / n4);
+    od[q] = mk((__dmul_rn(a0.y, b0.y) - c0.y * d0.y) / n4, (__dmul_rn(a1.y, b1.y) - c1.y * d1.y) / n4);
+  }
+  fft16k<-1>(ev, od, lds, t, tw);
+  cplx *zr = zrows + (size_t)blockIdx.x * N;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    zr[t + kBigT * q] = ev[q];
+    zr[t + kBigT * q + kBigH] = od[q];
+  }
+}
+
 // Column i of the half spectra, unpacked from the packed row transforms
 // (A = (Z[k] + conj Z[-k])/2 of row 2p, B = (Z[k] - conj Z[-k])/(2i) of row
 // 2p+1), forward transformed along y: S[i][:]
 __global__ __launch_bounds__(kBigT) void k_big_col_fwd(const cplx *__restrict__ zrows,
                                                         cplx *__restrict__ spec,
-                                                        const cplx *__restrict__ tw) {
+                                                        const cplx *__restrict__ tw, int ncols) {
   constexpr int N = kBigN;
   __shared__ cplx lds[Shape<kBigH>::LDSN];
-  const int i = blockIdx.x, b = threadIdx.x;
+  // The 8 columns of one 128-byte line go to consecutive workgroups of one
+  // XCD (workgroups go to the XCDs in turn), so 7 of them find their rows'
+  // lines in that XCD's L2.  The grid is rounded up to 64 workgroups.
+  const int xcd = blockIdx.x & 7, l = blockIdx.x >> 3;
+  const int i = ((l >> 3) << 6) + (xcd << 3) + (l & 7), b = threadIdx.x;
+  if (i >= ncols) return;
   cplx ev[8], od[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
@@ -2966,11 +3022,14 @@ __global__ __launch_bounds__(256) void k_real_gradmax(const double *__restrict__
 }
 
 // The stage update of src/Rot2D3C.para.f90:578-599 with the step's forcing:
-// rand_force draws (amplitude from the device dt) and/or dense arrays.
-__global__ void k_update_forced(const cplx *__restrict__ ps, const cplx *__restrict__ vz, cplx *c1,
-                                cplx *c2, const cplx *__restrict__ c3, const cplx *__restrict__ c4,
-                                const cplx *__restrict__ c5, const cplx *__restrict__ c6,
-                                StepConst c, const double *__restrict__ dtp, int o) {
+// rand_force draws (amplitude from the device dt) and/or dense arrays.  c1, c2
+// hold the stage's psi and vz; the rotation terms' d/dy of them (C3, C4 of the
+// reference, :575-576) are formed here.  The result goes to o1, o2, which may
+// be c1, c2 or ps, vz themselves (each element is read before it is written).
+__global__ void k_big_update(const cplx *ps, const cplx *vz, const cplx *c1, const cplx *c2,
+                             cplx *o1, cplx *o2, const cplx *__restrict__ c5,
+                             const cplx *__restrict__ c6, StepConst c,
+                             const double *__restrict__ dtp, int o) {
   const int n = c.g.n;
   const size_t total = (size_t)(n / 2 + 1) * n;
   const double dt = dtp ? *dtp : c.dt_fixed;
@@ -2987,32 +3046,45 @@ __global__ void k_update_forced(const cplx *__restrict__ ps, const cplx *__restr
         if (c.fz.e[q].i == i && c.fz.e[q].j == j) g = force_value(c.fz, q, dt, c.n2);
       if (c.fpd) f = cadd(f, c.fpd[e]);
       if (c.fzd) g = cadd(g, c.fzd[e]);
+      const cplx s1 = c1[e], s2 = c2[e];
+      const cplx t3 = mk(-ky * s1.y, ky * s1.x), t4 = mk(-ky * s2.y, ky * s2.x);
       const double tmp1 = dt / (double)o, tmp2 = 1.0 / ka2;
       double tmp3 = 1.0 + (c.nu * ipow(ka2, c.nn) + c.hnu * ipow(tmp2, c.mm)) * tmp1;
-      const cplx t5 = c5[e], t4 = c4[e];
+      const cplx t5 = c5[e];
       cplx t = mk((-t5.x + c.two_omega * t4.x) * tmp2 + f.x, (-t5.y + c.two_omega * t4.y) * tmp2 + f.y);
       t = cadd(ps[e], cscale(t, tmp1));
       a = mk(t.x / tmp3, t.y / tmp3);
       tmp3 = 1.0 + (c.nuv * ipow(ka2, c.nnv) + c.hnuv * ipow(tmp2, c.mmv)) * tmp1;
-      const cplx t6 = c6[e], t3 = c3[e];
+      const cplx t6 = c6[e];
       t = mk(t6.x + c.two_omega * t3.x + g.x, t6.y + c.two_omega * t3.y + g.y);
       t = cadd(vz[e], cscale(t, tmp1));
       b = mk(t.x / tmp3, t.y / tmp3);
     }
-    c1[e] = a;
-    c2[e] = b;
+    o1[e] = a;
+    o2[e] = b;
   }
 }
 
-static int big_c2r(r2d3c_plan *p, const cplx *in, int op, double *out, hipStream_t s) {
-  k_big_col_inv<<<kBigN / 2 + 1, kBigT, 0, s>>>(in, p->mfull, op, 1.0, p->g, p->tw);
-  k_big_row_c2r<<<kBigN / 2, kBigT, 0, s>>>(p->mfull, out, p->g, p->tw);
+// ncols < n/2 + 1: the input is zero in the columns kx >= ncols (dealiased)
+static int big_c2r(r2d3c_plan *p, const cplx *in, int op, double *out, hipStream_t s, int ncols) {
+  k_big_col_inv<<<ncols, kBigT, 0, s>>>(in, p->mfull, op, 1.0, p->g, p->tw);
+  k_big_row_c2r<<<kBigN / 2, kBigT, 0, s>>>(p->mfull, out, p->g, p->tw, ncols);
   LAUNCHCHK("c2r 16384");
+  return 0;
+}
+// columns kx < ncols of r2c((a b - c d) / n^4)
+static int big_r2c_jac(r2d3c_plan *p, const double *a, const double *b, const double *c,
+                       const double *d, cplx *out, int ncols, hipStream_t s) {
+  const double n4 = (double)kBigN * kBigN * (double)kBigN * kBigN;
+  k_big_row_r2c_jac<<<kBigN / 2, kBigT, 0, s>>>(a, b, c, d, n4, p->mfull, p->tw);
+  k_big_col_fwd<<<(ncols + 63) / 64 * 64, kBigT, 0, s>>>(p->mfull, out, p->tw, ncols);
+  LAUNCHCHK("r2c bracket 16384");
   return 0;
 }
 static int big_r2c(r2d3c_plan *p, const double *in, cplx *out, hipStream_t s) {
   k_big_row_r2c<<<kBigN / 2, kBigT, 0, s>>>(in, p->mfull, p->tw);
-  k_big_col_fwd<<<kBigN / 2 + 1, kBigT, 0, s>>>(p->mfull, out, p->tw);
+  k_big_col_fwd<<<(kBigN / 2 + 1 + 63) / 64 * 64, kBigT, 0, s>>>(p->mfull, out, p->tw,
+                                                                  kBigN / 2 + 1);
   LAUNCHCHK("r2c 16384");
   return 0;
 }
@@ -3039,7 +3111,14 @@ static int big_cfl(r2d3c_plan *p, const r2d3c_params *q, double cfl, const cplx 
   LAUNCHCHK("cfl 16384");
   return 0;
 }
-// one time step, stage by stage (src/Rot2D3C.para.f90:399,545-611)
+// One time step, stage by stage (src/Rot2D3C.para.f90:399,545-611).  Per
+// stage six c2r of the dealiased columns (d/dx, d/dy of psi, of laplacian(psi)
+// and of vz: the two brackets share psi's derivatives), the two brackets
+// formed inside their r2c row pass, and the update, which forms d/dy of psi
+// and vz itself.  Stage 4 reads psi, vz directly and stage 1 writes them, so
+// the step copies no state; the update's output is already the masked state.
+// In CFL mode the step's dt comes from stage 4's d/dx, d/dy of psi, which are
+// the fields CFL_condition transforms.
 static int big_step(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
                     const r2d3c_forcing *fo, int dt_mode, double dt_or_cfl, double *d_dt,
                     hipStream_t s) {
@@ -3052,27 +3131,35 @@ static int big_step(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
     c.fpd = (const cplx *)fo->dense_psi;
     c.fzd = (const cplx *)fo->dense_vz;
   }
-  const size_t sz = (size_t)(kBigN / 2 + 1) * kBigN;
+  const size_t sz = (size_t)(kBigN / 2 + 1) * kBigN, nn = (size_t)kBigN * kBigN;
   cplx *C1 = p->ci, *C2 = C1 + sz, *C3 = C2 + sz, *C4 = C3 + sz, *C5 = C4 + sz, *C6 = C5 + sz;
+  // six real fields: three real scratch fields, and C3, C4 and the first
+  // transfers scratch field (each holds n^2 doubles)
+  double *R1 = p->real, *R2 = p->real2, *R3 = p->real3, *R4 = (double *)C3, *R5 = (double *)C4,
+         *R6 = (double *)p->spec3;
+  const int nc = p->g.nc;
   double *dtp = d_dt ? d_dt : p->dtslot;
   int rc;
-  if (dt_mode == R2D3C_DT_CFL) {
-    if ((rc = big_cfl(p, q, dt_or_cfl, psi, dtp, s))) return rc;
-  } else {
-    k_set_f64<<<1, 1, 0, s>>>(dtp, dt_or_cfl);
-  }
-  HIPCHK(hipMemcpyAsync(C1, psi, sizeof(cplx) * sz, hipMemcpyDeviceToDevice, s));
-  HIPCHK(hipMemcpyAsync(C2, vz, sizeof(cplx) * sz, hipMemcpyDeviceToDevice, s));
+  if (dt_mode != R2D3C_DT_CFL) k_set_f64<<<1, 1, 0, s>>>(dtp, dt_or_cfl);
   const int g = launch_grid(sz);
   for (int o = 4; o >= 1; --o) {
-    k_spec_op<<<g, 256, 0, s>>>(C1, C4, OP_LAP, kBigN, p->g.kmax, p->g.tiny);
-    if ((rc = big_poisson(p, C1, C4, C5, s)) || (rc = big_poisson(p, C1, C2, C6, s))) return rc;
-    k_spec_op<<<g, 256, 0, s>>>(C1, C3, OP_DY, kBigN, p->g.kmax, p->g.tiny);
-    k_spec_op<<<g, 256, 0, s>>>(C2, C4, OP_DY, kBigN, p->g.kmax, p->g.tiny);
-    k_update_forced<<<g, 256, 0, s>>>(psi, vz, C1, C2, C3, C4, C5, C6, c, dtp, o);
+    const cplx *a = o == 4 ? psi : C1, *v = o == 4 ? vz : C2;
+    if ((rc = big_c2r(p, a, OP_DX, R1, s, nc)) || (rc = big_c2r(p, a, OP_DY, R2, s, nc))) return rc;
+    if (o == 4 && dt_mode == R2D3C_DT_CFL) {
+      k_set_u64<<<1, 1, 0, s>>>(p->maxbits, 0ull);
+      k_real_gradmax<<<kReduceBlocks, 256, 0, s>>>(R1, R2, nn, p->maxbits);
+      k_dt_from_max<<<1, 1, 0, s>>>(p->maxbits, make_dt_const(kBigN, q, dt_or_cfl), dtp);
+    }
+    if ((rc = big_c2r(p, a, OP_LDX, R3, s, nc)) || (rc = big_c2r(p, a, OP_LDY, R4, s, nc)) ||
+        (rc = big_c2r(p, v, OP_DX, R5, s, nc)) || (rc = big_c2r(p, v, OP_DY, R6, s, nc)))
+      return rc;
+    // poisson(psi, laplacian psi) and poisson(psi, vz) (:572-573)
+    if ((rc = big_r2c_jac(p, R1, R4, R2, R3, C5, nc, s)) ||
+        (rc = big_r2c_jac(p, R1, R6, R2, R5, C6, nc, s)))
+      return rc;
+    k_big_update<<<g, 256, 0, s>>>(psi, vz, a, v, o == 1 ? psi : C1, o == 1 ? vz : C2, C5, C6, c,
+                                   dtp, o);
   }
-  k_spec_op<<<g, 256, 0, s>>>(C1, psi, OP_MASK, kBigN, p->g.kmax, p->g.tiny);
-  k_spec_op<<<g, 256, 0, s>>>(C2, vz, OP_MASK, kBigN, p->g.kmax, p->g.tiny);
   LAUNCHCHK("step 16384");
   return 0;
 }

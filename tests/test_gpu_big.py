@@ -98,3 +98,27 @@ def test_step_matches_512_for_band_limited_state(cuda):
         del ps, vz
     for a, b in zip(out[512], out[N]):
         assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-10
+
+
+def test_cfl_step_takes_the_cfl_condition_dt(cuda):
+    """A CFL-mode step (its dt formed inside the first stage from the stage's
+    own d/dx, d/dy of psi) equals, bit for bit, a fixed-dt step with the dt
+    of CFL_condition on the pre-step state, and reports that dt."""
+    print("16384 CFL step", flush=True)
+    from rot2d3c_amd import spectral as S
+    p = S.Params(1e-4, 0.1, 1, 2, 2e-4, 0.05, 1, 2, 3.0)
+    ps0 = _modes_field(N, [(3, -2, 1.0, 0.3), (5, 4, 0.5, 1.0)], cuda)
+    vz0 = _modes_field(N, [(0, 7, 0.7, 0.2), (2, -3, 0.4, 2.0)], cuda)
+    draws = (S.ForceDraw(6, -5, 0.4, 1.0, 8.0), S.ForceDraw(3, 7, 1.3, 0.5, 1.0))
+    cfl = 0.3
+    dt_ref = S.CFL_condition(cfl, ps0, vz0, p.nu, p.nn, p.nuv, p.nnv, p.omega)
+    a = (ps0.clone(), vz0.clone())
+    t = torch.zeros(1, dtype=torch.float64, device=cuda)
+    S.rk4_step(a[0], a[1], p, *draws, cfl=cfl, dt_out=t)
+    del ps0, vz0
+    b = _modes_field(N, [(3, -2, 1.0, 0.3), (5, 4, 0.5, 1.0)], cuda), \
+        _modes_field(N, [(0, 7, 0.7, 0.2), (2, -3, 0.4, 2.0)], cuda)
+    S.rk4_step(b[0], b[1], p, *draws, dt=dt_ref)
+    torch.cuda.synchronize()
+    assert float(t.item()) == dt_ref and 0.0 < dt_ref < 1.0
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
