@@ -226,9 +226,14 @@ void r2d3c_rand_force_draw(double kup, int *seed, double *kx, double *ky, double
 
 /* ---- schedules and autotuning (no reference counterpart) --------------- */
 
-/* Schedule of the persistent row kernel of r2d3c_step / r2d3c_run for n >= 256:
- * rows per block (2, or 1 when n >= 512) and blocks per CU (1..4).  Every
- * schedule gives bit-identical results; only the speed differs.  Default 2, 1. */
+/* Schedule of the row kernel of r2d3c_step / r2d3c_run and of ensembles for
+ * n >= 256: blocks per CU 0 selects the non-persistent kernel (one block per
+ * 2 rows; rows must be 2); 1..4 select the persistent kernel with that many
+ * blocks per CU and 2 rows per block, or 1 when n >= 512.  Schedules that
+ * differ only in blocks per CU run the same kernel and give bit-identical
+ * results; the three kernels (non-persistent, persistent with 2 or 1 rows)
+ * agree within rounding (the compiler may fuse their multiply-adds
+ * differently).  Default 2, 0. */
 int r2d3c_plan_set_schedule(r2d3c_plan *plan, int row_rows, int row_blocks_per_cu);
 int r2d3c_plan_get_schedule(const r2d3c_plan *plan, int *row_rows, int *row_blocks_per_cu);
 /* Times every valid schedule on scratch state (best of three trials of

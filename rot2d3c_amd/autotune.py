@@ -1,10 +1,13 @@
 """Per-size schedule autotuner with a JSON cache.
 
-A schedule is the launch shape of the persistent row kernel for one grid size:
-rows per block (2, or 1 when n >= 512) and blocks per CU (1 or 2).  Schedules
-change only the thread and LDS layout, never the arithmetic, so a run gives
-bit-identical results under every schedule; the tuner only looks for the
-fastest.  r2d3c_autotune times each candidate on the device; the choice is
+A schedule is the launch shape of the row kernel for one grid size: (2, 0)
+is the non-persistent kernel (one block per 2 rows); (rows, blocks per CU)
+with blocks per CU 1 or 2 the persistent kernel with 2 rows per block, or 1
+when n >= 512.  Schedules
+change only the thread and LDS layout, never the arithmetic: schedules of one
+compiled kernel give bit-identical results, the three kernels agree within
+rounding; the tuner only looks for the fastest, and its cached choice keeps a
+machine's runs reproducible.  r2d3c_autotune times each candidate on the device; the choice is
 cached per (GPU architecture, CU count, n) in a JSON file, by default
 ~/.cache/rot2d3c_amd/schedules.json (R2D3C_SCHEDULE_CACHE overrides it).
 """
@@ -18,7 +21,7 @@ import torch
 from . import _lib
 from ._lib import check
 
-DEFAULT = (2, 1)
+DEFAULT = (2, 0)
 _lock = threading.Lock()
 
 

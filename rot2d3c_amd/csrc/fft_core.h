@@ -28,6 +28,11 @@ typedef double2 cplx;
 __device__ __forceinline__ cplx mk(double x, double y) { return make_double2(x, y); }
 __device__ __forceinline__ cplx cadd(cplx a, cplx b) { return mk(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ cplx csub(cplx a, cplx b) { return mk(a.x - b.x, a.y - b.y); }
+// a product rounded on its own: never fused into the add that uses it
+__device__ __forceinline__ double mul_rn(double a, double b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
 __device__ __forceinline__ cplx cmul(cplx a, cplx b) {
   return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -105,8 +110,9 @@ template <int R, int S> __device__ __forceinline__ void dftR(cplx *x) {
 // transform's LDS area; consecutive slots are IL entries apart, so IL
 // transforms can share one interleaved LDS array.  tw holds TS*N entries
 // (exp(-2 pi i t / (TS N))): a longer transform's table serves its halves.  All threads of the block
-// must call this together (it contains block barriers); the LDS area is free
-// again on return.
+// must call this together (it contains block barriers).  The last pass reads
+// the LDS area and a closing barrier follows it, so the area is free for any
+// thread to write again on return.
 template <int N, int S, int IL, int TS = 1>
 __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
                                           const cplx *__restrict__ tw) {
@@ -174,6 +180,7 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
 #pragma unroll
     for (int m = 0; m < RL; ++m) v[u + m * U] = x[m];
   }
+  __syncthreads();
   __syncthreads();
 }
 

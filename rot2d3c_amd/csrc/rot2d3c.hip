@@ -855,6 +855,10 @@ __global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
   __shared__ cplx lds[RW * Shape<N>::LDSN];
   const int r = threadIdx.x % RW, b = threadIdx.x / RW;
   const int y = xcd_group<RowStep<N>::G>(blockIdx.x) * RW + r;
+  // Each transform below takes an opaque copy of the thread index (b0..b7):
+  // the compiler cannot hoist one transform's index and address math over
+  // the others, which keeps the kernel at 128 VGPRs (4 waves per SIMD)
+  // instead of ~245.
   M += blockIdx.y * mstride;
   Q += blockIdx.y * mstride;
   const double two_omega_n2 = om_tab ? om_tab[blockIdx.y] : c.two_omega_n2;
@@ -862,8 +866,12 @@ __global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
   const size_t fs = c.fs;
   cplx v[8];
   double px[8], py[8], j1[8];
-  load_packed_step<N>(v, M, M + fs, y, b, nc, kp);
-  fft_canon<N, +1, RW>(v, lds + r, b, tw);
+  int b0 = b;
+  asm volatile("" : "+v"(b0));
+  load_packed_step<N>(v, M, M + fs, y, b0, nc, kp);
+  int b1 = b;
+  asm volatile("" : "+v"(b1));
+  fft_canon<N, +1, RW>(v, lds + r, b1, tw);
 #pragma unroll
   for (int q = 0; q < 8; ++q) { px[q] = v[q].x; py[q] = v[q].y; }
   if (maxbits) {
@@ -874,12 +882,20 @@ __global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
       if (off < (int)blockDim.x) m = fmax(m, __shfl_xor(m, off));
     if ((threadIdx.x & 63) == 0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(m));
   }
-  load_packed_step<N>(v, M + 2 * fs, M + 3 * fs, y, b, nc, kp);
-  fft_canon<N, +1, RW>(v, lds + r, b, tw);
+  int b2 = b;
+  asm volatile("" : "+v"(b2));
+  load_packed_step<N>(v, M + 2 * fs, M + 3 * fs, y, b2, nc, kp);
+  int b3 = b;
+  asm volatile("" : "+v"(b3));
+  fft_canon<N, +1, RW>(v, lds + r, b3, tw);
 #pragma unroll
   for (int q = 0; q < 8; ++q) j1[q] = px[q] * v[q].y - py[q] * v[q].x;
-  load_packed_step<N>(v, M + 4 * fs, M + 5 * fs, y, b, nc, kp);
-  fft_canon<N, +1, RW>(v, lds + r, b, tw);
+  int b4 = b;
+  asm volatile("" : "+v"(b4));
+  load_packed_step<N>(v, M + 4 * fs, M + 5 * fs, y, b4, nc, kp);
+  int b5 = b;
+  asm volatile("" : "+v"(b5));
+  fft_canon<N, +1, RW>(v, lds + r, b5, tw);
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const double vx = v[q].x, vy = v[q].y;
@@ -887,11 +903,15 @@ __global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
     const double p2 = ((px[q] * vy - py[q] * vx) + two_omega_n2 * py[q]) * c.inv_n4;
     v[q] = mk(p1, p2);
   }
-  fft_canon<N, -1, RW>(v, lds + r, b, tw);
+  int b6 = b;
+  asm volatile("" : "+v"(b6));
+  fft_canon<N, -1, RW>(v, lds + r, b6, tw);
 #pragma unroll
   for (int q = 0; q < 8; ++q) lds[pad(b + T * q) * RW + r] = v[q];
   __syncthreads();
-  unpack_store<N, RW>(lds + r, b, nc, kp, Q, y, Q + fs, y);
+  int b7 = b;
+  asm volatile("" : "+v"(b7));
+  unpack_store<N, RW>(lds + r, b7, nc, kp, Q, y, Q + fs, y);
 }
 
 
@@ -902,7 +922,9 @@ __global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
 // LDS: RW*(n + n/16) complex exchange slots + one staging buffer of 2*nc*RW.
 // RWQ (rows per block, 1 or 2) and the grid size are schedule choices: they
 // change neither the arithmetic nor its order, only the thread and LDS
-// layout, so every schedule gives bit-identical results.
+// layout.  The grid size leaves the compiled kernel unchanged, so it gives
+// bit-identical results; the RWQ variants are compiled separately and agree
+// within rounding (multiply-add fusion may differ).
 template <int N, int RWQ> struct RowV2 {
   static constexpr int T = N / 8;
   static constexpr int NCMAX = N / 3 + 2;
@@ -1251,9 +1273,10 @@ struct r2d3c_plan {
   double *dtslot = nullptr;
   double *partial = nullptr;      // reduction partials
   long long bytes = 0;
-  // schedule of the persistent row kernel (r2d3c_plan_set_schedule /
-  // r2d3c_autotune): rows per block and blocks per CU
-  int row_rw = 2, row_bpc = 1;
+  // schedule of the row kernel (r2d3c_plan_set_schedule / r2d3c_autotune):
+  // rows per block and blocks per CU of the persistent kernel, or bpc = 0
+  // for the non-persistent kernel (2 rows per block)
+  int row_rw = 2, row_bpc = 0;
   // kernel timing of r2d3c_step (r2d3c_timing_enable / r2d3c_timing_read)
   int timing = 0;
   std::vector<hipEvent_t> evpool;
@@ -1455,7 +1478,11 @@ static int step_seq(r2d3c_plan *p, StepConst &c, cplx *psi, cplx *vz, hipStream_
     unsigned long long *macc = (o == 4 && c.cfl_mode) ? mb : nullptr;
     ev_pair(p, 1, &ea, &eb);
     if constexpr (RowV2<N, 2>::THREADS >= 64) {
-      if (p->row_rw == 1 && RowV2<N, 1>::THREADS >= 64)
+      if (p->row_bpc == 0)
+        hipExtLaunchKernelGGL(k_step_row<N>, dim3(N / RowStep<N>::ROWS),
+                              dim3(N / 8 * RowStep<N>::ROWS), 0, s, ea, eb, 0, (const cplx *)M, Q,
+                              c, macc, tw, (size_t)0, (const double *)nullptr);
+      else if (p->row_rw == 1 && RowV2<N, 1>::THREADS >= 64)
         hipExtLaunchKernelGGL(k_step_row2<N, 1>, dim3(row2_grid<N, 1>(p->cus, p->row_bpc)),
                               dim3(RowV2<N, 1>::THREADS), 0, s, ea, eb, 0, (const cplx *)M, Q, c,
                               macc, tw, (size_t)0, (const double *)nullptr);
@@ -1989,6 +2016,8 @@ void r2d3c_rand_force_draw(double kup, int *seed, double *kx, double *ky, double
 // schedules and their autotuner
 // ---------------------------------------------------------------------------
 static bool schedule_valid(int n, int rw, int bpc) {
+  // bpc = 0: the non-persistent row kernel, one block per 2 rows
+  if (bpc == 0) return rw == 2;
   if (bpc < 1 || bpc > 4 || (rw != 1 && rw != 2)) return false;
   // one row per block needs n/8 >= 64 threads (whole waves)
   return rw == 2 || n >= 512;
@@ -2048,11 +2077,11 @@ int r2d3c_autotune(r2d3c_plan *p, int reps, double *ms_per_step, void *stream) {
   int rc = 0;
   if ((e = hipMemsetAsync(psi, 0, sz, s)) != hipSuccess || (e = hipMemsetAsync(vz, 0, sz, s)) != hipSuccess)
     rc = hip_err(e, "hipMemsetAsync");
-  const int cand[4][2] = {{2, 1}, {2, 2}, {1, 1}, {1, 2}};
+  const int cand[5][2] = {{2, 0}, {2, 1}, {2, 2}, {1, 1}, {1, 2}};
   int best = -1;
   float best_ms = 0.f;
   const int rw0 = p->row_rw, bpc0 = p->row_bpc;
-  for (int k = 0; k < 4 && !rc; ++k) {
+  for (int k = 0; k < 5 && !rc; ++k) {
     if (!schedule_valid(n, cand[k][0], cand[k][1])) continue;
     if (n < 256 && k > 0) break;  // no persistent row kernel below 256: nothing to tune
     p->row_rw = cand[k][0];
@@ -2169,7 +2198,10 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
     const cplx *M0 = k.work;
     cplx *Q0 = k.work + 6 * p->fs;
     if constexpr (RowV2<N, 2>::THREADS >= 64) {
-      if (p->row_rw == 1 && RowV2<N, 1>::THREADS >= 64)
+      if (p->row_bpc == 0)
+        k_step_row<N><<<dim3(N / RowStep<N>::ROWS, k.count), N / 8 * RowStep<N>::ROWS, 0, s>>>(
+            M0, Q0, k.c0, nullptr, tw, ms, k.d_om);
+      else if (p->row_rw == 1 && RowV2<N, 1>::THREADS >= 64)
         k_step_row2<N, 1><<<dim3(row2_grid<N, 1>(p->cus, p->row_bpc), k.count),
                             RowV2<N, 1>::THREADS, 0, s>>>(M0, Q0, k.c0, nullptr, tw, ms, k.d_om);
       else
@@ -2956,8 +2988,8 @@ __global__ __launch_bounds__(kBigT) void k_big_row_r2c_jac(const double *__restr
     const double2 b0 = *(const double2 *)(b + e0), b1 = *(const double2 *)(b + e1);
     const double2 c0 = *(const double2 *)(c + e0), c1 = *(const double2 *)(c + e1);
     const double2 d0 = *(const double2 *)(d + e0), d1 = *(const double2 *)(d + e1);
-    ev[q] = mk((__dmul_rn(a0.x, b0.x) - c0.x * d0.x) / n4, (__dmul_rn(a1.x, b1.x) - c1.x * d1.x) / n4);
-    od[q] = mk((__dmul_rn(a0.y, b0.y) - c0.y * d0.y) / n4, (__dmul_rn(a1.y, b1.y) - c1.y * d1.y) / n4);
+    ev[q] = mk((mul_rn(a0.x, b0.x) - c0.x * d0.x) / n4, (mul_rn(a1.x, b1.x) - c1.x * d1.x) / n4);
+    od[q] = mk((mul_rn(a0.y, b0.y) - c0.y * d0.y) / n4, (mul_rn(a1.y, b1.y) - c1.y * d1.y) / n4);
   }
   fft16k<-1>(ev, od, lds, t, tw);
   cplx *zr = zrows + (size_t)blockIdx.x * N;
