@@ -29,41 +29,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
 
-
-def algorithmic_bytes(n, k):
-    """Compulsory HBM bytes per launch of each step kernel class for a run of k
-    chained steps (one r2d3c_run call), dealiased schedule.
-
-    M = bytes of one mixed field restricted to the kx <= K columns every
-    dealiased field lives in, D' = number of (kx, ky) half-plane modes inside
-    the update mask tiny < k^2 <= kmax (D = D' + 1 with the k = 0 mode),
-    S = bytes of one full spectral field.
-      column inverse (once per run): read psi, vz on the disc, write 6 M
-      row kernel: read 6 M, write 2 M
-      fused column, stages 1-3: read 2 M + psi0, vz0 on the disc, write 6 M
-      fused column, last stage: read 2 M + psi0, vz0, write psi, vz on the disc
-        and, except after the run's last step, the next step's 6 M."""
-    kmax = (n / 3.0) ** 2
-    K = 0
-    while (K + 1) ** 2 <= kmax:
-        K += 1
-    M = n * (K + 1) * 16
-    ky = np.fft.fftfreq(n, 1.0 / n)
-    kx = np.arange(K + 1, dtype=np.float64)
-    D = int(np.count_nonzero(kx[:, None] ** 2 + ky[None, :] ** 2 <= kmax))
-    S = 16 * n * (n // 2 + 1)
-    disc = 2 * 16 * (D - 1)
-    last_chained = 2 * M + disc + disc + 6 * M
-    last_final = 2 * M + disc + disc
-    per = {
-        "k_step_colinv": 2 * 16 * D + 6 * M,
-        "k_step_row": 8 * M,
-        "k_step_colfused": 2 * M + disc + 6 * M,
-        "k_step_colfused_last": ((k - 1) * last_chained + last_final) / k,
-    }
-    step = per["k_step_colinv"] / k + 4 * per["k_step_row"] + 3 * per["k_step_colfused"] \
-        + per["k_step_colfused_last"]
-    return per, step, {"K": K, "M": M, "D": D, "S": S, "B76S": 76 * S}
+from rot2d3c_amd.traffic import algorithmic_bytes  # noqa: E402
 
 
 def cpu_baseline(n, steps, p, kf, f0, seed, ps0, vz0, dt):
@@ -81,8 +47,9 @@ def cpu_baseline(n, steps, p, kf, f0, seed, ps0, vz0, dt):
 def closure_bench(dns_ms_per_step, dns_dt, dev, steps=500, warmup=20):
     """GPU-seconds per unit simulated time of the 256^2 run with the learned
     closure (the committed trained network, rot2d3c_amd/data/closure_256.npz;
-    one-run ensemble, closure inside the step graph, fixed dt = 8 x the DNS dt:
-    the DNS's CFL number; chunks of 50 steps, each followed by the run-time
+    one-run ensemble, closure inside the step graphs with its committed
+    settings (network every 2nd step), fixed dt = 8 x the DNS dt: the DNS's
+    CFL number; chunks of 50 steps, each followed by the run-time
     stabiliser's CFL check) against the 2048^2 DNS timed above.  The spectrum
     and flux errors are those of the committed evaluation
     (rot2d3c_amd/data/closure_256_eval.json, made by rot2d3c_amd.closure_study)."""
@@ -102,6 +69,8 @@ def closure_bench(dns_ms_per_step, dns_dt, dev, steps=500, warmup=20):
         net.set_sign_clip(float(z["sign_clip"]))
     ens.set_closure(0, net, z["scale_in"], z["scale_out"], float(z["kmin"]),
                     float(z["kclip"]) if "kclip" in z else 0.0)
+    if "period" in z:
+        ens.set_closure_period(int(z["period"]))
     spec = [dict(kf=40.0, f0=1.0, seed=3)]
     warm = ens.forcing_array(E.forcing_table(spec, warmup))
     # chunks of 50 steps with the run-time stabiliser's CFL check after each,

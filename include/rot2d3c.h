@@ -363,6 +363,10 @@ int r2d3c_closure_clip(r2d3c_plan *plan, const double *psi, const double *vz, do
  * closure (the plain solver).  The net must take 1 image of n x n, serve
  * no other run (its buffers hold this run's activations) and stay alive while
  * the ensemble uses it. */
+/* The closure runs' forcing is evaluated on every `period`-th step of each
+ * r2d3c_ensemble_run call (its first step, then steps period, 2 period, ...)
+ * and held in between.  Default 1 (every step). */
+int r2d3c_ensemble_set_closure_period(r2d3c_ensemble *ens, int period);
 int r2d3c_ensemble_set_closure(r2d3c_ensemble *ens, int run, r2d3c_cnn *net,
                                const double *scale_in, const double *scale_out, double kmin,
                                double kclip);

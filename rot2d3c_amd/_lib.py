@@ -77,6 +77,7 @@ SIGNATURES = {
     "r2d3c_ensemble_destroy": (_c_int, [_vp]),
     "r2d3c_ensemble_set_schedule": (_c_int, [_vp, _c_int, _c_int, _c_int]),
     "r2d3c_ensemble_run": (_c_int, [_vp, ctypes.POINTER(Forcing), _c_int, _vp]),
+    "r2d3c_ensemble_set_closure_period": (_c_int, [_vp, _c_int]),
     "r2d3c_ensemble_set_closure": (_c_int, [_vp, _c_int, _vp, ctypes.POINTER(_c_double),
                                             ctypes.POINTER(_c_double), _c_double, _c_double]),
     "r2d3c_filter": (_c_int, [_vp, _vp, _vp, _vp, _c_double, _vp]),

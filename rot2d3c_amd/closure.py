@@ -345,15 +345,17 @@ class ClosureRun:
     inputs and outputs (1/std of the training inputs, std of the targets).
     With net=None (or after a fallback) it is the plain coarse solver; with
     kclip > 0 the closure forcing is backscatter-clipped on |k| >= kclip
-    (clip_forcing)."""
+    (clip_forcing); the forcing is evaluated on every period-th step (counted
+    from construction) and held in between."""
 
     def __init__(self, ps, vz, params, net=None, scale_in=None, scale_out=None, guard_every=50,
-                 energy_bound=None, kmin=0.0, kclip=0.0):
+                 energy_bound=None, kmin=0.0, kclip=0.0, period=1):
         self.ps, self.vz, self.p = ps, vz, params
         self.n = S._spec_n(ps)
         self.net = net
         self.scale_in, self.scale_out = scale_in, scale_out
         self.kmin, self.kclip = kmin, float(kclip)
+        self.period = int(period)  # forcing evaluated every period-th step, held between
         self.active = net is not None
         self.guard_every = guard_every
         self.energy_bound = energy_bound
@@ -368,7 +370,9 @@ class ClosureRun:
 
     def step(self, fp_draw=None, fz_draw=None, dt=None, cfl=None, dt_out=None):
         dense = None
-        if self.active:
+        if self.active and self._steps % self.period:
+            dense = (self._fp, self._fz)  # the held forcing
+        elif self.active:
             network_inputs(self.ps, self.vz, self.scale_in, out=self._inp[0])
             self.net.forward(self._inp, out=self._pred)
             closure_forcing(self._pred[0], self.n, self.scale_out, self._fp, self._fz, self.kmin)

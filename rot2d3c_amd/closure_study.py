@@ -87,7 +87,7 @@ def errors(les, dns, kmax):
 
 
 def timed_steps(ps0, vz0, p, dt, forcing, steps, net=None, sin=None, sout=None, kmin=0.0,
-                kclip=0.0, clamp=0.0, sign_clip=0.0):
+                kclip=0.0, clamp=0.0, sign_clip=0.0, period=1):
     """Seconds per step of a one-run ensemble (each step one replay of the
     captured step graph, the closure inside it when net is given), timed over
     one run() call of `steps` steps after a warm-up call; host-side table
@@ -99,6 +99,7 @@ def timed_steps(ps0, vz0, p, dt, forcing, steps, net=None, sin=None, sout=None, 
         net.set_output_clamp(clamp)
         net.set_sign_clip(sign_clip)
         ens.set_closure(0, net, sin, sout, kmin, kclip)
+        ens.set_closure_period(period)
     warm = ens.forcing_array([[pair] for pair in forcing.take(20)])
     table = ens.forcing_array([[pair] for pair in forcing.take(steps)])
     ens.run(20, warm)
@@ -114,7 +115,7 @@ def timed_steps(ps0, vz0, p, dt, forcing, steps, net=None, sin=None, sout=None, 
 def coarse_runs(ps0, vz0, p, t_end, dt, seeds, kf, f0, params=None, sin=None, sout=None,
                 kmin=0.0, samples=200, energy_bound=None, chunk=50, kclip=0.0,
                 enstrophy_bound=None, trace=None, clamp=0.0, cfl_bound=None, sign_clip=0.0,
-                safe=None):
+                safe=None, period=1):
     """len(seeds) coarse runs from (ps0, vz0), one coarse-grained forcing
     realisation each (the filtered equation's forcing G f),
     advanced together as one ensemble to simulated time t_end with fixed dt;
@@ -142,6 +143,7 @@ def coarse_runs(ps0, vz0, p, t_end, dt, seeds, kf, f0, params=None, sin=None, so
             nets[-1].set_output_clamp(clamp)
             nets[-1].set_sign_clip(sign_clip)
             ens.set_closure(r, nets[-1], sin, sout, kmin, kclip)
+        ens.set_closure_period(period)
     forcings = [Forcing(kf, f0, sd, S._spec_n(ps0)) for sd in seeds]
     nsteps = int(round(t_end / dt))
     every = max(1, nsteps // samples)
@@ -278,7 +280,7 @@ def compact_record(out, command):
                      "seconds": out["train"]["seconds"],
                      "epochs": None if out["config"]["params"] else out["config"]["epochs"]},
            "kmin": out["kmin"], "gain": out["gain"], "kclip": out["kclip"], "clamp": out["clamp"],
-           "sign_clip": out["sign_clip"],
+           "sign_clip": out["sign_clip"], "period": out["period"],
            "dns_cfl": out["dns"]["cfl"], "cfl_guard": out["config"]["cfl_guard"],
            "safe_cfl": out["config"]["safe_cfl"],
            "tune": {k: {m: v[m] for m in ("spectrum_rel_err_max", "spectrum_rel_err_mean",
@@ -328,6 +330,8 @@ def main(argv=None):
     ap.add_argument("--sign-clip", type=float, nargs="+", default=[0.0],
                     help="candidate strengths in [0, 1] of the network's sign clipping "
                          "(closure.Net.set_sign_clip), tuned with the others")
+    ap.add_argument("--period", type=int, nargs="+", default=[1],
+                    help="candidate closure refresh periods in steps (forcing held in between)")
     ap.add_argument("--safe-cfl", type=float, nargs=2, default=None, metavar=("ON", "OFF"),
                     help="switch a closure run to the fully sign-clipped network while the "
                          "CFL number of its fixed dt is above ON times the DNS's, back below "
@@ -420,11 +424,11 @@ def main(argv=None):
     dns_tune = dns_window(advance, ps, vz, nc, a.tune_steps)
     Tt = a.tune_steps * a.dt
     out["tune"] = {}
-    cands = [dict(kmin=km, gain=g, kclip=kc, clamp=cl, sign_clip=sc)
+    cands = [dict(kmin=km, gain=g, kclip=kc, clamp=cl, sign_clip=sc, period=pe)
              for km in a.kmin for g in a.gain for kc in a.kclip for cl in a.clamp
-             for sc in a.sign_clip]
-    key = lambda c: "kmin %g gain %g kclip %g clamp %g sign_clip %g" % (
-        c["kmin"], c["gain"], c["kclip"], c["clamp"], c["sign_clip"])
+             for sc in a.sign_clip for pe in a.period]
+    key = lambda c: "kmin %g gain %g kclip %g clamp %g sign_clip %g period %d" % (
+        c["kmin"], c["gain"], c["kclip"], c["clamp"], c["sign_clip"], c["period"])
     cfl_bound = a.cfl_guard * out["dns"]["cfl"]
     safe = None if a.safe_cfl is None else tuple(v * out["dns"]["cfl"] for v in a.safe_cfl)
     for c in cands:
@@ -433,7 +437,7 @@ def main(argv=None):
                                                energy_bound=20.0 * e0, kclip=c["kclip"],
                                                enstrophy_bound=a.guard * z0, clamp=c["clamp"],
                                                cfl_bound=cfl_bound, sign_clip=c["sign_clip"],
-                                               safe=safe)
+                                               safe=safe, period=c["period"])
         r = {"finite": finite, "fallbacks": fb, "cfl_max": coarse_runs.cfl_max,
              "safe_fraction": coarse_runs.safe_fraction}
         if finite and stats is not None:
@@ -445,8 +449,8 @@ def main(argv=None):
     best = min(cands, key=lambda c: out["tune"][key(c)].get("spectrum_rel_err_mean", 1e300)
                + (1e300 if out["tune"][key(c)]["fallbacks"] else 0.0))
     out.update(best)
-    kbest, gbest, kcbest, clbest, scbest = (best[k] for k in ("kmin", "gain", "kclip", "clamp",
-                                                              "sign_clip"))
+    kbest, gbest, kcbest, clbest, scbest, pbest = (
+        best[k] for k in ("kmin", "gain", "kclip", "clamp", "sign_clip", "period"))
     sy = [gbest * v for v in sy]
 
     # 5. evaluation window: coarse-grained DNS statistics, then the coarse runs
@@ -464,10 +468,10 @@ def main(argv=None):
                                                enstrophy_bound=a.guard * z0,
                                                trace=log if a.trace and prm is not None else None,
                                                clamp=clbest, cfl_bound=cfl_bound,
-                                               sign_clip=scbest, safe=safe)
+                                               sign_clip=scbest, safe=safe, period=pbest)
         net = L.Net(1, nc, nc, dev, params=params) if prm is not None else None
         sec = timed_steps(ps_c0, vz_c0, p, dtc, Forcing(kf, f0, a.seed + 11, nc), 500, net, scale_in,
-                          sy, kbest, kcbest, clbest, scbest)
+                          sy, kbest, kcbest, clbest, scbest, pbest)
         r = {"finite": finite, "steps": steps, "dt": dtc, "sim_time": steps * dtc,
              "realisations": len(seeds), "ms_per_step": sec * 1e3, "gpu_s_per_time": sec / dtc,
              "fallbacks": fb, "cfl_max": coarse_runs.cfl_max if prm is not None else None,
@@ -489,7 +493,7 @@ def main(argv=None):
         json.dump(out, f, indent=1)
     np.savez(os.path.join(a.out, "closure_params.npz"), params=params, scale_in=np.array(scale_in),
              scale_out=np.array(sy), kmin=kbest, gain=gbest, kclip=kcbest,
-             out_clamp=clbest, sign_clip=scbest, nf=nf, nc=nc,
+             out_clamp=clbest, sign_clip=scbest, period=pbest, nf=nf, nc=nc,
              **({} if safe is None else {"safe_cfl": np.array(safe)}), **{k: out["train"][k] for k in (
                  "val_loss", "val_loss_zero_model", "seconds") if out["train"].get(k) is not None})
     cmd = "python -m rot2d3c_amd.closure_study " + " ".join(

@@ -2168,16 +2168,22 @@ struct r2d3c_ensemble {
   size_t ftab_cap = 0;              // ForceSet entries allocated
   std::vector<ForceSet> h_ftab;
   int *d_step = nullptr;
-  hipGraph_t graph = nullptr;
+  int period = 1;                   // closures refreshed every period-th step of a run call
+  hipGraph_t graph = nullptr;       // one step with the closures
   hipGraphExec_t exec = nullptr;
+  hipGraph_t graph2 = nullptr;      // one step holding the closures' last forcing
+  hipGraphExec_t exec2 = nullptr;
   const ForceSet *graph_ftab = nullptr;
   bool graph_valid = false;
 };
 
 template <int N>
 static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, hipStream_t s,
-                          bool first, bool masked, bool chain) {
-  for (auto &cl : k.closures) {  // the closure forcing of this step, from the step's state
+                          bool first, bool masked, bool chain, bool closures) {
+  // the closure forcing of this step, from the step's state (else the
+  // forcing of the last evaluation is held)
+  for (size_t ci = 0; closures && ci < k.closures.size(); ++ci) {
+    EnsClosure &cl = k.closures[ci];
     int rc = r2d3c_closure_net_inputs(k.plan, cl.psi, cl.vz, cl.sin, cl.net, s);
     if (!rc) rc = r2d3c_cnn_forward_buffers(cl.net, 1, s);
     if (!rc)
@@ -2221,18 +2227,18 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
 }
 
 static int ens_dispatch(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, hipStream_t s,
-                        bool first, bool masked, bool chain) {
+                        bool first, bool masked, bool chain, bool closures = true) {
   int rc;
   switch (k.n) {
-    case 16: rc = ens_class_step<16>(e, k, ftab, s, first, masked, chain); break;
-    case 32: rc = ens_class_step<32>(e, k, ftab, s, first, masked, chain); break;
-    case 64: rc = ens_class_step<64>(e, k, ftab, s, first, masked, chain); break;
-    case 128: rc = ens_class_step<128>(e, k, ftab, s, first, masked, chain); break;
-    case 256: rc = ens_class_step<256>(e, k, ftab, s, first, masked, chain); break;
-    case 512: rc = ens_class_step<512>(e, k, ftab, s, first, masked, chain); break;
-    case 1024: rc = ens_class_step<1024>(e, k, ftab, s, first, masked, chain); break;
-    case 2048: rc = ens_class_step<2048>(e, k, ftab, s, first, masked, chain); break;
-    case 4096: rc = ens_class_step<4096>(e, k, ftab, s, first, masked, chain); break;
+    case 16: rc = ens_class_step<16>(e, k, ftab, s, first, masked, chain, closures); break;
+    case 32: rc = ens_class_step<32>(e, k, ftab, s, first, masked, chain, closures); break;
+    case 64: rc = ens_class_step<64>(e, k, ftab, s, first, masked, chain, closures); break;
+    case 128: rc = ens_class_step<128>(e, k, ftab, s, first, masked, chain, closures); break;
+    case 256: rc = ens_class_step<256>(e, k, ftab, s, first, masked, chain, closures); break;
+    case 512: rc = ens_class_step<512>(e, k, ftab, s, first, masked, chain, closures); break;
+    case 1024: rc = ens_class_step<1024>(e, k, ftab, s, first, masked, chain, closures); break;
+    case 2048: rc = ens_class_step<2048>(e, k, ftab, s, first, masked, chain, closures); break;
+    case 4096: rc = ens_class_step<4096>(e, k, ftab, s, first, masked, chain, closures); break;
     default: return set_err(R2D3C_ENOTSUP, "unsupported grid size");
   }
   if (rc) return rc;
@@ -2242,9 +2248,10 @@ static int ens_dispatch(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, hi
 
 // Captures one steady time step of every class (their launches are independent
 // branches of the graph) followed by the forcing-table tick.
-static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab) {
-  if (e->exec) { (void)hipGraphExecDestroy(e->exec); e->exec = nullptr; }
-  if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
+static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, bool closures,
+                       hipGraph_t *graph, hipGraphExec_t *exec) {
+  if (*exec) { (void)hipGraphExecDestroy(*exec); *exec = nullptr; }
+  if (*graph) { (void)hipGraphDestroy(*graph); *graph = nullptr; }
   e->graph_valid = false;
   HIPCHK(hipStreamBeginCapture(e->es, hipStreamCaptureModeThreadLocal));
   int rc = 0;
@@ -2258,7 +2265,7 @@ static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab) {
     EnsClass &k = *kp;
     if (he != hipSuccess || rc) break;
     he = hipStreamWaitEvent(k.st, e->fork, 0);
-    if (he == hipSuccess) rc = ens_dispatch(e, k, ftab, k.st, false, true, true);
+    if (he == hipSuccess) rc = ens_dispatch(e, k, ftab, k.st, false, true, true, closures);
     if (he == hipSuccess && !rc) he = hipEventRecord(k.done, k.st);
     if (he == hipSuccess && !rc) he = hipStreamWaitEvent(e->es, k.done, 0);
   }
@@ -2271,8 +2278,24 @@ static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab) {
   if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
   if (he != hipSuccess) { if (g) (void)hipGraphDestroy(g); return hip_err(he, "ensemble capture"); }
   if (ee != hipSuccess) return hip_err(ee, "hipStreamEndCapture");
-  e->graph = g;
-  HIPCHK(hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0));
+  *graph = g;
+  HIPCHK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+  return 0;
+}
+
+// (Re)captures the step graphs a run call needs: with the closures, and, when
+// closures are refreshed less often than every step, without them.
+static int ens_capture_all(r2d3c_ensemble *e, const ForceSet *ftab) {
+  bool any = false;
+  for (auto &k : e->cls) any |= !k.closures.empty();
+  int rc = ens_capture(e, ftab, true, &e->graph, &e->exec);
+  if (!rc && any && e->period > 1) {
+    rc = ens_capture(e, ftab, false, &e->graph2, &e->exec2);
+  } else {  // no held-forcing steps: drop a graph captured for an earlier set-up
+    if (e->exec2) { (void)hipGraphExecDestroy(e->exec2); e->exec2 = nullptr; }
+    if (e->graph2) { (void)hipGraphDestroy(e->graph2); e->graph2 = nullptr; }
+  }
+  if (rc) return rc;
   e->graph_ftab = ftab;
   e->graph_valid = true;
   return 0;
@@ -2282,6 +2305,8 @@ static void ens_free(r2d3c_ensemble *e) {
   if (!e) return;
   if (e->exec) (void)hipGraphExecDestroy(e->exec);
   if (e->graph) (void)hipGraphDestroy(e->graph);
+  if (e->exec2) (void)hipGraphExecDestroy(e->exec2);
+  if (e->graph2) (void)hipGraphDestroy(e->graph2);
   for (auto &k : e->cls) {
     for (auto &cl : k.closures) {
       (void)hipFree(cl.fp);
@@ -2444,6 +2469,15 @@ int r2d3c_ensemble_set_closure(r2d3c_ensemble *e, int run, r2d3c_cnn *net, const
   return 0;
 }
 
+int r2d3c_ensemble_set_closure_period(r2d3c_ensemble *e, int period) {
+  NEED(e);
+  if (period < 1) return set_err(R2D3C_EINVAL, "closure period must be >= 1");
+  HIPCHK(hipStreamSynchronize(e->es));
+  e->period = period;
+  e->graph_valid = false;
+  return 0;
+}
+
 int r2d3c_ensemble_run(r2d3c_ensemble *e, const r2d3c_forcing *forcing, int n_steps,
                        void *stream) {
   NEED(e);
@@ -2490,10 +2524,11 @@ int r2d3c_ensemble_run(r2d3c_ensemble *e, const r2d3c_forcing *forcing, int n_st
     k_ens_tick<<<1, 1, 0, e->es>>>(e->d_step);
     LAUNCHCHK("ensemble tick");
     if (!e->graph_valid || e->graph_ftab != ftab) {
-      int rc = ens_capture(e, ftab);
+      int rc = ens_capture_all(e, ftab);
       if (rc) return rc;
     }
-    for (int st = 1; st < n_steps; ++st) HIPCHK(hipGraphLaunch(e->exec, e->es));
+    for (int st = 1; st < n_steps; ++st)
+      HIPCHK(hipGraphLaunch(st % e->period && e->exec2 ? e->exec2 : e->exec, e->es));
   }
   HIPCHK(hipEventRecord(e->ev_out, e->es));
   HIPCHK(hipStreamWaitEvent(s, e->ev_out, 0));

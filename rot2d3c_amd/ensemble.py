@@ -74,6 +74,14 @@ class Ensemble:
                 "r2d3c_ensemble_set_closure")
         self._nets[run] = net  # keep the network alive while the ensemble uses it
 
+    def set_closure_period(self, period):
+        """Evaluate the closure runs' forcing on every period-th step of each
+        run() call (its first step, then steps period, 2 period, ...) and
+        hold it in between."""
+        with torch.cuda.device(self.device):
+            check(_lib.lib().r2d3c_ensemble_set_closure_period(self.handle, int(period)),
+                  "r2d3c_ensemble_set_closure_period")
+
     def forcing_array(self, forcing):
         """The C forcing table of `forcing` (see run), reusable across calls."""
         n_steps = len(forcing)
