@@ -272,3 +272,28 @@ def test_step_iflow2_10_steps(n, nprocs, cuda):
         assert abs(float(dto.item()) - dt) <= 1e-12 * dt
     assert rel_l2(_host(tps), ops) < TOL
     assert rel_l2(_host(tvz), ovz) < TOL
+
+
+def test_bench_workload_2048_10_steps(cuda):
+    """The 2048^2 workload the bench times (nu = nuv = 5e-5, nu+ = 0.5,
+    Omega = 5, random forcing on kf = 40 for psi and vz at injection 1.0,
+    fixed dt = 2e-4, seeded random-phase start), 10 chained steps through
+    r2d3c_run with the plan's default row schedule, against the oracle's
+    iflow = 3 loop on the same input: relative L2 <= 1e-12."""
+    print("bench workload 2048, 10 steps", flush=True)
+    import torch
+    from rot2d3c_amd import ic
+    S = _S()
+    n, steps, dt, kf, seed = 2048, 10, 2e-4, 40.0, 0
+    p = O.Params(5e-5, 0.5, 1, 2, 5e-5, 0.5, 1, 2, 5.0)
+    ps, vz = ic.random_phase_ic(n, seed=1234)
+    sched = ic.forcing_schedule(kf, 1.0, 1.0, seed, steps, S.rand_force_draw)
+    draws = [(S.ForceDraw(*a), S.ForceDraw(*b)) for a, b in sched]
+    tps, tvz = _dev(ps, cuda), _dev(vz, cuda)
+    sp = S.Params(p.nu, p.hnu, p.nn, p.mm, p.nuv, p.hnuv, p.nnv, p.mmv, p.omega)
+    S.rk4_run(tps, tvz, sp, draws, dt=dt)
+    torch.cuda.synchronize()
+    ops, ovz, *_ = O.run_random_forcing(ps.copy(), vz.copy(), steps, 0.0, p, kf, kf, 1.0, 1.0, seed,
+                                        fixed_dt=dt)
+    assert rel_l2(_host(tps), ops) < TOL
+    assert rel_l2(_host(tvz), ovz) < TOL
