@@ -229,10 +229,10 @@ def main():
     ps = torch.from_numpy(ps0).to(dev)
     vz = torch.from_numpy(vz0).to(dev)
     plan = S.Plan.get(n, dev)
-    schedule = None
+    from rot2d3c_amd import autotune
+    sch = autotune.DEFAULT
     if not args.no_autotune:
-        from rot2d3c_amd import autotune
-        schedule = autotune.apply(plan.handle, n, dev)
+        sch = autotune.apply(plan.handle, n, dev)
 
     def run(k0, k):
         # r2d3c_run: k chained steps (bitwise equal to k single steps)
@@ -327,7 +327,8 @@ def main():
         "config": {"workload": "Rot2D3C RK4 step, %d^2 FP64, dt=2e-4 fixed, nu=5e-5, nu+=0.5, "
                                "Omega=5, forcing kf=40 (psi and vz), injection 1.0" % n,
                    "grid": n, "dt": dt, "parallelism": "single GPU",
-                   "row_schedule": list(schedule) if schedule else [2, 1]},
+                   "schedule": {"row_rows": sch[0], "row_blocks_per_cu": sch[1],
+                                "column_radix_form": sch[2]}},
         "roofline": roof,
         "cpu_baseline": cpu,
         "step_hbm": {"algorithmic_bytes_per_step": step_bytes,

@@ -236,8 +236,16 @@ void r2d3c_rand_force_draw(double kup, int *seed, double *kx, double *ky, double
  * differently).  Default 2, 0. */
 int r2d3c_plan_set_schedule(r2d3c_plan *plan, int row_rows, int row_blocks_per_cu);
 int r2d3c_plan_get_schedule(const r2d3c_plan *plan, int *row_rows, int *row_blocks_per_cu);
-/* Times every valid schedule on scratch state (best of three trials of
- * `reps` fixed-dt steps each), keeps the fastest in the plan and returns its milliseconds per step (may be NULL).
+/* Radix schedule of the column transforms of r2d3c_step / r2d3c_run and of
+ * ensembles for n >= 512: 0 (default) the four-step form, 512-point
+ * wave-local sub-transforms and one radix n/512 exchange; 1 radix-8 Stockham
+ * passes with a last radix 2, 4 or 8 (below 512 the two coincide).  The two
+ * agree within rounding. */
+int r2d3c_plan_set_column_radix(r2d3c_plan *plan, int form);
+int r2d3c_plan_get_column_radix(const r2d3c_plan *plan, int *form);
+/* Times every valid schedule (row kernel and launch shape x column radix
+ * schedule) on scratch state (best of three trials of `reps` fixed-dt steps
+ * each), keeps the fastest in the plan and returns its milliseconds per step (may be NULL).
  * Allocates and frees its scratch; synchronizes the stream. */
 int r2d3c_autotune(r2d3c_plan *plan, int reps, double *ms_per_step, void *stream);
 
@@ -263,6 +271,8 @@ int r2d3c_ensemble_create(r2d3c_ensemble **ens, const r2d3c_member *members, int
 int r2d3c_ensemble_destroy(r2d3c_ensemble *ens);
 /* r2d3c_plan_set_schedule for the runs of size n. */
 int r2d3c_ensemble_set_schedule(r2d3c_ensemble *ens, int n, int row_rows, int row_blocks_per_cu);
+/* r2d3c_plan_set_column_radix for the runs of size n. */
+int r2d3c_ensemble_set_column_radix(r2d3c_ensemble *ens, int n, int form);
 /* Advances every run n_steps steps.  forcing: NULL, or an array of
  * n_steps*count forcing structs, step-major with the runs in creation order,
  * holding rand_force draws only; the table is uploaded before the steps, so the steps run without host

@@ -1,13 +1,15 @@
 """Per-size schedule autotuner with a JSON cache.
 
-A schedule is the launch shape of the row kernel for one grid size: (2, 0)
-is the non-persistent kernel (one block per 2 rows); (rows, blocks per CU)
-with blocks per CU 1 or 2 the persistent kernel with 2 rows per block, or 1
-when n >= 512.  Schedules
-change only the thread and LDS layout, never the arithmetic: schedules of one
-compiled kernel give bit-identical results, the three kernels agree within
-rounding; the tuner only looks for the fastest, and its cached choice keeps a
-machine's runs reproducible.  r2d3c_autotune times each candidate on the device; the choice is
+A schedule is (rows, blocks per CU, column radix form) for one grid size.
+The first two are the launch shape of the row kernel: (2, 0) is the
+non-persistent kernel (one block per 2 rows); (rows, blocks per CU) with
+blocks per CU 1 or 2 the persistent kernel with 2 rows per block, or 1 when
+n >= 512.  The third is the radix schedule of the column transforms from
+n = 512: 0 the four-step form (512-point wave-local sub-transforms and one
+radix n/512 exchange), 1 radix-8 Stockham passes.  Launch shapes of one
+compiled kernel give bit-identical results; different kernels or column
+forms agree within rounding.  The tuner only looks for the fastest, and its
+cached choice keeps a machine's runs reproducible.  r2d3c_autotune times each candidate on the device; the choice is
 cached per (GPU architecture, CU count, n) in a JSON file, by default
 ~/.cache/rot2d3c_amd/schedules.json (R2D3C_SCHEDULE_CACHE overrides it).
 """
@@ -21,7 +23,7 @@ import torch
 from . import _lib
 from ._lib import check
 
-DEFAULT = (2, 0)
+DEFAULT = (2, 0, 0)
 _lock = threading.Lock()
 
 
@@ -58,7 +60,8 @@ def _store(table):
 def tune(plan_handle, reps=20, device=None):
     """Time every schedule on the plan (scratch state, best of three trials
     of `reps` steps each),
-    keep the fastest in the plan and return ((rows, blocks_per_cu), ms/step)."""
+    keep the fastest in the plan and return ((rows, blocks_per_cu, column_form),
+    ms/step)."""
     ms = ctypes.c_double()
     stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
     check(_lib.lib().r2d3c_autotune(plan_handle, int(reps), ctypes.byref(ms), stream),
@@ -66,7 +69,10 @@ def tune(plan_handle, reps=20, device=None):
     rw, bpc = ctypes.c_int(), ctypes.c_int()
     check(_lib.lib().r2d3c_plan_get_schedule(plan_handle, ctypes.byref(rw), ctypes.byref(bpc)),
           "r2d3c_plan_get_schedule")
-    return (rw.value, bpc.value), ms.value
+    cs = ctypes.c_int()
+    check(_lib.lib().r2d3c_plan_get_column_radix(plan_handle, ctypes.byref(cs)),
+          "r2d3c_plan_get_column_radix")
+    return (rw.value, bpc.value, cs.value), ms.value
 
 
 def schedule(n, device, reps=20, retune=False):
@@ -76,18 +82,21 @@ def schedule(n, device, reps=20, retune=False):
     key = "%s:%d" % (device_key(device), int(n))
     with _lock:
         table = _load()
-        if not retune and key in table:
+        # an entry without the column form predates its tuning: tune again
+        if not retune and key in table and len(table[key]["schedule"]) == 3:
             return tuple(table[key]["schedule"])
         with torch.cuda.device(device):
-            (rw, bpc), ms = tune(Plan.get(n, device).handle, reps, device)
+            sched, ms = tune(Plan.get(n, device).handle, reps, device)
         table = _load()
-        table[key] = {"schedule": [rw, bpc], "ms_per_step": ms}
+        table[key] = {"schedule": list(sched), "ms_per_step": ms}
         _store(table)
-        return rw, bpc
+        return sched
 
 
 def apply(plan_handle, n, device, reps=20):
     """Set the cached (or freshly tuned) schedule of size n on a plan."""
-    rw, bpc = schedule(n, device, reps)
+    rw, bpc, cs = schedule(n, device, reps)
     check(_lib.lib().r2d3c_plan_set_schedule(plan_handle, rw, bpc), "r2d3c_plan_set_schedule")
-    return rw, bpc
+    check(_lib.lib().r2d3c_plan_set_column_radix(plan_handle, cs),
+          "r2d3c_plan_set_column_radix")
+    return rw, bpc, cs

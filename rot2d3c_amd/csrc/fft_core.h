@@ -300,26 +300,29 @@ __device__ __forceinline__ void fft_dif4(cplx v[8], cplx *P, int b, const cplx *
 }
 
 // Element index held in v[q] by thread b in the interleaved distribution
-// (the canonical one for N < 512, where fft_canon is used both ways).
-template <int N> __device__ __forceinline__ int ilv(int b, int q) {
-  if constexpr (N >= 512) {
+// (the canonical one for N < 512, or with CS, where fft_canon is used both
+// ways).
+template <int N, bool CS = false> __device__ __forceinline__ int ilv(int b, int q) {
+  if constexpr (N >= 512 && !CS) {
     return (b >> 6) + (N / 512) * ((b & 63) + 64 * q);
   } else {
     return b + (N / 8) * q;
   }
 }
 
-// DIF / DIT dispatch: four-step for N >= 512, fft_canon (canonical both ways)
-// below.  P: the transform's LDS area with slot stride IL (IL = 1 required
-// for N >= 512).
-template <int N, int S, int IL>
+// DIF / DIT dispatch, two radix schedules: the four-step form for N >= 512
+// (512-point wave-local sub-transforms and one radix N/512 exchange), or, below
+// 512 or with CS, fft_canon (radix-8 passes and a last radix 2, 4 or 8,
+// canonical order both ways).  P: the transform's LDS area with slot stride
+// IL (IL = 1 required for the four-step form).
+template <int N, int S, int IL, bool CS = false>
 __device__ __forceinline__ void fft_dif(cplx v[8], cplx *P, int b, const cplx *__restrict__ tw) {
-  if constexpr (N >= 512) fft_dif4<N, S>(v, P, b, tw);
+  if constexpr (N >= 512 && !CS) fft_dif4<N, S>(v, P, b, tw);
   else fft_canon<N, S, IL>(v, P, b, tw);
 }
-template <int N, int S, int IL>
+template <int N, int S, int IL, bool CS = false>
 __device__ __forceinline__ void fft_dit(cplx v[8], cplx *P, int b, const cplx *__restrict__ tw) {
-  if constexpr (N >= 512) fft_dit4<N, S>(v, P, b, tw);
+  if constexpr (N >= 512 && !CS) fft_dit4<N, S>(v, P, b, tw);
   else fft_canon<N, S, IL>(v, P, b, tw);
 }
 

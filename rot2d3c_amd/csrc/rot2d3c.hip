@@ -759,7 +759,7 @@ __global__ void k_set_f64(double *p, double v) { *p = v; }
 // -> M0 = psi~, M1 = d_y psi~, M2 = w~, M3 = d_y w~ with w = -k^2 psi
 // (laplak2, src/Rot2D3C.para.f90:571); blocks [ncp, ncp+nc): vz column kx ->
 // M4 = vz~, M5 = d_y vz~.  All inputs dealiased as derivk2 does.
-template <int N>
+template <int N, bool CS = false>
 __device__ __forceinline__ void colinv_body(const cplx *__restrict__ psi,
                                             const cplx *__restrict__ vz, cplx *__restrict__ M,
                                             const StepConst &c, unsigned long long *maxbits,
@@ -777,7 +777,7 @@ __device__ __forceinline__ void colinv_body(const cplx *__restrict__ psi,
   cplx s[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {  // inverse transforms take the interleaved order
-    const int j = ilv<N>(b, q), ky = kyof(j, N);
+    const int j = ilv<N, CS>(b, q), ky = kyof(j, N);
     const double ka2 = (double)kx * kx + (double)ky * ky;
     s[q] = ka2 <= c.g.kmax ? src[j] : mk(0.0, 0.0);
   }
@@ -792,27 +792,27 @@ __device__ __forceinline__ void colinv_body(const cplx *__restrict__ psi,
     cplx v[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const int ky = kyof(ilv<N>(bb, q), N);
+      const int ky = kyof(ilv<N, CS>(bb, q), N);
       const double ka2 = (double)kx * kx + (double)ky * ky;
       cplx a = s[q];
       if (f >= 2) a = mk(-ka2 * a.x, -ka2 * a.y);
       if (f & 1) a = mk(-(double)ky * a.y, (double)ky * a.x);
       v[q] = a;
     }
-    fft_dit<N, +1, 1>(v, lds, bb, tw);
+    fft_dit<N, +1, 1, CS>(v, lds, bb, tw);
     cplx *dst = dst0 + (size_t)f * c.fs;
 #pragma unroll
     for (int q = 0; q < 8; ++q) dst[tix(bb + T * q, kx, c.g.kp)] = v[q];
   }
 }
 
-template <int N>
+template <int N, bool CS = false>
 __global__ __launch_bounds__(N / 8) void k_step_colinv(const cplx *__restrict__ psi,
                                                         const cplx *__restrict__ vz,
                                                         cplx *__restrict__ M, StepConst c,
                                                         unsigned long long *maxbits,
                                                         const cplx *__restrict__ tw) {
-  colinv_body<N>(psi, vz, M, c, maxbits, tw, xcd_pairs(blockIdx.x));
+  colinv_body<N, CS>(psi, vz, M, c, maxbits, tw, xcd_pairs(blockIdx.x));
 }
 
 // Row kernel of one stage: for RW rows, z1 = psi_x + i psi_y,
@@ -1090,13 +1090,13 @@ __global__ __launch_bounds__((RowV2<N, RWQ>::THREADS)) void k_step_row2(
 // write dt and clear the CFL slot of the next step (mreset).
 enum { FM_EMIT = 1, FM_STATE = 2, FM_DISK = 4, FM_ZERO = 8 };
 
-template <int N>
+template <int N, bool CS = false>
 __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *psi, cplx *vz,
                                               cplx *__restrict__ M, const StepConst &c, int o,
                                               int mode, const unsigned long long *mread,
                                               unsigned long long *mreset, double *dt_out,
                                               const cplx *__restrict__ tw, int lb) {
-  auto pos = [](int b, int q) { return ilv<N>(b, q); };
+  auto pos = [](int b, int q) { return ilv<N, CS>(b, q); };
   constexpr int T = N / 8;
   __shared__ cplx lds[Shape<N>::LDSN];
   const int b = threadIdx.x;
@@ -1124,7 +1124,7 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
   const cplx *Qf = Q + (size_t)field * c.fs;
 #pragma unroll
   for (int q = 0; q < 8; ++q) v[q] = Qf[tix(b + T * q, kx, c.g.kp)];
-  fft_dif<N, -1, 1>(v, lds, b, tw);  // v[q] = spectrum at ky index pos(b, q)
+  fft_dif<N, -1, 1, CS>(v, lds, b, tw);  // v[q] = spectrum at ky index pos(b, q)
 
   const ForceSet &fo = field == 0 ? c.fp : c.fz;
   // at most two forced modes per field (scalars: no private-memory array)
@@ -1182,14 +1182,14 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
       cplx w[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const int ky = kyof(ilv<N>(bb, q), N);
+        const int ky = kyof(ilv<N, CS>(bb, q), N);
         const double ka2 = (double)kx * kx + (double)ky * ky;
         cplx a = s[q];
         if (f >= 2) a = mk(-ka2 * a.x, -ka2 * a.y);
         if (f & 1) a = mk(-(double)ky * a.y, (double)ky * a.x);
         w[q] = a;
       }
-      fft_dit<N, +1, 1>(w, lds, bb, tw);
+      fft_dit<N, +1, 1, CS>(w, lds, bb, tw);
       cplx *dst = dst0 + (size_t)f * c.fs;
 #pragma unroll
       for (int q = 0; q < 8; ++q) dst[tix(bb + T * q, kx, c.g.kp)] = w[q];
@@ -1197,12 +1197,13 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
   }
 }
 
-template <int N>
+template <int N, bool CS = false>
 __global__ __launch_bounds__(N / 8) void k_step_colfused(
     const cplx *__restrict__ Q, cplx *psi, cplx *vz, cplx *__restrict__ M, StepConst c, int o,
     int mode, const unsigned long long *mread, unsigned long long *mreset, double *dt_out,
     const cplx *__restrict__ tw) {
-  colfused_body<N>(Q, psi, vz, M, c, o, mode, mread, mreset, dt_out, tw, xcd_pairs(blockIdx.x));
+  colfused_body<N, CS>(Q, psi, vz, M, c, o, mode, mread, mreset, dt_out, tw,
+                       xcd_pairs(blockIdx.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -1231,18 +1232,18 @@ __device__ __forceinline__ StepConst ens_const(const EnsArgs &a, int m) {
   return c;
 }
 
-template <int N>
+template <int N, bool CS = false>
 __global__ __launch_bounds__(N / 8) void k_ens_colinv(EnsArgs a, const cplx *__restrict__ tw) {
   const EnsMember &e = a.mem[blockIdx.y];
   const StepConst c = ens_const(a, blockIdx.y);
-  colinv_body<N>(e.psi, e.vz, e.M, c, nullptr, tw, xcd_pairs(blockIdx.x));
+  colinv_body<N, CS>(e.psi, e.vz, e.M, c, nullptr, tw, xcd_pairs(blockIdx.x));
 }
-template <int N>
+template <int N, bool CS = false>
 __global__ __launch_bounds__(N / 8) void k_ens_colfused(EnsArgs a, int o, int mode,
                                                          const cplx *__restrict__ tw) {
   const EnsMember &e = a.mem[blockIdx.y];
   const StepConst c = ens_const(a, blockIdx.y);
-  colfused_body<N>(e.M + 6 * c.fs, e.psi, e.vz, e.M, c, o, mode, nullptr, nullptr, nullptr, tw,
+  colfused_body<N, CS>(e.M + 6 * c.fs, e.psi, e.vz, e.M, c, o, mode, nullptr, nullptr, nullptr, tw,
                    xcd_pairs(blockIdx.x));
 }
 // advances the forcing-table step at the end of a time step
@@ -1277,6 +1278,9 @@ struct r2d3c_plan {
   // rows per block and blocks per CU of the persistent kernel, or bpc = 0
   // for the non-persistent kernel (2 rows per block)
   int row_rw = 2, row_bpc = 0;
+  // radix schedule of the column transforms for n >= 512: 0 the four-step
+  // form, 1 radix-8 Stockham passes (fft_canon)
+  int col_cs = 0;
   // kernel timing of r2d3c_step (r2d3c_timing_enable / r2d3c_timing_read)
   int timing = 0;
   std::vector<hipEvent_t> evpool;
@@ -1471,7 +1475,8 @@ static int step_seq(r2d3c_plan *p, StepConst &c, cplx *psi, cplx *vz, hipStream_
   hipEvent_t ea, eb;
   if (first) {
     ev_pair(p, 0, &ea, &eb);
-    hipExtLaunchKernelGGL(k_step_colinv<N>, dim3(round16(ncp + nc)), dim3(N / 8), 0, s, ea, eb, 0,
+    hipExtLaunchKernelGGL((N >= 512 && p->col_cs) ? k_step_colinv<N, true> : k_step_colinv<N, false>,
+                          dim3(round16(ncp + nc)), dim3(N / 8), 0, s, ea, eb, 0,
                           (const cplx *)psi, (const cplx *)vz, M, c, mb, tw);
   }
   for (int o = 4; o >= 1; --o) {
@@ -1499,7 +1504,9 @@ static int step_seq(r2d3c_plan *p, StepConst &c, cplx *psi, cplx *vz, hipStream_
     const int nblk = round16(2 * ncp + ((mode & FM_ZERO) ? (N / 2 + 1 - nc) : 0));
     unsigned long long *mres = o == 4 ? mother : nullptr;
     ev_pair(p, o == 1 ? 3 : 2, &ea, &eb);
-    hipExtLaunchKernelGGL(k_step_colfused<N>, dim3(nblk), dim3(N / 8), 0, s, ea, eb, 0,
+    hipExtLaunchKernelGGL((N >= 512 && p->col_cs) ? k_step_colfused<N, true>
+                                                  : k_step_colfused<N, false>,
+                          dim3(nblk), dim3(N / 8), 0, s, ea, eb, 0,
                           (const cplx *)Q, psi, vz, M, c, o, mode,
                           (const unsigned long long *)mb, mres, dt_out, tw);
   }
@@ -2056,6 +2063,19 @@ int r2d3c_plan_set_schedule(r2d3c_plan *p, int row_rows, int row_blocks_per_cu) 
   return 0;
 }
 
+int r2d3c_plan_set_column_radix(r2d3c_plan *p, int form) {
+  NEED(p);
+  if (form != 0 && form != 1) return set_err(R2D3C_EINVAL, "column radix form: 0 or 1");
+  p->col_cs = form;
+  return 0;
+}
+
+int r2d3c_plan_get_column_radix(const r2d3c_plan *p, int *form) {
+  NEED(p); NEED(form);
+  *form = p->col_cs;
+  return 0;
+}
+
 int r2d3c_plan_get_schedule(const r2d3c_plan *p, int *row_rows, int *row_blocks_per_cu) {
   NEED(p); NEED(row_rows); NEED(row_blocks_per_cu);
   *row_rows = p->row_rw;
@@ -2077,15 +2097,20 @@ int r2d3c_autotune(r2d3c_plan *p, int reps, double *ms_per_step, void *stream) {
   int rc = 0;
   if ((e = hipMemsetAsync(psi, 0, sz, s)) != hipSuccess || (e = hipMemsetAsync(vz, 0, sz, s)) != hipSuccess)
     rc = hip_err(e, "hipMemsetAsync");
-  const int cand[5][2] = {{2, 0}, {2, 1}, {2, 2}, {1, 1}, {1, 2}};
+  // candidates: (rows, blocks per CU, column radix form); the column form
+  // matters only from n = 512, the row schedule only from n = 256
+  const int cand[10][3] = {{2, 0, 0}, {2, 1, 0}, {2, 2, 0}, {1, 1, 0}, {1, 2, 0},
+                           {2, 0, 1}, {2, 1, 1}, {2, 2, 1}, {1, 1, 1}, {1, 2, 1}};
   int best = -1;
   float best_ms = 0.f;
-  const int rw0 = p->row_rw, bpc0 = p->row_bpc;
-  for (int k = 0; k < 5 && !rc; ++k) {
+  const int rw0 = p->row_rw, bpc0 = p->row_bpc, cs0 = p->col_cs;
+  for (int k = 0; k < 10 && !rc; ++k) {
     if (!schedule_valid(n, cand[k][0], cand[k][1])) continue;
     if (n < 256 && k > 0) break;  // no persistent row kernel below 256: nothing to tune
+    if (n < 512 && cand[k][2]) break;  // one column radix schedule below 512
     p->row_rw = cand[k][0];
     p->row_bpc = cand[k][1];
+    p->col_cs = cand[k][2];
     float ms = 0.f;
     auto tune = [&](auto tag) -> int {
       constexpr int NN = decltype(tag)::value;
@@ -2111,10 +2136,12 @@ int r2d3c_autotune(r2d3c_plan *p, int reps, double *ms_per_step, void *stream) {
   if (rc || best < 0) {
     p->row_rw = rw0;
     p->row_bpc = bpc0;
+    p->col_cs = cs0;
     return rc ? rc : set_err(R2D3C_EINVAL, "no schedule could be timed");
   }
   p->row_rw = cand[best][0];
   p->row_bpc = cand[best][1];
+  p->col_cs = cand[best][2];
   if (ms_per_step) *ms_per_step = best_ms / reps;
   return 0;
 }
@@ -2197,7 +2224,11 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
   const int nc = p->g.nc, ncp = (nc + 1) & ~1;
   const cplx *tw = p->tw;
   EnsArgs a{k.d_mem, ftab, e->d_step, e->count, k.first};
-  if (first) k_ens_colinv<N><<<dim3(round16(ncp + nc), k.count), N / 8, 0, s>>>(a, tw);
+  const bool cs = N >= 512 && p->col_cs;  // the columns' radix schedule
+  if (first) {
+    if (cs) k_ens_colinv<N, true><<<dim3(round16(ncp + nc), k.count), N / 8, 0, s>>>(a, tw);
+    else k_ens_colinv<N, false><<<dim3(round16(ncp + nc), k.count), N / 8, 0, s>>>(a, tw);
+  }
   for (int o = 4; o >= 1; --o) {
     // one row launch covers the class: the runs' workspaces are 8 fs apart
     const size_t ms = 8 * p->fs;
@@ -2221,7 +2252,8 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
     int mode = FM_EMIT;
     if (o == 1) mode = FM_STATE | (masked ? FM_DISK : FM_ZERO) | (chain ? FM_EMIT : 0);
     const int nblk = round16(2 * ncp + ((mode & FM_ZERO) ? (N / 2 + 1 - nc) : 0));
-    k_ens_colfused<N><<<dim3(nblk, k.count), N / 8, 0, s>>>(a, o, mode, tw);
+    if (cs) k_ens_colfused<N, true><<<dim3(nblk, k.count), N / 8, 0, s>>>(a, o, mode, tw);
+    else k_ens_colfused<N, false><<<dim3(nblk, k.count), N / 8, 0, s>>>(a, o, mode, tw);
   }
   return 0;
 }
@@ -2250,6 +2282,7 @@ static int ens_dispatch(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, hi
 // branches of the graph) followed by the forcing-table tick.
 static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, bool closures,
                        hipGraph_t *graph, hipGraphExec_t *exec) {
+  HIPCHK(hipStreamSynchronize(e->es));  // no launch of the old graph may still run
   if (*exec) { (void)hipGraphExecDestroy(*exec); *exec = nullptr; }
   if (*graph) { (void)hipGraphDestroy(*graph); *graph = nullptr; }
   e->graph_valid = false;
@@ -2409,6 +2442,17 @@ int r2d3c_ensemble_set_schedule(r2d3c_ensemble *e, int n, int row_rows, int row_
   for (auto &k : e->cls)
     if (k.n == n) {
       int rc = r2d3c_plan_set_schedule(k.plan, row_rows, row_blocks_per_cu);
+      if (!rc) e->graph_valid = false;
+      return rc;
+    }
+  return set_err(R2D3C_EINVAL, "no run of that size in the ensemble");
+}
+
+int r2d3c_ensemble_set_column_radix(r2d3c_ensemble *e, int n, int form) {
+  NEED(e);
+  for (auto &k : e->cls)
+    if (k.n == n) {
+      int rc = r2d3c_plan_set_column_radix(k.plan, form);
       if (!rc) e->graph_valid = false;
       return rc;
     }

@@ -37,7 +37,8 @@ class Member:
 class Ensemble:
     def __init__(self, members, schedules="auto"):
         """members: list of Member on one device.  schedules: "auto" (cached
-        autotuned schedule per size), None (defaults) or {n: (rows, bpc)}."""
+        autotuned schedule per size), None (defaults) or {n: (rows, bpc)} /
+        {n: (rows, bpc, column_form)} (autotune module: schedule meanings)."""
         if not members:
             raise ValueError("empty ensemble")
         self.members = list(members)
@@ -51,13 +52,17 @@ class Ensemble:
         self.sizes = sorted({m.n for m in self.members})
         for n in self.sizes:
             if schedules == "auto":
-                rw, bpc = autotune.schedule(n, self.device)
+                sch = autotune.schedule(n, self.device)
             elif schedules:
-                rw, bpc = schedules.get(n, autotune.DEFAULT)
+                sch = schedules.get(n, autotune.DEFAULT)
             else:
                 continue
+            rw, bpc = sch[0], sch[1]
+            cs = sch[2] if len(sch) > 2 else 0
             check(_lib.lib().r2d3c_ensemble_set_schedule(self.handle, n, rw, bpc),
                   "r2d3c_ensemble_set_schedule")
+            check(_lib.lib().r2d3c_ensemble_set_column_radix(self.handle, n, cs),
+                  "r2d3c_ensemble_set_column_radix")
 
     def set_closure(self, run, net=None, scale_in=None, scale_out=None, kmin=0.0, kclip=0.0):
         """Make run `run` a closure run with network `net` (closure.Net for one

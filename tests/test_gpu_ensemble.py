@@ -1,6 +1,9 @@
 """GPU: ensembles of independent runs (rot2d3c_amd.ensemble) and row-kernel
 schedules.  Every run of an ensemble must equal the same run advanced alone
 (r2d3c_run, fixed dt, same forcing, same schedule) bit for bit."""
+import ctypes
+import json
+
 import numpy as np
 import pytest
 import torch
@@ -139,11 +142,83 @@ def test_schedules_of_one_kernel_agree_bit_for_bit(n, cuda):
         assert rel_l2(vz.cpu().numpy(), firsts[0][1].cpu().numpy()) < 1e-12
 
 
+@pytest.mark.parametrize("n", [256, 512, 2048])
+def test_column_radix_forms(n, cuda):
+    """The column transforms' two radix schedules (four-step, radix-8
+    Stockham) agree within rounding over 3 steps (1e-12 relative L2), each
+    repeats bit for bit, below 512 they are the same code (bit-identical),
+    and the 3-step result stays within 1e-12 of the oracle at 512."""
+    print("column forms n=%d" % n, flush=True)
+    from rot2d3c_amd import spectral as S, _lib
+    from rot2d3c_amd._lib import check
+    ps0, vz0 = random_state(n, 78)
+    p = S.Params(1e-3, 0.1, 1, 2, 1e-3, 0.1, 1, 2, 2.0)
+    pl = S.Plan.get(n, cuda)
+    assert _lib.lib().r2d3c_plan_set_column_radix(pl.handle, 2) != 0
+    outs = {}
+    try:
+        for cs in (0, 1, 1):
+            check(_lib.lib().r2d3c_plan_set_column_radix(pl.handle, cs), "set_column_radix")
+            got = ctypes.c_int(-1)
+            check(_lib.lib().r2d3c_plan_get_column_radix(pl.handle, ctypes.byref(got)), "get")
+            assert got.value == cs
+            ps = torch.from_numpy(ps0).to(cuda)
+            vz = torch.from_numpy(vz0).to(cuda)
+            S.rk4_run(ps, vz, p, None, 3, dt=2e-4)
+            torch.cuda.synchronize()
+            if cs in outs:
+                assert torch.equal(ps, outs[cs][0]) and torch.equal(vz, outs[cs][1])
+            outs[cs] = (ps, vz)
+    finally:
+        check(_lib.lib().r2d3c_plan_set_column_radix(pl.handle, 0), "set_column_radix")
+    (a0, b0), (a1, b1) = outs[0], outs[1]
+    if n < 512:
+        assert torch.equal(a0, a1) and torch.equal(b0, b1)
+    assert rel_l2(a1.cpu().numpy(), a0.cpu().numpy()) < 1e-12
+    assert rel_l2(b1.cpu().numpy(), b0.cpu().numpy()) < 1e-12
+    if n == 512:
+        q = O.Params(1e-3, 0.1, 1, 2, 1e-3, 0.1, 1, 2, 2.0)
+        ps, vz = ps0.copy(), vz0.copy()
+        for _ in range(3):
+            ps, vz = O.rk_stages(ps, vz, 0 * ps, 0 * vz, 2e-4, q)
+        assert rel_l2(a1.cpu().numpy(), ps) < 1e-12 and rel_l2(b1.cpu().numpy(), vz) < 1e-12
+
+
+def test_ensemble_with_stockham_columns_equals_solo_runs(cuda):
+    """An ensemble's runs under column form 1 equal their solo runs under
+    the same schedule bit for bit."""
+    print("ensemble column form 1", flush=True)
+    from rot2d3c_amd import spectral as S, _lib, ensemble as E
+    from rot2d3c_amd._lib import check
+    n, nsteps = 512, 4
+    ps0, vz0 = random_state(n, 79)
+    p = S.Params(1e-3, 0.1, 1, 2, 1e-3, 0.1, 1, 2, 2.0)
+    ps_e, vz_e = torch.from_numpy(ps0).to(cuda), torch.from_numpy(vz0).to(cuda)
+    ens = E.Ensemble([E.Member(ps_e, vz_e, p, 2e-4)], schedules={n: (2, 0, 1)})
+    ens.run(nsteps, None)
+    torch.cuda.synchronize()
+    ens.close()
+    pl = S.Plan.get(n, cuda)
+    check(_lib.lib().r2d3c_plan_set_schedule(pl.handle, 2, 0), "set_schedule")
+    check(_lib.lib().r2d3c_plan_set_column_radix(pl.handle, 1), "set_column_radix")
+    try:
+        ps, vz = torch.from_numpy(ps0).to(cuda), torch.from_numpy(vz0).to(cuda)
+        S.rk4_run(ps, vz, p, None, nsteps, dt=2e-4)
+        torch.cuda.synchronize()
+    finally:
+        check(_lib.lib().r2d3c_plan_set_column_radix(pl.handle, 0), "set_column_radix")
+    assert torch.equal(ps, ps_e) and torch.equal(vz, vz_e)
+
+
 def test_autotune_picks_a_valid_schedule(cuda, tmp_path, monkeypatch):
     print("autotune n=512", flush=True)
-    monkeypatch.setenv("R2D3C_SCHEDULE_CACHE", str(tmp_path / "s.json"))
+    cache = tmp_path / "s.json"
+    monkeypatch.setenv("R2D3C_SCHEDULE_CACHE", str(cache))
     from rot2d3c_amd import autotune
-    rw, bpc = autotune.schedule(512, cuda, reps=2)
-    assert (rw, bpc) in ((2, 0), (2, 1), (2, 2), (1, 1), (1, 2))
-    assert autotune.schedule(512, cuda) == (rw, bpc)  # from the cache
-    assert (tmp_path / "s.json").exists()
+    key = "%s:512" % autotune.device_key(cuda)
+    # an entry cached before the column form was tuned is tuned again
+    cache.write_text(json.dumps({key: {"schedule": [1, 1], "ms_per_step": 1.0}}))
+    sch = autotune.schedule(512, cuda, reps=2)
+    assert sch[:2] in ((2, 0), (2, 1), (2, 2), (1, 1), (1, 2)) and sch[2] in (0, 1)
+    assert autotune.schedule(512, cuda) == sch  # from the cache
+    assert json.loads(cache.read_text())[key]["schedule"] == list(sch)
