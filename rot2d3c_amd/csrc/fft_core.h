@@ -181,7 +181,6 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
     for (int m = 0; m < RL; ++m) v[u + m * U] = x[m];
   }
   __syncthreads();
-  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------

@@ -824,31 +824,73 @@ __global__ __launch_bounds__(N / 8) void k_step_colinv(const cplx *__restrict__ 
 // src/Rot2D3C.para.f90:571-576,586,591.  P1 + i P2 is forward transformed as
 // one packed row and unpacked into Q0, Q1 for kx < nc.  With maxbits set, the
 // max of psi_x^2 + psi_y^2 is accumulated (CFL on the pre-step state).
+// Packed row inputs of one transform: the i*kx derivative of field A and
+// field B at the kept columns kx < nc = n/3 + 1 (and their mirrors), zero
+// elsewhere.  The loads of a group of LG slots are issued before any of its
+// arithmetic, without branches (slots in the zero band read column 0 and are
+// discarded), so they are in flight together; slots that no thread can keep
+// (by nc = n/3 + 1) are not loaded at all.  Smaller groups hold fewer
+// registers: the row kernel keeps 4 waves per SIMD at n = 2048.
+#ifndef R2D_ROW_LG
+#define R2D_ROW_LG 4
+#endif
+template <int N> constexpr int row_load_group() { return N <= 1024 ? 8 : R2D_ROW_LG; }
 template <int N>
 __device__ __forceinline__ void load_packed_step(cplx v[8], const cplx *__restrict__ FA,
                                                  const cplx *__restrict__ FB, int y, int b, int nc,
                                                  int kp) {
-  constexpr int T = N / 8;
+  constexpr int T = N / 8, NC = N / 3 + 1, LG = row_load_group<N>();
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int k = b + T * q;
-    cplx z = mk(0.0, 0.0);
-    if (k < nc) {
-      const cplx f = FA[tix(y, k, kp)], B = FB[tix(y, k, kp)];
-      const cplx A = mk(-(double)k * f.y, (double)k * f.x);  // i*kx*f
-      z = packed_z<N>(k, A, B, false);
-    } else if (k > N - nc) {
-      const int kk = N - k;
-      const cplx f = FA[tix(y, kk, kp)], B = FB[tix(y, kk, kp)];
-      const cplx A = mk(-(double)kk * f.y, (double)kk * f.x);
-      z = packed_z<N>(k, A, B, true);
+  for (int g = 0; g < 8; g += LG) {
+    cplx fa[LG], fb[LG];
+#pragma unroll
+    for (int u = 0; u < LG; ++u) {
+      const int q = g + u;
+      if (T * q >= NC && T * q + T - 1 <= N - NC) continue;  // only zero-band slots
+      const int k = b + T * q;
+      const bool lo = k < nc, hi = k > N - nc;
+      const int kk = lo ? k : (hi ? N - k : 0);
+      fa[u] = FA[tix(y, kk, kp)];
+      fb[u] = FB[tix(y, kk, kp)];
     }
-    v[q] = z;
+#pragma unroll
+    for (int u = 0; u < LG; ++u) {
+      const int q = g + u;
+      if (T * q >= NC && T * q + T - 1 <= N - NC) { v[q] = mk(0.0, 0.0); continue; }
+      const int k = b + T * q;
+      const bool lo = k < nc, hi = k > N - nc;
+      // z = packed_z(A = i kx f, B) with its roundings spelled out: the
+      // real part of A fused into its add, the imaginary part rounded on
+      // its own where it is zeroed at k = 0 and k = n/2
+      const double dk = (double)(lo ? k : N - k);
+      const cplx f = fa[u], B = fb[u];
+      cplx z;
+      if (lo) {  // Z[k] = A + i B
+        const bool edge = k == 0 || k == N / 2;
+        const double ay = edge ? 0.0 : mul_rn(dk, f.x), by = edge ? 0.0 : B.y;
+        z = mk(fma(-f.y, dk, -by), ay + B.x);
+      } else {  // Z[n-k] = conj(A) + i conj(B)
+        z = mk(fma(-f.y, dk, B.y), fma(-f.x, dk, B.x));
+      }
+      v[q] = (lo || hi) ? z : mk(0.0, 0.0);
+    }
+    if (LG < 8) __builtin_amdgcn_sched_barrier(0);  // next group's loads stay below
   }
 }
 
+// Minimum waves per SIMD the row kernel is compiled for (0: the compiler's
+// choice); caps its registers at 512 / R2D_ROW_WPE.
+#ifndef R2D_ROW_WPE
+#define R2D_ROW_WPE 0
+#endif
+#if R2D_ROW_WPE > 0
+#define R2D_ROW_ATTR __attribute__((amdgpu_waves_per_eu(R2D_ROW_WPE)))
+#else
+#define R2D_ROW_ATTR
+#endif
+
 template <int N>
-__global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
+__global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) R2D_ROW_ATTR void k_step_row(
     const cplx *__restrict__ M, cplx *__restrict__ Q, StepConst c, unsigned long long *maxbits,
     const cplx *__restrict__ tw, size_t mstride, const double *__restrict__ om_tab) {
   constexpr int T = N / 8, RW = RowStep<N>::ROWS;
@@ -1657,7 +1699,8 @@ int r2d3c_plan_create(r2d3c_plan **out, int n) {
   int K = 0;
   while ((double)(K + 1) * (K + 1) <= kmax) ++K;
   p->g.n = n;
-  p->g.nc = K + 1;
+  p->g.nc = K + 1;  // = n/3 + 1 for every power of two (the row loaders rely on it)
+  if (p->g.nc != n / 3 + 1) { delete p; return set_err(R2D3C_EINVAL, "dealias cutoff"); }
   p->g.kp = (K + 2) & ~1;
   p->g.ncf = n / 2 + 1;
   p->g.kpf = n / 2 + 2;
