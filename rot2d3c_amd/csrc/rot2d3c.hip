@@ -832,7 +832,7 @@ __global__ __launch_bounds__(N / 8) void k_step_colinv(const cplx *__restrict__ 
 // (by nc = n/3 + 1) are not loaded at all.  Smaller groups hold fewer
 // registers: the row kernel keeps 4 waves per SIMD at n = 2048.
 #ifndef R2D_ROW_LG
-#define R2D_ROW_LG 4
+#define R2D_ROW_LG 2
 #endif
 template <int N> constexpr int row_load_group() { return N <= 1024 ? 8 : R2D_ROW_LG; }
 template <int N>
@@ -1179,25 +1179,40 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
   const double tmp1 = dt / (double)o;
   const double nu = field == 0 ? c.nu : c.nuv, hnu = field == 0 ? c.hnu : c.hnuv;
   const int nn = field == 0 ? c.nn : c.nnv, mm = field == 0 ? c.mm : c.mmv;
+  // the state on the disc, in two groups of four slots: each group's loads
+  // are issued before its arithmetic, so they are in flight together
+  const cplx *fd = field == 0 ? c.fpd : c.fzd;
   cplx s[8];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int j = pos(b, q), ky = kyof(j, N);
-    const double ka2 = (double)kx * kx + (double)ky * ky;
-    cplx cc = mk(0.0, 0.0);
-    if (ka2 <= c.g.kmax && ka2 > c.g.tiny) {
-      const double tmp2 = 1.0 / ka2;
-      const double tmp3 = 1.0 + (nu * ipow(ka2, nn) + hnu * ipow(tmp2, mm)) * tmp1;
-      cplx f = j == fj0 ? fv0 : (j == fj1 ? fv1 : mk(0.0, 0.0));
-      const cplx *fd = field == 0 ? c.fpd : c.fzd;
-      if (fd) f = cadd(f, fd[(size_t)kx * N + j]);
-      const cplx s0 = state[(size_t)kx * N + j];
-      cplx t = field == 0 ? cscale(v[q], tmp2) : v[q];
-      t = cadd(t, f);
-      t = cadd(s0, cscale(t, tmp1));
-      cc = mk(t.x / tmp3, t.y / tmp3);
+  for (int g = 0; g < 8; g += 4) {
+    cplx s0v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = pos(b, g + u), ky = kyof(j, N);
+      const double ka2 = (double)kx * kx + (double)ky * ky;
+      s0v[u] = mk(0.0, 0.0);
+      if (ka2 <= c.g.kmax && ka2 > c.g.tiny) s0v[u] = state[(size_t)kx * N + j];
     }
-    s[q] = cc;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = g + u;
+      const int j = pos(b, q), ky = kyof(j, N);
+      const double ka2 = (double)kx * kx + (double)ky * ky;
+      cplx cc = mk(0.0, 0.0);
+      if (ka2 <= c.g.kmax && ka2 > c.g.tiny) {
+        const double tmp2 = 1.0 / ka2;
+        const double tmp3 = 1.0 + (nu * ipow(ka2, nn) + hnu * ipow(tmp2, mm)) * tmp1;
+        cplx f = j == fj0 ? fv0 : (j == fj1 ? fv1 : mk(0.0, 0.0));
+        if (fd) f = cadd(f, fd[(size_t)kx * N + j]);
+        const cplx s0 = s0v[u];
+        cplx t = field == 0 ? cscale(v[q], tmp2) : v[q];
+        t = cadd(t, f);
+        t = cadd(s0, cscale(t, tmp1));
+        cc = mk(t.x / tmp3, t.y / tmp3);
+      }
+      s[q] = cc;
+    }
+    __builtin_amdgcn_sched_barrier(0);  // the next group's loads stay below
   }
   if (mode & FM_STATE) {
     // interleaved -> canonical through LDS, then coalesced stores
