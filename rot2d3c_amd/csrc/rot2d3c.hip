@@ -3057,18 +3057,31 @@ __global__ __launch_bounds__(kBigT) void k_big_row_c2r(const cplx *__restrict__ 
   // chunks in that XCD's L2.
   const int xcd = blockIdx.x & 7, l = blockIdx.x >> 3;
   const int b = threadIdx.x, y = 2 * ((((l >> 1) << 3) + xcd) * 2 + (l & 1));
-  auto z = [&](int k) {
-    const bool mir = k > N / 2;
-    const int kk = mir ? N - k : k;
-    if (kk >= ncols) return mk(0.0, 0.0);
-    return packed_z<N>(k, mix[tix(y, kk, g.kpf)], mix[tix(y + 1, kk, g.kpf)], mir);
-  };
+  // Packed inputs, two q slots (four points) at a time: a group's loads are
+  // issued before its arithmetic, without branches (points past ncols read
+  // column 0 and are discarded), so they are in flight together.
   cplx ev[8], od[8];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int k = 2 * (b + kBigT * q);
-    ev[q] = z(k);
-    od[q] = z(k + 1);
+  for (int q0 = 0; q0 < 8; q0 += 2) {
+    cplx ra[4], rb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = 2 * (b + kBigT * (q0 + (u >> 1))) + (u & 1);
+      const int kk = k > N / 2 ? N - k : k;
+      const int kl = kk < ncols ? kk : 0;
+      ra[u] = mix[tix(y, kl, g.kpf)];
+      rb[u] = mix[tix(y + 1, kl, g.kpf)];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = 2 * (b + kBigT * (q0 + (u >> 1))) + (u & 1);
+      const bool mir = k > N / 2;
+      const int kk = mir ? N - k : k;
+      const cplx zz = kk < ncols ? packed_z<N>(k, ra[u], rb[u], mir) : mk(0.0, 0.0);
+      if (u & 1) od[q0 + (u >> 1)] = zz;
+      else ev[q0 + (u >> 1)] = zz;
+    }
+    __builtin_amdgcn_sched_barrier(0);  // the next group's loads stay below
   }
   fft16k<+1>(ev, od, lds, b, tw);
 #pragma unroll
