@@ -24,6 +24,10 @@ from . import _lib
 from ._lib import check
 
 DEFAULT = (2, 0, 0)
+# every schedule r2d3c_autotune may try, in its order (invalid ones for a
+# size are rejected by the library)
+CANDIDATES = tuple((rw, bpc, cs) for cs in (0, 1)
+                   for rw, bpc in ((2, 0), (2, 1), (2, 2), (1, 1), (1, 2)))
 _lock = threading.Lock()
 
 

@@ -10,6 +10,7 @@ as one table, so the steps need no host round trip.  Each run equals the same
 run advanced alone by rk4_run with its fixed dt, bit for bit.
 """
 import ctypes
+import time
 import math
 
 import numpy as np
@@ -50,6 +51,7 @@ class Ensemble:
                   "r2d3c_ensemble_create")
         self.handle = h
         self.sizes = sorted({m.n for m in self.members})
+        self.schedules = {n: autotune.DEFAULT for n in self.sizes}
         for n in self.sizes:
             if schedules == "auto":
                 sch = autotune.schedule(n, self.device)
@@ -57,12 +59,63 @@ class Ensemble:
                 sch = schedules.get(n, autotune.DEFAULT)
             else:
                 continue
-            rw, bpc = sch[0], sch[1]
-            cs = sch[2] if len(sch) > 2 else 0
-            check(_lib.lib().r2d3c_ensemble_set_schedule(self.handle, n, rw, bpc),
-                  "r2d3c_ensemble_set_schedule")
-            check(_lib.lib().r2d3c_ensemble_set_column_radix(self.handle, n, cs),
-                  "r2d3c_ensemble_set_column_radix")
+            check(self._set_schedule(n, sch), "r2d3c_ensemble_set_schedule")
+
+    def _set_schedule(self, n, sch):
+        """Set size n's schedule (rows, blocks per CU[, column form]); 0 or
+        the library's error code (the schedule is then unchanged)."""
+        rw, bpc = int(sch[0]), int(sch[1])
+        cs = int(sch[2]) if len(sch) > 2 else 0
+        old = self.schedules[n]
+        rc = _lib.lib().r2d3c_ensemble_set_schedule(self.handle, n, rw, bpc)
+        if rc == 0:
+            rc = _lib.lib().r2d3c_ensemble_set_column_radix(self.handle, n, cs)
+            if rc != 0:
+                _lib.lib().r2d3c_ensemble_set_schedule(self.handle, n, old[0], old[1])
+        if rc == 0:
+            self.schedules[n] = (rw, bpc, cs)
+        return rc
+
+    def tune(self, steps=10, trials=3):
+        """Choose every size's schedule by timing the whole ensemble step
+        graph, where the sizes share the GPU (a solo plan's fastest schedule
+        need not be the fastest there): one coordinate-descent sweep over the
+        sizes, largest first, each trying every valid schedule with the other
+        sizes fixed (best of `trials` timings of `steps` unforced steps).
+        The runs' states are restored afterwards, so tuning leaves the
+        ensemble's trajectories as they were.  Returns the schedules and the
+        final milliseconds per ensemble step."""
+        saved = [(m.ps.clone(), m.vz.clone()) for m in self.members]
+
+        def timed():
+            self.run(2)  # recapture and warm up
+            torch.cuda.synchronize(self.device)
+            best = float("inf")
+            for _ in range(trials):
+                t0 = time.perf_counter()
+                self.run(steps)
+                torch.cuda.synchronize(self.device)
+                best = min(best, (time.perf_counter() - t0) * 1e3 / steps)
+            return best
+
+        best_ms = timed()
+        for n in sorted(self.sizes, reverse=True):
+            for cand in autotune.CANDIDATES:
+                if cand == self.schedules[n] or (cand[2] and n < 512):
+                    continue
+                keep = self.schedules[n]
+                if self._set_schedule(n, cand) != 0:
+                    continue
+                ms = timed()
+                if ms < best_ms:
+                    best_ms = ms
+                else:
+                    check(self._set_schedule(n, keep), "r2d3c_ensemble_set_schedule")
+        for m, (ps, vz) in zip(self.members, saved):
+            m.ps.copy_(ps)
+            m.vz.copy_(vz)
+        torch.cuda.synchronize(self.device)
+        return dict(self.schedules), best_ms
 
     def set_closure(self, run, net=None, scale_in=None, scale_out=None, kmin=0.0, kclip=0.0):
         """Make run `run` a closure run with network `net` (closure.Net for one

@@ -210,6 +210,43 @@ def test_ensemble_with_stockham_columns_equals_solo_runs(cuda):
     assert torch.equal(ps, ps_e) and torch.equal(vz, vz_e)
 
 
+def test_ensemble_tune_keeps_state_and_runs_match_solo(cuda):
+    """Ensemble.tune times schedules inside the ensemble's step graph: it
+    leaves every run's state as it was (bitwise), keeps valid schedules, and
+    afterwards each run still equals its solo run under the chosen schedule."""
+    print("ensemble tune", flush=True)
+    from rot2d3c_amd import ensemble as E, spectral as S, _lib, autotune
+    from rot2d3c_amd._lib import check
+    spec = _spec([256, 512, 256, 64])
+    ms, host = _members(spec, cuda)
+    ens = E.Ensemble(ms, schedules=None)
+    before = [(m.ps.clone(), m.vz.clone()) for m in ms]
+    sch, t = ens.tune(steps=3, trials=1)
+    assert t > 0 and set(sch) == {64, 256, 512}
+    for n, v in sch.items():
+        assert tuple(v) in autotune.CANDIDATES and (v[2] == 0 or n >= 512)
+    for m, (ps, vz) in zip(ms, before):
+        assert torch.equal(m.ps, ps) and torch.equal(m.vz, vz)
+    nsteps = 3
+    ens.run(nsteps, None)
+    torch.cuda.synchronize()
+    try:
+        for m, h in zip(ms, host):
+            pl = S.Plan.get(m.n, cuda)
+            rw, bpc, cs = sch[m.n]
+            check(_lib.lib().r2d3c_plan_set_schedule(pl.handle, rw, bpc), "set_schedule")
+            check(_lib.lib().r2d3c_plan_set_column_radix(pl.handle, cs), "set_column_radix")
+            ps, vz = _solo(m, h, None, nsteps, cuda)
+            torch.cuda.synchronize()
+            assert torch.equal(ps, m.ps) and torch.equal(vz, m.vz), m.n
+    finally:
+        for n in sch:
+            pl = S.Plan.get(n, cuda)
+            check(_lib.lib().r2d3c_plan_set_schedule(pl.handle, 2, 0), "set_schedule")
+            check(_lib.lib().r2d3c_plan_set_column_radix(pl.handle, 0), "set_column_radix")
+    ens.close()
+
+
 def test_autotune_picks_a_valid_schedule(cuda, tmp_path, monkeypatch):
     print("autotune n=512", flush=True)
     cache = tmp_path / "s.json"
