@@ -74,17 +74,23 @@ def _copy_reading(disp):
     return max(vals)
 
 
+def _column(name, kern):
+    """A 2048 column kernel of the default (four-step) radix form; the
+    autotuner's runs of the other form are left out."""
+    return (kern + "<2048>") in name or (kern + "<2048, false>") in name
+
+
 def _classes(disp):
     """Readings (KiB) per step-kernel class; fused-column dispatches are
     stages 4, 3, 2, 1 in turn, the fourth of each step being the last stage."""
     out = {"k_step_colinv": [], "k_step_row": [], "k_step_colfused": [], "k_step_colfused_last": []}
     fused = 0
     for name, v in disp:
-        if name.startswith("void k_step_colinv<2048>") or name.startswith("k_step_colinv<2048>"):
+        if _column(name, "k_step_colinv"):
             out["k_step_colinv"].append(v)
         elif "k_step_row<2048" in name or "k_step_row2<2048" in name:
             out["k_step_row"].append(v)
-        elif "k_step_colfused<2048>" in name:
+        elif _column(name, "k_step_colfused"):
             out["k_step_colfused_last" if fused % 4 == 3 else "k_step_colfused"].append(v)
             fused += 1
     return out

@@ -54,6 +54,34 @@ def test_classes_scaling_and_ratios(tmp_path):
                                                           abs=1e-4)
 
 
+def test_radix_form_template_names(tmp_path):
+    """Column kernels named with their radix form: the default form
+    (`<2048, false>`) is counted, the other form's tuning runs are not, and
+    they do not shift the stage count."""
+    kib = lambda b: b / 1024.0
+    fetch = [("void at::native::vectorized_elementwise_kernel<copy>", kib(T.COPY_BYTES) / 2)]
+    write = [("void at::native::vectorized_elementwise_kernel<copy>", kib(T.COPY_BYTES))]
+    for form in ("true", "false", "false"):
+        for stage in range(4):
+            fused = 70e6 if stage < 3 else 90e6
+            if form == "true":
+                fused *= 10
+            name = "void k_step_colfused<2048, %s>(...)" % form
+            fetch.append((name, kib(fused) / 2))
+            write.append((name, kib(140e6)))
+    fetch.append(("void k_step_colfused<2048, true>(...)", 1e9))  # a lone other-form dispatch
+    write.append(("void k_step_colfused<2048, true>(...)", 1e9))
+    fetch.append(("void k_step_colinv<2048, false>(...)", kib(100e6) / 2))
+    write.append(("void k_step_colinv<2048, false>(...)", kib(50e6)))
+    fp, wp = tmp_path / "f.csv", tmp_path / "w.csv"
+    _write(fp, fetch, "FETCH_SIZE")
+    _write(wp, write, "WRITE_SIZE")
+    k = T.build(str(fp), str(wp), 10)["kernels"]
+    assert k["k_step_colfused"]["fetch_bytes_per_launch"] == pytest.approx(70e6, rel=1e-9)
+    assert k["k_step_colfused_last"]["fetch_bytes_per_launch"] == pytest.approx(90e6, rel=1e-9)
+    assert k["k_step_colinv"]["hbm_bytes_per_launch"] == pytest.approx(150e6, rel=1e-9)
+
+
 def test_missing_copy_probe_is_an_error(tmp_path):
     fp = tmp_path / "f.csv"
     _write(fp, [("void k_step_row<2048>(...)", 1.0)], "FETCH_SIZE")
