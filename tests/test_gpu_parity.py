@@ -35,7 +35,7 @@ def _rand_spec(n, seed, hermitian=True):
     return a * n
 
 
-@pytest.mark.parametrize("n", SIZES + [512, 2048])
+@pytest.mark.parametrize("n", SIZES + [512, 2048, 4096])
 def test_c2r_matches_oracle(n, cuda):
     print("c2r n=%d" % n, flush=True)
     S = _S()
@@ -44,7 +44,7 @@ def test_c2r_matches_oracle(n, cuda):
     assert rel_l2(r, O.fftp2d_complex_to_real(a)) < TOL
 
 
-@pytest.mark.parametrize("n", SIZES + [512, 2048])
+@pytest.mark.parametrize("n", SIZES + [512, 2048, 4096])
 def test_r2c_matches_oracle(n, cuda):
     print("r2c n=%d" % n, flush=True)
     S = _S()
@@ -65,7 +65,7 @@ def test_spectral_operators_exact(n, cuda):
     assert rel_l2(_host(S.inv_laplak2(ta)), O.inv_laplak2(a)) < 1e-15
 
 
-@pytest.mark.parametrize("n", SIZES)
+@pytest.mark.parametrize("n", SIZES + [4096])
 def test_poisson_matches_oracle(n, cuda):
     print("poisson n=%d" % n, flush=True)
     S = _S()
@@ -287,6 +287,30 @@ def test_bench_workload_2048_10_steps(cuda):
     n, steps, dt, kf, seed = 2048, 10, 2e-4, 40.0, 0
     p = O.Params(5e-5, 0.5, 1, 2, 5e-5, 0.5, 1, 2, 5.0)
     ps, vz = ic.random_phase_ic(n, seed=1234)
+    sched = ic.forcing_schedule(kf, 1.0, 1.0, seed, steps, S.rand_force_draw)
+    draws = [(S.ForceDraw(*a), S.ForceDraw(*b)) for a, b in sched]
+    tps, tvz = _dev(ps, cuda), _dev(vz, cuda)
+    sp = S.Params(p.nu, p.hnu, p.nn, p.mm, p.nuv, p.hnuv, p.nnv, p.mmv, p.omega)
+    S.rk4_run(tps, tvz, sp, draws, dt=dt)
+    torch.cuda.synchronize()
+    ops, ovz, *_ = O.run_random_forcing(ps.copy(), vz.copy(), steps, 0.0, p, kf, kf, 1.0, 1.0, seed,
+                                        fixed_dt=dt)
+    assert rel_l2(_host(tps), ops) < TOL
+    assert rel_l2(_host(tvz), ovz) < TOL
+
+
+def test_largest_fused_size_4096_one_step(cuda):
+    """n = 4096, the largest size of the fused step (its row kernels run at
+    the register limit of 1024-thread blocks): one fixed-dt step with
+    random forcing through r2d3c_run against the oracle's iflow = 3 loop,
+    relative L2 <= 1e-12."""
+    print("4096, 1 step", flush=True)
+    import torch
+    from rot2d3c_amd import ic
+    S = _S()
+    n, steps, dt, kf, seed = 4096, 1, 1e-4, 64.0, 5
+    p = O.Params(2e-5, 0.5, 1, 2, 2e-5, 0.5, 1, 2, 3.0)
+    ps, vz = ic.random_phase_ic(n, seed=77)
     sched = ic.forcing_schedule(kf, 1.0, 1.0, seed, steps, S.rand_force_draw)
     draws = [(S.ForceDraw(*a), S.ForceDraw(*b)) for a, b in sched]
     tps, tvz = _dev(ps, cuda), _dev(vz, cuda)
