@@ -118,17 +118,35 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
                                           const cplx *__restrict__ tw) {
   using F = Shape<N>;
   constexpr int T = F::T;
+  // LDS slots as one base per pass plus compile-time offsets: with T a
+  // multiple of 16, pad(i + m*K) = pad(i) + m*(K + K/16) for K = T, L >= 16
+  // or N/RL, and the L = 8 writes have offsets 8m + m/2 (their bases are
+  // 0..7 modulo 64).  Used where every offset fits the 16-bit immediate of
+  // an LDS instruction; otherwise (small or very long transforms) pad() per
+  // slot.
+  constexpr int RL0 = F::RL;
+  constexpr bool LIN = T % 16 == 0 && 16 * IL * 7 * (T + T / 16) <= 65535 &&
+                       16 * IL * (RL0 - 1) * (N / RL0 + N / RL0 / 16) <= 65535;
   // pass 1: radix 8, span 1; inputs b + m*T are exactly v[m]
   dft8<S>(v);
+  {
+    cplx *w = lds + (b * 8 + (b >> 1)) * IL;  // pad(8b + m) = 8b + b/2 + m
 #pragma unroll
-  for (int m = 0; m < 8; ++m) lds[pad(b * 8 + m) * IL] = v[m];
+    for (int m = 0; m < 8; ++m) w[m * IL] = v[m];
+  }
   __syncthreads();
   int L = 8;
 #pragma unroll
   for (int p = 0; p < F::NMID; ++p) {
     cplx x[8];
+    if constexpr (LIN) {
+      const cplx *r = lds + pad(b) * IL;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) x[m] = lds[pad(b + m * T) * IL];
+      for (int m = 0; m < 8; ++m) x[m] = r[m * (T + T / 16) * IL];
+    } else {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) x[m] = lds[pad(b + m * T) * IL];
+    }
     __syncthreads();
     const int j = b & (L - 1);
     const int st = (N / (8 * L)) * j;
@@ -146,8 +164,15 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
 #endif
     dft8<S>(x);
     const int base = (b - j) * 8 + j;
+    if constexpr (LIN) {
+      cplx *w = lds + pad(base) * IL;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) lds[pad(base + m * L) * IL] = x[m];
+      for (int m = 0; m < 8; ++m)
+        w[(L >= 16 ? m * (L + L / 16) : 8 * m + (m >> 1)) * IL] = x[m];
+    } else {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) lds[pad(base + m * L) * IL] = x[m];
+    }
     __syncthreads();
     L *= 8;
   }
@@ -157,8 +182,14 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
   for (int u = 0; u < U; ++u) {
     const int j = b + T * u;
     cplx x[RL];
+    if constexpr (LIN) {
+      const cplx *r = lds + pad(j) * IL;
 #pragma unroll
-    for (int m = 0; m < RL; ++m) x[m] = lds[pad(j + m * (N / RL)) * IL];
+      for (int m = 0; m < RL; ++m) x[m] = r[m * (N / RL + N / RL / 16) * IL];
+    } else {
+#pragma unroll
+      for (int m = 0; m < RL; ++m) x[m] = lds[pad(j + m * (N / RL)) * IL];
+    }
 #if R2D_TW_DERIVE
     if constexpr (RL == 2) {
       x[1] = cmul(x[1], twid<S>(tw, TS * (j)));
