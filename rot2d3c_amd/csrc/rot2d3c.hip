@@ -45,6 +45,16 @@ __device__ __forceinline__ int kxof(int i, int n) { return i < n / 2 ? i : i - n
 __device__ __forceinline__ size_t tix(int y, int kx, int kp) {
   return (size_t)(y >> 2) * (size_t)(kp * 4) + (size_t)kx * 4 + (y & 3);
 }
+// tix as a byte offset in 32 bits, for n <= 4096 (a tiled field then holds
+// fewer than 2^24 16-byte slots): the access takes the field's uniform base
+// address plus a 32-bit lane offset instead of a 64-bit address per lane.
+__device__ __forceinline__ unsigned tix_bytes(int y, int kx, int kp) {
+  return ((unsigned)(y >> 2) * (unsigned)(kp * 4) + (unsigned)kx * 4u + (unsigned)(y & 3)) * 16u;
+}
+template <class Tp> __device__ __forceinline__ Tp *at_bytes(Tp *base, unsigned off) {
+  return reinterpret_cast<Tp *>(reinterpret_cast<typename std::conditional<
+      std::is_const<Tp>::value, const char, char>::type *>(base) + off);
+}
 // integer power by binary exponentiation (Fortran x**n with integer n >= 0)
 __host__ __device__ __forceinline__ double ipow(double x, int e) {
   double r = 1.0;
@@ -205,8 +215,8 @@ __device__ __forceinline__ void unpack_store(const cplx *lds, int b, int ncols, 
       const cplx w = lds[pad(k) * IL], wm = lds[pad((N - k) & (N - 1)) * IL];
       const cplx A = mk(0.5 * (w.x + wm.x), 0.5 * (w.y - wm.y));
       const cplx B = mk(0.5 * (w.y + wm.y), -0.5 * (w.x - wm.x));
-      mixa[tix(ya, k, kp)] = A;
-      mixb[tix(yb_, k, kp)] = B;
+      *at_bytes(mixa, tix_bytes(ya, k, kp)) = A;
+      *at_bytes(mixb, tix_bytes(yb_, k, kp)) = B;
     }
   }
 }
@@ -850,8 +860,8 @@ __device__ __forceinline__ void load_packed_step(cplx v[8], const cplx *__restri
       const int k = b + T * q;
       const bool lo = k < nc, hi = k > N - nc;
       const int kk = lo ? k : (hi ? N - k : 0);
-      fa[u] = FA[tix(y, kk, kp)];
-      fb[u] = FB[tix(y, kk, kp)];
+      fa[u] = *at_bytes(FA, tix_bytes(y, kk, kp));
+      fb[u] = *at_bytes(FB, tix_bytes(y, kk, kp));
     }
 #pragma unroll
     for (int u = 0; u < LG; ++u) {
