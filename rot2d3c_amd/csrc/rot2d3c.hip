@@ -888,19 +888,8 @@ __device__ __forceinline__ void load_packed_step(cplx v[8], const cplx *__restri
   }
 }
 
-// Minimum waves per SIMD the row kernel is compiled for (0: the compiler's
-// choice); caps its registers at 512 / R2D_ROW_WPE.
-#ifndef R2D_ROW_WPE
-#define R2D_ROW_WPE 0
-#endif
-#if R2D_ROW_WPE > 0
-#define R2D_ROW_ATTR __attribute__((amdgpu_waves_per_eu(R2D_ROW_WPE)))
-#else
-#define R2D_ROW_ATTR
-#endif
-
 template <int N>
-__global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) R2D_ROW_ATTR void k_step_row(
+__global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
     const cplx *__restrict__ M, cplx *__restrict__ Q, StepConst c, unsigned long long *maxbits,
     const cplx *__restrict__ tw, size_t mstride, const double *__restrict__ om_tab) {
   constexpr int T = N / 8, RW = RowStep<N>::ROWS;
