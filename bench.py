@@ -147,6 +147,34 @@ def large_bench(steps, dev, warmup=2):
             "finite": finite}
 
 
+def copy_probe(dev, nbytes=1 << 30, reps=20):
+    """Achievable HBM bandwidth: the library's streaming copy kernel
+    (r2d3c_copy_probe) on nbytes, best of `reps` calls timed with events on
+    the stream it runs on; read + write = 2 nbytes per call."""
+    import torch
+    from rot2d3c_amd import _lib, spectral as S
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    lib = _lib.lib()
+    stream = S._stream(dev)
+    for _ in range(3):
+        _lib.check(lib.r2d3c_copy_probe(dst.data_ptr(), src.data_ptr(), nbytes, stream), "copy_probe")
+    best = float("inf")
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        _lib.check(lib.r2d3c_copy_probe(dst.data_ptr(), src.data_ptr(), nbytes, stream), "copy_probe")
+        b.record()
+        b.synchronize()
+        best = min(best, a.elapsed_time(b))
+    ok = bool(torch.equal(dst, src))
+    del src, dst
+    torch.cuda.empty_cache()
+    if not ok:
+        raise RuntimeError("copy probe produced a wrong copy")
+    return 2 * nbytes / (best * 1e-3) / 1e9
+
+
 def ensemble_bench(steps, dev, tuned, warmup=10):
     """The batched workload: 24 x 256^2 + 12 x 512^2 + 4 x 1024^2 independent
     runs (per-run seed, Omega, nu, forcing shell, fixed dt; see
@@ -292,6 +320,14 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic, "kernel": dom,
             "bytes_per_launch": per[dom], "avg_launch_us": round(kern[dom]["avg_us"], 2)}
 
+    copy_gbps = copy_probe(dev)
+    achievable = {"copy_GBps": round(copy_gbps, 1),
+                  "what": "r2d3c_copy_probe: 1 GiB device-to-device streaming copy, "
+                          "2 x 1 GiB of HBM traffic per call, best of 20 event-timed calls",
+                  "dominant_kernel_frac_of_copy": round(achieved / copy_gbps, 4),
+                  "step_76S_frac_of_copy": round(geo["B76S"] / (ms_per_step * 1e-3) / 1e9
+                                                 / copy_gbps, 4)}
+
     cpu = None
     if args.cpu_steps > 0:
         el = cpu_baseline(n, args.cpu_steps, p, kf, f0, seed, ps0, vz0, dt)
@@ -333,6 +369,7 @@ def main():
                    "schedule": {"row_rows": sch[0], "row_blocks_per_cu": sch[1],
                                 "column_radix_form": sch[2]}},
         "roofline": roof,
+        "achievable_hbm": achievable,
         "cpu_baseline": cpu,
         "step_hbm": {"algorithmic_bytes_per_step": step_bytes,
                      "achieved_GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),

@@ -214,6 +214,12 @@ int r2d3c_timing_enable(r2d3c_plan *plan, int enable);
  * launch count per kernel class (arrays of R2D3C_NKERNELS), and clears them. */
 int r2d3c_timing_read(r2d3c_plan *plan, double *ms, long long *launches);
 
+/* Streaming device copy of `bytes` (a multiple of 16; 16-byte aligned
+ * pointers) from src to dst on
+ * `stream`, the bandwidth yardstick of the benchmark (no reference
+ * counterpart): 2 x bytes of HBM traffic per call. */
+int r2d3c_copy_probe(void *dst, const void *src, long long bytes, void *stream);
+
 /* ---- host random numbers (no device work) ----------------------------- */
 
 /* Park-Miller minimal standard with the reference's XOR mask; float32 value in
