@@ -253,7 +253,7 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     ps0, vz0 = ic.random_phase_ic(n, seed=1234)
-    total = args.warmup + args.steps + min(args.steps, 100)
+    total = args.warmup + args.steps + 6 * min(args.steps, 100)
     sched = ic.forcing_schedule(kf, f0, f0, seed, total, S.rand_force_draw)
     draws = [(S.ForceDraw(*a), S.ForceDraw(*b)) for a, b in sched]
     sp = S.Params(**p)
@@ -296,6 +296,16 @@ def main():
     _lib.check(lib.r2d3c_timing_read(plan.handle, ms, cnt), "timing_read")
     _lib.check(lib.r2d3c_timing_enable(plan.handle, 0), "timing_disable")
     torch.cuda.synchronize()
+    # run-to-run spread: five more timed runs of ktimed steps each
+    reps = []
+    for r in range(5):
+        k0 = args.warmup + args.steps + (1 + r) * ktimed
+        t0 = time.perf_counter()
+        run(k0, ktimed)
+        torch.cuda.synchronize()
+        reps.append(1e3 * (time.perf_counter() - t0) / ktimed)
+    repeats = {"runs": 5, "steps_each": ktimed, "ms_per_step_median": round(float(np.median(reps)), 4),
+               "ms_per_step_min": round(min(reps), 4), "ms_per_step_max": round(max(reps), 4)}
     finite = bool(torch.isfinite(torch.view_as_real(ps)).all() and torch.isfinite(torch.view_as_real(vz)).all())
 
     per, step_bytes, geo = algorithmic_bytes(n, args.steps)
@@ -370,6 +380,7 @@ def main():
                                 "column_radix_form": sch[2]}},
         "roofline": roof,
         "achievable_hbm": achievable,
+        "repeats": repeats,
         "cpu_baseline": cpu,
         "step_hbm": {"algorithmic_bytes_per_step": step_bytes,
                      "achieved_GBps": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
