@@ -94,7 +94,7 @@ def timed_steps(ps0, vz0, p, dt, forcing, steps, net=None, sin=None, sout=None, 
     building is outside the timed region."""
     from . import ensemble as E
     ps, vz = ps0.clone(), vz0.clone()
-    ens = E.Ensemble([E.Member(ps, vz, p, dt)], schedules="auto")
+    ens = E.Ensemble([E.Member(ps, vz, p, dt)], schedules=None)  # the default kernels
     if net is not None:
         net.set_output_clamp(clamp)
         net.set_sign_clip(sign_clip)
@@ -135,7 +135,10 @@ def coarse_runs(ps0, vz0, p, t_end, dt, seeds, kf, f0, params=None, sin=None, so
     from . import ensemble as E
     R = len(seeds)
     runs = [(ps0.clone(), vz0.clone()) for _ in range(R)]
-    ens = E.Ensemble([E.Member(a, b, p, dt) for a, b in runs], schedules="auto")
+    # the default schedule: the same kernels on every GPU, so the study is
+    # reproducible bit for bit (an autotuned choice may pick kernels that
+    # differ in rounding, and the coarse runs are chaotic)
+    ens = E.Ensemble([E.Member(a, b, p, dt) for a, b in runs], schedules=None)
     nets = []
     if params is not None:
         for r in range(R):

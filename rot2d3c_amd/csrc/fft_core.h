@@ -138,6 +138,17 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
   int L = 8;
 #pragma unroll
   for (int p = 0; p < F::NMID; ++p) {
+    const int j = b & (L - 1);
+    const int st = (N / (8 * L)) * j;
+    // the pass's table twiddles are loaded first: they stay in flight over
+    // the LDS reads and the barrier
+#if R2D_TW_DERIVE
+    const cplx w1 = twid<S>(tw, TS * (st)), w2 = twid<S>(tw, TS * (2 * st)), w4 = twid<S>(tw, TS * (4 * st));
+#else
+    cplx wt[8];
+#pragma unroll
+    for (int m = 1; m < 8; ++m) wt[m] = twid<S>(tw, TS * (st * m));
+#endif
     cplx x[8];
     if constexpr (LIN) {
       const cplx *r = lds + pad(b) * IL;
@@ -148,11 +159,8 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
       for (int m = 0; m < 8; ++m) x[m] = lds[pad(b + m * T) * IL];
     }
     __syncthreads();
-    const int j = b & (L - 1);
-    const int st = (N / (8 * L)) * j;
 #if R2D_TW_DERIVE
     {  // three table loads; the other powers by products (error <= 2 ulp more)
-      const cplx w1 = twid<S>(tw, TS * (st)), w2 = twid<S>(tw, TS * (2 * st)), w4 = twid<S>(tw, TS * (4 * st));
       const cplx w3 = cmul(w1, w2);
       x[1] = cmul(x[1], w1); x[2] = cmul(x[2], w2); x[3] = cmul(x[3], w3);
       x[4] = cmul(x[4], w4); x[5] = cmul(x[5], cmul(w1, w4));
@@ -160,7 +168,7 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
     }
 #else
 #pragma unroll
-    for (int m = 1; m < 8; ++m) x[m] = cmul(x[m], twid<S>(tw, TS * (st * m)));
+    for (int m = 1; m < 8; ++m) x[m] = cmul(x[m], wt[m]);
 #endif
     dft8<S>(x);
     const int base = (b - j) * 8 + j;
@@ -181,6 +189,16 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int j = b + T * u;
+    // table twiddles ahead of the LDS reads
+#if R2D_TW_DERIVE
+    const cplx w1 = twid<S>(tw, TS * (j));
+    const cplx w2 = RL >= 4 ? twid<S>(tw, TS * (2 * j)) : w1;
+    const cplx w4 = RL == 8 ? twid<S>(tw, TS * (4 * j)) : w1;
+#else
+    cplx wt[RL];
+#pragma unroll
+    for (int m = 1; m < RL; ++m) wt[m] = twid<S>(tw, TS * (j * m));
+#endif
     cplx x[RL];
     if constexpr (LIN) {
       const cplx *r = lds + pad(j) * IL;
@@ -192,12 +210,10 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
     }
 #if R2D_TW_DERIVE
     if constexpr (RL == 2) {
-      x[1] = cmul(x[1], twid<S>(tw, TS * (j)));
+      x[1] = cmul(x[1], w1);
     } else if constexpr (RL == 4) {
-      const cplx w1 = twid<S>(tw, TS * (j)), w2 = twid<S>(tw, TS * (2 * j));
       x[1] = cmul(x[1], w1); x[2] = cmul(x[2], w2); x[3] = cmul(x[3], cmul(w1, w2));
     } else {
-      const cplx w1 = twid<S>(tw, TS * (j)), w2 = twid<S>(tw, TS * (2 * j)), w4 = twid<S>(tw, TS * (4 * j));
       const cplx w3 = cmul(w1, w2);
       x[1] = cmul(x[1], w1); x[2] = cmul(x[2], w2); x[3] = cmul(x[3], w3);
       x[4] = cmul(x[4], w4); x[5] = cmul(x[5], cmul(w1, w4));
@@ -205,7 +221,7 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
     }
 #else
 #pragma unroll
-    for (int m = 1; m < RL; ++m) x[m] = cmul(x[m], twid<S>(tw, TS * (j * m)));
+    for (int m = 1; m < RL; ++m) x[m] = cmul(x[m], wt[m]);
 #endif
     dftR<RL, S>(x);
 #pragma unroll
@@ -237,9 +253,17 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The three table twiddles w^st, w^2st, w^4st of a radix-8 pass; loaded
+// ahead of the pass's LDS reads so that they are in flight meanwhile.
+struct Tw3 {
+  cplx w1, w2, w4;
+};
 template <int S>
-__device__ __forceinline__ void twiddle8(cplx *x, const cplx *__restrict__ tw, int st) {
-  const cplx w1 = twid<S>(tw, st), w2 = twid<S>(tw, 2 * st), w4 = twid<S>(tw, 4 * st);
+__device__ __forceinline__ Tw3 tw3_load(const cplx *__restrict__ tw, int st) {
+  return Tw3{twid<S>(tw, st), twid<S>(tw, 2 * st), twid<S>(tw, 4 * st)};
+}
+__device__ __forceinline__ void twiddle8(cplx *x, const Tw3 &t) {
+  const cplx w1 = t.w1, w2 = t.w2, w4 = t.w4;
   const cplx w3 = cmul(w1, w2);
   x[1] = cmul(x[1], w1); x[2] = cmul(x[2], w2); x[3] = cmul(x[3], w3);
   x[4] = cmul(x[4], w4); x[5] = cmul(x[5], cmul(w1, w4));
@@ -257,12 +281,13 @@ __device__ __forceinline__ void fft_wave512(cplx v[8], cplx *L, int l,
   for (int m = 0; m < 8; ++m) L[pad(l * 8 + m)] = v[m];
   wave_lds_sync();
   {  // span 8
+    const int j = l & 7;
+    const Tw3 t = tw3_load<S>(tw, 8 * j * TS);
     cplx x[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) x[m] = L[pad(l + 64 * m)];
     wave_lds_sync();
-    const int j = l & 7;
-    twiddle8<S>(x, tw, 8 * j * TS);
+    twiddle8(x, t);
     dft8<S>(x);
     const int base = (l - j) * 8 + j;
 #pragma unroll
@@ -270,9 +295,10 @@ __device__ __forceinline__ void fft_wave512(cplx v[8], cplx *L, int l,
     wave_lds_sync();
   }
   {  // span 64, last
+    const Tw3 t = tw3_load<S>(tw, l * TS);
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = L[pad(l + 64 * m)];
-    twiddle8<S>(v, tw, l * TS);
+    twiddle8(v, t);
     dft8<S>(v);
     wave_lds_sync();
   }
