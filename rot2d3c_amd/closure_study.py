@@ -284,6 +284,7 @@ def compact_record(out, command):
                      "epochs": None if out["config"]["params"] else out["config"]["epochs"]},
            "kmin": out["kmin"], "gain": out["gain"], "kclip": out["kclip"], "clamp": out["clamp"],
            "sign_clip": out["sign_clip"], "period": out["period"],
+           "selection": out.get("selection", "min tuning error"),
            "dns_cfl": out["dns"]["cfl"], "cfl_guard": out["config"]["cfl_guard"],
            "safe_cfl": out["config"]["safe_cfl"],
            "tune": {k: {m: v[m] for m in ("spectrum_rel_err_max", "spectrum_rel_err_mean",
@@ -351,6 +352,9 @@ def main(argv=None):
                          "closure_params.npz (the DNS still runs to reach the same windows)")
     ap.add_argument("--target-kmin", type=float, default=0.0,
                     help="train on the targets' modes |k| >= target_kmin only")
+    ap.add_argument("--err-target", type=float, default=None,
+                    help="pick the candidate with the largest period whose mean spectrum error on "
+                         "the tuning window is at most this (default: the smallest error)")
     ap.add_argument("--realisations", type=int, default=4,
                     help="forcing realisations of each coarse run (averaged)")
     ap.add_argument("--epochs", type=int, default=30)
@@ -449,8 +453,20 @@ def main(argv=None):
         log("tune %s: spectrum err max %s mean %s, fallbacks %d, max CFL %.3f, safe %.3f" % (
             key(c), r.get("spectrum_rel_err_max"), r.get("spectrum_rel_err_mean"), fb,
             r["cfl_max"], r["safe_fraction"]))
-    best = min(cands, key=lambda c: out["tune"][key(c)].get("spectrum_rel_err_mean", 1e300)
-               + (1e300 if out["tune"][key(c)]["fallbacks"] else 0.0))
+    def err(c):
+        r = out["tune"][key(c)]
+        return r.get("spectrum_rel_err_mean", 1e300) + (1e300 if r["fallbacks"] else 0.0)
+
+    best = min(cands, key=err)
+    if a.err_target is not None:
+        # the cheapest candidate (fewest network evaluations: largest period)
+        # that meets the accuracy target on the tuning window, then the most
+        # accurate of those; the most accurate overall if none meets it
+        ok = [c for c in cands if err(c) <= a.err_target]
+        if ok:
+            best = min(ok, key=lambda c: (-c["period"], err(c)))
+    out["selection"] = ("min tuning error" if a.err_target is None else
+                        "largest period with tuning error <= %g, then min error" % a.err_target)
     out.update(best)
     kbest, gbest, kcbest, clbest, scbest, pbest = (
         best[k] for k in ("kmin", "gain", "kclip", "clamp", "sign_clip", "period"))
