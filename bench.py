@@ -219,8 +219,6 @@ def main():
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--cpu-steps", type=int, default=3,
                     help="steps of the oracle timed on the host (0 disables)")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="also run one 512 MiB device copy (PMC byte calibration)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default row schedule instead of the cached autotuned one")
     ap.add_argument("--large-steps", type=int, default=200,
@@ -269,13 +267,6 @@ def main():
     def run(k0, k):
         # r2d3c_run: k chained steps (bitwise equal to k single steps)
         S.rk4_run(ps, vz, sp, draws[k0:k0 + k], dt=dt)
-
-    if args.copy_probe:
-        src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
 
     run(0, args.warmup)
     torch.cuda.synchronize()

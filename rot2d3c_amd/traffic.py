@@ -2,15 +2,17 @@
 runs, and the algorithmic (compulsory) bytes they are judged against.
 
 Usage (FETCH_SIZE and WRITE_SIZE each from its own run of
-`python bench.py --steps 10 --warmup 2 --cpu-steps 0 --copy-probe
---ensemble-steps 0 --no-closure --large-steps 0` under
-`rocprofv3 --pmc <counter> --output-format csv`):
+`python bench.py --steps 10 --warmup 2 --cpu-steps 0 --ensemble-steps 0
+--no-closure --large-steps 0` under `rocprofv3 --pmc <counter> --output-format
+csv`):
 
     python -m rot2d3c_amd.traffic FETCH.csv WRITE.csv --steps 10 --out traffic.json
 
-Each counter is scaled by what it reads for the bench's 512 MiB device copy
-(which moves 512 MiB each way): FETCH_SIZE reads about half the bytes fetched
-on this GPU, WRITE_SIZE all the bytes written.
+Each counter is scaled by what it reads for the bench's streaming copy probe
+(`k_copy_probe`, 1 GiB each way per call): FETCH_SIZE reads about half the
+bytes fetched on this GPU, WRITE_SIZE all the bytes written.  The column
+kernels come in two radix forms; the form with the most dispatches (the one
+the bench's schedule used) is reported.
 """
 import argparse
 import csv
@@ -19,7 +21,7 @@ import sys
 
 import numpy as np
 
-COPY_BYTES = 512 * 1024 * 1024
+COPY_BYTES = 1 << 30  # bytes each way per copy-probe call
 
 
 def algorithmic_bytes(n, k):
@@ -66,31 +68,39 @@ def _dispatches(path):
 
 
 def _copy_reading(disp):
-    """The counter's reading for the copy probe: the largest reading of a copy
-    or elementwise kernel (the only 512 MiB one in the run)."""
-    vals = [v for name, v in disp if "copy" in name.lower() or "elementwise" in name]
+    """The counter's median reading (KiB) over the copy-probe dispatches."""
+    vals = [v for name, v in disp if "k_copy_probe" in name]
     if not vals:
-        raise ValueError("no copy kernel in the counter run (run bench.py with --copy-probe)")
-    return max(vals)
+        raise ValueError("no k_copy_probe dispatch in the counter run (bench.py runs one)")
+    return float(np.median(vals))
 
 
-def _column(name, kern):
-    """A 2048 column kernel of the default (four-step) radix form; the
-    autotuner's runs of the other form are left out."""
-    return (kern + "<2048>") in name or (kern + "<2048, false>") in name
+def _column(name, kern, form):
+    """A 2048 column kernel of the given radix form ("false": four-step,
+    "true": radix-8 Stockham; a name without the form is the four-step)."""
+    if form == "false" and (kern + "<2048>") in name:
+        return True
+    return (kern + "<2048, %s>" % form) in name
 
 
-def _classes(disp):
+def _form(disp):
+    """The column form with the most fused-column dispatches."""
+    n = {f: sum(1 for name, _ in disp if _column(name, "k_step_colfused", f))
+         for f in ("false", "true")}
+    return max(n, key=lambda f: n[f])
+
+
+def _classes(disp, form):
     """Readings (KiB) per step-kernel class; fused-column dispatches are
     stages 4, 3, 2, 1 in turn, the fourth of each step being the last stage."""
     out = {"k_step_colinv": [], "k_step_row": [], "k_step_colfused": [], "k_step_colfused_last": []}
     fused = 0
     for name, v in disp:
-        if _column(name, "k_step_colinv"):
+        if _column(name, "k_step_colinv", form):
             out["k_step_colinv"].append(v)
         elif "k_step_row<2048" in name or "k_step_row2<2048" in name:
             out["k_step_row"].append(v)
-        elif _column(name, "k_step_colfused"):
+        elif _column(name, "k_step_colfused", form):
             out["k_step_colfused_last" if fused % 4 == 3 else "k_step_colfused"].append(v)
             fused += 1
     return out
@@ -100,7 +110,8 @@ def build(fetch_csv, write_csv, steps, n=2048):
     fd, wd = _dispatches(fetch_csv), _dispatches(write_csv)
     ff = COPY_BYTES / (_copy_reading(fd) * 1024.0)
     wf = COPY_BYTES / (_copy_reading(wd) * 1024.0)
-    fc, wc = _classes(fd), _classes(wd)
+    form = _form(fd)
+    fc, wc = _classes(fd, form), _classes(wd, form)
     per, _, _ = algorithmic_bytes(n, steps)
     kern = {}
     for k in fc:
@@ -113,7 +124,8 @@ def build(fetch_csv, write_csv, steps, n=2048):
                    "algorithmic_bytes_per_launch": int(round(per[k])),
                    "ratio_to_algorithmic": round((fb + wb) / per[k], 4)}
     return {"workload": "%d^2 FP64 RK4 step (bench.py --steps %d --warmup 2 --cpu-steps 0 "
-                        "--copy-probe --ensemble-steps 0 --no-closure --large-steps 0)" % (n, steps),
+                        "--ensemble-steps 0 --no-closure --large-steps 0)" % (n, steps),
+            "column_form": "radix-8 Stockham" if form == "true" else "four-step",
             "calibration": {"copy_bytes": COPY_BYTES, "FETCH_SIZE_KiB": _copy_reading(fd),
                             "WRITE_SIZE_KiB": _copy_reading(wd), "fetch_factor": ff,
                             "write_factor": wf},
