@@ -101,18 +101,36 @@ def _n_of(a):
 # --------------------------------------------------------------------------
 # 2D FFTs: src/fftp2D.f90:133-290
 # --------------------------------------------------------------------------
+# FFT threads for timing runs only (the CPU baseline of the benchmark): with a
+# thread count set, the 2D transforms go through scipy.fft's pocketfft with
+# that many workers, equal to numpy.fft's within rounding.  Tests keep None.
+_FFT_WORKERS = None
+
+
+def set_fft_workers(workers):
+    """Thread count of the 2D transforms (None: numpy.fft, one thread)."""
+    global _FFT_WORKERS
+    _FFT_WORKERS = workers
+
+
 def fftp2d_complex_to_real(c):
     """Unnormalised inverse 2D transform, e^{+i}.  Complex backward transform
     along ky for every kx column, then real c2r along x, which ignores the
     imaginary parts at kx = 0 and kx = n/2 (src/fftp2D.f90:251,286-287).
     numpy's irfft2 over [ky][kx] applies exactly that order."""
     n = _n_of(c)
+    if _FFT_WORKERS:
+        import scipy.fft
+        return scipy.fft.irfft2(c.T, s=(n, n), workers=_FFT_WORKERS) * float(n * n)
     return np.fft.irfft2(c.T, s=(n, n)) * float(n * n)
 
 
 def fftp2d_real_to_complex(r):
     """Unnormalised forward 2D transform, e^{-i}: real r2c along x, then
     complex along y; output transposed to [kx][ky] (src/fftp2D.f90:171-208)."""
+    if _FFT_WORKERS:
+        import scipy.fft
+        return np.ascontiguousarray(scipy.fft.rfft2(r, workers=_FFT_WORKERS).T)
     return np.ascontiguousarray(np.fft.rfft2(r).T)
 
 
