@@ -483,3 +483,20 @@ def test_max_min_takes_max_of_slab_minima():
     assert one[1] == four[1] == w.max() and one[2] == w.min()
     assert four[2] == max(w[0:4].min(), w[4:8].min(), w[8:12].min(), w[12:16].min())
     assert four[2] > one[2]
+
+
+def test_threaded_fft_timing_path_matches_numpy():
+    """The oracle's threaded 2D transforms (set for the benchmark's CPU
+    baseline only) are the same transforms as numpy.fft's: bitwise equal
+    outputs, and the default is restored."""
+    rng = np.random.default_rng(11)
+    r = rng.standard_normal((64, 64))
+    c = O.fftp2d_real_to_complex(r)
+    O.set_fft_workers(2)
+    try:
+        c2 = O.fftp2d_real_to_complex(r)
+        r2 = O.fftp2d_complex_to_real(c)
+    finally:
+        O.set_fft_workers(None)
+    assert np.array_equal(c, c2)
+    assert np.array_equal(r2, O.fftp2d_complex_to_real(c))
