@@ -840,16 +840,22 @@ __global__ __launch_bounds__(N / 8) void k_step_colinv(const cplx *__restrict__ 
 // arithmetic, without branches (slots in the zero band read column 0 and are
 // discarded), so they are in flight together; slots that no thread can keep
 // (by nc = n/3 + 1) are not loaded at all.  Smaller groups hold fewer
-// registers: the row kernel keeps 4 waves per SIMD at n = 2048.
+// registers: from n = 2048 one slot per group keeps the row kernel at 4
+// waves per SIMD (groups of 4 or 8 slots take 136-142 VGPRs) and measured
+// fastest there.
 #ifndef R2D_ROW_LG
-#define R2D_ROW_LG 2
+#define R2D_ROW_LG 1
 #endif
 template <int N> constexpr int row_load_group() { return N <= 1024 ? 8 : R2D_ROW_LG; }
 template <int N>
 __device__ __forceinline__ void load_packed_step(cplx v[8], const cplx *__restrict__ FA,
                                                  const cplx *__restrict__ FB, int y, int b, int nc,
                                                  int kp) {
+  // nc = n/3 + 1 (checked when the plan is made): with 0 <= b < T, slot q
+  // holds k in [T q, T q + T - 1], so whole slots are known to be kept
+  // columns (all_lo), their mirrors (all_hi) or zero at compile time
   constexpr int T = N / 8, NC = N / 3 + 1, LG = row_load_group<N>();
+  (void)nc;
 #pragma unroll
   for (int g = 0; g < 8; g += LG) {
     cplx fa[LG], fb[LG];
@@ -858,7 +864,8 @@ __device__ __forceinline__ void load_packed_step(cplx v[8], const cplx *__restri
       const int q = g + u;
       if (T * q >= NC && T * q + T - 1 <= N - NC) continue;  // only zero-band slots
       const int k = b + T * q;
-      const bool lo = k < nc, hi = k > N - nc;
+      const bool all_lo = T * q + T - 1 < NC, all_hi = T * q > N - NC;
+      const bool lo = all_lo || (!all_hi && k < NC), hi = all_hi || (!all_lo && k > N - NC);
       const int kk = lo ? k : (hi ? N - k : 0);
       fa[u] = *at_bytes(FA, tix_bytes(y, kk, kp));
       fb[u] = *at_bytes(FB, tix_bytes(y, kk, kp));
@@ -868,7 +875,8 @@ __device__ __forceinline__ void load_packed_step(cplx v[8], const cplx *__restri
       const int q = g + u;
       if (T * q >= NC && T * q + T - 1 <= N - NC) { v[q] = mk(0.0, 0.0); continue; }
       const int k = b + T * q;
-      const bool lo = k < nc, hi = k > N - nc;
+      const bool all_lo = T * q + T - 1 < NC, all_hi = T * q > N - NC;
+      const bool lo = all_lo || (!all_hi && k < NC), hi = all_hi || (!all_lo && k > N - NC);
       // z = packed_z(A = i kx f, B) with its roundings spelled out: the
       // real part of A fused into its add, the imaginary part rounded on
       // its own where it is zeroed at k = 0 and k = n/2
@@ -876,7 +884,7 @@ __device__ __forceinline__ void load_packed_step(cplx v[8], const cplx *__restri
       const cplx f = fa[u], B = fb[u];
       cplx z;
       if (lo) {  // Z[k] = A + i B
-        const bool edge = k == 0 || k == N / 2;
+        const bool edge = (q == 0 && k == 0) || (T * q <= N / 2 && N / 2 < T * q + T && k == N / 2);
         const double ay = edge ? 0.0 : mul_rn(dk, f.x), by = edge ? 0.0 : B.y;
         z = mk(fma(-f.y, dk, -by), ay + B.x);
       } else {  // Z[n-k] = conj(A) + i conj(B)

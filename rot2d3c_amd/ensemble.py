@@ -76,7 +76,7 @@ class Ensemble:
             self.schedules[n] = (rw, bpc, cs)
         return rc
 
-    def tune(self, steps=10, trials=3):
+    def tune(self, steps=20, trials=5):
         """Choose every size's schedule by timing the whole ensemble step
         graph, where the sizes share the GPU (a solo plan's fastest schedule
         need not be the fastest there): one coordinate-descent sweep over the
