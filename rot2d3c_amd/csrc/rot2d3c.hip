@@ -203,15 +203,19 @@ __global__ __launch_bounds__(N / 8 * RowFull<N>::P) void k_row_c2r(const cplx *_
 
 // Unpack a packed forward row transform held in LDS (canonical k order) into
 // the half spectra of its two real rows, writing kx = k for k < ncols.
-template <int N, int IL>
+// KC: the column count ncols when it is known at compile time (n/3 + 1 for
+// the step's kept columns, n/2 + 1 for full rows), 0 otherwise; whole slots
+// past it are then skipped and whole slots below it stored without a test.
+template <int N, int IL, int KC = 0>
 __device__ __forceinline__ void unpack_store(const cplx *lds, int b, int ncols, int kp,
                                              cplx *__restrict__ mixa, int ya,
                                              cplx *__restrict__ mixb, int yb_) {
   constexpr int T = N / 8;
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
+    if (KC > 0 && T * q >= KC) continue;
     const int k = b + T * q;
-    if (k < ncols) {
+    if ((KC > 0 && T * q + T - 1 < KC) || k < ncols) {
       const cplx w = lds[pad(k) * IL], wm = lds[pad((N - k) & (N - 1)) * IL];
       const cplx A = mk(0.5 * (w.x + wm.x), 0.5 * (w.y - wm.y));
       const cplx B = mk(0.5 * (w.y + wm.y), -0.5 * (w.x - wm.x));
@@ -240,7 +244,7 @@ __global__ __launch_bounds__(N / 8 * RowFull<N>::P) void k_row_r2c(const double 
 #pragma unroll
   for (int q = 0; q < 8; ++q) lds[pad(b + T * q) * P + r] = v[q];
   __syncthreads();
-  unpack_store<N, P>(lds + r, b, g.ncf, g.kpf, mix, y, mix, y + 1);
+  unpack_store<N, P, N / 2 + 1>(lds + r, b, g.ncf, g.kpf, mix, y, mix, y + 1);
 }
 
 // rows per block of the step row kernel and the single-field full-range
@@ -960,7 +964,7 @@ __global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
   __syncthreads();
   int b7 = b;
   asm volatile("" : "+v"(b7));
-  unpack_store<N, RW>(lds + r, b7, nc, kp, Q, y, Q + fs, y);
+  unpack_store<N, RW, N / 3 + 1>(lds + r, b7, nc, kp, Q, y, Q + fs, y);
 }
 
 
@@ -1109,7 +1113,7 @@ __global__ __launch_bounds__((RowV2<N, RWQ>::THREADS)) void k_step_row2(
 #pragma unroll
     for (int q = 0; q < 8; ++q) P[pad(b + T * q) * RW] = v[q];
     __syncthreads();
-    unpack_store<N, RW>(P, b, nc, kp, Q, y, Q + fs, y);
+    unpack_store<N, RW, N / 3 + 1>(P, b, nc, kp, Q, y, Q + fs, y);
     if (tn >= NTILES) break;
     wait_vm();
     __syncthreads();
@@ -2918,7 +2922,7 @@ __global__ __launch_bounds__(N / 8) void k_cf_row(const float *__restrict__ z, G
 #pragma unroll
     for (int q = 0; q < 8; ++q) lds[pad(b + T * q)] = v[q];
     __syncthreads();
-    unpack_store<N, 1>(lds, b, g.ncf, g.kpf, mix + ch * fs, y, mix + ch * fs, y + 1);
+    unpack_store<N, 1, N / 2 + 1>(lds, b, g.ncf, g.kpf, mix + ch * fs, y, mix + ch * fs, y + 1);
     __syncthreads();
   }
 }
