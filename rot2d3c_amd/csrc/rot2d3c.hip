@@ -3088,10 +3088,25 @@ __global__ __launch_bounds__(kBigT) void k_big_col_inv(const cplx *__restrict__ 
 
 // rows y, y+1 of M (half spectra) -> R[y][:], R[y+1][:] as one packed
 // transform; columns kx >= ncols are taken as zero (not read)
+// Column range of packed slot (q, parity) of k_big_row_c2r over its
+// threads b = 0..kBigT-1: k = 2 (b + kBigT q) + parity, read at kk = k (k <=
+// n/2) or n - k.
+__host__ __device__ constexpr int big_kk_min(int q, int par) {
+  return q < 4 ? 2 * kBigT * q + par : kBigN - (2 * kBigT * q + 2 * kBigT - 2 + par);
+}
+__host__ __device__ constexpr int big_kk_max(int q, int par) {
+  return q < 4 ? 2 * kBigT * q + 2 * kBigT - 2 + par
+               : (q == 4 && par == 0 ? kBigN / 2 : kBigN - (2 * kBigT * q + par));
+}
+
+// KC: the number of columns read (n/3 + 1 for the step's dealiased fields,
+// n/2 + 1 for full transforms); slots wholly past it are neither loaded nor
+// tested, slots wholly below it are not tested.
+template <int KC>
 __global__ __launch_bounds__(kBigT) void k_big_row_c2r(const cplx *__restrict__ mix,
                                                         double *__restrict__ out, Geo g,
-                                                        const cplx *__restrict__ tw, int ncols) {
-  constexpr int N = kBigN;
+                                                        const cplx *__restrict__ tw) {
+  constexpr int N = kBigN, ncols = KC;
   __shared__ cplx lds[Shape<kBigH>::LDSN];
   // Consecutive workgroups go to the 8 XCDs in turn: the two row pairs of one
   // 4-row tile (which share every 64-byte chunk of the tiled layout) are
@@ -3108,18 +3123,22 @@ __global__ __launch_bounds__(kBigT) void k_big_row_c2r(const cplx *__restrict__ 
     cplx ra[4], rb[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int k = 2 * (b + kBigT * (q0 + (u >> 1))) + (u & 1);
+      const int q = q0 + (u >> 1), par = u & 1;
+      if (big_kk_min(q, par) >= ncols) continue;  // no thread reads this slot
+      const int k = 2 * (b + kBigT * q) + par;
       const int kk = k > N / 2 ? N - k : k;
-      const int kl = kk < ncols ? kk : 0;
+      const int kl = (big_kk_max(q, par) < ncols || kk < ncols) ? kk : 0;
       ra[u] = mix[tix(y, kl, g.kpf)];
       rb[u] = mix[tix(y + 1, kl, g.kpf)];
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int k = 2 * (b + kBigT * (q0 + (u >> 1))) + (u & 1);
+      const int q = q0 + (u >> 1), par = u & 1;
+      const int k = 2 * (b + kBigT * q) + par;
       const bool mir = k > N / 2;
       const int kk = mir ? N - k : k;
-      const cplx zz = kk < ncols ? packed_z<N>(k, ra[u], rb[u], mir) : mk(0.0, 0.0);
+      const bool in = big_kk_min(q, par) < ncols && (big_kk_max(q, par) < ncols || kk < ncols);
+      const cplx zz = in ? packed_z<N>(k, ra[u], rb[u], mir) : mk(0.0, 0.0);
       if (u & 1) od[q0 + (u >> 1)] = zz;
       else ev[q0 + (u >> 1)] = zz;
     }
@@ -3291,8 +3310,13 @@ __global__ void k_big_update(const cplx *ps, const cplx *vz, const cplx *c1, con
 
 // ncols < n/2 + 1: the input is zero in the columns kx >= ncols (dealiased)
 static int big_c2r(r2d3c_plan *p, const cplx *in, int op, double *out, hipStream_t s, int ncols) {
+  if (ncols != kBigN / 3 + 1 && ncols != kBigN / 2 + 1)
+    return set_err(R2D3C_EINVAL, "16384 c2r: column count n/3 + 1 or n/2 + 1");
   k_big_col_inv<<<ncols, kBigT, 0, s>>>(in, p->mfull, op, 1.0, p->g, p->tw);
-  k_big_row_c2r<<<kBigN / 2, kBigT, 0, s>>>(p->mfull, out, p->g, p->tw, ncols);
+  if (ncols == kBigN / 3 + 1)
+    k_big_row_c2r<kBigN / 3 + 1><<<kBigN / 2, kBigT, 0, s>>>(p->mfull, out, p->g, p->tw);
+  else
+    k_big_row_c2r<kBigN / 2 + 1><<<kBigN / 2, kBigT, 0, s>>>(p->mfull, out, p->g, p->tw);
   LAUNCHCHK("c2r 16384");
   return 0;
 }
