@@ -3086,8 +3086,6 @@ __global__ __launch_bounds__(kBigT) void k_big_col_inv(const cplx *__restrict__ 
   }
 }
 
-// rows y, y+1 of M (half spectra) -> R[y][:], R[y+1][:] as one packed
-// transform; columns kx >= ncols are taken as zero (not read)
 // Column range of packed slot (q, parity) of k_big_row_c2r over its
 // threads b = 0..kBigT-1: k = 2 (b + kBigT q) + parity, read at kk = k (k <=
 // n/2) or n - k.
@@ -3099,9 +3097,11 @@ __host__ __device__ constexpr int big_kk_max(int q, int par) {
                : (q == 4 && par == 0 ? kBigN / 2 : kBigN - (2 * kBigT * q + par));
 }
 
-// KC: the number of columns read (n/3 + 1 for the step's dealiased fields,
-// n/2 + 1 for full transforms); slots wholly past it are neither loaded nor
-// tested, slots wholly below it are not tested.
+// rows y, y+1 of M (half spectra) -> R[y][:], R[y+1][:] as one packed
+// transform; columns kx >= KC are taken as zero (not read).  KC: the number
+// of columns read (n/3 + 1 for the step's dealiased fields, n/2 + 1 for full
+// transforms); slots wholly past it are neither loaded nor tested, slots
+// wholly below it are not tested.
 template <int KC>
 __global__ __launch_bounds__(kBigT) void k_big_row_c2r(const cplx *__restrict__ mix,
                                                         double *__restrict__ out, Geo g,
