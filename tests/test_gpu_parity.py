@@ -145,7 +145,7 @@ def _run_both(n, steps, p, dtmode, seed=9, kup=4.0, cuda=None):
     return _host(tps), _host(tvz), ops, ovz, np.array(dts), np.array(odts)
 
 
-@pytest.mark.parametrize("n", SIZES)
+@pytest.mark.parametrize("n", SIZES + [512, 1024])
 def test_step_cfl_10_steps(n, cuda):
     print("10 CFL steps n=%d" % n, flush=True)
     p = O.Params(1e-3, 0.1, 1, 2, 2e-3, 0.05, 2, 1, 2.0)
