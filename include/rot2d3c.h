@@ -249,6 +249,15 @@ int r2d3c_plan_get_schedule(const r2d3c_plan *plan, int *row_rows, int *row_bloc
  * agree within rounding. */
 int r2d3c_plan_set_column_radix(r2d3c_plan *plan, int form);
 int r2d3c_plan_get_column_radix(const r2d3c_plan *plan, int *form);
+/* Split emission of the fused column kernel in stages 1-3 of r2d3c_step /
+ * r2d3c_run and of ensembles: each kx column's block is repeated for each
+ * emitted field (up to 4), every repeat doing the forward transform and the
+ * update and emitting one field, so a latency-bound launch (a small grid)
+ * finishes sooner.  -1 (default): split when the launch holds fewer threads
+ * than the GPU has SIMDs x 64; 0 never; 1 always.  Results are bit-identical
+ * either way. */
+int r2d3c_plan_set_column_split(r2d3c_plan *plan, int mode);
+int r2d3c_plan_get_column_split(const r2d3c_plan *plan, int *mode);
 /* Times every valid schedule (row kernel and launch shape x column radix
  * schedule) on scratch state (best of three trials of `reps` fixed-dt steps
  * each), keeps the fastest in the plan and returns its milliseconds per step (may be NULL).
@@ -279,6 +288,8 @@ int r2d3c_ensemble_destroy(r2d3c_ensemble *ens);
 int r2d3c_ensemble_set_schedule(r2d3c_ensemble *ens, int n, int row_rows, int row_blocks_per_cu);
 /* r2d3c_plan_set_column_radix for the runs of size n. */
 int r2d3c_ensemble_set_column_radix(r2d3c_ensemble *ens, int n, int form);
+/* r2d3c_plan_set_column_split for the runs of size n. */
+int r2d3c_ensemble_set_column_split(r2d3c_ensemble *ens, int n, int mode);
 /* Advances every run n_steps steps.  forcing: NULL, or an array of
  * n_steps*count forcing structs, step-major with the runs in creation order,
  * holding rand_force draws only; the table is uploaded before the steps, so the steps run without host

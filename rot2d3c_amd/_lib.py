@@ -80,6 +80,9 @@ SIGNATURES = {
     "r2d3c_ensemble_set_column_radix": (_c_int, [_vp, _c_int, _c_int]),
     "r2d3c_plan_set_column_radix": (_c_int, [_vp, _c_int]),
     "r2d3c_plan_get_column_radix": (_c_int, [_vp, ctypes.POINTER(_c_int)]),
+    "r2d3c_plan_set_column_split": (_c_int, [_vp, _c_int]),
+    "r2d3c_plan_get_column_split": (_c_int, [_vp, ctypes.POINTER(_c_int)]),
+    "r2d3c_ensemble_set_column_split": (_c_int, [_vp, _c_int, _c_int]),
     "r2d3c_ensemble_run": (_c_int, [_vp, ctypes.POINTER(Forcing), _c_int, _vp]),
     "r2d3c_ensemble_set_closure_period": (_c_int, [_vp, _c_int]),
     "r2d3c_ensemble_set_closure": (_c_int, [_vp, _c_int, _vp, ctypes.POINTER(_c_double),

@@ -1153,11 +1153,16 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
   constexpr int T = N / 8;
   __shared__ cplx lds[Shape<N>::LDSN];
   const int b = threadIdx.x;
+  // Split emission (gridDim.z > 1, never with FM_STATE): slice z of a column
+  // repeats the forward transform and the update and emits only field z, so
+  // a small grid's blocks each run two transforms instead of up to five.
+  const int slice = blockIdx.z, nslices = gridDim.z;
   const int nc = c.g.nc, ncp = (nc + 1) & ~1;
   int field, kx;
   if (lb < nc) { field = 0; kx = lb; }
   else if (lb >= ncp && lb < ncp + nc) { field = 1; kx = lb - ncp; }
   else if ((mode & FM_ZERO) && lb >= 2 * ncp && lb < 2 * ncp + (N / 2 + 1 - nc)) {
+    if (slice) return;
     kx = nc + (lb - 2 * ncp);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -1166,9 +1171,10 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
     }
     return;
   } else return;
+  if (nslices > 1 && slice >= (field == 0 ? 4 : 2)) return;  // no field left to emit
 
   const double dt = c.cfl_mode ? dt_from_max(c.dc, *mread) : c.dt_fixed;
-  if (lb == 0 && b == 0 && o == 4) {
+  if (lb == 0 && b == 0 && o == 4 && slice == 0) {
     if (dt_out) *dt_out = dt;
     if (mreset) *mreset = 0ull;
   }
@@ -1245,6 +1251,7 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
     const int nf = field == 0 ? 4 : 2;
     cplx *dst0 = M + (size_t)(field == 0 ? 0 : 4) * c.fs;
     for (int f = 0; f < nf; ++f) {
+      if (nslices > 1 && f != slice) continue;
       int bb = b;  // opaque thread index, as in k_step_colinv
       asm volatile("" : "+v"(bb));
       cplx w[8];
@@ -1349,6 +1356,9 @@ struct r2d3c_plan {
   // radix schedule of the column transforms for n >= 512: 0 the four-step
   // form, 1 radix-8 Stockham passes (fft_canon)
   int col_cs = 0;
+  // split emission of the fused column kernel's stages 1-3 (-1: when the
+  // launch holds fewer threads than one wave per SIMD; 0 off; 1 on)
+  int col_split = -1;
   // kernel timing of r2d3c_step (r2d3c_timing_enable / r2d3c_timing_read)
   int timing = 0;
   std::vector<hipEvent_t> evpool;
@@ -1499,6 +1509,14 @@ static int make_force_set(const r2d3c_force_draw *d, int n, ForceSet *f) {
 
 // Persistent row grid: bpc blocks per CU, a multiple of 8 G (the tile map),
 // at most one block per tile slot.
+// gridDim.z of a fused column launch of stages 1-3 holding `threads` threads:
+// 4 emission slices when split (a latency-bound launch), else 1
+static int col_slices(const r2d3c_plan *p, long long threads) {
+  if (p->col_split == 0) return 1;
+  if (p->col_split == 1) return 4;
+  return threads < (long long)p->cus * 256 ? 4 : 1;
+}
+
 template <int N, int RWQ> static int row2_grid(int cus, int bpc) {
   constexpr int G = RowV2<N, RWQ>::G, Q8 = 8 * G;
   int g = cus * bpc / Q8 * Q8;
@@ -1572,9 +1590,10 @@ static int step_seq(r2d3c_plan *p, StepConst &c, cplx *psi, cplx *vz, hipStream_
     const int nblk = round16(2 * ncp + ((mode & FM_ZERO) ? (N / 2 + 1 - nc) : 0));
     unsigned long long *mres = o == 4 ? mother : nullptr;
     ev_pair(p, o == 1 ? 3 : 2, &ea, &eb);
+    const int nz = (mode & FM_STATE) ? 1 : col_slices(p, (long long)nblk * (N / 8));
     hipExtLaunchKernelGGL((N >= 512 && p->col_cs) ? k_step_colfused<N, true>
                                                   : k_step_colfused<N, false>,
-                          dim3(nblk), dim3(N / 8), 0, s, ea, eb, 0,
+                          dim3(nblk, 1, nz), dim3(N / 8), 0, s, ea, eb, 0,
                           (const cplx *)Q, psi, vz, M, c, o, mode,
                           (const unsigned long long *)mb, mres, dt_out, tw);
   }
@@ -2163,6 +2182,19 @@ int r2d3c_plan_set_schedule(r2d3c_plan *p, int row_rows, int row_blocks_per_cu) 
   return 0;
 }
 
+int r2d3c_plan_set_column_split(r2d3c_plan *p, int mode) {
+  NEED(p);
+  if (mode < -1 || mode > 1) return set_err(R2D3C_EINVAL, "column split: -1, 0 or 1");
+  p->col_split = mode;
+  return 0;
+}
+
+int r2d3c_plan_get_column_split(const r2d3c_plan *p, int *mode) {
+  NEED(p); NEED(mode);
+  *mode = p->col_split;
+  return 0;
+}
+
 int r2d3c_plan_set_column_radix(r2d3c_plan *p, int form) {
   NEED(p);
   if (form != 0 && form != 1) return set_err(R2D3C_EINVAL, "column radix form: 0 or 1");
@@ -2352,8 +2384,9 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
     int mode = FM_EMIT;
     if (o == 1) mode = FM_STATE | (masked ? FM_DISK : FM_ZERO) | (chain ? FM_EMIT : 0);
     const int nblk = round16(2 * ncp + ((mode & FM_ZERO) ? (N / 2 + 1 - nc) : 0));
-    if (cs) k_ens_colfused<N, true><<<dim3(nblk, k.count), N / 8, 0, s>>>(a, o, mode, tw);
-    else k_ens_colfused<N, false><<<dim3(nblk, k.count), N / 8, 0, s>>>(a, o, mode, tw);
+    const int nz = (mode & FM_STATE) ? 1 : col_slices(p, (long long)nblk * k.count * (N / 8));
+    if (cs) k_ens_colfused<N, true><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(a, o, mode, tw);
+    else k_ens_colfused<N, false><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(a, o, mode, tw);
   }
   return 0;
 }
@@ -2542,6 +2575,17 @@ int r2d3c_ensemble_set_schedule(r2d3c_ensemble *e, int n, int row_rows, int row_
   for (auto &k : e->cls)
     if (k.n == n) {
       int rc = r2d3c_plan_set_schedule(k.plan, row_rows, row_blocks_per_cu);
+      if (!rc) e->graph_valid = false;
+      return rc;
+    }
+  return set_err(R2D3C_EINVAL, "no run of that size in the ensemble");
+}
+
+int r2d3c_ensemble_set_column_split(r2d3c_ensemble *e, int n, int mode) {
+  NEED(e);
+  for (auto &k : e->cls)
+    if (k.n == n) {
+      int rc = r2d3c_plan_set_column_split(k.plan, mode);
       if (!rc) e->graph_valid = false;
       return rc;
     }

@@ -247,6 +247,61 @@ def test_ensemble_tune_keeps_state_and_runs_match_solo(cuda):
     ens.close()
 
 
+@pytest.mark.parametrize("n", [64, 256, 1024, 2048])
+def test_column_split_is_bitwise(n, cuda):
+    """Split emission of the fused column kernel (each column's block
+    repeated per emitted field) gives the same bits as the unsplit kernel:
+    forced on, forced off and automatic, 4 CFL steps with random forcing
+    through r2d3c_run, dt included."""
+    print("column split n=%d" % n, flush=True)
+    from rot2d3c_amd import spectral as S, _lib, ic
+    from rot2d3c_amd._lib import check
+    ps0, vz0 = random_state(n, 81)
+    p = S.Params(1e-3, 0.1, 1, 2, 2e-3, 0.05, 2, 1, 2.0)
+    sched = ic.forcing_schedule(max(2.0, n / 16.0), 1.0, 1.0, 5, 4, S.rand_force_draw)
+    draws = [(S.ForceDraw(*a), S.ForceDraw(*b)) for a, b in sched]
+    pl = S.Plan.get(n, cuda)
+    assert _lib.lib().r2d3c_plan_set_column_split(pl.handle, 2) != 0
+    outs = []
+    try:
+        for mode in (0, 1, -1):
+            check(_lib.lib().r2d3c_plan_set_column_split(pl.handle, mode), "set_column_split")
+            got = ctypes.c_int(7)
+            check(_lib.lib().r2d3c_plan_get_column_split(pl.handle, ctypes.byref(got)), "get")
+            assert got.value == mode
+            ps, vz = torch.from_numpy(ps0).to(cuda), torch.from_numpy(vz0).to(cuda)
+            dts = torch.zeros(4, dtype=torch.float64, device=cuda)
+            S.rk4_run(ps, vz, p, draws, cfl=0.5, dt_out=dts)
+            torch.cuda.synchronize()
+            outs.append((ps, vz, dts))
+    finally:
+        check(_lib.lib().r2d3c_plan_set_column_split(pl.handle, -1), "set_column_split")
+    for ps, vz, dts in outs[1:]:
+        assert torch.equal(ps, outs[0][0]) and torch.equal(vz, outs[0][1])
+        assert torch.equal(dts, outs[0][2])
+
+
+def test_ensemble_column_split_is_bitwise(cuda):
+    """Ensemble runs with split emission forced on equal the unsplit ones."""
+    print("ensemble column split", flush=True)
+    from rot2d3c_amd import ensemble as E
+    from rot2d3c_amd._lib import check
+    spec = _spec([256, 512, 256])
+    res = []
+    for mode in (0, 1):
+        ms, host = _members(spec, cuda)
+        ens = E.Ensemble(ms, schedules=None)
+        for n in ens.sizes:
+            check(E._lib.lib().r2d3c_ensemble_set_column_split(ens.handle, n, mode), "split")
+        table = E.forcing_table(spec, 5)
+        ens.run(5, table)
+        torch.cuda.synchronize()
+        res.append([(m.ps.clone(), m.vz.clone()) for m in ms])
+        ens.close()
+    for (a, b), (c, d) in zip(res[0], res[1]):
+        assert torch.equal(a, c) and torch.equal(b, d)
+
+
 def test_autotune_picks_a_valid_schedule(cuda, tmp_path, monkeypatch):
     print("autotune n=512", flush=True)
     cache = tmp_path / "s.json"
