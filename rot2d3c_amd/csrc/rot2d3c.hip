@@ -809,8 +809,11 @@ __device__ __forceinline__ void colinv_body(const cplx *__restrict__ psi,
       const int ky = kyof(ilv<N, CS>(bb, q), N);
       const double ka2 = (double)kx * kx + (double)ky * ky;
       cplx a = s[q];
-      if (f >= 2) a = mk(-ka2 * a.x, -ka2 * a.y);
-      if (f & 1) a = mk(-(double)ky * a.y, (double)ky * a.x);
+      // products rounded on their own (never fused into the transform's
+      // first additions), so every kernel that emits these fields, here and
+      // in the fused column kernel, produces the same bits
+      if (f >= 2) a = mk(mul_rn(-ka2, a.x), mul_rn(-ka2, a.y));
+      if (f & 1) a = mk(mul_rn(-(double)ky, a.y), mul_rn((double)ky, a.x));
       v[q] = a;
     }
     fft_dit<N, +1, 1, CS>(v, lds, bb, tw);
@@ -1260,8 +1263,9 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
         const int ky = kyof(ilv<N, CS>(bb, q), N);
         const double ka2 = (double)kx * kx + (double)ky * ky;
         cplx a = s[q];
-        if (f >= 2) a = mk(-ka2 * a.x, -ka2 * a.y);
-        if (f & 1) a = mk(-(double)ky * a.y, (double)ky * a.x);
+        // products rounded on their own, as in colinv_body
+        if (f >= 2) a = mk(mul_rn(-ka2, a.x), mul_rn(-ka2, a.y));
+        if (f & 1) a = mk(mul_rn(-(double)ky, a.y), mul_rn((double)ky, a.x));
         w[q] = a;
       }
       fft_dit<N, +1, 1, CS>(w, lds, bb, tw);
