@@ -194,6 +194,25 @@ int r2d3c_run(r2d3c_plan *plan, const r2d3c_params *p, double *psi, double *vz,
               const r2d3c_forcing *forcing, int n_steps, int dt_mode, double dt_or_cfl,
               double *d_dt, void *stream);
 
+/* r2d3c_run without a plan and without any device allocation, for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * arguments and results as r2d3c_run on a plan of size n with the default
+ * schedules (the same kernels, bit for bit), except: 16 <= n <=
+ * R2D3C_NOALLOC_MAX_N, and iflow = 2 (ci_enabled) is rejected.  The
+ * workspace is a static device array of about 45 MB that every device the
+ * library runs on holds until the process ends.  The first call on a device
+ * looks the array up and copies the twiddle tables into it synchronously, so
+ * make it outside stream capture; later calls only launch kernels (a call
+ * whose n differs from the previous call's on that device also clears the
+ * fields it uses with a fill kernel).  All calls on one device share the
+ * array: calls that overlap in time on one device (different streams or
+ * host threads without ordering) race and give wrong results. */
+#define R2D3C_NOALLOC_MAX_N 1024
+int r2d3c_run_noalloc(int n, const r2d3c_params *p, double *psi, double *vz,
+                      const r2d3c_forcing *forcing, int n_steps, int dt_mode, double dt_or_cfl,
+                      double *d_dt, void *stream);
+
 /* Dense forcing arrays fp (kin = 1) and fz (kin = 0) of one draw for a step
  * of length dt (host value), as rand_force fills them
  * (src/pseudospec2D_Rot2D3C.f90:1100-1122).  Either output may be NULL. */

@@ -67,6 +67,8 @@ SIGNATURES = {
                             _c_int, _c_double, _vp, _vp]),
     "r2d3c_run": (_c_int, [_vp, ctypes.POINTER(Params), _vp, _vp, ctypes.POINTER(Forcing),
                            _c_int, _c_int, _c_double, _vp, _vp]),
+    "r2d3c_run_noalloc": (_c_int, [_c_int, ctypes.POINTER(Params), _vp, _vp,
+                                   ctypes.POINTER(Forcing), _c_int, _c_int, _c_double, _vp, _vp]),
     "r2d3c_forcing_fill": (_c_int, [_vp, ctypes.POINTER(Forcing), _c_double, _vp, _vp, _vp]),
     "r2d3c_timing_enable": (_c_int, [_vp, _c_int]),
     "r2d3c_timing_read": (_c_int, [_vp, ctypes.POINTER(_c_double), ctypes.POINTER(ctypes.c_longlong)]),

@@ -27,6 +27,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1730,6 +1731,89 @@ static int big_step(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
                     const r2d3c_forcing *fo, int dt_mode, double dt_or_cfl, double *d_dt,
                     hipStream_t s);
 
+// Geometry of an n x n plan: dealiasing ka2 <= kmax = (n/3)^2
+// (src/Rot2D3C.para.f90:213-214), the fused step's kept columns kx <= K and
+// the field pitches.  Nonzero if the cutoff is not n/3 (never for a power of
+// two; the row loaders rely on it).
+static int plan_geometry(r2d3c_plan *p, int n) {
+  const double kmax = ((double)n / 3.0) * ((double)n / 3.0);
+  int K = 0;
+  while ((double)(K + 1) * (K + 1) <= kmax) ++K;
+  p->g.n = n;
+  p->g.nc = K + 1;
+  if (p->g.nc != n / 3 + 1) return 1;
+  p->g.kp = (K + 2) & ~1;
+  p->g.ncf = n / 2 + 1;
+  p->g.kpf = n / 2 + 2;
+  p->g.kmax = kmax;
+  p->g.tiny = 0.000001;
+  p->fs = (size_t)n * p->g.kp;
+  p->fsf = (size_t)n * p->g.kpf;
+  return 0;
+}
+
+// Twiddles exp(-2 pi i t/n), t < n, from the extended-precision circle,
+// rounded once to double.
+static void twiddles_host(int n, cplx *tw) {
+  const long double twopi = 6.283185307179586476925286766559005768L;
+  for (int t = 0; t < n; ++t) {
+    const long double a = twopi * (long double)t / (long double)n;
+    tw[t] = make_double2((double)cosl(a), (double)(-sinl(a)));
+  }
+}
+
+// ---- r2d3c_run_noalloc: the fused step on a static device scratch array ----
+//
+// Layout of `scratch` (every offset a multiple of 256 bytes): the twiddle
+// tables of n = 16, 32, ..., kNaMaxN back to back; the CFL maximum slots and
+// the dt slot (512 bytes); the fused step's 6 mixed + 2 bracket fields of the
+// largest grid.  Each device holds its own copy for the life of the process.
+constexpr int kNaMaxN = R2D3C_NOALLOC_MAX_N;
+constexpr size_t kNaTwBytes = ((size_t)(2 * kNaMaxN - 16) * sizeof(cplx) + 255) & ~(size_t)255;
+constexpr size_t kNaFlagBytes = 512;
+constexpr size_t kNaMaxK = kNaMaxN / 3;  // K = n/3 for every power of two
+constexpr size_t kNaMstepBytes = 8 * sizeof(cplx) * kNaMaxN * ((kNaMaxK + 2) & ~(size_t)1);
+constexpr size_t kNaScratchBytes = kNaTwBytes + kNaFlagBytes + kNaMstepBytes;
+static __device__ __align__(256) unsigned char scratch[kNaScratchBytes];
+
+constexpr int kNaMaxDevices = 64;
+struct NaDevice {
+  unsigned char *ptr = nullptr;  // device address of this device's `scratch`
+  int cus = 0;
+  int last_n = 0;                // grid of the previous call (0: none yet)
+};
+static std::mutex g_na_mu;
+static NaDevice g_na[kNaMaxDevices];
+
+static size_t na_tw_offset(int n) {  // entries before n's table: 16 + ... + n/2
+  return (size_t)(n - 16);
+}
+
+__global__ void k_zero_cplx(cplx *a, size_t count) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (size_t)gridDim.x * blockDim.x)
+    a[i] = mk(0.0, 0.0);
+}
+
+// Set-up once per device, with that device current: the device address of
+// `scratch` (the host-side name is not a device address) and the twiddle
+// tables, copied synchronously.
+static int na_device(int dev, NaDevice **out) {
+  std::lock_guard<std::mutex> lock(g_na_mu);
+  NaDevice &d = g_na[dev];
+  if (!d.ptr) {
+    void *ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    HIPCHK(hipDeviceGetAttribute(&d.cus, hipDeviceAttributeMultiprocessorCount, dev));
+    std::vector<cplx> tw(2 * kNaMaxN - 16);
+    for (int n = 16; n <= kNaMaxN; n *= 2) twiddles_host(n, tw.data() + na_tw_offset(n));
+    HIPCHK(hipMemcpy(ptr, tw.data(), sizeof(cplx) * tw.size(), hipMemcpyHostToDevice));
+    d.ptr = (unsigned char *)ptr;
+  }
+  *out = &d;
+  return 0;
+}
+
 extern "C" {
 
 const char *r2d3c_last_error(void) { return g_err.c_str(); }
@@ -1743,27 +1827,9 @@ int r2d3c_plan_create(r2d3c_plan **out, int n) {
   p->n = n;
   HIPCHK(hipGetDevice(&p->device));
   HIPCHK(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device));
-  // dealiasing: ka2 <= kmax = (n/3)^2 (src/Rot2D3C.para.f90:213-214)
-  const double kmax = ((double)n / 3.0) * ((double)n / 3.0);
-  int K = 0;
-  while ((double)(K + 1) * (K + 1) <= kmax) ++K;
-  p->g.n = n;
-  p->g.nc = K + 1;  // = n/3 + 1 for every power of two (the row loaders rely on it)
-  if (p->g.nc != n / 3 + 1) { delete p; return set_err(R2D3C_EINVAL, "dealias cutoff"); }
-  p->g.kp = (K + 2) & ~1;
-  p->g.ncf = n / 2 + 1;
-  p->g.kpf = n / 2 + 2;
-  p->g.kmax = kmax;
-  p->g.tiny = 0.000001;
-  p->fs = (size_t)n * p->g.kp;
-  p->fsf = (size_t)n * p->g.kpf;
-  // twiddles from the extended-precision circle, rounded once to double
+  if (plan_geometry(p, n)) { delete p; return set_err(R2D3C_EINVAL, "dealias cutoff"); }
   std::vector<cplx> tw(n);
-  const long double twopi = 6.283185307179586476925286766559005768L;
-  for (int t = 0; t < n; ++t) {
-    const long double a = twopi * (long double)t / (long double)n;
-    tw[t] = make_double2((double)cosl(a), (double)(-sinl(a)));
-  }
+  twiddles_host(n, tw.data());
   // n = 16384 steps stage by stage: no fused-step workspace, a third real field
   const bool big = n == 16384;
   size_t b_tw = sizeof(cplx) * n, b_ms = big ? 256 : sizeof(cplx) * 8 * p->fs,
@@ -1990,6 +2056,63 @@ int r2d3c_run(r2d3c_plan *p, const r2d3c_params *q, double *psi, double *vz,
   }
   R2D_DISPATCH(p->n, run_steps, p, q, (cplx *)psi, (cplx *)vz, forcing, n_steps, dt_mode,
                dt_or_cfl, d_dt, (hipStream_t)stream);
+}
+
+int r2d3c_run_noalloc(int n, const r2d3c_params *q, double *psi, double *vz,
+                      const r2d3c_forcing *forcing, int n_steps, int dt_mode, double dt_or_cfl,
+                      double *d_dt, void *stream) {
+  NEED(psi); NEED(vz);
+  if (n < 16 || n > kNaMaxN || (n & (n - 1)))
+    return set_err(R2D3C_EINVAL, "r2d3c_run_noalloc: n must be a power of two in [16, " +
+                                     std::to_string(kNaMaxN) + "]");
+  int rc = check_params(q);
+  if (rc) return rc;
+  if (dt_mode != R2D3C_DT_CFL && dt_mode != R2D3C_DT_FIXED)
+    return set_err(R2D3C_EINVAL, "dt_mode must be R2D3C_DT_CFL or R2D3C_DT_FIXED");
+  if (psi == vz) return set_err(R2D3C_EINVAL, "psi and vz must not alias");
+  if (n_steps < 0) return set_err(R2D3C_EINVAL, "n_steps must be >= 0");
+  if (forcing)
+    for (int st = 0; st < n_steps; ++st)
+      if (forcing[st].ci_enabled)
+        return set_err(R2D3C_ENOTSUP, "r2d3c_run_noalloc: iflow = 2 needs a plan (r2d3c_run)");
+  if (n_steps == 0) return 0;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNaMaxDevices) return set_err(R2D3C_EINVAL, "device index out of range");
+  NaDevice *d = nullptr;
+  if ((rc = na_device(dev, &d))) return rc;
+  // a plan over the scratch array: the same kernels and launch shapes as a
+  // plan of r2d3c_plan_create with the default schedules
+  r2d3c_plan pl;
+  pl.n = n;
+  pl.device = dev;
+  pl.cus = d->cus;
+  if (plan_geometry(&pl, n)) return set_err(R2D3C_EINVAL, "dealias cutoff");
+  pl.tw = (cplx *)d->ptr + na_tw_offset(n);
+  pl.maxbits = (unsigned long long *)(d->ptr + kNaTwBytes);
+  pl.dtslot = (double *)(d->ptr + kNaTwBytes + 256);
+  pl.mstep = (cplx *)(d->ptr + kNaTwBytes + kNaFlagBytes);
+  hipStream_t s = (hipStream_t)stream;
+  {
+    // a new plan's fields start zeroed; after a call at another n the
+    // array holds that grid's layout, so clear the fields this grid uses
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    if (d->last_n != n) {
+      k_zero_cplx<<<1024, 256, 0, s>>>(pl.mstep, 8 * pl.fs);
+      LAUNCHCHK("r2d3c_run_noalloc clear");
+      d->last_n = n;
+    }
+  }
+  switch (n) {
+    case 16: return run_steps<16>(&pl, q, (cplx *)psi, (cplx *)vz, forcing, n_steps, dt_mode, dt_or_cfl, d_dt, s);
+    case 32: return run_steps<32>(&pl, q, (cplx *)psi, (cplx *)vz, forcing, n_steps, dt_mode, dt_or_cfl, d_dt, s);
+    case 64: return run_steps<64>(&pl, q, (cplx *)psi, (cplx *)vz, forcing, n_steps, dt_mode, dt_or_cfl, d_dt, s);
+    case 128: return run_steps<128>(&pl, q, (cplx *)psi, (cplx *)vz, forcing, n_steps, dt_mode, dt_or_cfl, d_dt, s);
+    case 256: return run_steps<256>(&pl, q, (cplx *)psi, (cplx *)vz, forcing, n_steps, dt_mode, dt_or_cfl, d_dt, s);
+    case 512: return run_steps<512>(&pl, q, (cplx *)psi, (cplx *)vz, forcing, n_steps, dt_mode, dt_or_cfl, d_dt, s);
+    case 1024: return run_steps<1024>(&pl, q, (cplx *)psi, (cplx *)vz, forcing, n_steps, dt_mode, dt_or_cfl, d_dt, s);
+    default: return set_err(R2D3C_ENOTSUP, "unsupported grid size");
+  }
 }
 
 int r2d3c_forcing_fill(r2d3c_plan *p, const r2d3c_forcing *fo, double dt, double *fp, double *fz,
