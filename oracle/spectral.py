@@ -42,14 +42,29 @@ Pinning (checks that do not restate the formula; see tests/test_oracle_*.py):
                             (to 1e-3 with steady forcing, statistically with
                             random forcing) as the reference's BalanceTest
                             script checks them; max_min per slab by hand
+  initial_condition ....... the cosine sums evaluated point by point from the
+                            Park-Miller phases, normalised by real-space
+                            Parseval means (tests/test_oracle_loop_pins.py)
+  static_forcing .......... real-space closed form (fp0/kup)(sin kup x +
+                            sin kup y), fz0 (cos kup x + sin kup y)
   reference_loop /
-  run_random_forcing ...... the forcing lag of cond_check is pinned by the
-                            random-forcing budget closure (a mutation test shows
-                            it); the rest of the statement order (outputs,
-                            time += dt, counters of a fresh start and of a
-                            restart) is parity unpinned: only a run of the
-                            reference would pin it, and the reference cannot be
-                            built here (it needs FFTW 2.x, absent from the image)
+  run_random_forcing ...... closed-form runs: a fresh start that only decays
+                            (u0 = 0, Omega = 0: constant dt, backward-Euler
+                            factor per mode), a restart from one rotating
+                            Fourier mode (each step the fourth-order Taylor
+                            polynomial of exp(dt L), CFL dt from the exact
+                            state) and an iflow = 1 start growing the forcing
+                            pattern; they fix the output schedule, output
+                            numbers, times written, restart counters and
+                            iteration count (mutation tests plant slips in
+                            each); the forcing lag of cond_check is pinned by
+                            the random-forcing budget closure.  Parity
+                            unpinned: the two pre-loop rand_force draws that
+                            only advance the seeds (:376-380) -- any offset of
+                            the forcing sequence is equally valid
+                            statistically, and only a run of the reference
+                            would fix it (the reference cannot be built here:
+                            it needs FFTW 2.x, absent from the image)
 """
 import math
 
@@ -529,6 +544,28 @@ def initial_condition(n, kup, u0, v0, seed):
 
 
 # --------------------------------------------------------------------------
+# static forcing of iflow = 1: src/Rot2D3C.para.f90:343-371
+# --------------------------------------------------------------------------
+def static_forcing(n, kup, fp0, fz0):
+    """fp from sin(2 kup pi x/n) + sin(2 kup pi y/n) normalised to
+    energy(fp,1) = fp0**2, fz from cos(2 kup pi x/n) + sin(2 kup pi y/n)
+    normalised to energy(fz,0) = fz0**2, both then kept only on
+    tiny <= ka2 <= kmax (x = dble(i)-1 along the fast index)."""
+    g = grid(n)
+    x = np.arange(n, dtype=np.float64)
+    sx = np.sin(2 * kup * PI * x / float(n))
+    cx = np.cos(2 * kup * PI * x / float(n))
+    R1 = sx[None, :] + sx[:, None]   # [y][x]: sin(..x..) + sin(..y..)
+    R2 = cx[None, :] + sx[:, None]
+    fp = fftp2d_real_to_complex(R1)
+    fz = fftp2d_real_to_complex(R2)
+    tmp = fp0 / math.sqrt(energy(fp, 1))
+    tmp1 = fz0 / math.sqrt(energy(fz, 0))
+    keep = (g.ka2 <= g.kmax) & (g.ka2 >= g.tiny)
+    return np.where(keep, tmp * fp, 0.0), np.where(keep, tmp1 * fz, 0.0)
+
+
+# --------------------------------------------------------------------------
 # the main loop for iflow = 3 (src/Rot2D3C.para.f90:376-415,532-611)
 # --------------------------------------------------------------------------
 def run_random_forcing(ps, vz, steps, cfl, p, kdn, kup, fp0, fz0, seed,
@@ -567,10 +604,10 @@ def run_random_forcing(ps, vz, steps, cfl, p, kdn, kup, fp0, fz0, seed,
 
 # --------------------------------------------------------------------------
 # the main loop with its output cadence (src/Rot2D3C.para.f90:245-250,
-# 300-308, 376-615); parity unpinned beyond the statement order read there
+# 300-308, 343-381, 393-615)
 # --------------------------------------------------------------------------
 def reference_loop(n, prm, stat=0, time=0.0, ps=None, vz=None):
-    """Run the reference loop for iflow = 3 (or 0) and return its events:
+    """Run the reference loop for iflow = 0, 1 or 3 and return its events:
     balance rows (every cstep), spectra (every sstep) and field dumps (every
     tstep), each with the output index and time the reference writes.
     prm: dict with the parameter.inp names.  A fresh start (stat = 0) builds
@@ -592,6 +629,8 @@ def reference_loop(n, prm, stat=0, time=0.0, ps=None, vz=None):
     fp, fz = zero, zero
     seed1 = seed + 1
     cfl = prm["CFL"]
+    if prm["iflow"] == 1:  # (:343-371)
+        fp, fz = static_forcing(n, prm["kup"], prm["fp0"], prm["fz0"])
     if prm["iflow"] == 3:
         dt = CFL_condition(cfl, ps, vz, p.nu, p.nn, p.nuv, p.nnv, p.omega)
         fp, seed = rand_force(prm["kdn"], prm["kup"], prm["fp0"], dt, seed, 1, n)

@@ -321,3 +321,71 @@ def test_largest_fused_size_4096_one_step(cuda):
                                         fixed_dt=dt)
     assert rel_l2(_host(tps), ops) < TOL
     assert rel_l2(_host(tvz), ovz) < TOL
+
+
+def test_bench_workload_2048_10_cfl_steps(cuda):
+    """The 2048^2 workload with the reference's CFL-adaptive dt (CFL = 0.27
+    gives dt near 2e-4), 10 chained steps through r2d3c_run against the
+    oracle's iflow = 3 loop: every step's dt to 1e-12, fields to 1e-12."""
+    print("bench workload 2048, 10 CFL steps", flush=True)
+    import torch
+    from rot2d3c_amd import ic
+    S = _S()
+    n, steps, cfl, kf, seed = 2048, 10, 0.27, 40.0, 0
+    p = O.Params(5e-5, 0.5, 1, 2, 5e-5, 0.5, 1, 2, 5.0)
+    ps, vz = ic.random_phase_ic(n, seed=1234)
+    sched = ic.forcing_schedule(kf, 1.0, 1.0, seed, steps, S.rand_force_draw)
+    draws = [(S.ForceDraw(*a), S.ForceDraw(*b)) for a, b in sched]
+    tps, tvz = _dev(ps, cuda), _dev(vz, cuda)
+    dts = torch.zeros(steps, dtype=torch.float64, device=cuda)
+    sp = S.Params(p.nu, p.hnu, p.nn, p.mm, p.nuv, p.hnuv, p.nnv, p.mmv, p.omega)
+    S.rk4_run(tps, tvz, sp, draws, cfl=cfl, dt_out=dts)
+    torch.cuda.synchronize()
+    O.set_fft_workers(8)  # bitwise the same transforms, threaded
+    try:
+        ops, ovz, _, odts, _, _ = O.run_random_forcing(ps.copy(), vz.copy(), steps, cfl, p, kf,
+                                                       kf, 1.0, 1.0, seed)
+    finally:
+        O.set_fft_workers(None)
+    gdt = _host(dts)
+    assert np.max(np.abs(gdt - np.array(odts)) / np.array(odts)) < 1e-12
+    assert rel_l2(_host(tps), ops) < TOL
+    assert rel_l2(_host(tvz), ovz) < TOL
+
+
+@pytest.mark.parametrize("n,mode", [(32, "cfl"), (256, 1e-3), (1024, "cfl")])
+def test_step_dense_forcing_matches_oracle(n, mode, cuda):
+    """The dense forcing arrays of r2d3c_step (iflow = 1 and the closure's
+    forcing), together with a rand_force draw, against the oracle's stages
+    with fp = dense + the draw's array: three chained steps.  The dense
+    arrays are not band-limited, so the update mask's handling of them is
+    exercised too."""
+    print("dense forcing n=%d mode=%s" % (n, mode), flush=True)
+    import torch
+    S = _S()
+    rng = np.random.default_rng(71)
+    ps, vz = random_state(n, 72)
+    fpd = O.fftp2d_real_to_complex(rng.standard_normal((n, n))) * (3.0 / n)
+    fzd = O.fftp2d_real_to_complex(rng.standard_normal((n, n))) * (2.0 / n)
+    p = O.Params(1e-3, 0.1, 1, 2, 2e-3, 0.05, 2, 1, 1.5)
+    sp = S.Params(p.nu, p.hnu, p.nn, p.mm, p.nuv, p.hnuv, p.nnv, p.mmv, p.omega)
+    tps, tvz = _dev(ps, cuda), _dev(vz, cuda)
+    tfp, tfz = _dev(fpd, cuda), _dev(fzd, cuda)
+    dto = torch.zeros(1, dtype=torch.float64, device=cuda)
+    kup, seed = n / 8.0, 19
+    ops, ovz = ps.copy(), vz.copy()
+    for _ in range(3):
+        if mode == "cfl":
+            dt = O.CFL_condition(0.5, ops, ovz, p.nu, p.nn, p.nuv, p.nnv, p.omega)
+        else:
+            dt = mode
+        kx, ky, ph, s_next = S.rand_force_draw(kup, seed)
+        fdraw, _ = O.rand_force(kup, kup, 0.7, dt, seed, 1, n)
+        seed = s_next
+        ops, ovz = O.rk_stages(ops, ovz, fpd + fdraw, fzd, dt, p)
+        kw = dict(cfl=0.5) if mode == "cfl" else dict(dt=mode)
+        S.rk4_step(tps, tvz, sp, S.ForceDraw(kx, ky, ph, 0.7, kup), None, dt_out=dto,
+                   dense=(tfp, tfz), **kw)
+        assert abs(float(dto.item()) - dt) <= 1e-12 * dt
+    assert rel_l2(_host(tps), ops) < TOL
+    assert rel_l2(_host(tvz), ovz) < TOL

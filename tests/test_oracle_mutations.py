@@ -1,7 +1,9 @@
 """The oracle pins have teeth: plant plausible mistakes in a copy of the
 oracle (on the CPU, in a temporary directory) and check that
-tests/test_oracle_pins.py fails for each one.  The last two are caught only
-by the budget-closure pins."""
+tests/test_oracle_pins.py (operators, scheme, diagnostics; the last two
+mistakes are caught only by its budget-closure pins) or
+tests/test_oracle_loop_pins.py (main loop, initial condition, iflow = 1
+forcing) fails for each one."""
 import os
 import shutil
 import subprocess
@@ -30,19 +32,51 @@ MUTATIONS = {
 }
 
 
-@pytest.mark.parametrize("name", sorted(MUTATIONS))
-def test_planted_mistake_is_caught(name, tmp_path):
-    before, after = MUTATIONS[name]
+LOOP_MUTATIONS = {
+    "fresh-start output counter": (
+        'ini, timet, timec, times = 1, prm["tstep"], prm["cstep"], prm["sstep"]',
+        'ini, timet, timec, times = 1, 0, prm["cstep"], prm["sstep"]'),
+    "restart first iteration": ('        ini = (stat - 1) * prm["tstep"]',
+                                '        ini = stat * prm["tstep"]'),
+    "restart spectrum index": ('spec_idx = int(float(ini) / float(prm["sstep"]) + 1)',
+                               'spec_idx = int(float(ini) / float(prm["sstep"]))'),
+    "IC phase pairing": ("cx1.append(np.cos(ax + ph[0]))", "cx1.append(np.cos(ax + ph[1]))"),
+    "IC axes": ("R1 = np.array(cy1).T @ np.array(cx1)  # [y][x]",
+                "R1 = np.array(cx1).T @ np.array(cy1)  # [y][x]"),
+    "IC normalisation": ("    ener = energy(ps, 1)\n    enerv", "    ener = energy(ps, 0)\n    enerv"),
+    "static forcing pattern": ("R2 = cx[None, :] + sx[:, None]", "R2 = sx[None, :] + cx[:, None]"),
+    "CFL dt after the outputs": (
+        '    for _ in range(ini, prm["step"] + 1):\n        dt = CFL_condition(cfl, ps, vz, p.nu, p.nn, p.nuv, p.nnv, p.omega)',
+        '    dt = CFL_condition(cfl, ps, vz, p.nu, p.nn, p.nuv, p.nnv, p.omega)\n'
+        '    for _ in range(ini, prm["step"] + 1):'),
+}
+
+
+def _run_mutated(tmp_path, before, after, test_files):
     src = open(os.path.join(ROOT, "oracle", "spectral.py")).read()
-    assert before in src
+    assert before in src, before
     shutil.copytree(os.path.join(ROOT, "oracle"), tmp_path / "oracle")
     os.makedirs(tmp_path / "rot2d3c_amd")
     for f in ("__init__.py", "_lib.py", "balance.py"):  # the budget restatement
         shutil.copy(os.path.join(ROOT, "rot2d3c_amd", f), tmp_path / "rot2d3c_amd" / f)
     os.makedirs(tmp_path / "tests")
-    for f in ("conftest.py", "test_oracle_pins.py"):
+    for f in ("conftest.py", "closed_forms.py") + tuple(test_files):
         shutil.copy(os.path.join(ROOT, "tests", f), tmp_path / "tests" / f)
     (tmp_path / "oracle" / "spectral.py").write_text(src.replace(before, after))
-    r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider",
-                        "tests/test_oracle_pins.py"], cwd=tmp_path, capture_output=True, text=True)
+    return subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider",
+                           *["tests/" + f for f in test_files]], cwd=tmp_path,
+                          capture_output=True, text=True)
+
+
+@pytest.mark.parametrize("name", sorted(MUTATIONS))
+def test_planted_mistake_is_caught(name, tmp_path):
+    before, after = MUTATIONS[name]
+    r = _run_mutated(tmp_path, before, after, ["test_oracle_pins.py"])
+    assert r.returncode != 0, "mutation '%s' not caught:\n%s" % (name, r.stdout[-2000:])
+
+
+@pytest.mark.parametrize("name", sorted(LOOP_MUTATIONS))
+def test_planted_loop_mistake_is_caught(name, tmp_path):
+    before, after = LOOP_MUTATIONS[name]
+    r = _run_mutated(tmp_path, before, after, ["test_oracle_loop_pins.py"])
     assert r.returncode != 0, "mutation '%s' not caught:\n%s" % (name, r.stdout[-2000:])
