@@ -314,3 +314,60 @@ def test_autotune_picks_a_valid_schedule(cuda, tmp_path, monkeypatch):
     assert sch[:2] in ((2, 0), (2, 1), (2, 2), (1, 1), (1, 2)) and sch[2] in (0, 1)
     assert autotune.schedule(512, cuda) == sch  # from the cache
     assert json.loads(cache.read_text())[key]["schedule"] == list(sch)
+
+
+def _set_plan_schedule(n, sch, cuda):
+    from rot2d3c_amd import spectral as S, _lib
+    from rot2d3c_amd._lib import check
+    pl = S.Plan.get(n, cuda)
+    check(_lib.lib().r2d3c_plan_set_schedule(pl.handle, sch[0], sch[1]), "set_schedule")
+    check(_lib.lib().r2d3c_plan_set_column_radix(pl.handle, sch[2]), "set_column_radix")
+
+
+@pytest.mark.parametrize("schedules", [None, {256: (2, 0, 0), 512: (2, 0, 1), 1024: (2, 0, 1)}],
+                         ids=["default", "stockham-columns"])
+def test_config3_mix_runs_equal_solo_runs_and_oracle(schedules, cuda):
+    """The batched workload itself: 24 x 256^2 + 12 x 512^2 + 4 x 1024^2 runs
+    (own seed, Omega ~ U[0,10], nu = 1e-3*256/n, kf = n/32, fixed dt), 3 steps
+    in one ensemble, with the default schedules and with the Stockham column
+    form at 512 and 1024 (the schedules the ensemble tuner picks).  Every run
+    equals its solo r2d3c_run under the same schedule bit for bit, and one run
+    of each size (all four 1024^2 runs) matches the oracle's iflow = 3 loop
+    at 1e-12."""
+    from rot2d3c_amd import ensemble as E, spectral as S
+    from rot2d3c_amd.ic import random_phase_ic
+    nsteps = 3
+    spec = E.mixed_ensemble_spec()
+    print("config-3 mix, %d runs, %d steps, schedules=%s" % (len(spec), nsteps, schedules),
+          flush=True)
+    ens = E.build(spec, cuda, schedules=schedules)
+    table = E.forcing_table(spec, nsteps)
+    ens.run(nsteps, table)
+    torch.cuda.synchronize()
+    assert sorted(ens.sizes) == [256, 512, 1024]
+    try:
+        for n in ens.sizes:
+            _set_plan_schedule(n, ens.schedules[n], cuda)
+        for r, (m, sp) in enumerate(zip(ens.members, spec)):
+            ps, vz = random_phase_ic(sp["n"], sp["seed"])
+            sps, svz = _solo(m, (ps, vz), [table[st][r] for st in range(nsteps)], nsteps, cuda)
+            assert torch.equal(m.ps, sps) and torch.equal(m.vz, svz), "run %d (n=%d)" % (r, m.n)
+    finally:
+        for n in ens.sizes:
+            _set_plan_schedule(n, (2, 0, 0), cuda)
+    checked = [0, 24] + [r for r, sp in enumerate(spec) if sp["n"] == 1024]
+    O.set_fft_workers(8)
+    try:
+        for r in checked:
+            sp, m = spec[r], ens.members[r]
+            print("  oracle run %d n=%d" % (r, sp["n"]), flush=True)
+            q = m.params
+            p = O.Params(q.nu, q.hnu, q.nn, q.mm, q.nuv, q.hnuv, q.nnv, q.mmv, q.omega)
+            ps, vz = random_phase_ic(sp["n"], sp["seed"])
+            ops, ovz, *_ = O.run_random_forcing(ps, vz, nsteps, 0.0, p, sp["kf"], sp["kf"],
+                                                sp["f0"], sp["f0"], sp["seed"], fixed_dt=m.dt)
+            assert rel_l2(m.ps.cpu().numpy(), ops) < 1e-12, r
+            assert rel_l2(m.vz.cpu().numpy(), ovz) < 1e-12, r
+    finally:
+        O.set_fft_workers(None)
+    ens.close()
