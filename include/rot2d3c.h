@@ -376,6 +376,15 @@ int r2d3c_cnn_set_output_clamp(r2d3c_cnn *net, double clamp);
  * production of vorticity^2 and vz^2.  Gradients and training steps are
  * unaffected. */
 int r2d3c_cnn_set_sign_clip(r2d3c_cnn *net, double strength);
+/* Gain per shell of the closure forcing made from this network's prediction
+ * (r2d3c_closure_net_forcing, and the ensemble's closure runs): the forcing at
+ * a mode of shell int(|k| + 0.5) < nshells is multiplied by gain[shell]
+ * (0 <= gain <= 16); other shells keep gain 1.  gain = NULL or nshells = 0
+ * removes the table (the default).  nshells <= max(height, width)/2 + 2.
+ * The first call allocates the table's device memory and the copy is
+ * synchronous; set it before r2d3c_ensemble_set_closure, whose step graph
+ * keeps the table size it was captured with. */
+int r2d3c_cnn_set_forcing_gain(r2d3c_cnn *net, const double *gain, int nshells);
 /* Forward, MSE loss (mean over pixels and both output channels, in d_loss),
  * backward; the parameter gradient goes to d_grad when it is not NULL. */
 int r2d3c_cnn_gradients(r2d3c_cnn *net, const float *d_in, const float *d_target, int batch,

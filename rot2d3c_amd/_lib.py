@@ -109,6 +109,7 @@ SIGNATURES = {
                                  ctypes.POINTER(_c_int)]),
     "r2d3c_cnn_forward_buffers": (_c_int, [_vp, _c_int, _vp]),
     "r2d3c_cnn_set_output_clamp": (_c_int, [_vp, _c_double]),
+    "r2d3c_cnn_set_forcing_gain": (_c_int, [_vp, ctypes.POINTER(_c_double), _c_int]),
     "r2d3c_cnn_set_sign_clip": (_c_int, [_vp, _c_double]),
     "r2d3c_closure_net_inputs": (_c_int, [_vp, _vp, _vp, ctypes.POINTER(_c_double), _vp, _vp]),
     "r2d3c_closure_net_forcing": (_c_int, [_vp, _vp, ctypes.POINTER(_c_double), _c_double, _vp,

@@ -114,6 +114,18 @@ class Net:
                                         self._stream()), "r2d3c_cnn_forward")
         return out
 
+    def set_forcing_gain(self, gain=None):
+        """Gain per shell int(|k| + 0.5) of the closure forcing made from this
+        network (array indexed by shell; None: no gain).  Set it before the
+        network is attached to an ensemble run."""
+        if gain is None:
+            check(_lib_().r2d3c_cnn_set_forcing_gain(self.handle, None, 0),
+                  "r2d3c_cnn_set_forcing_gain")
+            return
+        g = np.asarray(gain, dtype=np.float64)
+        check(_lib_().r2d3c_cnn_set_forcing_gain(self.handle, _dvec(g), len(g)),
+              "r2d3c_cnn_set_forcing_gain")
+
     def set_output_clamp(self, clamp):
         """Saturate inference outputs to [-clamp, clamp] (0: none); training
         and gradients are unaffected."""
@@ -245,6 +257,15 @@ def clip_forcing(ps, vz, fp, fz, kclip=1.0):
 
 
 # --- training ----------------------------------------------------------------
+def taper_gain(kmin, ktaper, nshells):
+    """Closure forcing gain rising linearly from 0 at shell kmin to 1 at
+    shell ktaper (a hard cutoff at kmin when ktaper <= kmin)."""
+    k = np.arange(nshells, dtype=np.float64)
+    if ktaper <= kmin:
+        return np.where(k >= kmin, 1.0, 0.0)
+    return np.clip((k - kmin) / (ktaper - kmin), 0.0, 1.0)
+
+
 def channel_std(x):
     """Per-channel standard deviation of [N, h, w, C] (float64 host values)."""
     flat = x.reshape(-1, x.shape[-1]).double()

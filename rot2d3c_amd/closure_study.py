@@ -9,7 +9,11 @@
 2. Training of the closure network on the GPU (90% / 10% split).
 3. Coarse runs (default 256^2, same parameters, fixed dt = dt nf/nc: the
    DNS's CFL number), each as several forcing realisations averaged,
-   started from the coarse-grained DNS state at the start of a window.  On
+   started from the coarse-grained DNS state at the start of a window.  The
+   DNS of each tuning or evaluation window is as many realisations from the
+   window's start, forced by the same draws as the coarse runs (the first
+   continues into the next window); the judge compares the means and takes
+   the realisations' spread as the noise floor per shell.  On
    the tuning window each candidate low-k cutoff of the closure forcing is
    scored; on the evaluation window the plain coarse solver and the closure
    run with the chosen cutoff are judged by their time-averaged energy
@@ -115,7 +119,7 @@ def timed_steps(ps0, vz0, p, dt, forcing, steps, net=None, sin=None, sout=None, 
 def coarse_runs(ps0, vz0, p, t_end, dt, seeds, kf, f0, params=None, sin=None, sout=None,
                 kmin=0.0, samples=200, energy_bound=None, chunk=50, kclip=0.0,
                 enstrophy_bound=None, trace=None, clamp=0.0, cfl_bound=None, sign_clip=0.0,
-                safe=None, period=1):
+                safe=None, period=1, gain=None):
     """len(seeds) coarse runs from (ps0, vz0), one coarse-grained forcing
     realisation each (the filtered equation's forcing G f),
     advanced together as one ensemble to simulated time t_end with fixed dt;
@@ -126,7 +130,9 @@ def coarse_runs(ps0, vz0, p, t_end, dt, seeds, kf, f0, params=None, sin=None, so
     cfl_bound times the CFL condition's dt, restarts the chunk from its last
     good state as the plain solver (the fallback).  The networks' outputs
     are saturated at +-clamp (0: none) and sign-clipped with strength
-    sign_clip (closure.Net.set_sign_clip).  safe = (cfl_on, cfl_off): a
+    sign_clip (closure.Net.set_sign_clip), and the closure forcing is scaled
+    per shell by `gain` (closure.Net.set_forcing_gain; None: none).  safe =
+    (cfl_on, cfl_off): a
     closure run whose fixed dt's CFL number exceeds cfl_on continues with the
     fully sign-clipped (damping only) network until it falls below cfl_off
     (checked every chunk; coarse_runs.safe_fraction is the fraction of
@@ -145,12 +151,13 @@ def coarse_runs(ps0, vz0, p, t_end, dt, seeds, kf, f0, params=None, sin=None, so
             nets.append(L.Net(1, S._spec_n(ps0), S._spec_n(ps0), ps0.device, params=params))
             nets[-1].set_output_clamp(clamp)
             nets[-1].set_sign_clip(sign_clip)
+            nets[-1].set_forcing_gain(gain)
             ens.set_closure(r, nets[-1], sin, sout, kmin, kclip)
         ens.set_closure_period(period)
     forcings = [Forcing(kf, f0, sd, S._spec_n(ps0)) for sd in seeds]
     nsteps = int(round(t_end / dt))
     every = max(1, nsteps // samples)
-    acc, nsamp, steps, finite, fallbacks, cfl_max = None, 0, 0, True, 0, 0.0
+    accs, nsamp, steps, finite, fallbacks, cfl_max = [None] * R, 0, 0, True, 0, 0.0
     good = [(a.clone(), b.clone()) for a, b in runs]
     on = [params is not None] * R
     sm = None if safe is None else L.SafeMode(safe[0], safe[1], sign_clip)
@@ -198,12 +205,15 @@ def coarse_runs(ps0, vz0, p, t_end, dt, seeds, kf, f0, params=None, sin=None, so
                 ens.set_closure(r, nets[r], sin, sout, kmin, kclip)  # recaptures the step graph
         steps += c
         if steps // every > (steps - c) // every:
-            for a, b in runs:
+            for r, (a, b) in enumerate(runs):
                 s_ = spectra(a, b)
-                acc = [np.array(x) for x in s_] if acc is None else [u + v for u, v in zip(acc, s_)]
-                nsamp += 1
+                accs[r] = ([np.array(x) for x in s_] if accs[r] is None
+                           else [u + v for u, v in zip(accs[r], s_)])
+            nsamp += 1
     ens.close()
-    stats = None if acc is None else [u / nsamp for u in acc]
+    per_run = None if accs[0] is None else [[u / nsamp for u in acc] for acc in accs]
+    stats = None if per_run is None else [sum(x) / R for x in zip(*per_run)]
+    coarse_runs.per_run = per_run  # each run's time-averaged statistics
     coarse_runs.cfl_max = cfl_max  # largest CFL number of a closure run at a check
     coarse_runs.safe_fraction = 0.0 if sm is None else sm.fraction
     return stats, steps, finite, fallbacks
@@ -220,6 +230,146 @@ def dns_window(advance, ps, vz, nc, steps, samples=200):
         acc = [np.array(x) for x in s_] if acc is None else [u + v for u, v in zip(acc, s_)]
         nsamp += 1
     return [u / nsamp for u in acc]
+
+
+def dns_realisations(ps, vz, p, dt, nc, steps, seeds, kf, f0, samples=200):
+    """From the state (ps, vz), one DNS run per forcing seed over `steps`
+    steps; run r is forced by the draws of seeds[r], the same draws (not
+    coarse-grained) as coarse run r of coarse_runs.  Returns each run's
+    time-averaged coarse-grained statistics (dns_window) and leaves ps, vz at
+    the end state of the first run."""
+    start = (ps.clone(), vz.clone())
+    out, end0 = [], None
+    for r, sd in enumerate(seeds):
+        ps.copy_(start[0])
+        vz.copy_(start[1])
+        f = Forcing(kf, f0, sd)
+
+        def advance(k):
+            done = 0
+            while done < k:
+                c = min(500, k - done)
+                S.rk4_run(ps, vz, p, f.take(c), c, dt=dt)
+                done += c
+
+        out.append(dns_window(advance, ps, vz, nc, steps, samples))
+        if r == 0:
+            end0 = (ps.clone(), vz.clone())
+    ps.copy_(end0[0])
+    vz.copy_(end0[1])
+    return out
+
+
+def mean_stats(runs):
+    return [sum(x) / len(runs) for x in zip(*runs)]
+
+
+def judge(les_runs, dns_runs, kmax, tol=0.05):
+    """The closure's judge: the time-averaged energy spectrum E(k) = Ek + Ezk
+    of the coarse runs (mean over their forcing realisations) against the
+    mean of the coarse-grained DNS realisations forced by the same draws,
+    for k = 1..kmax, and the 2D energy flux.  A shell agrees when its
+    relative error is at most `tol`, or at most the noise floor 2 sigma of
+    the difference of the two means (sigma from the realisations' spread),
+    where that is larger.  The flux error is max_k |difference of the mean
+    fluxes| / max_k |DNS flux|, and its noise floor 2 sigma of the largest
+    difference of the means in the same units."""
+    e_les = np.array([(s[0] + s[1])[:kmax] for s in les_runs])
+    e_dns = np.array([(s[0] + s[1])[:kmax] for s in dns_runs])
+    m_les, m_dns = e_les.mean(0), e_dns.mean(0)
+
+    def se(e):
+        return e.std(0, ddof=1) / math.sqrt(len(e)) if len(e) > 1 else np.zeros(e.shape[1])
+
+    noise = 2.0 * np.sqrt(se(e_les) ** 2 + se(e_dns) ** 2) / m_dns
+    rel = np.abs(m_les - m_dns) / m_dns
+    f_les = np.array([s[2][:kmax] for s in les_runs])
+    f_dns = np.array([s[2][:kmax] for s in dns_runs])
+    fx_les, fx_dns = f_les.mean(0), f_dns.mean(0)
+    fscale = max(np.max(np.abs(fx_dns)), 1e-300)
+    flux = np.max(np.abs(fx_les - fx_dns)) / fscale
+    flux_noise = 2.0 * np.max(np.sqrt(se(f_les) ** 2 + se(f_dns) ** 2)) / fscale
+    ok = rel <= np.maximum(tol, noise)
+    return {"spectrum_rel_err_mean": float(rel.mean()), "spectrum_rel_err_max": float(rel.max()),
+            "flux_err": float(flux), "flux_noise_floor": float(flux_noise), "shells": int(kmax),
+            "shells_within_5pct": int(np.sum(rel <= tol)),
+            "shells_within_5pct_or_noise": int(np.sum(ok)),
+            "shells_outside": [int(k + 1) for k in np.nonzero(~ok)[0]],
+            "dns_realisations": len(dns_runs), "les_realisations": len(les_runs),
+            "dns_spread_rel": [float(x) for x in se(e_dns) / m_dns],
+            "noise_floor_rel": [float(x) for x in noise], "per_k": [float(x) for x in rel],
+            "ratio": [float(x) for x in m_les / m_dns]}
+
+
+def closure_settings(z):
+    """The inference settings stored with a trained network (closure_params.npz)."""
+    g = lambda k, d: float(z[k]) if k in z else d
+    return dict(kmin=g("kmin", 0.0), kclip=g("kclip", 0.0), clamp=g("out_clamp", 0.0),
+                sign_clip=g("sign_clip", 0.0), period=int(g("period", 1)),
+                ktaper=g("ktaper", 0.0),
+                safe=tuple(float(v) for v in z["safe_cfl"]) if "safe_cfl" in z else None)
+
+
+def live_evaluation(npz_path, dev, spinup=15000, window=10000, realisations=2, samples=100,
+                    dt=2e-4, seed=0, cfl_guard=3.0, guard=4.0, log=None):
+    """Judge a trained closure on a DNS window generated here: the 2048^2 DNS
+    of the study (same parameters and seeded start) spun up `spinup` steps,
+    then `realisations` DNS runs over `window` steps from that state, forced
+    by the same draws as the coarse runs; the plain 256^2 solver and the
+    closure run (its stored inference settings, stabiliser and fallback
+    guards as in the study), each as `realisations` runs from the
+    coarse-grained start state.  Returns {"plain": judge(...), "closure":
+    judge(...), ...}."""
+    log = log or (lambda *a: None)
+    z = np.load(npz_path)
+    nf, nc = int(z["nf"]), int(z["nc"])
+    cs = closure_settings(z)
+    kf, f0 = 40.0, 1.0
+    p = S.Params(5e-5, 0.5, 1, 2, 5e-5, 0.5, 1, 2, 5.0)
+    ps0, vz0 = ic.random_phase_ic(nf, seed=1234 + seed)
+    ps = torch.from_numpy(ps0).to(dev)
+    vz = torch.from_numpy(vz0).to(dev)
+    del ps0, vz0
+    forcing = Forcing(kf, f0, seed)
+    done = 0
+    while done < spinup:
+        c = min(500, spinup - done)
+        S.rk4_run(ps, vz, p, forcing.take(c), c, dt=dt)
+        done += c
+    dns_cfl = L.cfl_number(ps, vz, p, dt)
+    log("closure judge: DNS %d^2 spun up %d steps (CFL number %.3f)" % (nf, spinup, dns_cfl))
+    ps_c0, vz_c0 = L.coarse_grain(ps, nc), L.coarse_grain(vz, nc)
+    e0 = S.energy(ps_c0, 1) + S.energy(vz_c0, 0)
+    z0 = S.energy(ps_c0, 2) + S.energy(vz_c0, 1)
+    seeds = [seed + 100 + r for r in range(realisations)]
+    dns = dns_realisations(ps, vz, p, dt, nc, window, seeds, kf, f0, samples)
+    log("closure judge: %d DNS realisations of %d steps" % (realisations, window))
+    del ps, vz
+    torch.cuda.empty_cache()
+    dtc = dt * nf / nc
+    kmax = nc // 3
+    out = {"window_time": window * dt, "spinup_steps": spinup, "dns_cfl": dns_cfl,
+           "coarse_dt": dtc, "settings": {k: v for k, v in cs.items() if k != "safe"}}
+    safe = cs["safe"]
+    for name, prm in (("plain", None), ("closure", z["params"])):
+        stats, steps, finite, fb = coarse_runs(
+            ps_c0, vz_c0, p, window * dt, dtc, seeds, kf, f0, prm, z["scale_in"], z["scale_out"],
+            cs["kmin"], samples=samples, energy_bound=20.0 * e0, kclip=cs["kclip"],
+            enstrophy_bound=guard * z0, clamp=cs["clamp"], cfl_bound=cfl_guard * dns_cfl,
+            sign_clip=cs["sign_clip"], safe=safe, period=cs["period"],
+            gain=L.taper_gain(cs["kmin"], cs["ktaper"], nc // 2 + 2) if cs["ktaper"] > 0 else None)
+        r = {"finite": bool(finite), "fallbacks": fb, "steps": steps}
+        if finite and stats is not None:
+            r.update(judge(coarse_runs.per_run, dns, kmax))
+        if prm is not None:
+            r["safe_fraction"] = coarse_runs.safe_fraction
+        out[name] = r
+        log("closure judge: %s finite %s, spectrum err mean %s max %s, flux err %s, "
+            "shells within 5%% %s, within 5%% or noise %s" % (
+                name, finite, r.get("spectrum_rel_err_mean"), r.get("spectrum_rel_err_max"),
+                r.get("flux_err"), r.get("shells_within_5pct"),
+                r.get("shells_within_5pct_or_noise")))
+    return out
 
 
 def train_window(a, advance, ps, vz, out, log):
@@ -282,18 +432,27 @@ def compact_record(out, command):
                      "val_loss_zero_model": out["train"]["val_loss_zero_model"],
                      "seconds": out["train"]["seconds"],
                      "epochs": None if out["config"]["params"] else out["config"]["epochs"]},
-           "kmin": out["kmin"], "gain": out["gain"], "kclip": out["kclip"], "clamp": out["clamp"],
+           "kmin": out["kmin"], "ktaper": out["ktaper"], "gain": out["gain"],
+           "kclip": out["kclip"], "clamp": out["clamp"],
            "sign_clip": out["sign_clip"], "period": out["period"],
            "selection": out.get("selection", "min tuning error"),
            "dns_cfl": out["dns"]["cfl"], "cfl_guard": out["config"]["cfl_guard"],
            "safe_cfl": out["config"]["safe_cfl"],
            "tune": {k: {m: v[m] for m in ("spectrum_rel_err_max", "spectrum_rel_err_mean",
-                                           "flux_err") if m in v}
+                                           "flux_err", "shells_within_5pct",
+                                           "shells_within_5pct_or_noise") if m in v}
                     for k, v in out["tune"].items()},
            "evaluation_window": out["config"]["eval_steps"] * out["config"]["dt"],
            "coarse_forcing": "coarse-grained draws (injection f0 G(k)^2)"}
     for name in ("plain", "closure"):
         rec[name] = {m: out["coarse"][name][m] for m in keep if m in out["coarse"][name]}
+    for name in ("plain", "closure"):
+        j = out["coarse"][name].get("judge")
+        if j is not None:
+            rec[name]["judge"] = {k: j[k] for k in (
+                "spectrum_rel_err_mean", "spectrum_rel_err_max", "flux_err", "shells",
+                "shells_within_5pct", "shells_within_5pct_or_noise", "shells_outside",
+                "dns_realisations", "noise_floor_rel", "ratio")}
     rec["speedup_gpu_s_per_time"] = out["speedup_gpu_s_per_time"]
     rec["shells"] = int(dns.size)
     for name in ("closure", "plain"):
@@ -334,6 +493,10 @@ def main(argv=None):
     ap.add_argument("--sign-clip", type=float, nargs="+", default=[0.0],
                     help="candidate strengths in [0, 1] of the network's sign clipping "
                          "(closure.Net.set_sign_clip), tuned with the others")
+    ap.add_argument("--ktaper", type=float, nargs="+", default=[0.0],
+                    help="candidate shells at which the closure forcing reaches full strength, "
+                         "rising linearly from 0 at the cutoff kmin (0 or <= kmin: a hard "
+                         "cutoff), tuned with the others")
     ap.add_argument("--period", type=int, nargs="+", default=[1],
                     help="candidate closure refresh periods in steps (forcing held in between)")
     ap.add_argument("--safe-cfl", type=float, nargs=2, default=None, metavar=("ON", "OFF"),
@@ -428,14 +591,20 @@ def main(argv=None):
     ps_c0, vz_c0 = L.coarse_grain(ps, nc), L.coarse_grain(vz, nc)
     e0 = S.energy(ps_c0, 1) + S.energy(vz_c0, 0)
     z0 = S.energy(ps_c0, 2) + S.energy(vz_c0, 1)
-    dns_tune = dns_window(advance, ps, vz, nc, a.tune_steps)
+    # the DNS judge: realisations forced by the coarse runs' draws (the first
+    # continues the DNS into the next window)
+    dns_tune_runs = dns_realisations(ps, vz, p, a.dt, nc, a.tune_steps, seeds, kf, f0)
+    dns_tune = mean_stats(dns_tune_runs)
+    log("dns tuning window done (%d realisations)" % len(seeds))
     Tt = a.tune_steps * a.dt
     out["tune"] = {}
-    cands = [dict(kmin=km, gain=g, kclip=kc, clamp=cl, sign_clip=sc, period=pe)
+    cands = [dict(kmin=km, gain=g, kclip=kc, clamp=cl, sign_clip=sc, period=pe, ktaper=kt)
              for km in a.kmin for g in a.gain for kc in a.kclip for cl in a.clamp
-             for sc in a.sign_clip for pe in a.period]
-    key = lambda c: "kmin %g gain %g kclip %g clamp %g sign_clip %g period %d" % (
-        c["kmin"], c["gain"], c["kclip"], c["clamp"], c["sign_clip"], c["period"])
+             for sc in a.sign_clip for pe in a.period for kt in a.ktaper]
+    key = lambda c: "kmin %g gain %g kclip %g clamp %g sign_clip %g period %d ktaper %g" % (
+        c["kmin"], c["gain"], c["kclip"], c["clamp"], c["sign_clip"], c["period"], c["ktaper"])
+    taper = lambda c: (L.taper_gain(c["kmin"], c["ktaper"], nc // 2 + 2)
+                       if c["ktaper"] > c["kmin"] else None)
     cfl_bound = a.cfl_guard * out["dns"]["cfl"]
     safe = None if a.safe_cfl is None else tuple(v * out["dns"]["cfl"] for v in a.safe_cfl)
     for c in cands:
@@ -444,39 +613,50 @@ def main(argv=None):
                                                energy_bound=20.0 * e0, kclip=c["kclip"],
                                                enstrophy_bound=a.guard * z0, clamp=c["clamp"],
                                                cfl_bound=cfl_bound, sign_clip=c["sign_clip"],
-                                               safe=safe, period=c["period"])
+                                               safe=safe, period=c["period"], gain=taper(c))
         r = {"finite": finite, "fallbacks": fb, "cfl_max": coarse_runs.cfl_max,
              "safe_fraction": coarse_runs.safe_fraction}
         if finite and stats is not None:
             r.update(errors(stats, dns_tune, kmax))
+            j = judge(coarse_runs.per_run, dns_tune_runs, kmax)
+            r["shells_within_5pct_or_noise"] = j["shells_within_5pct_or_noise"]
+            r["shells_within_5pct"] = j["shells_within_5pct"]
         out["tune"][key(c)] = r
-        log("tune %s: spectrum err max %s mean %s, fallbacks %d, max CFL %.3f, safe %.3f" % (
-            key(c), r.get("spectrum_rel_err_max"), r.get("spectrum_rel_err_mean"), fb,
-            r["cfl_max"], r["safe_fraction"]))
+        log("tune %s: spectrum err max %s mean %s, shells within 5%% %s (or noise %s), "
+            "fallbacks %d, max CFL %.3f, safe %.3f" % (
+                key(c), r.get("spectrum_rel_err_max"), r.get("spectrum_rel_err_mean"),
+                r.get("shells_within_5pct"), r.get("shells_within_5pct_or_noise"), fb,
+                r["cfl_max"], r["safe_fraction"]))
     def err(c):
         r = out["tune"][key(c)]
         return r.get("spectrum_rel_err_mean", 1e300) + (1e300 if r["fallbacks"] else 0.0)
 
+    def agree(c):
+        return out["tune"][key(c)].get("shells_within_5pct_or_noise", -1)
+
     best = min(cands, key=err)
     if a.err_target is not None:
         # the cheapest candidate (fewest network evaluations: largest period)
-        # that meets the accuracy target on the tuning window, then the most
-        # accurate of those; the most accurate overall if none meets it
+        # that meets the accuracy target on the tuning window, then the one
+        # with the most shells agreeing with the DNS (5% or its noise floor),
+        # then the most accurate; the most accurate overall if none meets it
         ok = [c for c in cands if err(c) <= a.err_target]
         if ok:
-            best = min(ok, key=lambda c: (-c["period"], err(c)))
+            best = min(ok, key=lambda c: (-c["period"], -agree(c), err(c)))
     out["selection"] = ("min tuning error" if a.err_target is None else
-                        "largest period with tuning error <= %g, then min error" % a.err_target)
+                        "largest period with tuning error <= %g, then most shells within 5%% "
+                        "or the DNS noise floor, then min error" % a.err_target)
     out.update(best)
-    kbest, gbest, kcbest, clbest, scbest, pbest = (
-        best[k] for k in ("kmin", "gain", "kclip", "clamp", "sign_clip", "period"))
+    kbest, gbest, kcbest, clbest, scbest, pbest, ktbest = (
+        best[k] for k in ("kmin", "gain", "kclip", "clamp", "sign_clip", "period", "ktaper"))
     sy = [gbest * v for v in sy]
 
     # 5. evaluation window: coarse-grained DNS statistics, then the coarse runs
     ps_c0, vz_c0 = L.coarse_grain(ps, nc), L.coarse_grain(vz, nc)
     e0 = S.energy(ps_c0, 1) + S.energy(vz_c0, 0)
     z0 = S.energy(ps_c0, 2) + S.energy(vz_c0, 1)
-    dns_stats = dns_window(advance, ps, vz, nc, a.eval_steps)
+    dns_runs = dns_realisations(ps, vz, p, a.dt, nc, a.eval_steps, seeds, kf, f0)
+    dns_stats = mean_stats(dns_runs)
     T = a.eval_steps * a.dt
     log("dns evaluation window done (t = %.3f); closure %s" % (T, key(best)))
     del ps, vz
@@ -487,8 +667,11 @@ def main(argv=None):
                                                enstrophy_bound=a.guard * z0,
                                                trace=log if a.trace and prm is not None else None,
                                                clamp=clbest, cfl_bound=cfl_bound,
-                                               sign_clip=scbest, safe=safe, period=pbest)
+                                               sign_clip=scbest, safe=safe, period=pbest,
+                                               gain=taper(best))
         net = L.Net(1, nc, nc, dev, params=params) if prm is not None else None
+        if net is not None:
+            net.set_forcing_gain(taper(best))
         sec = timed_steps(ps_c0, vz_c0, p, dtc, Forcing(kf, f0, a.seed + 11, nc), 500, net, scale_in,
                           sy, kbest, kcbest, clbest, scbest, pbest)
         r = {"finite": finite, "steps": steps, "dt": dtc, "sim_time": steps * dtc,
@@ -498,6 +681,7 @@ def main(argv=None):
         if finite and stats is not None:
             r.update(errors(stats, dns_stats, kmax))
             r["spectrum"] = [float(x) for x in (stats[0] + stats[1])[:kmax]]
+            r["judge"] = judge(coarse_runs.per_run, dns_runs, kmax)
         out["coarse"][name] = r
         log("%s: finite %s, %d steps, t %.3f, %.3f GPU-s/time, spectrum err max %s mean %s "
             "k<=40 max %s, flux err %s" % (
@@ -512,7 +696,7 @@ def main(argv=None):
         json.dump(out, f, indent=1)
     np.savez(os.path.join(a.out, "closure_params.npz"), params=params, scale_in=np.array(scale_in),
              scale_out=np.array(sy), kmin=kbest, gain=gbest, kclip=kcbest,
-             out_clamp=clbest, sign_clip=scbest, period=pbest, nf=nf, nc=nc,
+             out_clamp=clbest, sign_clip=scbest, period=pbest, ktaper=ktbest, nf=nf, nc=nc,
              **({} if safe is None else {"safe_cfl": np.array(safe)}), **{k: out["train"][k] for k in (
                  "val_loss", "val_loss_zero_model", "seconds") if out["train"].get(k) is not None})
     cmd = "python -m rot2d3c_amd.closure_study " + " ".join(

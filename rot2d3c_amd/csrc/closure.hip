@@ -368,6 +368,8 @@ struct r2d3c_cnn {
   bool weights_ready = false;
   float out_clamp = 0.f;        // inference output saturation (0: none)
   float sign_clip = 0.f;        // inference sign clipping strength of outputs 0, 1
+  double *fgain = nullptr;      // closure forcing gain per shell (NULL: none)
+  int nfgain = 0;
 };
 
 namespace {
@@ -490,6 +492,7 @@ void cnn_free(r2d3c_cnn *c) {
   if (c->g) (void)hipFree(c->g);
   if (c->g2) (void)hipFree(c->g2);
   if (c->lpart) (void)hipFree(c->lpart);
+  if (c->fgain) (void)hipFree(c->fgain);
   for (auto *a : c->act) if (a) (void)hipFree(a);
   for (auto *z : c->z) if (z) (void)hipFree(z);
   delete c;
@@ -602,6 +605,22 @@ int r2d3c_cnn_set_sign_clip(r2d3c_cnn *c, double strength) {
   return 0;
 }
 
+int r2d3c_cnn_set_forcing_gain(r2d3c_cnn *c, const double *gain, int nshells) {
+  if (!c) return err(R2D3C_EINVAL, "cnn: NULL argument");
+  if (!gain || nshells == 0) {
+    c->nfgain = 0;
+    return 0;
+  }
+  const int cap = (c->H > c->W ? c->H : c->W) / 2 + 2;
+  if (nshells < 0 || nshells > cap) return err(R2D3C_EINVAL, "cnn: nshells in [0, max(h, w)/2 + 2]");
+  for (int k = 0; k < nshells; ++k)
+    if (!(gain[k] >= 0.0 && gain[k] <= 16.0)) return err(R2D3C_EINVAL, "cnn: gains in [0, 16]");
+  if (!c->fgain) CHK(hipMalloc((void **)&c->fgain, sizeof(double) * cap));
+  CHK(hipMemcpy(c->fgain, gain, sizeof(double) * nshells, hipMemcpyHostToDevice));
+  c->nfgain = nshells;
+  return 0;
+}
+
 int r2d3c_cnn_set_output_clamp(r2d3c_cnn *c, double clamp) {
   if (!c) return err(R2D3C_EINVAL, "cnn: NULL argument");
   if (!(clamp >= 0.0)) return err(R2D3C_EINVAL, "cnn: clamp must be >= 0");
@@ -651,3 +670,9 @@ int r2d3c_cnn_train_step(r2d3c_cnn *c, const float *d_in, const float *d_target,
 }
 
 }  // extern "C"
+
+// The closure forcing gain table of a network (rot2d3c.hip's forcing epilogue).
+void r2d3c_internal_cnn_forcing_gain(const r2d3c_cnn *c, const double **gain, int *n) {
+  *gain = c && c->nfgain ? c->fgain : nullptr;
+  *n = c ? c->nfgain : 0;
+}

@@ -3100,10 +3100,11 @@ __global__ __launch_bounds__(N / 8) void k_cf_row(const float *__restrict__ z, G
 
 // column forwards of the two predicted fields with the closure epilogue:
 // y = 0: fp = -tau5/k^2, y = 1: fz = tau6, both on the update mask and
-// k^2 >= kmin2 only
+// k^2 >= kmin2 only, times gain[int(|k| + 0.5)] for shells below ngain
 template <int N>
 __global__ __launch_bounds__(N / 8) void k_cf_col(const cplx *__restrict__ mix, Geo g,
                                                    const cplx *__restrict__ tw, double kmin2,
+                                                   const double *__restrict__ gain, int ngain,
                                                    cplx *__restrict__ fp, cplx *__restrict__ fz) {
   constexpr int T = N / 8;
   __shared__ cplx lds[Shape<N>::LDSN];
@@ -3120,8 +3121,11 @@ __global__ __launch_bounds__(N / 8) void k_cf_col(const cplx *__restrict__ mix, 
     const int j = b + T * q, ky = kyof(j, N);
     const double ka2 = (double)kx * kx + (double)ky * ky;
     cplx o = mk(0.0, 0.0);
-    if (ka2 <= g.kmax && ka2 > g.tiny && ka2 >= kmin2)
+    if (ka2 <= g.kmax && ka2 > g.tiny && ka2 >= kmin2) {
       o = f == 0 ? mk(-v[q].x / ka2, -v[q].y / ka2) : v[q];
+      const int shell = (int)(sqrt(ka2) + 0.5);
+      if (shell < ngain) o = mk(o.x * gain[shell], o.y * gain[shell]);
+    }
     dst[(size_t)i * N + j] = o;
   }
 }
@@ -3149,10 +3153,12 @@ __global__ void k_closure_clip(const cplx *__restrict__ psi, const cplx *__restr
   }
 }
 
+void r2d3c_internal_cnn_forcing_gain(const r2d3c_cnn *c, const double **gain, int *n);
+
 template <int N>
 static int run_closure_io(r2d3c_plan *p, int dir, const cplx *psi, const cplx *vz, const double *sc,
                           double kmin, unsigned short *in_bf16, const float *z, cplx *fp, cplx *fz,
-                          hipStream_t s) {
+                          hipStream_t s, const r2d3c_cnn *net = nullptr) {
   if (dir == 0) {
     k_ci_colinv<N><<<dim3(N / 2 + 1, 3), N / 8, 0, s>>>(psi, vz, p->mfull, p->g, p->tw);
     const double inv = 1.0 / ((double)N * N);
@@ -3160,7 +3166,11 @@ static int run_closure_io(r2d3c_plan *p, int dir, const cplx *psi, const cplx *v
                                           sc[2] * inv, in_bf16);
   } else {
     k_cf_row<N><<<N / 2, N / 8, 0, s>>>(z, p->g, p->tw, sc[0], sc[1], p->mfull);
-    k_cf_col<N><<<dim3(N / 2 + 1, 2), N / 8, 0, s>>>(p->mfull, p->g, p->tw, kmin * kmin, fp, fz);
+    const double *gain = nullptr;
+    int ngain = 0;
+    r2d3c_internal_cnn_forcing_gain(net, &gain, &ngain);
+    k_cf_col<N><<<dim3(N / 2 + 1, 2), N / 8, 0, s>>>(p->mfull, p->g, p->tw, kmin * kmin, gain,
+                                                      ngain, fp, fz);
   }
   LAUNCHCHK("closure io");
   return 0;
@@ -3187,7 +3197,7 @@ int r2d3c_closure_net_forcing(r2d3c_plan *p, r2d3c_cnn *net, const double *scale
   if (!rc) rc = r2d3c_cnn_buffers(net, nullptr, &z);
   if (rc) return rc;
   R2D_DISPATCH(p->n, run_closure_io, p, 1, nullptr, nullptr, scale, kmin, nullptr, z, (cplx *)fp,
-               (cplx *)fz, (hipStream_t)stream);
+               (cplx *)fz, (hipStream_t)stream, net);
 }
 
 int r2d3c_closure_clip(r2d3c_plan *p, const double *psi, const double *vz, double kclip,

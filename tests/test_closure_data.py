@@ -46,3 +46,21 @@ def test_evaluation_record_is_consistent():
     assert ev["closure"]["fallbacks"] == 0
     assert 0.0 <= ev["closure"]["safe_fraction"] < 0.5
     assert ev["closure"]["spectrum_rel_err_mean"] <= 0.05
+
+
+def test_evaluation_judge_is_consistent():
+    """The judge against the mean of DNS realisations forced by the coarse
+    runs' draws: a shell agrees when its relative error is within 5% or
+    within the noise floor (2 sigma of the difference of the two means)."""
+    ev = json.load(open(os.path.join(DATA, "closure_256_eval.json")))
+    for name in ("closure", "plain"):
+        j = ev[name]["judge"]
+        r, nz = np.array(j["ratio"]), np.array(j["noise_floor_rel"])
+        assert len(r) == len(nz) == j["shells"] == 85
+        ok = np.abs(r - 1) <= np.maximum(0.05, nz)
+        assert j["shells_within_5pct_or_noise"] == int(ok.sum())
+        assert j["shells_within_5pct"] == int(np.sum(np.abs(r - 1) <= 0.05))
+        assert j["shells_outside"] == [int(k) + 1 for k in np.nonzero(~ok)[0]]
+        assert j["dns_realisations"] >= 2
+    assert (ev["closure"]["judge"]["shells_within_5pct_or_noise"] >
+            ev["plain"]["judge"]["shells_within_5pct_or_noise"])

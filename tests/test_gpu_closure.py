@@ -202,6 +202,36 @@ def test_fused_closure_io_matches_the_separate_steps(cuda):
     net.close()
 
 
+def test_forcing_gain_scales_each_shell(cuda):
+    """r2d3c_cnn_set_forcing_gain: the closure forcing made from the network's
+    prediction is the one without the table times gain[int(|k| + 0.5)] (bit
+    for bit: one multiplication per mode), shells past the table keep gain 1,
+    and removing the table restores the original bits."""
+    print("closure forcing gain", flush=True)
+    from rot2d3c_amd import closure as L
+    n = 64
+    ps, vz = random_state(n, 33)
+    ps, vz = torch.from_numpy(ps).to(cuda), torch.from_numpy(vz).to(cuda)
+    net = L.Net(1, n, n, cuda, seed=5)
+    sin, sout = (0.02, 0.5, 3.0), (7.0, 0.3)
+    fp0, fz0 = _fused_forcing(n, ps, vz, net, sin, sout, cuda)
+    gain = L.taper_gain(4.0, 9.0, 15) * 1.25
+    net.set_forcing_gain(gain)
+    fp1, fz1 = _fused_forcing(n, ps, vz, net, sin, sout, cuda)
+    g = O.grid(n)
+    shell = np.floor(np.sqrt(g.ka2) + 0.5).astype(int)
+    w = np.where(shell < len(gain), gain[np.minimum(shell, len(gain) - 1)], 1.0)
+    for a, b in ((fp0, fp1), (fz0, fz1)):
+        a, b = a.cpu().numpy(), b.cpu().numpy()
+        assert np.array_equal(b.real, a.real * w) and np.array_equal(b.imag, a.imag * w)
+    net.set_forcing_gain(None)
+    fp2, fz2 = _fused_forcing(n, ps, vz, net, sin, sout, cuda)
+    assert torch.equal(fp2, fp0) and torch.equal(fz2, fz0)
+    with pytest.raises(RuntimeError):
+        net.set_forcing_gain(np.full(40, 1.0))  # more shells than 64/2 + 2
+    net.close()
+
+
 def test_ensemble_closure_run_tracks_the_stepwise_closure_run(cuda):
     """A closure run inside the ensemble's step graph against ClosureRun
     (separate kernels, same network and forcing): equal up to the inputs'
