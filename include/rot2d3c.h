@@ -14,9 +14,13 @@
  * Complex arrays are passed as `double *` to interleaved (re, im) pairs.
  *
  * All device work is asynchronous on the caller's HIP stream (passed as
- * `void *`, i.e. a hipStream_t; NULL = the default stream).  No entry point
- * allocates device memory except r2d3c_plan_create.  A plan is not
- * thread-safe: use one host thread per plan.  Every function returns 0 on
+ * `void *`, i.e. a hipStream_t; NULL = the default stream).  Device memory
+ * is allocated only by r2d3c_plan_create, r2d3c_autotune (scratch, freed
+ * before it returns), r2d3c_ensemble_create, r2d3c_ensemble_set_closure,
+ * r2d3c_ensemble_run (when the forcing table grows), r2d3c_cnn_create and
+ * r2d3c_cnn_set_forcing_gain (its first call); the plan operators,
+ * r2d3c_step and r2d3c_run allocate nothing, and r2d3c_run_noalloc allocates
+ * nothing at all.  A plan is not thread-safe: use one host thread per plan.  Every function returns 0 on
  * success, a negative R2D3C_E* code on an argument error, or
  * -(1000 + hipError_t) when the HIP runtime fails; r2d3c_last_error()
  * returns a description of the last failure on the calling thread.
@@ -67,7 +71,11 @@ typedef struct r2d3c_forcing {
    * fz = const_inj(C2, ci_kdn, ci_kup, ci_fz0, 0) at every stage
    * (src/Rot2D3C.para.f90:565-569); the step then runs stage by stage.
    * ci_nprocs: the MPI rank count whose random numbers to reproduce
-   * (<= 1: one rank). */
+   * (<= 1: one rank).  In one r2d3c_run call every step must have the same
+   * ci_enabled and (with iflow = 2) the same ci_kdn, ci_kup, ci_nprocs.  The
+   * first iflow = 2 step of a plan, and the first after a change of band or
+   * rank count, uploads the random-number table and synchronizes the stream
+   * (not capturable); later ones only launch kernels. */
   int ci_enabled;
   double ci_kdn, ci_kup, ci_fp0, ci_fz0;
   int ci_nprocs;

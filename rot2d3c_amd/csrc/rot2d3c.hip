@@ -1686,6 +1686,15 @@ template <int N>
 static int run_steps(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
                      const r2d3c_forcing *fo, int nsteps, int dt_mode, double dt_or_cfl,
                      double *d_dt, hipStream_t s) {
+  if (fo) {  // one forcing kind per call: the iflow = 2 table is built from fo[0]
+    for (int st = 1; st < nsteps; ++st)
+      if (fo[st].ci_enabled != fo[0].ci_enabled ||
+          (fo[0].ci_enabled && (fo[st].ci_kdn != fo[0].ci_kdn || fo[st].ci_kup != fo[0].ci_kup ||
+                                fo[st].ci_nprocs != fo[0].ci_nprocs)))
+        return set_err(R2D3C_EINVAL,
+                       "every step of one call must have the same ci_enabled and, with "
+                       "iflow = 2, the same ci_kdn, ci_kup and ci_nprocs");
+  }
   if (fo && fo[0].ci_enabled) {  // iflow = 2 runs stage by stage
     if ((fo[0].ci_kdn > fo[0].ci_kup) || fo[0].ci_kup < 0.0)
       return set_err(R2D3C_EINVAL, "const_inj band must satisfy 0 <= kdn <= kup");

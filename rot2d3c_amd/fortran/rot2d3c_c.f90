@@ -13,9 +13,11 @@ module rot2d3c_c
     integer(c_int) :: nn, mm, nnv, mmv
   end type
 
+  ! defaults: no forcing (enabled = 0), kupk = 1 so an enabled draw never
+  ! divides by an unset value
   type, bind(c) :: r2d3c_force_draw
-    real(c_double) :: kx, ky, phase, f0, kupk
-    integer(c_int) :: enabled
+    real(c_double) :: kx = 0, ky = 0, phase = 0, f0 = 0, kupk = 1
+    integer(c_int) :: enabled = 0
   end type
 
   type, bind(c) :: r2d3c_forcing

@@ -89,3 +89,4 @@ def test_struct_layouts_match_the_header(tmp_path):
         expect.append(ctypes.sizeof(ct))
         expect += [getattr(ct, f).offset for f, _ in ct._fields_]
     assert got == expect
+

@@ -389,3 +389,29 @@ def test_step_dense_forcing_matches_oracle(n, mode, cuda):
         assert abs(float(dto.item()) - dt) <= 1e-12 * dt
     assert rel_l2(_host(tps), ops) < TOL
     assert rel_l2(_host(tvz), ovz) < TOL
+
+
+def test_run_rejects_steps_with_different_iflow2_settings(cuda):
+    """r2d3c_run builds the iflow = 2 random-number table from the first
+    step's band, so steps with another band or forcing kind are rejected
+    (before any device work) instead of being applied with the wrong table."""
+    print("r2d3c_run mixed iflow=2 settings", flush=True)
+    import ctypes
+    import torch
+    from rot2d3c_amd import _lib
+    S = _S()
+    n = 32
+    pl = S.Plan.get(n, cuda)
+    ps = torch.zeros((n // 2 + 1, n), dtype=torch.complex128, device=cuda)
+    vz = torch.zeros_like(ps)
+    cp = S.Params(1e-3, 0.1, 1, 2, 1e-3, 0.1, 1, 2, 1.0).c()
+    for second in (dict(ci_enabled=1, ci_kdn=2.0, ci_kup=6.0), dict(ci_enabled=0)):
+        arr = (_lib.Forcing * 2)()
+        arr[0].ci_enabled, arr[0].ci_kdn, arr[0].ci_kup = 1, 2.0, 5.0
+        for k, v in second.items():
+            setattr(arr[1], k, v)
+        rc = _lib.lib().r2d3c_run(pl.handle, ctypes.byref(cp), ctypes.c_void_p(ps.data_ptr()),
+                                  ctypes.c_void_p(vz.data_ptr()), arr, 2, _lib.R2D3C_DT_FIXED,
+                                  1e-3, None, None)
+        assert rc == -1
+        assert b"same ci_enabled" in _lib.lib().r2d3c_last_error()
