@@ -33,10 +33,10 @@ from rot2d3c_amd.traffic import algorithmic_bytes  # noqa: E402
 
 
 def cpu_baseline(n, steps, p, kf, f0, seed, ps0, vz0, dt, workers=None):
-    """The oracle on the same workload for `steps` steps, forcing drawn from
-    the same seed; its 2D FFTs on `workers` threads (None: numpy, one core;
-    the pointwise work stays single-threaded numpy).  Returns the elapsed
-    seconds."""
+    """The numpy oracle on the same workload for `steps` steps, forcing drawn
+    from the same seed; its 2D FFTs on `workers` threads (None: numpy, one
+    core; the pointwise work stays single-threaded numpy).  Returns the
+    elapsed seconds."""
     from oracle import spectral as O
     op = O.Params(p["nu"], p["hnu"], p["nn"], p["mm"], p["nuv"], p["hnuv"], p["nnv"],
                   p["mmv"], p["omega"])
@@ -48,6 +48,33 @@ def cpu_baseline(n, steps, p, kf, f0, seed, ps0, vz0, dt, workers=None):
         return time.perf_counter() - t0
     finally:
         O.set_fft_workers(None)
+
+
+def cpu_baseline_compiled(n, steps, p, kf, f0, seed, ps0, vz0, dt, threads):
+    """The compiled C++/OpenMP restatement of the step (oracle/cpu_step.cpp:
+    the reference's 40 full 2D transforms per fixed-dt step, its own FFT) on
+    `threads` host threads, same workload and forcing draws.  Returns the
+    elapsed seconds."""
+    from oracle import cpu_step as C
+    from oracle import spectral as O
+    if not os.path.exists(C.LIB):
+        C.build()
+    C.threads(threads)
+    op = O.Params(p["nu"], p["hnu"], p["nn"], p["mm"], p["nuv"], p["hnuv"], p["nnv"],
+                  p["mmv"], p["omega"])
+    s0, s1 = seed, seed + 1
+    _, _, _, s0 = O.rand_force_draw(kf, s0)  # the pre-loop pair advances both seeds
+    _, _, _, s1 = O.rand_force_draw(kf, s1)
+    draws = []
+    for _ in range(steps):
+        kx, ky, ph, s0 = O.rand_force_draw(kf, s0)
+        a = (kx, ky, ph, f0, kf)
+        kx, ky, ph, s1 = O.rand_force_draw(kf, s1)
+        draws.append((a, (kx, ky, ph, f0, 1.0)))
+    a, b = ps0.copy(), vz0.copy()
+    t0 = time.perf_counter()
+    C.run(a, b, op, steps, dt, draws)
+    return time.perf_counter() - t0
 
 
 def closure_bench(dns_ms_per_step, dns_dt, dev, steps=500, warmup=20, judge_window=10000,
@@ -363,16 +390,18 @@ def main():
         # host cores this process may use (the GPU's share of the host)
         cores = min(16, int(os.environ.get("OMP_NUM_THREADS") or 0) or os.cpu_count() or 1,
                     len(os.sched_getaffinity(0)))
-        el1 = cpu_baseline(n, args.cpu_steps, p, kf, f0, seed, ps0, vz0, dt)
-        eln = cpu_baseline(n, args.cpu_steps, p, kf, f0, seed, ps0, vz0, dt, workers=cores) \
-            if cores > 1 else el1
-        cpu = {"value": round(n * n * args.cpu_steps / eln / 1e6, 4), "unit": "Mpts·RK4-steps/s",
-               "cores": cores, "kind": "port",
-               "sample": "%d steps of the same %d^2 workload through the oracle, its 2D FFTs "
-                         "threaded over %d cores (scipy.fft pocketfft), pointwise work numpy: "
-                         "%.1f s; one core (numpy.fft): %.1f s"
-                         % (args.cpu_steps, n, cores, eln, el1),
-               "value_one_core": round(n * n * args.cpu_steps / el1 / 1e6, 4)}
+        k = args.cpu_steps
+        eln = cpu_baseline_compiled(n, k, p, kf, f0, seed, ps0, vz0, dt, cores)
+        el1 = cpu_baseline_compiled(n, 1, p, kf, f0, seed, ps0, vz0, dt, 1)
+        elp = cpu_baseline(n, 1, p, kf, f0, seed, ps0, vz0, dt, workers=cores)
+        mp = lambda steps, sec: round(n * n * steps / sec / 1e6, 4)
+        cpu = {"value": mp(k, eln), "unit": "Mpts·RK4-steps/s", "cores": cores, "kind": "port",
+               "sample": "%d steps of the same %d^2 workload through the compiled C++/OpenMP "
+                         "restatement (oracle/cpu_step.cpp: 40 full 2D FFTs per step, own "
+                         "radix-4 FFT) on %d threads: %.1f s; 1 step on one thread: %.1f s; "
+                         "1 step of the numpy oracle (FFTs threaded over %d cores): %.1f s"
+                         % (k, n, cores, eln, el1, cores, elp),
+               "value_one_core": mp(1, el1), "value_numpy_oracle": mp(1, elp)}
 
     clo = None
     if not args.no_closure:
