@@ -1,7 +1,8 @@
 // closure.hip -- gfx950 kernels of the learned subgrid closure: a 6-layer
 // periodic 3x3 CNN (32 channels, GELU) with BF16 weights and activations and
-// FP32 accumulation on the matrix cores (v_mfma_f32_16x16x16_bf16), its
-// backward pass, an MSE loss and Adam.  No reference counterpart.
+// FP32 accumulation on the matrix cores (gfx950's v_mfma_f32_16x16x32_bf16
+// for the 32-channel inputs, v_mfma_f32_16x16x16_bf16 for the 16-channel
+// ones), its backward pass, an MSE loss and Adam.  No reference counterpart.
 //
 // Layouts (all per sample, pixels row-major, channels fastest, "NHWC"):
 //   activations  bf16 [B][H][W][C], C = 16 (input, 3 used) or 32
@@ -29,6 +30,8 @@ int r2d3c_internal_set_error(int code, const char *msg);
 namespace {
 
 typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short bf16;
 
@@ -86,7 +89,10 @@ __global__ __launch_bounds__(64 * kConvWaves) void k_conv3x3(const bf16 *__restr
                                                   float *__restrict__ Zout, bf16 *__restrict__ Y,
                                                   int B, int H, int W, float clamp,
                                                   const bf16 *__restrict__ X0, float keep) {
-  constexpr int CI16 = CI / 16, CO16 = CO / 16, WS = CI + 4, TW = kTilesPerWave;
+  // weight rows padded by 4 (16x16x16 path) or 8 (16x16x32 path: keeps the
+  // 16-byte operand reads aligned) bf16 to spread the LDS banks
+  constexpr int CI16 = CI / 16, CO16 = CO / 16, WS = CI + (CI % 32 == 0 ? 8 : 4),
+                TW = kTilesPerWave;
   __shared__ bf16 ws[9 * CO * WS];
   for (int e = threadIdx.x; e < 9 * CO * (CI / 4); e += 64 * kConvWaves) {
     const int row = e / (CI / 4), c4 = e - row * (CI / 4);
@@ -117,6 +123,35 @@ __global__ __launch_bounds__(64 * kConvWaves) void k_conv3x3(const bf16 *__restr
 #pragma unroll
   for (int t = 0; t < 9; ++t) {
     const int dy = t / 3 - 1, dx = t % 3 - 1;
+    if constexpr (CI % 32 == 0) {
+      // gfx950's 16x16x32 BF16 MFMA: lane l holds 8 consecutive channels
+      // 8(l/16)..8(l/16)+7 of pixel l%16 (A) and the same channels of the
+      // weights of output channel l%16 (B); A and B use one channel order,
+      // so the reduction over K is the layer's channel sum
+#pragma unroll
+      for (int c = 0; c < CI / 32; ++c) {
+        s16x8 a[TW];
+#pragma unroll
+        for (int i = 0; i < TW; ++i) {
+          int yy = ty[i] + dy, xx = tx[i] + m + dx;
+          yy += yy < 0 ? H : (yy >= H ? -H : 0);
+          xx += xx < 0 ? W : (xx >= W ? -W : 0);
+          a[i] = ok[i] ? *(const s16x8 *)(X + ((size_t)((size_t)tb[i] * H + yy) * W + xx) * CI +
+                                          32 * c + 8 * q)
+                       : s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int n = 0; n < CO16; ++n) {
+          const s16x8 w = *(const s16x8 *)(ws + (t * CO + 16 * n + m) * WS + 32 * c + 8 * q);
+#pragma unroll
+          for (int i = 0; i < TW; ++i)
+            acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                __builtin_bit_cast(bf16x8, a[i]), __builtin_bit_cast(bf16x8, w), acc[i][n], 0, 0,
+                0);
+        }
+      }
+      continue;
+    }
 #pragma unroll
     for (int c = 0; c < CI16; ++c) {
       s16x4 a[TW];
@@ -420,11 +455,16 @@ void conv(const r2d3c_cnn *c, int l, const bf16 *X, const bf16 *Wt, const float 
 // inference options: clamp > 0 saturates the outputs to [-clamp, clamp]; with
 // sign_clip > 0 the output channels 0, 1 are scaled by 1 - sign_clip where
 // they share the sign of input channels 0, 1
-int forward(r2d3c_cnn *c, int B, hipStream_t s, float clamp = 0.f, float sign_clip = 0.f) {
+// keep_z: store the pre-activations of layers 0-4 (the backward pass needs
+// them for GELU'); the last layer's Z (the prediction) is always stored.
+int forward(r2d3c_cnn *c, int B, hipStream_t s, float clamp = 0.f, float sign_clip = 0.f,
+            bool keep_z = false) {
   const bf16 *wf = c->wf;
-  conv<16, 32, CONV_GELU>(c, 0, c->act[0], wf + woff(0), nullptr, c->z[0], c->act[1], B, s);
+  conv<16, 32, CONV_GELU>(c, 0, c->act[0], wf + woff(0), nullptr, keep_z ? c->z[0] : nullptr,
+                          c->act[1], B, s);
   for (int l = 1; l < 5; ++l)
-    conv<32, 32, CONV_GELU>(c, l, c->act[l], wf + woff(l), nullptr, c->z[l], c->act[l + 1], B, s);
+    conv<32, 32, CONV_GELU>(c, l, c->act[l], wf + woff(l), nullptr, keep_z ? c->z[l] : nullptr,
+                            c->act[l + 1], B, s);
   conv<32, 16, CONV_LINEAR>(c, 5, c->act[5], wf + woff(5), nullptr, c->z[5], c->act[6], B, s,
                             clamp, sign_clip > 0.f ? c->act[0] : nullptr, 1.f - sign_clip);
   KCHK("cnn forward");
@@ -647,7 +687,7 @@ int r2d3c_cnn_gradients(r2d3c_cnn *c, const float *d_in, const float *d_target, 
   int rc = check_batch(c, batch);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
-  if ((rc = pack(c, d_in, batch, s)) || (rc = forward(c, batch, s)) ||
+  if ((rc = pack(c, d_in, batch, s)) || (rc = forward(c, batch, s, 0.f, 0.f, true)) ||
       (rc = loss_grad(c, d_target, batch, d_loss, s)) || (rc = backward(c, batch, s)))
     return rc;
   if (d_grad)
