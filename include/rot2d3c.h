@@ -285,6 +285,14 @@ int r2d3c_plan_get_column_radix(const r2d3c_plan *plan, int *form);
  * either way. */
 int r2d3c_plan_set_column_split(r2d3c_plan *plan, int mode);
 int r2d3c_plan_get_column_split(const r2d3c_plan *plan, int *mode);
+/* Form of the non-persistent row kernel (schedule (2, 0)) for n <= 512, in
+ * r2d3c_step / r2d3c_run and ensembles: 1 the three field pairs' inverse
+ * transforms run in parallel by three thread groups of one block (a
+ * latency-bound launch finishes sooner), 0 one after another; -1 (default)
+ * automatic: parallel for n <= 512.  The two forms agree within rounding;
+ * larger grids always use the sequential form. */
+int r2d3c_plan_set_row_split(r2d3c_plan *plan, int mode);
+int r2d3c_plan_get_row_split(const r2d3c_plan *plan, int *mode);
 /* Times every valid schedule (row kernel and launch shape x column radix
  * schedule) on scratch state (best of three trials of `reps` fixed-dt steps
  * each), keeps the fastest in the plan and returns its milliseconds per step (may be NULL).
@@ -315,6 +323,8 @@ int r2d3c_ensemble_destroy(r2d3c_ensemble *ens);
 int r2d3c_ensemble_set_schedule(r2d3c_ensemble *ens, int n, int row_rows, int row_blocks_per_cu);
 /* r2d3c_plan_set_column_radix for the runs of size n. */
 int r2d3c_ensemble_set_column_radix(r2d3c_ensemble *ens, int n, int form);
+/* r2d3c_plan_set_row_split for the runs of size n. */
+int r2d3c_ensemble_set_row_split(r2d3c_ensemble *ens, int n, int mode);
 /* r2d3c_plan_set_column_split for the runs of size n. */
 int r2d3c_ensemble_set_column_split(r2d3c_ensemble *ens, int n, int mode);
 /* Advances every run n_steps steps.  forcing: NULL, or an array of

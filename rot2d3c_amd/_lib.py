@@ -73,6 +73,9 @@ SIGNATURES = {
     "r2d3c_timing_enable": (_c_int, [_vp, _c_int]),
     "r2d3c_timing_read": (_c_int, [_vp, ctypes.POINTER(_c_double), ctypes.POINTER(ctypes.c_longlong)]),
     "r2d3c_copy_probe": (_c_int, [_vp, _vp, ctypes.c_longlong, _vp]),
+    "r2d3c_plan_set_row_split": (_c_int, [_vp, _c_int]),
+    "r2d3c_plan_get_row_split": (_c_int, [_vp, ctypes.POINTER(_c_int)]),
+    "r2d3c_ensemble_set_row_split": (_c_int, [_vp, _c_int, _c_int]),
     "r2d3c_plan_set_schedule": (_c_int, [_vp, _c_int, _c_int]),
     "r2d3c_plan_get_schedule": (_c_int, [_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),
     "r2d3c_autotune": (_c_int, [_vp, _c_int, ctypes.POINTER(_c_double), _vp]),

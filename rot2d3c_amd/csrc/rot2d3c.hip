@@ -972,6 +972,81 @@ __global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
 }
 
 
+// Row kernel with the three field pairs in parallel, for small grids (n <=
+// 512), where a launch is latency-bound: its chain is one load -> inverse
+// transform -> products -> forward transform instead of three loads and
+// inverse transforms in turn.  Thread group g = threadIdx.x / (T RW) loads
+// field pair g (psi, w = -k^2 psi, vz) of the RW rows, runs its inverse
+// transform in LDS region g and publishes the result there; every group then
+// forms the products from the three regions and runs the forward transform
+// in its own region (the same arithmetic in every group, so no thread waits
+// at a barrier the others skip); group 0 stores.  The arithmetic is
+// k_step_row's (results agree within the rounding of multiply-add fusion).
+template <int N>
+__global__ __launch_bounds__(3 * (N / 8) * RowStep<N>::ROWS) void k_step_row3(
+    const cplx *__restrict__ M, cplx *__restrict__ Q, StepConst c, unsigned long long *maxbits,
+    const cplx *__restrict__ tw, size_t mstride, const double *__restrict__ om_tab) {
+  constexpr int T = N / 8, RW = RowStep<N>::ROWS, GT = T * RW, LN = RW * Shape<N>::LDSN;
+  __shared__ cplx lds[3 * LN];
+  const int g = threadIdx.x / GT, t = threadIdx.x - g * GT;
+  const int r = t % RW, b = t / RW;
+  const int y = xcd_group<RowStep<N>::G>(blockIdx.x) * RW + r;
+  M += blockIdx.y * mstride;
+  Q += blockIdx.y * mstride;
+  const double two_omega_n2 = om_tab ? om_tab[blockIdx.y] : c.two_omega_n2;
+  const int nc = c.g.nc, kp = c.g.kp;
+  const size_t fs = c.fs;
+  cplx *P = lds + g * LN + r;
+  cplx v[8];
+  int b0 = b;
+  asm volatile("" : "+v"(b0));
+  load_packed_step<N>(v, M + (size_t)(2 * g) * fs, M + (size_t)(2 * g + 1) * fs, y, b0, nc, kp);
+  int b1 = b;
+  asm volatile("" : "+v"(b1));
+  fft_canon<N, +1, RW>(v, P, b1, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) P[pad(b + T * q) * RW] = v[q];
+  __syncthreads();
+  double m = 0.0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int o = pad(b + T * q) * RW + r;
+    const cplx z1 = lds[o], z2 = lds[LN + o], z3 = lds[2 * LN + o];
+    const double px = z1.x, py = z1.y, vx = z3.x, vy = z3.y;
+    m = fmax(m, px * px + py * py);
+    const double j1 = px * z2.y - py * z2.x;
+    const double p1 = (-j1 + two_omega_n2 * vy) * c.inv_n4;
+    const double p2 = ((px * vy - py * vx) + two_omega_n2 * py) * c.inv_n4;
+    v[q] = mk(p1, p2);
+  }
+  if (maxbits) {  // every group holds the same values: a redundant, exact max
+    for (int off = 32; off > 0; off >>= 1)
+      if (off < (int)blockDim.x) m = fmax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(m));
+  }
+  __syncthreads();  // every region read before the forward transforms reuse them
+  int b6 = b;
+  asm volatile("" : "+v"(b6));
+  fft_canon<N, -1, RW>(v, P, b6, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) P[pad(b + T * q) * RW] = v[q];
+  __syncthreads();
+  if (g == 0) {
+    int b7 = b;
+    asm volatile("" : "+v"(b7));
+    unpack_store<N, RW, N / 3 + 1>(P, b7, nc, kp, Q, y, Q + fs, y);
+  }
+}
+
+// Row kernel of a stage: the parallel-pair form for small grids when the
+// plan's row split allows it (-1: automatic, n <= 512; 1: always; 0: never),
+// else the non-persistent kernel.  blockIdx.y: the run (count runs).
+template <int N>
+static void launch_row_np(const r2d3c_plan *p, const cplx *M, cplx *Q, const StepConst &c,
+                          unsigned long long *macc, const cplx *tw, size_t ms, const double *om,
+                          int count, hipStream_t s, hipEvent_t ea, hipEvent_t eb);
+
+
 // Persistent variant of the row kernel for n >= 256.  A block walks its y
 // tiles; the inputs of each packed transform (a field pair of RW rows) are
 // copied straight into LDS with global_load_lds_dwordx4 while the previous
@@ -1358,6 +1433,9 @@ struct r2d3c_plan {
   // rows per block and blocks per CU of the persistent kernel, or bpc = 0
   // for the non-persistent kernel (2 rows per block)
   int row_rw = 2, row_bpc = 0;
+  // the non-persistent row kernel's form for n <= 512: -1 automatic (the
+  // parallel field pairs of k_step_row3), 0 sequential (k_step_row), 1 parallel
+  int row_split = -1;
   // radix schedule of the column transforms for n >= 512: 0 the four-step
   // form, 1 radix-8 Stockham passes (fft_canon)
   int col_cs = 0;
@@ -1549,6 +1627,19 @@ static int make_step_const(r2d3c_plan *p, const r2d3c_params *q, int dt_mode, do
   return 0;
 }
 
+template <int N>
+static void launch_row_np(const r2d3c_plan *p, const cplx *M, cplx *Q, const StepConst &c,
+                          unsigned long long *macc, const cplx *tw, size_t ms, const double *om,
+                          int count, hipStream_t s, hipEvent_t ea, hipEvent_t eb) {
+  constexpr int RW = RowStep<N>::ROWS;
+  if (N <= 512 && p->row_split != 0)
+    hipExtLaunchKernelGGL(k_step_row3<(N <= 512 ? N : 512)>, dim3(N / RW, count),
+                          dim3(3 * (N / 8) * RW), 0, s, ea, eb, 0, M, Q, c, macc, tw, ms, om);
+  else
+    hipExtLaunchKernelGGL(k_step_row<N>, dim3(N / RW, count), dim3(N / 8 * RW), 0, s, ea, eb, 0,
+                          M, Q, c, macc, tw, ms, om);
+}
+
 // Kernels of one time step.  first: the state's column transforms are made
 // here (otherwise the previous step's last stage left them in M); chain: the
 // last stage also emits the next step's column transforms; masked: the state
@@ -1575,9 +1666,7 @@ static int step_seq(r2d3c_plan *p, StepConst &c, cplx *psi, cplx *vz, hipStream_
     ev_pair(p, 1, &ea, &eb);
     if constexpr (RowV2<N, 2>::THREADS >= 64) {
       if (p->row_bpc == 0)
-        hipExtLaunchKernelGGL(k_step_row<N>, dim3(N / RowStep<N>::ROWS),
-                              dim3(N / 8 * RowStep<N>::ROWS), 0, s, ea, eb, 0, (const cplx *)M, Q,
-                              c, macc, tw, (size_t)0, (const double *)nullptr);
+        launch_row_np<N>(p, M, Q, c, macc, tw, 0, nullptr, 1, s, ea, eb);
       else if (p->row_rw == 1 && RowV2<N, 1>::THREADS >= 64)
         hipExtLaunchKernelGGL(k_step_row2<N, 1>, dim3(row2_grid<N, 1>(p->cus, p->row_bpc)),
                               dim3(RowV2<N, 1>::THREADS), 0, s, ea, eb, 0, (const cplx *)M, Q, c,
@@ -1587,8 +1676,7 @@ static int step_seq(r2d3c_plan *p, StepConst &c, cplx *psi, cplx *vz, hipStream_
                               dim3(RowV2<N, 2>::THREADS), 0, s, ea, eb, 0, (const cplx *)M, Q, c,
                               macc, tw, (size_t)0, (const double *)nullptr);
     } else {
-      hipExtLaunchKernelGGL(k_step_row<N>, dim3(N / RW), dim3(N / 8 * RW), 0, s, ea, eb, 0,
-                            (const cplx *)M, Q, c, macc, tw, (size_t)0, (const double *)nullptr);
+      launch_row_np<N>(p, M, Q, c, macc, tw, 0, nullptr, 1, s, ea, eb);
     }
     int mode = FM_EMIT;
     if (o == 1) mode = FM_STATE | (masked ? FM_DISK : FM_ZERO) | (chain ? FM_EMIT : 0);
@@ -2325,6 +2413,19 @@ int r2d3c_plan_set_column_split(r2d3c_plan *p, int mode) {
   return 0;
 }
 
+int r2d3c_plan_set_row_split(r2d3c_plan *p, int mode) {
+  NEED(p);
+  if (mode < -1 || mode > 1) return set_err(R2D3C_EINVAL, "row split: -1, 0 or 1");
+  p->row_split = mode;
+  return 0;
+}
+
+int r2d3c_plan_get_row_split(const r2d3c_plan *p, int *mode) {
+  NEED(p); NEED(mode);
+  *mode = p->row_split;
+  return 0;
+}
+
 int r2d3c_plan_get_column_split(const r2d3c_plan *p, int *mode) {
   NEED(p); NEED(mode);
   *mode = p->col_split;
@@ -2504,8 +2605,7 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
     cplx *Q0 = k.work + 6 * p->fs;
     if constexpr (RowV2<N, 2>::THREADS >= 64) {
       if (p->row_bpc == 0)
-        k_step_row<N><<<dim3(N / RowStep<N>::ROWS, k.count), N / 8 * RowStep<N>::ROWS, 0, s>>>(
-            M0, Q0, k.c0, nullptr, tw, ms, k.d_om);
+        launch_row_np<N>(p, M0, Q0, k.c0, nullptr, tw, ms, k.d_om, k.count, s, nullptr, nullptr);
       else if (p->row_rw == 1 && RowV2<N, 1>::THREADS >= 64)
         k_step_row2<N, 1><<<dim3(row2_grid<N, 1>(p->cus, p->row_bpc), k.count),
                             RowV2<N, 1>::THREADS, 0, s>>>(M0, Q0, k.c0, nullptr, tw, ms, k.d_om);
@@ -2513,9 +2613,7 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
         k_step_row2<N, 2><<<dim3(row2_grid<N, 2>(p->cus, p->row_bpc), k.count),
                             RowV2<N, 2>::THREADS, 0, s>>>(M0, Q0, k.c0, nullptr, tw, ms, k.d_om);
     } else {
-      constexpr int RW = RowStep<N>::ROWS;
-      k_step_row<N><<<dim3(N / RW, k.count), N / 8 * RW, 0, s>>>(M0, Q0, k.c0, nullptr, tw, ms,
-                                                                 k.d_om);
+      launch_row_np<N>(p, M0, Q0, k.c0, nullptr, tw, ms, k.d_om, k.count, s, nullptr, nullptr);
     }
     int mode = FM_EMIT;
     if (o == 1) mode = FM_STATE | (masked ? FM_DISK : FM_ZERO) | (chain ? FM_EMIT : 0);
@@ -2711,6 +2809,17 @@ int r2d3c_ensemble_set_schedule(r2d3c_ensemble *e, int n, int row_rows, int row_
   for (auto &k : e->cls)
     if (k.n == n) {
       int rc = r2d3c_plan_set_schedule(k.plan, row_rows, row_blocks_per_cu);
+      if (!rc) e->graph_valid = false;
+      return rc;
+    }
+  return set_err(R2D3C_EINVAL, "no run of that size in the ensemble");
+}
+
+int r2d3c_ensemble_set_row_split(r2d3c_ensemble *e, int n, int mode) {
+  NEED(e);
+  for (auto &k : e->cls)
+    if (k.n == n) {
+      int rc = r2d3c_plan_set_row_split(k.plan, mode);
       if (!rc) e->graph_valid = false;
       return rc;
     }

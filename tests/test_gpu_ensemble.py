@@ -371,3 +371,36 @@ def test_config3_mix_runs_equal_solo_runs_and_oracle(schedules, cuda):
     finally:
         O.set_fft_workers(None)
     ens.close()
+
+
+@pytest.mark.parametrize("n", [16, 64, 256, 512])
+def test_row_split_forms_agree(n, cuda):
+    """The row kernel with the three field pairs' inverse transforms in
+    parallel (row split 1, the default for n <= 512) and the sequential one
+    (0) agree within rounding over 3 CFL steps; with either form a chained
+    run equals repeated single steps bit for bit."""
+    print("row split forms n=%d" % n, flush=True)
+    from rot2d3c_amd import spectral as S, _lib
+    from rot2d3c_amd._lib import check
+    pl = S.Plan.get(n, cuda)
+    mode = ctypes.c_int()
+    check(_lib.lib().r2d3c_plan_get_row_split(pl.handle, ctypes.byref(mode)), "get_row_split")
+    assert mode.value == -1
+    assert _lib.lib().r2d3c_plan_set_row_split(pl.handle, 2) != 0
+    ps0, vz0 = random_state(n, 91)
+    p = S.Params(1e-3, 0.1, 1, 2, 2e-3, 0.05, 1, 2, 1.5)
+    out = {}
+    try:
+        for m in (0, 1):
+            check(_lib.lib().r2d3c_plan_set_row_split(pl.handle, m), "set_row_split")
+            a = (torch.from_numpy(ps0).to(cuda), torch.from_numpy(vz0).to(cuda))
+            b = (torch.from_numpy(ps0).to(cuda), torch.from_numpy(vz0).to(cuda))
+            for _ in range(3):
+                S.rk4_step(a[0], a[1], p, cfl=0.5)
+            S.rk4_run(b[0], b[1], p, None, 3, cfl=0.5)
+            torch.cuda.synchronize()
+            assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1]), m
+            out[m] = (a[0].cpu().numpy(), a[1].cpu().numpy())
+    finally:
+        check(_lib.lib().r2d3c_plan_set_row_split(pl.handle, -1), "set_row_split")
+    assert rel_l2(out[1][0], out[0][0]) < 1e-12 and rel_l2(out[1][1], out[0][1]) < 1e-12
