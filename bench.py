@@ -266,7 +266,8 @@ def ensemble_bench(steps, dev, tuned, warmup=10):
             "algorithmic_bytes_per_step": alg,
             "frac_of_peak": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
             "model_76S_frac_of_peak": round(b76 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
-            "finite": finite, "schedules": schedules}
+            "finite": finite, "schedules": schedules,
+            "size_classes": "concurrent graph branches" if ens.concurrent else "one after another"}
 
 
 def main():

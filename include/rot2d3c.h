@@ -314,15 +314,20 @@ typedef struct r2d3c_ensemble r2d3c_ensemble;
 
 /* Groups the runs by grid size and allocates each run's workspace.  Runs of
  * one size advance together: one launch per kernel and stage covers all of
- * them, and the sizes are independent branches of one captured HIP graph per
- * time step.  Each run's result equals r2d3c_run of that run alone (fixed dt,
- * same forcing) bit for bit. */
+ * them, and the sizes follow each other (or run as concurrent branches,
+ * r2d3c_ensemble_set_concurrent) in one captured HIP graph per time step.
+ * Each run's result equals r2d3c_run of that run alone (fixed dt, same
+ * forcing) bit for bit. */
 int r2d3c_ensemble_create(r2d3c_ensemble **ens, const r2d3c_member *members, int count);
 int r2d3c_ensemble_destroy(r2d3c_ensemble *ens);
 /* r2d3c_plan_set_schedule for the runs of size n. */
 int r2d3c_ensemble_set_schedule(r2d3c_ensemble *ens, int n, int row_rows, int row_blocks_per_cu);
 /* r2d3c_plan_set_column_radix for the runs of size n. */
 int r2d3c_ensemble_set_column_radix(r2d3c_ensemble *ens, int n, int form);
+/* How the size classes of a time step are ordered: 0 (default) one after
+ * another, each class's launches having the whole GPU; 1 as concurrent
+ * branches of the step graph.  Results are identical either way. */
+int r2d3c_ensemble_set_concurrent(r2d3c_ensemble *ens, int concurrent);
 /* r2d3c_plan_set_row_split for the runs of size n. */
 int r2d3c_ensemble_set_row_split(r2d3c_ensemble *ens, int n, int mode);
 /* r2d3c_plan_set_column_split for the runs of size n. */

@@ -2571,6 +2571,9 @@ struct r2d3c_ensemble {
   hipGraphExec_t exec2 = nullptr;
   const ForceSet *graph_ftab = nullptr;
   bool graph_valid = false;
+  // size classes of a step: 0 (default) one after another on the ensemble's
+  // stream, 1 as concurrent branches of the step graph
+  int concurrent = 0;
 };
 
 template <int N>
@@ -2656,7 +2659,7 @@ static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, bool closures,
   HIPCHK(hipStreamBeginCapture(e->es, hipStreamCaptureModeThreadLocal));
   int rc = 0;
   hipError_t he = hipEventRecord(e->fork, e->es);
-  // largest sizes first: their branch is the longest path of the step
+  // largest sizes first: with concurrent branches theirs is the longest path
   std::vector<EnsClass *> by_size;
   for (auto &k : e->cls) by_size.push_back(&k);
   std::stable_sort(by_size.begin(), by_size.end(),
@@ -2664,6 +2667,10 @@ static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, bool closures,
   for (EnsClass *kp : by_size) {
     EnsClass &k = *kp;
     if (he != hipSuccess || rc) break;
+    if (!e->concurrent) {  // one class after another, each with the whole GPU
+      rc = ens_dispatch(e, k, ftab, e->es, false, true, true, closures);
+      continue;
+    }
     he = hipStreamWaitEvent(k.st, e->fork, 0);
     if (he == hipSuccess) rc = ens_dispatch(e, k, ftab, k.st, false, true, true, closures);
     if (he == hipSuccess && !rc) he = hipEventRecord(k.done, k.st);
@@ -2813,6 +2820,14 @@ int r2d3c_ensemble_set_schedule(r2d3c_ensemble *e, int n, int row_rows, int row_
       return rc;
     }
   return set_err(R2D3C_EINVAL, "no run of that size in the ensemble");
+}
+
+int r2d3c_ensemble_set_concurrent(r2d3c_ensemble *e, int concurrent) {
+  NEED(e);
+  if (concurrent != 0 && concurrent != 1) return set_err(R2D3C_EINVAL, "concurrent: 0 or 1");
+  if (e->concurrent != concurrent) e->graph_valid = false;
+  e->concurrent = concurrent;
+  return 0;
 }
 
 int r2d3c_ensemble_set_row_split(r2d3c_ensemble *e, int n, int mode) {

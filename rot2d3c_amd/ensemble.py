@@ -51,6 +51,7 @@ class Ensemble:
                   "r2d3c_ensemble_create")
         self.handle = h
         self.sizes = sorted({m.n for m in self.members})
+        self.concurrent = 0
         self.schedules = {n: autotune.DEFAULT for n in self.sizes}
         for n in self.sizes:
             if schedules == "auto":
@@ -77,9 +78,10 @@ class Ensemble:
         return rc
 
     def tune(self, steps=20, trials=5):
-        """Choose every size's schedule by timing the whole ensemble step
-        graph, where the sizes share the GPU (a solo plan's fastest schedule
-        need not be the fastest there): one coordinate-descent sweep over the
+        """Choose the order of the size classes (one after another or
+        concurrent) and every size's schedule by timing the whole ensemble
+        step graph (a solo plan's fastest schedule need not be the fastest
+        there): the order first, then one coordinate-descent sweep over the
         sizes, largest first, each trying every valid schedule with the other
         sizes fixed (best of `trials` timings of `steps` unforced steps).
         The runs' states are restored afterwards, so tuning leaves the
@@ -99,6 +101,16 @@ class Ensemble:
             return best
 
         best_ms = timed()
+        # the order of the size classes in a step: one after another (the
+        # default) or as concurrent graph branches
+        if len(self.sizes) > 1:
+            keep = self.concurrent
+            self.set_concurrent(1 - keep)
+            ms = timed()
+            if ms < best_ms:
+                best_ms = ms
+            else:
+                self.set_concurrent(keep)
         for n in sorted(self.sizes, reverse=True):
             for cand in autotune.CANDIDATES:
                 if cand == self.schedules[n] or (cand[2] and n < 512):
@@ -116,6 +128,13 @@ class Ensemble:
             m.vz.copy_(vz)
         torch.cuda.synchronize(self.device)
         return dict(self.schedules), best_ms
+
+    def set_concurrent(self, concurrent):
+        """Size classes of a step one after another (0, the default) or as
+        concurrent branches of the step graph (1); results are identical."""
+        check(_lib.lib().r2d3c_ensemble_set_concurrent(self.handle, int(concurrent)),
+              "r2d3c_ensemble_set_concurrent")
+        self.concurrent = int(concurrent)
 
     def set_closure(self, run, net=None, scale_in=None, scale_out=None, kmin=0.0, kclip=0.0):
         """Make run `run` a closure run with network `net` (closure.Net for one

@@ -404,3 +404,29 @@ def test_row_split_forms_agree(n, cuda):
     finally:
         check(_lib.lib().r2d3c_plan_set_row_split(pl.handle, -1), "set_row_split")
     assert rel_l2(out[1][0], out[0][0]) < 1e-12 and rel_l2(out[1][1], out[0][1]) < 1e-12
+
+
+def test_ensemble_order_of_size_classes_does_not_change_results(cuda):
+    """Size classes one after another (the default) and as concurrent graph
+    branches give the same bits, each run equal to its solo run."""
+    print("ensemble class order", flush=True)
+    from rot2d3c_amd import ensemble as E
+    spec = _spec([32, 64, 32, 128, 64], seed=13)
+    nsteps = 4
+    table = E.forcing_table(spec, nsteps)
+    outs = []
+    for conc in (0, 1):
+        ms, host = _members(spec, cuda)
+        ens = E.Ensemble(ms, schedules=None)
+        assert ens.concurrent == 0
+        ens.set_concurrent(conc)
+        ens.run(nsteps, table)
+        torch.cuda.synchronize()
+        outs.append([(m.ps.clone(), m.vz.clone()) for m in ms])
+        ens.close()
+    for r, ((a0, b0), (a1, b1)) in enumerate(zip(*outs)):
+        assert torch.equal(a0, a1) and torch.equal(b0, b1), r
+    ms, host = _members(spec, cuda)
+    for r, (m, h) in enumerate(zip(ms, host)):
+        ps, vz = _solo(m, h, [table[st][r] for st in range(nsteps)], nsteps, cuda)
+        assert torch.equal(outs[0][r][0], ps) and torch.equal(outs[0][r][1], vz), r
