@@ -173,34 +173,51 @@ def closure_bench(dns_ms_per_step, dns_dt, dev, steps=500, warmup=20, judge_wind
 def large_bench(steps, dev, warmup=2):
     """The large single run: 16384^2 FP64, fixed dt = 2e-5, nu = nuv = 5e-6,
     hnu = hnuv = 1.0, Omega = 10, forcing kf = 256 (psi and vz, injection
-    1.0), random-phase start.  At this size the step runs stage by stage from
-    2D transforms of the dealiased columns (two 8192-point halves per
-    transform)."""
+    1.0), random-phase start.  At this size the step runs from 2D transforms
+    of the dealiased columns (two 8192-point halves per transform) with the
+    fused bracket pass (both Jacobians and their row transforms in one pass,
+    the update inside the second column transform); the stage-by-stage form
+    (six real derivative fields, separate update) is timed beside it on
+    min(steps, 20) further steps."""
     import torch
-    from rot2d3c_amd import ic, spectral as S
+    from rot2d3c_amd import _lib, ic, spectral as S
     n, dt = 16384, 2e-5
     ps0, vz0 = ic.random_phase_ic(n, seed=1234)
     ps = torch.from_numpy(ps0).to(dev)
     vz = torch.from_numpy(vz0).to(dev)
     del ps0, vz0
     p = S.Params(5e-6, 1.0, 1, 2, 5e-6, 1.0, 1, 2, 10.0)
-    sched = ic.forcing_schedule(256.0, 1.0, 1.0, 0, warmup + steps, S.rand_force_draw)
+    nref = min(steps, 20)
+    sched = ic.forcing_schedule(256.0, 1.0, 1.0, 0, warmup + steps + nref, S.rand_force_draw)
     draws = [(S.ForceDraw(*a), S.ForceDraw(*b)) for a, b in sched]
+    plan = S.Plan.get(n, dev).handle
+    _lib.check(_lib.lib().r2d3c_plan_set_bracket_pass(plan, 1), "set_bracket_pass")
     S.rk4_run(ps, vz, p, draws[:warmup], warmup, dt=dt)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    S.rk4_run(ps, vz, p, draws[warmup:], steps, dt=dt)
+    S.rk4_run(ps, vz, p, draws[warmup:warmup + steps], steps, dt=dt)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     finite = bool(torch.isfinite(torch.view_as_real(ps)).all())
     ms = wall * 1e3 / steps
+    ms_ref = None
+    if nref > 0:  # the stage-by-stage form on the steps that follow
+        _lib.check(_lib.lib().r2d3c_plan_set_bracket_pass(plan, 0), "set_bracket_pass")
+        t0 = time.perf_counter()
+        S.rk4_run(ps, vz, p, draws[warmup + steps:], nref, dt=dt)
+        torch.cuda.synchronize()
+        ms_ref = round((time.perf_counter() - t0) * 1e3 / nref, 3)
+        _lib.check(_lib.lib().r2d3c_plan_set_bracket_pass(plan, 1), "set_bracket_pass")
+        finite = finite and bool(torch.isfinite(torch.view_as_real(ps)).all())
     b76 = 76 * 16 * n * (n // 2 + 1)
     S.Plan.release(n, dev)
     del ps, vz
     torch.cuda.empty_cache()
-    return {"workload": "16384^2 FP64, dt=2e-5, nu=5e-6, nu+=1.0, Omega=10, kf=256, stage-by-stage step",
+    return {"workload": "16384^2 FP64, dt=2e-5, nu=5e-6, nu+=1.0, Omega=10, kf=256, "
+                        "step with the fused bracket pass",
             "kernels": "profiles/step16384_kernels.md",
             "steps": steps, "ms_per_step": round(ms, 3),
+            "stage_by_stage_ms_per_step": ms_ref, "stage_by_stage_steps": nref,
             "value": round(n * n * steps / wall / 1e6, 2), "unit": "Mpts·RK4-steps/s",
             "model_76S_frac_of_peak": round(b76 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
             "finite": finite}

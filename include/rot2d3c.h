@@ -293,6 +293,13 @@ int r2d3c_plan_get_column_split(const r2d3c_plan *plan, int *mode);
  * larger grids always use the sequential form. */
 int r2d3c_plan_set_row_split(r2d3c_plan *plan, int mode);
 int r2d3c_plan_get_row_split(const r2d3c_plan *plan, int *mode);
+/* Form of the n = 16384 step (other sizes ignore it): 1 (default) the fused
+ * bracket pass, where both Jacobians of a stage and their row transforms are
+ * formed in one pass over the rows and the second one's column transform
+ * carries the update; 0 stage by stage through six real derivative fields.
+ * The two agree within rounding. */
+int r2d3c_plan_set_bracket_pass(r2d3c_plan *plan, int fused);
+int r2d3c_plan_get_bracket_pass(const r2d3c_plan *plan, int *fused);
 /* Times every valid schedule (row kernel and launch shape x column radix
  * schedule) on scratch state (best of three trials of `reps` fixed-dt steps
  * each), keeps the fastest in the plan and returns its milliseconds per step (may be NULL).

@@ -75,6 +75,8 @@ SIGNATURES = {
     "r2d3c_copy_probe": (_c_int, [_vp, _vp, ctypes.c_longlong, _vp]),
     "r2d3c_plan_set_row_split": (_c_int, [_vp, _c_int]),
     "r2d3c_plan_get_row_split": (_c_int, [_vp, ctypes.POINTER(_c_int)]),
+    "r2d3c_plan_set_bracket_pass": (_c_int, [_vp, _c_int]),
+    "r2d3c_plan_get_bracket_pass": (_c_int, [_vp, ctypes.POINTER(_c_int)]),
     "r2d3c_ensemble_set_concurrent": (_c_int, [_vp, _c_int]),
     "r2d3c_ensemble_set_row_split": (_c_int, [_vp, _c_int, _c_int]),
     "r2d3c_plan_set_schedule": (_c_int, [_vp, _c_int, _c_int]),

@@ -1442,6 +1442,9 @@ struct r2d3c_plan {
   // split emission of the fused column kernel's stages 1-3 (-1: when the
   // launch holds fewer threads than one wave per SIMD; 0 off; 1 on)
   int col_split = -1;
+  // n = 16384 step: 1 the fused bracket pass (k_big_row_jac2), 0 stage by
+  // stage through six real fields
+  int big_fused = 1;
   // kernel timing of r2d3c_step (r2d3c_timing_enable / r2d3c_timing_read)
   int timing = 0;
   std::vector<hipEvent_t> evpool;
@@ -1819,7 +1822,7 @@ static int run_steps(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
 // n = 16384 (definitions at the end of this file)
 constexpr int kBigN = 16384, kBigH = 8192, kBigT = 1024;
 static int big_c2r(r2d3c_plan *p, const cplx *in, int op, double *out, hipStream_t s,
-                   int ncols = kBigN / 2 + 1);
+                   int ncols = kBigN / 2 + 1, cplx *mix = nullptr);
 static int big_r2c(r2d3c_plan *p, const double *in, cplx *out, hipStream_t s);
 static int big_poisson(r2d3c_plan *p, const cplx *a, const cplx *b, cplx *c, hipStream_t s);
 static int big_cfl(r2d3c_plan *p, const r2d3c_params *q, double cfl, const cplx *psi, double *d_dt,
@@ -2423,6 +2426,19 @@ int r2d3c_plan_set_row_split(r2d3c_plan *p, int mode) {
 int r2d3c_plan_get_row_split(const r2d3c_plan *p, int *mode) {
   NEED(p); NEED(mode);
   *mode = p->row_split;
+  return 0;
+}
+
+int r2d3c_plan_set_bracket_pass(r2d3c_plan *p, int fused) {
+  NEED(p);
+  if (fused != 0 && fused != 1) return set_err(R2D3C_EINVAL, "bracket pass: 0 or 1");
+  p->big_fused = fused;
+  return 0;
+}
+
+int r2d3c_plan_get_bracket_pass(const r2d3c_plan *p, int *fused) {
+  NEED(p); NEED(fused);
+  *fused = p->big_fused;
   return 0;
 }
 
@@ -3411,27 +3427,21 @@ __host__ __device__ constexpr int big_kk_max(int q, int par) {
                : (q == 4 && par == 0 ? kBigN / 2 : kBigN - (2 * kBigT * q + par));
 }
 
-// rows y, y+1 of M (half spectra) -> R[y][:], R[y+1][:] as one packed
-// transform; columns kx >= KC are taken as zero (not read).  KC: the number
-// of columns read (n/3 + 1 for the step's dealiased fields, n/2 + 1 for full
+// Packed inputs of one 16384-point row transform in fft16k's input order
+// (ev[q]: k = 2(b + 1024 q), od[q]: k + 1): Z[k] = A[k] + i B[k] for k <= n/2
+// and conj(A[n-k]) + i conj(B[n-k]) above (packed_z), with A from row ya of
+// the mixed field FA and B from row yb of FB (tiled layout, pitch kp).
+// Columns kx >= KC are taken as zero (not read): KC is the number of columns
+// read (n/3 + 1 for the step's dealiased fields, n/2 + 1 for full
 // transforms); slots wholly past it are neither loaded nor tested, slots
-// wholly below it are not tested.
+// wholly below it are not tested.  Two q slots (four points) at a time: a
+// group's loads are issued before its arithmetic, without branches (points
+// past KC read column 0 and are discarded), so they are in flight together.
 template <int KC>
-__global__ __launch_bounds__(kBigT) void k_big_row_c2r(const cplx *__restrict__ mix,
-                                                        double *__restrict__ out, Geo g,
-                                                        const cplx *__restrict__ tw) {
+__device__ __forceinline__ void big_load_packed(cplx ev[8], cplx od[8], const cplx *__restrict__ FA,
+                                                int ya, const cplx *__restrict__ FB, int yb, int b,
+                                                int kp) {
   constexpr int N = kBigN, ncols = KC;
-  __shared__ cplx lds[Shape<kBigH>::LDSN];
-  // Consecutive workgroups go to the 8 XCDs in turn: the two row pairs of one
-  // 4-row tile (which share every 64-byte chunk of the tiled layout) are
-  // given to consecutive workgroups of one XCD, so the second finds the
-  // chunks in that XCD's L2.
-  const int xcd = blockIdx.x & 7, l = blockIdx.x >> 3;
-  const int b = threadIdx.x, y = 2 * ((((l >> 1) << 3) + xcd) * 2 + (l & 1));
-  // Packed inputs, two q slots (four points) at a time: a group's loads are
-  // issued before its arithmetic, without branches (points past ncols read
-  // column 0 and are discarded), so they are in flight together.
-  cplx ev[8], od[8];
 #pragma unroll
   for (int q0 = 0; q0 < 8; q0 += 2) {
     cplx ra[4], rb[4];
@@ -3442,8 +3452,8 @@ __global__ __launch_bounds__(kBigT) void k_big_row_c2r(const cplx *__restrict__ 
       const int k = 2 * (b + kBigT * q) + par;
       const int kk = k > N / 2 ? N - k : k;
       const int kl = (big_kk_max(q, par) < ncols || kk < ncols) ? kk : 0;
-      ra[u] = mix[tix(y, kl, g.kpf)];
-      rb[u] = mix[tix(y + 1, kl, g.kpf)];
+      ra[u] = FA[tix(ya, kl, kp)];
+      rb[u] = FB[tix(yb, kl, kp)];
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -3458,6 +3468,24 @@ __global__ __launch_bounds__(kBigT) void k_big_row_c2r(const cplx *__restrict__ 
     }
     __builtin_amdgcn_sched_barrier(0);  // the next group's loads stay below
   }
+}
+
+// rows y, y+1 of M (half spectra) -> R[y][:], R[y+1][:] as one packed
+// transform; columns kx >= KC are taken as zero (not read).
+template <int KC>
+__global__ __launch_bounds__(kBigT) void k_big_row_c2r(const cplx *__restrict__ mix,
+                                                        double *__restrict__ out, Geo g,
+                                                        const cplx *__restrict__ tw) {
+  constexpr int N = kBigN;
+  __shared__ cplx lds[Shape<kBigH>::LDSN];
+  // Consecutive workgroups go to the 8 XCDs in turn: the two row pairs of one
+  // 4-row tile (which share every 64-byte chunk of the tiled layout) are
+  // given to consecutive workgroups of one XCD, so the second finds the
+  // chunks in that XCD's L2.
+  const int xcd = blockIdx.x & 7, l = blockIdx.x >> 3;
+  const int b = threadIdx.x, y = 2 * ((((l >> 1) << 3) + xcd) * 2 + (l & 1));
+  cplx ev[8], od[8];
+  big_load_packed<KC>(ev, od, mix, y, mix, y + 1, b, g.kpf);
   fft16k<+1>(ev, od, lds, b, tw);
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
@@ -3555,6 +3583,164 @@ __global__ __launch_bounds__(kBigT) void k_big_col_fwd(const cplx *__restrict__ 
   }
 }
 
+// ---- the fused bracket pass of the n = 16384 step ----
+constexpr int kBigNC = kBigN / 3 + 1;  // kept columns kx < nc
+
+// DFT of length 16384, sign S, by decimation in frequency, for inputs in
+// natural halves: in ev[q] = x[b + 1024 q], od[q] = x[b + 1024 q + 8192];
+// out ev[q] = X[2(b + 1024 q)], od[q] = X[2(b + 1024 q) + 1].  The first
+// radix-2 pass pairs x[j] with x[j + n/2] in registers; the halves are then
+// two 8192-point transforms (even and odd outputs).
+template <int S>
+__device__ __forceinline__ void fft16k_dif(cplx ev[8], cplx od[8], cplx *lds, int b,
+                                           const cplx *__restrict__ tw) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const cplx e = ev[q], o = od[q];
+    ev[q] = cadd(e, o);
+    od[q] = cmul(csub(e, o), twid<S>(tw, b + kBigT * q));
+  }
+  int b1 = b, b2 = b;
+  asm volatile("" : "+v"(b1));
+  fft_canon<kBigH, S, 1, 2>(ev, lds, b1, tw);
+  asm volatile("" : "+v"(b2));
+  fft_canon<kBigH, S, 1, 2>(od, lds, b2, tw);
+}
+
+// psi_x + i psi_y along row y from the kept columns of its two column
+// transforms MX (d/dx) and MY (d/dy): ev[q] at x = b + 1024 q, od[q] at
+// x + 8192 (unnormalised, as c2r).  The step's bracket pass and the n = 16384
+// CFL_condition share it, so both see the same values.
+__device__ __forceinline__ void big_grad_row(cplx ev[8], cplx od[8], const cplx *__restrict__ MX,
+                                             const cplx *__restrict__ MY, int y, int b, int kp,
+                                             cplx *lds, const cplx *__restrict__ tw) {
+  big_load_packed<kBigNC>(ev, od, MX, y, MY, y, b, kp);
+  fft16k<+1>(ev, od, lds, b, tw);
+}
+// max over the block of psi_x^2 + psi_y^2 into maxbits (bit pattern, atomicMax)
+__device__ __forceinline__ void big_fold_max(const cplx ev[8], const cplx od[8],
+                                             unsigned long long *maxbits) {
+  double m = 0.0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    m = fmax(m, ev[q].x * ev[q].x + ev[q].y * ev[q].y);
+    m = fmax(m, od[q].x * od[q].x + od[q].y * od[q].y);
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(m));
+}
+
+// The two brackets of a stage along row y, in one pass (poisson's
+// arithmetic, src/pseudospec2D_Rot2D3C.f90:195-216, of
+// src/Rot2D3C.para.f90:572-573): z = psi_x + i psi_y is inverse transformed
+// in registers (big_grad_row); w_x, w_y (w = laplacian psi) and vz_x, vz_y
+// come as real rows; in place of z the block forms
+// W = J1 + i J2,  J1 = (psi_x w_y - psi_y w_x)/n^4,  J2 = (psi_x vz_y - psi_y vz_x)/n^4
+// and transforms it forward (fft16k_dif): row y of Z = FFT(J1) + i FFT(J2).
+// The two row spectra are split through LDS, A = (Z[k] + conj Z[n-k])/2 =
+// FFT(J1)[k] and B = (Z[k] - conj Z[n-k])/(2i) = FFT(J2)[k], and stored for
+// the kept k < nc in the tiled mixed layout (fa, fb; pitch kp), from which
+// the column passes read whole 64-byte chunks.  With maxbits the row's
+// maximum of psi_x^2 + psi_y^2 is folded in (CFL_condition's maximum).
+__global__ __launch_bounds__(kBigT) void k_big_row_jac2(
+    const cplx *__restrict__ MX, const cplx *__restrict__ MY, const double *__restrict__ wx,
+    const double *__restrict__ wy, const double *__restrict__ vx, const double *__restrict__ vy,
+    double n4, cplx *__restrict__ fa, cplx *__restrict__ fb, Geo g, const cplx *__restrict__ tw,
+    unsigned long long *maxbits) {
+  constexpr int N = kBigN;
+  __shared__ cplx lds[Shape<kBigH>::LDSN];
+  // the 4 rows of one tile (one 64-byte chunk per kx) run on consecutive
+  // workgroups of one XCD, so three of them find the chunks in its L2
+  const int xcd = blockIdx.x & 7, l = blockIdx.x >> 3;
+  const int b = threadIdx.x, y = 4 * (((l >> 2) << 3) + xcd) + (l & 3);
+  cplx ev[8], od[8];
+  big_grad_row(ev, od, MX, MY, y, b, g.kpf, lds, tw);
+  if (maxbits) big_fold_max(ev, od, maxbits);
+  const size_t r = (size_t)y * N;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const size_t x0 = r + b + kBigT * q, x1 = x0 + kBigH;
+    const double a0 = wx[x0], b0 = wy[x0], c0 = vx[x0], d0 = vy[x0];
+    const double a1 = wx[x1], b1 = wy[x1], c1 = vx[x1], d1 = vy[x1];
+    const double px0 = ev[q].x, py0 = ev[q].y, px1 = od[q].x, py1 = od[q].y;
+    ev[q] = mk((mul_rn(px0, b0) - py0 * a0) / n4, (mul_rn(px0, d0) - py0 * c0) / n4);
+    od[q] = mk((mul_rn(px1, b1) - py1 * a1) / n4, (mul_rn(px1, d1) - py1 * c1) / n4);
+    if (q & 1) __builtin_amdgcn_sched_barrier(0);  // two slots' loads in flight at a time
+  }
+  fft16k_dif<-1>(ev, od, lds, b, tw);
+  // even k (ev, index m = k/2), then odd k (od, m = (k-1)/2), through the
+  // LDS the transform has freed; Z[n-k] sits at index (n/2 - m) mod n/2 of
+  // the even half and n/2 - 1 - m of the odd half
+  const int kp = g.kp;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) lds[b + kBigT * q] = h ? od[q] : ev[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {  // 2 (b + 1024 q) < nc only for q < 3
+      const int m = b + kBigT * q, k = 2 * m + h;
+      if (k < kBigNC) {
+        const cplx z = h ? od[q] : ev[q], w = lds[h ? kBigH - 1 - m : (kBigH - m) & (kBigH - 1)];
+        fa[tix(y, k, kp)] = mk(0.5 * (z.x + w.x), 0.5 * (z.y - w.y));
+        fb[tix(y, k, kp)] = mk(0.5 * (z.y + w.y), -0.5 * (z.x - w.x));
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// CFL_condition's maximum at n = 16384 from the column transforms MX, MY of
+// d/dx psi and d/dy psi (the bracket pass's first transform)
+__global__ __launch_bounds__(kBigT) void k_big_row_gradmax(const cplx *__restrict__ MX,
+                                                            const cplx *__restrict__ MY, Geo g,
+                                                            const cplx *__restrict__ tw,
+                                                            unsigned long long *maxbits) {
+  __shared__ cplx lds[Shape<kBigH>::LDSN];
+  cplx ev[8], od[8];
+  big_grad_row(ev, od, MX, MY, blockIdx.x, threadIdx.x, g.kpf, lds, tw);
+  big_fold_max(ev, od, maxbits);
+}
+
+// Column i < nc of a bracket's row spectra (tiled mixed layout, pitch kp),
+// forward transformed along y: ev[q] = S[i][j], od[q] = S[i][j + 8192],
+// j = b + 1024 q.  Rows 2p and 2p+1 share a 64-byte chunk: one 32-byte load.
+__device__ __forceinline__ void big_col_fwd_tiled(cplx ev[8], cplx od[8],
+                                                  const cplx *__restrict__ f, int i, int kp, int b,
+                                                  cplx *lds, const cplx *__restrict__ tw) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const size_t t = tix(2 * (b + kBigT * q), i, kp);
+    ev[q] = f[t];
+    od[q] = f[t + 1];
+  }
+  fft16k<-1>(ev, od, lds, b, tw);
+}
+// Neighbouring columns (whose 64-byte chunks share 128-byte lines) go to
+// consecutive workgroups of one XCD, 8 at a time (workgroups go to the XCDs
+// in turn); grids are rounded up to 64 workgroups.
+__device__ __forceinline__ int big_col_of_block() {
+  const int xcd = blockIdx.x & 7, l = blockIdx.x >> 3;
+  return ((l >> 3) << 6) + (xcd << 3) + (l & 7);
+}
+
+// FFT(J1) of the kept columns -> spec[i][:] (the reference's C5)
+__global__ __launch_bounds__(kBigT) void k_big_col_fwd_j1(const cplx *__restrict__ fa, int kp,
+                                                           cplx *__restrict__ spec,
+                                                           const cplx *__restrict__ tw) {
+  constexpr int N = kBigN;
+  __shared__ cplx lds[Shape<kBigH>::LDSN];
+  const int i = big_col_of_block(), b = threadIdx.x;
+  if (i >= kBigNC) return;
+  cplx ev[8], od[8];
+  big_col_fwd_tiled(ev, od, fa, i, kp, b, lds, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    spec[(size_t)i * N + b + kBigT * q] = ev[q];
+    spec[(size_t)i * N + b + kBigT * q + kBigH] = od[q];
+  }
+}
+
 // a = a * b;  a = (a - b * c) / n4;  max of a^2 + b^2 (bit pattern, atomicMax)
 __global__ void k_real_mul(double *__restrict__ a, const double *__restrict__ b, size_t nn) {
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn;
@@ -3578,11 +3764,48 @@ __global__ __launch_bounds__(256) void k_real_gradmax(const double *__restrict__
   if ((threadIdx.x & 63) == 0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(m));
 }
 
-// The stage update of src/Rot2D3C.para.f90:578-599 with the step's forcing:
-// rand_force draws (amplitude from the device dt) and/or dense arrays.  c1, c2
-// hold the stage's psi and vz; the rotation terms' d/dy of them (C3, C4 of the
-// reference, :575-576) are formed here.  The result goes to o1, o2, which may
-// be c1, c2 or ps, vz themselves (each element is read before it is written).
+// The stage update of src/Rot2D3C.para.f90:578-599 at element e = (i, j) with
+// the step's forcing: rand_force draws (amplitude from the device dt) and/or
+// dense arrays.  c1, c2 hold the stage's psi and vz; the rotation terms' d/dy
+// of them (C3, C4 of the reference, :575-576) are formed here.  C6 comes from
+// c6[e] or, with REG, is t6r.  The result goes to o1, o2, which may be c1, c2
+// or ps, vz themselves (the element is read before it is written).
+template <bool REG>
+__device__ __forceinline__ void big_update_at(size_t e, int i, int j, const cplx *ps,
+                                              const cplx *vz, const cplx *c1, const cplx *c2,
+                                              cplx *o1, cplx *o2, const cplx *__restrict__ c5,
+                                              const cplx *__restrict__ c6, cplx t6r,
+                                              const StepConst &c, double dt, int o) {
+  const int n = kBigN;
+  const double kx = kxof(i, n), ky = kyof(j, n), ka2 = kx * kx + ky * ky;
+  cplx a = mk(0.0, 0.0), b = mk(0.0, 0.0);
+  if (ka2 <= c.g.kmax && ka2 > c.g.tiny) {
+    cplx f = mk(0.0, 0.0), g = mk(0.0, 0.0);
+    for (int q = 0; q < c.fp.nent; ++q)
+      if (c.fp.e[q].i == i && c.fp.e[q].j == j) f = force_value(c.fp, q, dt, c.n2);
+    for (int q = 0; q < c.fz.nent; ++q)
+      if (c.fz.e[q].i == i && c.fz.e[q].j == j) g = force_value(c.fz, q, dt, c.n2);
+    if (c.fpd) f = cadd(f, c.fpd[e]);
+    if (c.fzd) g = cadd(g, c.fzd[e]);
+    const cplx s1 = c1[e], s2 = c2[e];
+    const cplx t3 = mk(-ky * s1.y, ky * s1.x), t4 = mk(-ky * s2.y, ky * s2.x);
+    const double tmp1 = dt / (double)o, tmp2 = 1.0 / ka2;
+    double tmp3 = 1.0 + (c.nu * ipow(ka2, c.nn) + c.hnu * ipow(tmp2, c.mm)) * tmp1;
+    const cplx t5 = c5[e];
+    cplx t = mk((-t5.x + c.two_omega * t4.x) * tmp2 + f.x, (-t5.y + c.two_omega * t4.y) * tmp2 + f.y);
+    t = cadd(ps[e], cscale(t, tmp1));
+    a = mk(t.x / tmp3, t.y / tmp3);
+    tmp3 = 1.0 + (c.nuv * ipow(ka2, c.nnv) + c.hnuv * ipow(tmp2, c.mmv)) * tmp1;
+    const cplx t6 = REG ? t6r : c6[e];
+    t = mk(t6.x + c.two_omega * t3.x + g.x, t6.y + c.two_omega * t3.y + g.y);
+    t = cadd(vz[e], cscale(t, tmp1));
+    b = mk(t.x / tmp3, t.y / tmp3);
+  }
+  o1[e] = a;
+  o2[e] = b;
+}
+
+// The stage update over the whole half plane (stage-by-stage form)
 __global__ void k_big_update(const cplx *ps, const cplx *vz, const cplx *c1, const cplx *c2,
                              cplx *o1, cplx *o2, const cplx *__restrict__ c5,
                              const cplx *__restrict__ c6, StepConst c,
@@ -3591,46 +3814,64 @@ __global__ void k_big_update(const cplx *ps, const cplx *vz, const cplx *c1, con
   const size_t total = (size_t)(n / 2 + 1) * n;
   const double dt = dtp ? *dtp : c.dt_fixed;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(e / n), j = (int)(e % n);
-    const double kx = kxof(i, n), ky = kyof(j, n), ka2 = kx * kx + ky * ky;
-    cplx a = mk(0.0, 0.0), b = mk(0.0, 0.0);
-    if (ka2 <= c.g.kmax && ka2 > c.g.tiny) {
-      cplx f = mk(0.0, 0.0), g = mk(0.0, 0.0);
-      for (int q = 0; q < c.fp.nent; ++q)
-        if (c.fp.e[q].i == i && c.fp.e[q].j == j) f = force_value(c.fp, q, dt, c.n2);
-      for (int q = 0; q < c.fz.nent; ++q)
-        if (c.fz.e[q].i == i && c.fz.e[q].j == j) g = force_value(c.fz, q, dt, c.n2);
-      if (c.fpd) f = cadd(f, c.fpd[e]);
-      if (c.fzd) g = cadd(g, c.fzd[e]);
-      const cplx s1 = c1[e], s2 = c2[e];
-      const cplx t3 = mk(-ky * s1.y, ky * s1.x), t4 = mk(-ky * s2.y, ky * s2.x);
-      const double tmp1 = dt / (double)o, tmp2 = 1.0 / ka2;
-      double tmp3 = 1.0 + (c.nu * ipow(ka2, c.nn) + c.hnu * ipow(tmp2, c.mm)) * tmp1;
-      const cplx t5 = c5[e];
-      cplx t = mk((-t5.x + c.two_omega * t4.x) * tmp2 + f.x, (-t5.y + c.two_omega * t4.y) * tmp2 + f.y);
-      t = cadd(ps[e], cscale(t, tmp1));
-      a = mk(t.x / tmp3, t.y / tmp3);
-      tmp3 = 1.0 + (c.nuv * ipow(ka2, c.nnv) + c.hnuv * ipow(tmp2, c.mmv)) * tmp1;
-      const cplx t6 = c6[e];
-      t = mk(t6.x + c.two_omega * t3.x + g.x, t6.y + c.two_omega * t3.y + g.y);
-      t = cadd(vz[e], cscale(t, tmp1));
-      b = mk(t.x / tmp3, t.y / tmp3);
+       e += (size_t)gridDim.x * blockDim.x)
+    big_update_at<false>(e, (int)(e / n), (int)(e % n), ps, vz, c1, c2, o1, o2, c5, c6,
+                         mk(0.0, 0.0), c, dt, o);
+}
+
+// FFT(J2) of kept column i fused with the stage update of that column (C6 =
+// FFT(J2) stays in registers; C5 = FFT(J1) from k_big_col_fwd_j1).  With
+// zero_tail the grid also covers the columns nc..n/2, which it sets to zero
+// in o1, o2 (the last stage, which writes the state: the update mask).
+__global__ __launch_bounds__(kBigT) void k_big_col_update(
+    const cplx *__restrict__ fb, int kp, const cplx *ps, const cplx *vz, const cplx *c1, const cplx *c2,
+    cplx *o1, cplx *o2, const cplx *__restrict__ c5, StepConst c, const double *__restrict__ dtp,
+    int o, int zero_tail, const cplx *__restrict__ tw) {
+  constexpr int N = kBigN;
+  __shared__ cplx lds[Shape<kBigH>::LDSN];
+  const int i = big_col_of_block(), b = threadIdx.x;
+  if (i >= kBigNC) {
+    if (zero_tail && i <= N / 2) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const size_t e = (size_t)i * N + b + kBigT * q;
+        o1[e] = mk(0.0, 0.0);
+        o2[e] = mk(0.0, 0.0);
+      }
     }
-    o1[e] = a;
-    o2[e] = b;
+    return;
+  }
+  cplx ev[8], od[8];
+  big_col_fwd_tiled(ev, od, fb, i, kp, b, lds, tw);
+  const double dt = dtp ? *dtp : c.dt_fixed;
+  // The transform's closing barrier has freed the LDS.  Each half of the
+  // column goes to slots that only this thread reads back, so the element
+  // loop runs one element at a time (unrolled it would spill) with no barrier.
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) lds[b + kBigT * q] = h ? od[q] : ev[q];
+#pragma unroll 1
+    for (int q = 0; q < 8; ++q) {
+      const int j = b + kBigT * q + h * kBigH;
+      big_update_at<true>((size_t)i * N + j, i, j, ps, vz, c1, c2, o1, o2, c5, nullptr,
+                          lds[b + kBigT * q], c, dt, o);
+    }
   }
 }
 
 // ncols < n/2 + 1: the input is zero in the columns kx >= ncols (dealiased)
-static int big_c2r(r2d3c_plan *p, const cplx *in, int op, double *out, hipStream_t s, int ncols) {
+// mix: the mixed field between the two passes (default: the plan's first)
+static int big_c2r(r2d3c_plan *p, const cplx *in, int op, double *out, hipStream_t s, int ncols,
+                   cplx *mix) {
   if (ncols != kBigN / 3 + 1 && ncols != kBigN / 2 + 1)
     return set_err(R2D3C_EINVAL, "16384 c2r: column count n/3 + 1 or n/2 + 1");
-  k_big_col_inv<<<ncols, kBigT, 0, s>>>(in, p->mfull, op, 1.0, p->g, p->tw);
+  if (!mix) mix = p->mfull;
+  k_big_col_inv<<<ncols, kBigT, 0, s>>>(in, mix, op, 1.0, p->g, p->tw);
   if (ncols == kBigN / 3 + 1)
-    k_big_row_c2r<kBigN / 3 + 1><<<kBigN / 2, kBigT, 0, s>>>(p->mfull, out, p->g, p->tw);
+    k_big_row_c2r<kBigN / 3 + 1><<<kBigN / 2, kBigT, 0, s>>>(mix, out, p->g, p->tw);
   else
-    k_big_row_c2r<kBigN / 2 + 1><<<kBigN / 2, kBigT, 0, s>>>(p->mfull, out, p->g, p->tw);
+    k_big_row_c2r<kBigN / 2 + 1><<<kBigN / 2, kBigT, 0, s>>>(mix, out, p->g, p->tw);
   LAUNCHCHK("c2r 16384");
   return 0;
 }
@@ -3666,21 +3907,34 @@ static int big_cfl(r2d3c_plan *p, const r2d3c_params *q, double cfl, const cplx 
                    hipStream_t s) {
   const size_t nn = (size_t)kBigN * kBigN;
   k_set_u64<<<1, 1, 0, s>>>(p->maxbits, 0ull);
-  int rc;
-  if ((rc = big_c2r(p, psi, OP_DX, p->real, s)) || (rc = big_c2r(p, psi, OP_DY, p->real2, s))) return rc;
-  k_real_gradmax<<<kReduceBlocks, 256, 0, s>>>(p->real, p->real2, nn, p->maxbits);
+  if (p->big_fused) {  // the fused step's own maximum (its first row transform)
+    cplx *MX = p->mfull, *MY = MX + p->fsf;
+    k_big_col_inv<<<kBigNC, kBigT, 0, s>>>(psi, MX, OP_DX, 1.0, p->g, p->tw);
+    k_big_col_inv<<<kBigNC, kBigT, 0, s>>>(psi, MY, OP_DY, 1.0, p->g, p->tw);
+    k_big_row_gradmax<<<kBigN, kBigT, 0, s>>>(MX, MY, p->g, p->tw, p->maxbits);
+  } else {
+    int rc;
+    if ((rc = big_c2r(p, psi, OP_DX, p->real, s, kBigNC)) ||
+        (rc = big_c2r(p, psi, OP_DY, p->real2, s, kBigNC)))
+      return rc;
+    k_real_gradmax<<<kReduceBlocks, 256, 0, s>>>(p->real, p->real2, nn, p->maxbits);
+  }
   k_dt_from_max<<<1, 1, 0, s>>>(p->maxbits, make_dt_const(kBigN, q, cfl), d_dt);
   LAUNCHCHK("cfl 16384");
   return 0;
 }
-// One time step, stage by stage (src/Rot2D3C.para.f90:399,545-611).  Per
-// stage six c2r of the dealiased columns (d/dx, d/dy of psi, of laplacian(psi)
-// and of vz: the two brackets share psi's derivatives), the two brackets
-// formed inside their r2c row pass, and the update, which forms d/dy of psi
+// One time step at n = 16384 (src/Rot2D3C.para.f90:399,545-611) on the
+// dealiased columns, in one of two forms.  Fused bracket pass (default): per
+// stage four c2r (d/dx, d/dy of laplacian(psi) and of vz) into real fields,
+// the column transforms of d/dx, d/dy psi, k_big_row_jac2 (psi's derivatives
+// in registers, both brackets and their row transforms), and two column
+// forwards, the second carrying the update.  Stage by stage: six c2r (the two
+// brackets share psi's derivatives), the two brackets formed inside their r2c
+// row pass, and the update over the half plane.  The update forms d/dy of psi
 // and vz itself.  Stage 4 reads psi, vz directly and stage 1 writes them, so
-// the step copies no state; the update's output is already the masked state.
-// In CFL mode the step's dt comes from stage 4's d/dx, d/dy of psi, which are
-// the fields CFL_condition transforms.
+// the step copies no state; its output is already the masked state.  In CFL
+// mode the step's dt comes from stage 4's d/dx, d/dy of psi, which are the
+// fields CFL_condition transforms.
 static int big_step(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
                     const r2d3c_forcing *fo, int dt_mode, double dt_or_cfl, double *d_dt,
                     hipStream_t s) {
@@ -3704,6 +3958,37 @@ static int big_step(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
   int rc;
   if (dt_mode != R2D3C_DT_CFL) k_set_f64<<<1, 1, 0, s>>>(dtp, dt_or_cfl);
   const int g = launch_grid(sz);
+  if (p->big_fused) {
+    // Fused form: d/dx, d/dy of psi stay column transforms (MX, MY); the four
+    // other derivative fields go through HBM as real fields; k_big_row_jac2
+    // forms both brackets and their row transforms in one pass, and the
+    // second bracket's column transform carries the update.
+    cplx *MX = p->mfull, *MY = MX + p->fsf, *MS = MY + p->fsf, *FA = MS, *FB = FA + p->fs;
+    const double n4 = (double)kBigN * kBigN * (double)kBigN * kBigN;
+    const bool cfl = dt_mode == R2D3C_DT_CFL;
+    for (int o = 4; o >= 1; --o) {
+      const cplx *a = o == 4 ? psi : C1, *v = o == 4 ? vz : C2;
+      if ((rc = big_c2r(p, a, OP_LDX, R1, s, nc, MS)) || (rc = big_c2r(p, a, OP_LDY, R2, s, nc, MS)) ||
+          (rc = big_c2r(p, v, OP_DX, R3, s, nc, MS)) || (rc = big_c2r(p, v, OP_DY, R4, s, nc, MS)))
+        return rc;
+      k_big_col_inv<<<kBigNC, kBigT, 0, s>>>(a, MX, OP_DX, 1.0, p->g, p->tw);
+      k_big_col_inv<<<kBigNC, kBigT, 0, s>>>(a, MY, OP_DY, 1.0, p->g, p->tw);
+      if (o == 4 && cfl) k_set_u64<<<1, 1, 0, s>>>(p->maxbits, 0ull);
+      // FA, FB overlay MS (two fields of n kp entries in 2 fsf): the four
+      // c2r above are done with it
+      k_big_row_jac2<<<kBigN, kBigT, 0, s>>>(MX, MY, R1, R2, R3, R4, n4, FA, FB, p->g, p->tw,
+                                             o == 4 && cfl ? p->maxbits : nullptr);
+      if (o == 4 && cfl)
+        k_dt_from_max<<<1, 1, 0, s>>>(p->maxbits, make_dt_const(kBigN, q, dt_or_cfl), dtp);
+      k_big_col_fwd_j1<<<(kBigNC + 63) / 64 * 64, kBigT, 0, s>>>(FA, p->g.kp, C5, p->tw);
+      const int ncol = o == 1 ? kBigN / 2 + 1 : kBigNC;
+      k_big_col_update<<<(ncol + 63) / 64 * 64, kBigT, 0, s>>>(
+          FB, p->g.kp, psi, vz, a, v, o == 1 ? psi : C1, o == 1 ? vz : C2, C5, c, dtp, o, o == 1,
+          p->tw);
+    }
+    LAUNCHCHK("step 16384");
+    return 0;
+  }
   for (int o = 4; o >= 1; --o) {
     const cplx *a = o == 4 ? psi : C1, *v = o == 4 ? vz : C2;
     if ((rc = big_c2r(p, a, OP_DX, R1, s, nc)) || (rc = big_c2r(p, a, OP_DY, R2, s, nc))) return rc;

@@ -4,6 +4,7 @@ do not depend on it: transforms of a few Fourier modes against their closed
 form, the r2c/c2r round trip, single-mode Jacobians, and a step from a
 band-limited state against the same step at 512^2 (whose path is checked
 against the oracle elsewhere), rescaled by the grid factor."""
+import ctypes
 import math
 
 import numpy as np
@@ -69,12 +70,31 @@ def test_poisson_of_two_modes(cuda):
     del a, b, c, r
 
 
-def test_step_matches_512_for_band_limited_state(cuda):
+def _set_bracket_pass(cuda, fused):
+    from rot2d3c_amd import _lib, spectral as S
+    h = S.Plan.get(N, cuda).handle
+    _lib.check(_lib.lib().r2d3c_plan_set_bracket_pass(h, int(fused)), "set_bracket_pass")
+    got = ctypes.c_int(-1)
+    _lib.check(_lib.lib().r2d3c_plan_get_bracket_pass(h, ctypes.byref(got)), "get_bracket_pass")
+    assert got.value == int(fused)
+
+
+@pytest.mark.parametrize("fused", [1, 0])
+def test_step_matches_512_for_band_limited_state(cuda, fused):
     """Two fixed-dt steps with rotation, dissipation and forcing from a state
-    on |k| <= 12: the 16384 path against the 512 fused path (both dealias far
-    above the band the steps reach)."""
-    print("16384 step vs 512", flush=True)
+    on |k| <= 12: the 16384 path (fused bracket pass, and stage by stage)
+    against the 512 fused path (both dealias far above the band the steps
+    reach)."""
+    print("16384 step vs 512, bracket pass %d" % fused, flush=True)
     from rot2d3c_amd import spectral as S
+    _set_bracket_pass(cuda, fused)
+    try:
+        _step_vs_512(cuda, S)
+    finally:
+        _set_bracket_pass(cuda, 1)
+
+
+def _step_vs_512(cuda, S):
     rng = np.random.default_rng(2)
     # amplitudes 1e-2: the brackets still move the result by ~1e-4, while the
     # harmonics the 8 stages push past the 512 grid's cutoff stay ~1e-13
@@ -122,3 +142,51 @@ def test_cfl_step_takes_the_cfl_condition_dt(cuda):
     torch.cuda.synchronize()
     assert float(t.item()) == dt_ref and 0.0 < dt_ref < 1.0
     assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+
+
+def _broadband(n, seed, device):
+    """Random half-plane spectrum over every mode (also past the dealiasing
+    disc and in the columns past n/3), amplitude ~ n^2 / (1 + |k|^2)."""
+    g = torch.Generator(device=device).manual_seed(seed)
+    kx = torch.arange(n // 2 + 1, device=device, dtype=torch.float64)[:, None]
+    ky = torch.fft.fftfreq(n, 1.0 / n, device=device, dtype=torch.float64)[None, :]
+    amp = float(n) * n / (1.0 + kx * kx + ky * ky)
+    re = torch.randn((n // 2 + 1, n), generator=g, device=device, dtype=torch.float64)
+    im = torch.randn((n // 2 + 1, n), generator=g, device=device, dtype=torch.float64)
+    a = torch.complex(re * amp, im * amp)
+    del re, im, amp
+    return a
+
+
+def test_bracket_pass_forms_agree_on_broadband_state(cuda):
+    """The fused bracket pass and the stage-by-stage form give the same step
+    within rounding from a state with every mode set (the update mask,
+    including the columns past n/3, comes out zero in both), with forcing, in
+    fixed-dt and CFL mode (the CFL dt: same maximum within rounding)."""
+    print("16384 fused vs stage-by-stage", flush=True)
+    from rot2d3c_amd import spectral as S
+    p = S.Params(1e-4, 0.1, 1, 2, 2e-4, 0.05, 1, 2, 3.0)
+    draws = (S.ForceDraw(6, -5, 0.4, 1.0, 8.0), S.ForceDraw(3, 7, 1.3, 0.5, 1.0))
+    out = {}
+    try:
+        for fused in (1, 0):
+            _set_bracket_pass(cuda, fused)
+            a = (_broadband(N, 5, cuda), _broadband(N, 6, cuda))
+            S.rk4_step(a[0], a[1], p, *draws, dt=1e-6)
+            t = torch.zeros(1, dtype=torch.float64, device=cuda)
+            b = (_broadband(N, 7, cuda), _broadband(N, 8, cuda))
+            S.rk4_step(b[0], b[1], p, *draws, cfl=0.3, dt_out=t)
+            torch.cuda.synchronize()
+            out[fused] = (a, b, float(t.item()))
+            del a, b
+        kx = torch.arange(N // 2 + 1, device=cuda, dtype=torch.float64)[:, None]
+        ky = torch.fft.fftfreq(N, 1.0 / N, device=cuda, dtype=torch.float64)[None, :]
+        outside = (kx * kx + ky * ky) > (N / 3.0) ** 2
+        for x, y in zip(out[1][0] + out[1][1], out[0][0] + out[0][1]):
+            rel = float(torch.linalg.vector_norm(x - y) / torch.linalg.vector_norm(y))
+            assert rel < 1e-12, rel
+            assert not bool(torch.any(x[outside] != 0)) and not bool(torch.any(y[outside] != 0))
+        assert abs(out[1][2] - out[0][2]) <= 1e-13 * out[0][2]
+    finally:
+        out.clear()
+        _set_bracket_pass(cuda, 1)
