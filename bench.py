@@ -264,13 +264,37 @@ def ensemble_bench(steps, dev, tuned, warmup=10):
         ens.tune()
     table = E.forcing_table(spec, warmup + steps)
     ens.run(warmup, table[:warmup])
+    # the draws as the library's forcing array before the timed region (the
+    # conversion of 20 000 Python draws is harness work, ~2.4 us each); the
+    # library's own table set-up and upload stay inside
+    fo = ens.forcing_array(table[warmup:])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ens.run(steps, table[warmup:])
+    ens.run(steps, fo)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     finite = all(bool(torch.isfinite(torch.view_as_real(m.ps)).all()) for m in ens.members)
     schedules = {str(n): list(v) for n, v in ens.schedules.items()}
+    row_split = {str(n): v for n, v in ens.row_split.items()}
+    form = ens.stage_launch
+    names = {0: "per-size kernels", 1: "one persistent launch per stage", 2: "one launch per phase"}
+    forms_ms = {names[form]: round(1e3 * wall / steps, 4)}
+    # the other stage launch forms on the steps that follow (their own
+    # schedules), for comparison
+    nalt = min(steps, 100)
+    talt = E.forcing_table([dict(r, seed=r["seed"] + 1000) for r in spec], nalt + 2)
+    foalt = ens.forcing_array(talt[2:])
+    for f in (0, 1, 2):
+        if f == form:
+            continue
+        ens.set_stage_launch(f)
+        ens.run(2, talt[:2])
+        ens.sync()
+        t0 = time.perf_counter()
+        ens.run(nalt, foalt)
+        ens.sync()
+        forms_ms[names[f] + (" (their schedules)" if f == 0 else "")] = round(
+            1e3 * (time.perf_counter() - t0) / nalt, 4)
     ens.close()
     pts = E.points(spec)
     alg = sum(algorithmic_bytes(r["n"], steps)[1] for r in spec)
@@ -284,7 +308,9 @@ def ensemble_bench(steps, dev, tuned, warmup=10):
             "frac_of_peak": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
             "model_76S_frac_of_peak": round(b76 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
             "finite": finite, "schedules": schedules,
-            "size_classes": "concurrent graph branches" if ens.concurrent else "one after another"}
+            "size_classes": "concurrent graph branches" if ens.concurrent else "one after another",
+            "row_split": row_split, "stage_launch": names[form],
+            "stage_launch_forms_ms_per_step": forms_ms}
 
 
 def main():

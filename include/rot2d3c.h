@@ -35,6 +35,7 @@ extern "C" {
 #define R2D3C_OK 0
 #define R2D3C_EINVAL (-1)     /* bad argument (size, pointer, exponent) */
 #define R2D3C_ENOTSUP (-2)    /* grid size without a compiled kernel    */
+#define R2D3C_EFAIL (-3)      /* a device-side check failed             */
 #define R2D3C_EHIP_BASE 1000  /* -(1000 + hipError_t)                   */
 
 typedef struct r2d3c_plan r2d3c_plan;
@@ -335,6 +336,25 @@ int r2d3c_ensemble_set_column_radix(r2d3c_ensemble *ens, int n, int form);
  * another, each class's launches having the whole GPU; 1 as concurrent
  * branches of the step graph.  Results are identical either way. */
 int r2d3c_ensemble_set_concurrent(r2d3c_ensemble *ens, int concurrent);
+/* How a time step's stages are launched: 0 (default) the per-size kernels
+ * (a row and a fused column launch per size and stage); 1 one persistent
+ * launch per RK4 stage for every run of every size, whose blocks, all
+ * resident, walk a work-list of (size, run, tile) units: the row transforms
+ * of every run, a grid barrier, then the column transforms and updates;
+ * 2 the same work-list as one launch per phase (rows of every size, then
+ * columns of every size).  Forms 1 and 2: sizes 64 to 1024, at most 5 sizes,
+ * runs without closures; they set every size's schedule to their arithmetic
+ * (sequential row kernel, radix-8 Stockham columns), so each run still
+ * equals its solo r2d3c_run with that schedule bit for bit.  Synchronizes
+ * the ensemble.  No reference counterpart (the reference runs one
+ * simulation per job). */
+int r2d3c_ensemble_set_stage_launch(r2d3c_ensemble *ens, int form);
+/* The stage launch form and, for form 1, its grid size (blocks). */
+int r2d3c_ensemble_get_stage_launch(const r2d3c_ensemble *ens, int *form, int *grid);
+/* Waits for the ensemble's work; R2D3C_EFAIL if a grid barrier of the
+ * persistent launch (form 1) timed out since the last call (not every block
+ * was resident: the state advanced by that run call is invalid). */
+int r2d3c_ensemble_sync(r2d3c_ensemble *ens);
 /* r2d3c_plan_set_row_split for the runs of size n. */
 int r2d3c_ensemble_set_row_split(r2d3c_ensemble *ens, int n, int mode);
 /* r2d3c_plan_set_column_split for the runs of size n. */

@@ -78,6 +78,9 @@ SIGNATURES = {
     "r2d3c_plan_set_bracket_pass": (_c_int, [_vp, _c_int]),
     "r2d3c_plan_get_bracket_pass": (_c_int, [_vp, ctypes.POINTER(_c_int)]),
     "r2d3c_ensemble_set_concurrent": (_c_int, [_vp, _c_int]),
+    "r2d3c_ensemble_set_stage_launch": (_c_int, [_vp, _c_int]),
+    "r2d3c_ensemble_get_stage_launch": (_c_int, [_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),
+    "r2d3c_ensemble_sync": (_c_int, [_vp]),
     "r2d3c_ensemble_set_row_split": (_c_int, [_vp, _c_int, _c_int]),
     "r2d3c_plan_set_schedule": (_c_int, [_vp, _c_int, _c_int]),
     "r2d3c_plan_get_schedule": (_c_int, [_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),

@@ -904,21 +904,21 @@ __device__ __forceinline__ void load_packed_step(cplx v[8], const cplx *__restri
   }
 }
 
+// Rows y (this thread's, r = y % RW) of one run: three packed inverse
+// transforms, the brackets and the rotation terms, one forward transform,
+// unpacked into the bracket fields Q.  b: the thread's index in its group of
+// T*RW threads; lds: the group's RW * LDSN slots.  Every thread of the block
+// must call it together (block barriers).
 template <int N>
-__global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
-    const cplx *__restrict__ M, cplx *__restrict__ Q, StepConst c, unsigned long long *maxbits,
-    const cplx *__restrict__ tw, size_t mstride, const double *__restrict__ om_tab) {
+__device__ __forceinline__ void step_row_body(const cplx *__restrict__ M, cplx *__restrict__ Q,
+                                              const StepConst &c, unsigned long long *maxbits,
+                                              const cplx *__restrict__ tw, double two_omega_n2,
+                                              int y, int b, int r, cplx *lds) {
   constexpr int T = N / 8, RW = RowStep<N>::ROWS;
-  __shared__ cplx lds[RW * Shape<N>::LDSN];
-  const int r = threadIdx.x % RW, b = threadIdx.x / RW;
-  const int y = xcd_group<RowStep<N>::G>(blockIdx.x) * RW + r;
   // Each transform below takes an opaque copy of the thread index (b0..b7):
   // the compiler cannot hoist one transform's index and address math over
   // the others, which keeps the kernel at 128 VGPRs (4 waves per SIMD)
   // instead of ~245.
-  M += blockIdx.y * mstride;
-  Q += blockIdx.y * mstride;
-  const double two_omega_n2 = om_tab ? om_tab[blockIdx.y] : c.two_omega_n2;
   const int nc = c.g.nc, kp = c.g.kp;
   const size_t fs = c.fs;
   cplx v[8];
@@ -969,6 +969,20 @@ __global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
   int b7 = b;
   asm volatile("" : "+v"(b7));
   unpack_store<N, RW, N / 3 + 1>(lds + r, b7, nc, kp, Q, y, Q + fs, y);
+}
+
+template <int N>
+__global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
+    const cplx *__restrict__ M, cplx *__restrict__ Q, StepConst c, unsigned long long *maxbits,
+    const cplx *__restrict__ tw, size_t mstride, const double *__restrict__ om_tab) {
+  constexpr int RW = RowStep<N>::ROWS;
+  __shared__ cplx lds[RW * Shape<N>::LDSN];
+  const int r = threadIdx.x % RW, b = threadIdx.x / RW;
+  const int y = xcd_group<RowStep<N>::G>(blockIdx.x) * RW + r;
+  M += blockIdx.y * mstride;
+  Q += blockIdx.y * mstride;
+  const double two_omega_n2 = om_tab ? om_tab[blockIdx.y] : c.two_omega_n2;
+  step_row_body<N>(M, Q, c, maxbits, tw, two_omega_n2, y, b, r, lds);
 }
 
 
@@ -1222,20 +1236,22 @@ __global__ __launch_bounds__((RowV2<N, RWQ>::THREADS)) void k_step_row2(
 // write dt and clear the CFL slot of the next step (mreset).
 enum { FM_EMIT = 1, FM_STATE = 2, FM_DISK = 4, FM_ZERO = 8 };
 
+// b: the thread's index in its group of N/8 threads; lds: the group's LDSN
+// slots.  Split emission (nslices > 1, never with FM_STATE): slice z of a
+// column repeats the forward transform and the update and emits only field
+// z, so a small grid's blocks each run two transforms instead of up to five.
+// With act false the group runs every transform and barrier on its (valid)
+// column but stores nothing: a group of a block that has no column of its
+// own keeps the block's barrier sequence.
 template <int N, bool CS = false>
 __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *psi, cplx *vz,
                                               cplx *__restrict__ M, const StepConst &c, int o,
                                               int mode, const unsigned long long *mread,
                                               unsigned long long *mreset, double *dt_out,
-                                              const cplx *__restrict__ tw, int lb) {
+                                              const cplx *__restrict__ tw, int lb, int b,
+                                              cplx *lds, int slice, int nslices, bool act) {
   auto pos = [](int b, int q) { return ilv<N, CS>(b, q); };
   constexpr int T = N / 8;
-  __shared__ cplx lds[Shape<N>::LDSN];
-  const int b = threadIdx.x;
-  // Split emission (gridDim.z > 1, never with FM_STATE): slice z of a column
-  // repeats the forward transform and the update and emits only field z, so
-  // a small grid's blocks each run two transforms instead of up to five.
-  const int slice = blockIdx.z, nslices = gridDim.z;
   const int nc = c.g.nc, ncp = (nc + 1) & ~1;
   int field, kx;
   if (lb < nc) { field = 0; kx = lb; }
@@ -1320,7 +1336,7 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
     for (int q = 0; q < 8; ++q) {
       const int j = b + T * q, ky = kyof(j, N);
       const double ka2 = (double)kx * kx + (double)ky * ky;
-      if (!disk || (ka2 <= c.g.kmax && ka2 > c.g.tiny))
+      if (act && (!disk || (ka2 <= c.g.kmax && ka2 > c.g.tiny)))
         state[(size_t)kx * N + j] = lds[pad(j)];
     }
     __syncthreads();
@@ -1346,8 +1362,10 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
       }
       fft_dit<N, +1, 1, CS>(w, lds, bb, tw);
       cplx *dst = dst0 + (size_t)f * c.fs;
+      if (act) {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) dst[tix(bb + T * q, kx, c.g.kp)] = w[q];
+        for (int q = 0; q < 8; ++q) dst[tix(bb + T * q, kx, c.g.kp)] = w[q];
+      }
     }
   }
 }
@@ -1357,8 +1375,9 @@ __global__ __launch_bounds__(N / 8) void k_step_colfused(
     const cplx *__restrict__ Q, cplx *psi, cplx *vz, cplx *__restrict__ M, StepConst c, int o,
     int mode, const unsigned long long *mread, unsigned long long *mreset, double *dt_out,
     const cplx *__restrict__ tw) {
+  __shared__ cplx lds[Shape<N>::LDSN];
   colfused_body<N, CS>(Q, psi, vz, M, c, o, mode, mread, mreset, dt_out, tw,
-                       xcd_pairs(blockIdx.x));
+                       xcd_pairs(blockIdx.x), threadIdx.x, lds, blockIdx.z, gridDim.z, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -1398,12 +1417,237 @@ __global__ __launch_bounds__(N / 8) void k_ens_colfused(EnsArgs a, int o, int mo
                                                          const cplx *__restrict__ tw) {
   const EnsMember &e = a.mem[blockIdx.y];
   const StepConst c = ens_const(a, blockIdx.y);
+  __shared__ cplx lds[Shape<N>::LDSN];
   colfused_body<N, CS>(e.M + 6 * c.fs, e.psi, e.vz, e.M, c, o, mode, nullptr, nullptr, nullptr, tw,
-                   xcd_pairs(blockIdx.x));
+                       xcd_pairs(blockIdx.x), threadIdx.x, lds, blockIdx.z, gridDim.z, true);
 }
-// advances the forcing-table step at the end of a time step
-__global__ void k_ens_tick(int *step) { step[0] = step[0] + 1; }
-__global__ void k_set_i32(int *p, int v) { p[0] = v; }
+// ---------------------------------------------------------------------------
+// ensembles: one persistent launch per RK4 stage for every size class
+// ---------------------------------------------------------------------------
+// The launch's blocks (kPsT threads, all resident: the grid is at most the
+// occupancy limit) walk a work-list of units.  Row phase: a unit is G = 1024/n
+// row pairs of one run, one group of (n/8)*2 threads each (step_row_body);
+// then a grid barrier; column phase: a unit is G = 2048/n columns of one
+// field of one run, one group of n/8 threads each (colfused_body).  Every
+// unit holds 2048 points, so the units of all sizes cost about the same, and
+// every group of a block runs the same size's body, so the block's barriers
+// line up.  The arithmetic is the per-size kernels' (row: k_step_row; columns:
+// the radix-8 Stockham form for n >= 512), so each run equals the same run
+// advanced by those kernels, bit for bit.
+constexpr int kPsT = 256;
+constexpr int kPsMaxClasses = 5;           // sizes 64..1024
+// the block's LDS, passed to the unit bodies as an LDS-space pointer
+typedef __attribute__((address_space(3))) cplx *PsLds;
+constexpr unsigned kPsSpin = 1u << 22;     // barrier polls before the time-out
+constexpr int kPsLds = 2176;  // complex slots: G * 2 * (n + n/16) rows, G * (n + n/16) columns
+struct PsClass {
+  EnsArgs a;            // the class's runs, forcing table and step
+  const cplx *tw;       // twiddles of size n
+  const double *om;     // 2 Omega n^2 per run
+  int n, count, ucol;   // size, runs, column units per (run, field)
+  int row_end, col_end; // units of the classes up to and including this one
+};
+
+// Grid-wide barrier of a launch whose blocks are all resident.  bar[0] counts
+// arrivals, bar[1] is the generation: the last block to arrive resets the
+// count and advances the generation.  Every thread releases its writes at
+// device (agent) scope before the barrier and acquires after it, which
+// writes back and invalidates the XCDs' non-coherent L2s.  The wait is
+// bounded: after `spin` polls a block records the time-out in *err and goes
+// on, so the launch always ends (its results are then invalid; later launches
+// see *err and do nothing, and r2d3c_ensemble_sync reports it).
+__device__ __forceinline__ void ps_grid_sync(unsigned *bar, int *err, unsigned spin) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned gen = __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned arrived =
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+    if (arrived == gridDim.x) {
+      __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(bar + 1, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      unsigned k = 0;
+      while (__hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++k > spin) {
+          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+
+template <int N>
+__device__ __forceinline__ void ps_row_unit(const PsClass *k, int u, int tid, PsLds lds3) {
+  cplx *lds = (cplx *)lds3;
+  constexpr int RW = RowStep<N>::ROWS, GT = (N / 8) * RW, G = kPsT / GT, UR = (N / 2) / G;
+  static_assert(RW == 2 && G * GT == kPsT && G * RW * Shape<N>::LDSN <= kPsLds, "row unit");
+  const int m = u / UR, up = u - m * UR;
+  const int g = tid / GT, t = tid - g * GT;
+  const int r = t % RW, b = t / RW;
+  const EnsMember *e = k->a.mem + m;
+  const StepConst &c = e->c;
+  step_row_body<N>(e->M, e->M + 6 * c.fs, c, nullptr, k->tw, k->om[m], (up * G + g) * RW + r, b,
+                   r, lds + g * RW * Shape<N>::LDSN);
+}
+
+template <int N>
+__device__ __forceinline__ void ps_col_unit(const PsClass *k, int u, int o, int mode, int tid, PsLds lds3) {
+  cplx *lds = (cplx *)lds3;
+  constexpr int GT = N / 8, G = kPsT / GT;
+  static_assert(G * GT == kPsT && G * Shape<N>::LDSN <= kPsLds, "column unit");
+  const int ucol = k->ucol, per_run = 2 * ucol;
+  const int m = u / per_run, rem = u - m * per_run;
+  const int field = rem >= ucol, j = rem - field * ucol;
+  const int g = tid / GT, b = tid - g * GT;
+  const EnsMember *e = k->a.mem + m;
+  const StepConst c = ens_const(k->a, m);
+  const int nc = c.g.nc, ncp = (nc + 1) & ~1;
+  const int kx = j * G + g;
+  const bool act = kx < nc;
+  const int lb = (act ? kx : nc - 1) + (field ? ncp : 0);
+  colfused_body<N, (N >= 512)>(e->M + 6 * c.fs, e->psi, e->vz, e->M, c, o, mode, nullptr,
+                                nullptr, nullptr, k->tw, lb, b, lds + g * Shape<N>::LDSN, 0, 1,
+                                act);
+}
+
+// Logical block of hardware block hb: pairs (2p, 2p+1) on one XCD in
+// consecutive slots (the grid is a multiple of 16), so the two units of a
+// 4-row tile at n = 1024 share that XCD's L2.
+__device__ __forceinline__ int ps_logical_block(int hb) {
+  const int x = hb & 7, s = hb >> 3;
+  return ((s >> 1) << 4) + (x << 1) + (s & 1);
+}
+
+__global__ __launch_bounds__(kPsT) __attribute__((amdgpu_waves_per_eu(3))) void k_ens_stage(
+    const PsClass *__restrict__ cls, int nk, unsigned *bar, int *err, unsigned spin, int o,
+    int mode) {
+  __shared__ cplx lds_arr[kPsLds];
+  const PsLds lds = (PsLds)lds_arr;
+  if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const int lb = ps_logical_block(blockIdx.x);
+  const int row_total = cls[nk - 1].row_end, col_total = cls[nk - 1].col_end;
+  for (int u = lb; u < row_total; u += gridDim.x) {
+    int ci = 0;
+    while (cls[ci].row_end <= u) ++ci;
+    const PsClass *k = cls + ci;
+    const int v = u - (ci ? cls[ci - 1].row_end : 0);
+    // an opaque copy of the thread index per unit: no body's index math is
+    // hoisted out of the loop (ten bodies' worth would not fit the registers)
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    switch (k->n) {
+      case 64: ps_row_unit<64>(k, v, tid, lds); break;
+      case 128: ps_row_unit<128>(k, v, tid, lds); break;
+      case 256: ps_row_unit<256>(k, v, tid, lds); break;
+      case 512: ps_row_unit<512>(k, v, tid, lds); break;
+      case 1024: ps_row_unit<1024>(k, v, tid, lds); break;
+      default: break;
+    }
+  }
+  ps_grid_sync(bar, err, spin);
+  for (int u = lb; u < col_total; u += gridDim.x) {
+    int ci = 0;
+    while (cls[ci].col_end <= u) ++ci;
+    const PsClass *k = cls + ci;
+    const int v = u - (ci ? cls[ci - 1].col_end : 0);
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    switch (k->n) {
+      case 64: ps_col_unit<64>(k, v, o, mode, tid, lds); break;
+      case 128: ps_col_unit<128>(k, v, o, mode, tid, lds); break;
+      case 256: ps_col_unit<256>(k, v, o, mode, tid, lds); break;
+      case 512: ps_col_unit<512>(k, v, o, mode, tid, lds); break;
+      case 1024: ps_col_unit<1024>(k, v, o, mode, tid, lds); break;
+      default: break;
+    }
+  }
+}
+
+// The same work-list as k_ens_stage, one unit per block and one launch per
+// phase (the kernel boundary is the barrier): rows of every size, then
+// columns of every size.
+__global__ __launch_bounds__(kPsT) __attribute__((amdgpu_waves_per_eu(3))) void k_ens_rows(
+    const PsClass *__restrict__ cls, int nk) {
+  __shared__ cplx lds_arr[kPsLds];
+  const PsLds lds = (PsLds)lds_arr;
+  const int u = ps_logical_block(blockIdx.x);
+  if (u >= cls[nk - 1].row_end) return;
+  int ci = 0;
+  while (cls[ci].row_end <= u) ++ci;
+  const PsClass *k = cls + ci;
+  const int v = u - (ci ? cls[ci - 1].row_end : 0), tid = threadIdx.x;
+  switch (k->n) {
+    case 64: ps_row_unit<64>(k, v, tid, lds); break;
+    case 128: ps_row_unit<128>(k, v, tid, lds); break;
+    case 256: ps_row_unit<256>(k, v, tid, lds); break;
+    case 512: ps_row_unit<512>(k, v, tid, lds); break;
+    case 1024: ps_row_unit<1024>(k, v, tid, lds); break;
+    default: break;
+  }
+}
+__global__ __launch_bounds__(kPsT) __attribute__((amdgpu_waves_per_eu(3))) void k_ens_cols(
+    const PsClass *__restrict__ cls, int nk, int o, int mode) {
+  __shared__ cplx lds_arr[kPsLds];
+  const PsLds lds = (PsLds)lds_arr;
+  const int u = ps_logical_block(blockIdx.x);
+  if (u >= cls[nk - 1].col_end) return;
+  int ci = 0;
+  while (cls[ci].col_end <= u) ++ci;
+  const PsClass *k = cls + ci;
+  const int v = u - (ci ? cls[ci - 1].col_end : 0), tid = threadIdx.x;
+  switch (k->n) {
+    case 64: ps_col_unit<64>(k, v, o, mode, tid, lds); break;
+    case 128: ps_col_unit<128>(k, v, o, mode, tid, lds); break;
+    case 256: ps_col_unit<256>(k, v, o, mode, tid, lds); break;
+    case 512: ps_col_unit<512>(k, v, o, mode, tid, lds); break;
+    case 1024: ps_col_unit<1024>(k, v, o, mode, tid, lds); break;
+    default: break;
+  }
+}
+
+// The forcing of a step, copied into every run's constants (EnsMember::c.fp,
+// c.fz) before the step's kernels run, so that they read it with the run's
+// other constants instead of through the step counter and the table (a
+// dependent load at the start of every column block).  step[0]: the step,
+// step[1]: the table's steps.  advance 0 (a run call's start): step 0;
+// advance 1 (the end of a step, inside the step graph): the next step.
+// Past the table, or without one, the runs are unforced.
+struct EnsForceClass {
+  EnsMember *mem;
+  int base, count;  // index of the class's first run in the table, runs
+};
+__global__ __launch_bounds__(256) void k_ens_force(int *step, int advance, int table_steps,
+                                                   const ForceSet *__restrict__ ftab, int stride,
+                                                   const EnsForceClass *__restrict__ cls, int nk) {
+  __shared__ int cur;
+  if (threadIdx.x == 0) {
+    const int v = advance ? step[0] + 1 : 0;
+    step[0] = v;
+    if (!advance) step[1] = table_steps;
+    cur = v < (advance ? step[1] : table_steps) ? v : -1;
+  }
+  __syncthreads();
+  const int st = cur;
+  for (int ci = 0; ci < nk; ++ci) {
+    const EnsForceClass k = cls[ci];
+    for (int m = threadIdx.x; m < k.count; m += blockDim.x) {
+      ForceSet fp{}, fz{};
+      if (ftab && st >= 0) {
+        const ForceSet *f = ftab + ((size_t)st * stride + k.base + m) * 2;
+        fp = f[0];
+        fz = f[1];
+      }
+      k.mem[m].c.fp = fp;
+      k.mem[m].c.fz = fz;
+    }
+  }
+}
 
 // ---------------------------------------------------------------------------
 // host side: plan, dispatch, C ABI
@@ -2579,7 +2823,8 @@ struct r2d3c_ensemble {
   ForceSet *d_ftab = nullptr;
   size_t ftab_cap = 0;              // ForceSet entries allocated
   std::vector<ForceSet> h_ftab;
-  int *d_step = nullptr;
+  int *d_step = nullptr;             // [0] step of the forcing table, [1] its steps
+  EnsForceClass *d_fcls = nullptr;  // the classes' member tables (k_ens_force)
   int period = 1;                   // closures refreshed every period-th step of a run call
   hipGraph_t graph = nullptr;       // one step with the closures
   hipGraphExec_t exec = nullptr;
@@ -2590,6 +2835,15 @@ struct r2d3c_ensemble {
   // size classes of a step: 0 (default) one after another on the ensemble's
   // stream, 1 as concurrent branches of the step graph
   int concurrent = 0;
+  // stage launch form: 0 per-size kernels; 1 one persistent launch per RK4
+  // stage for every class (k_ens_stage); 2 one launch per phase for every
+  // class (k_ens_rows, k_ens_cols).  The class table, the grid barrier, its
+  // time-out flag and the persistent grid size.
+  int persistent = 0;
+  PsClass *d_ps = nullptr;
+  unsigned *d_bar = nullptr;
+  int *d_err = nullptr;
+  int ps_grid = 0, ps_rows = 0, ps_cols = 0;
 };
 
 template <int N>
@@ -2611,7 +2865,8 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
   const r2d3c_plan *p = k.plan;
   const int nc = p->g.nc, ncp = (nc + 1) & ~1;
   const cplx *tw = p->tw;
-  EnsArgs a{k.d_mem, ftab, e->d_step, e->count, k.first};
+  (void)ftab;  // the step's forcing is in the runs' constants (k_ens_force)
+  EnsArgs a{k.d_mem, nullptr, e->d_step, e->count, k.first};
   const bool cs = N >= 512 && p->col_cs;  // the columns' radix schedule
   if (first) {
     if (cs) k_ens_colinv<N, true><<<dim3(round16(ncp + nc), k.count), N / 8, 0, s>>>(a, tw);
@@ -2672,9 +2927,48 @@ static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, bool closures,
   if (*exec) { (void)hipGraphExecDestroy(*exec); *exec = nullptr; }
   if (*graph) { (void)hipGraphDestroy(*graph); *graph = nullptr; }
   e->graph_valid = false;
+  int nk = 0;
+  if (e->persistent) {  // the class table, largest size first
+    std::vector<PsClass> h;
+    for (auto &k : e->cls) {
+      PsClass c{};
+      c.a = EnsArgs{k.d_mem, nullptr, e->d_step, e->count, k.first};
+      c.tw = k.plan->tw;
+      c.om = k.d_om;
+      c.n = k.n;
+      c.count = k.count;
+      c.ucol = (k.plan->g.nc + 2048 / k.n - 1) / (2048 / k.n);
+      h.push_back(c);
+    }
+    std::stable_sort(h.begin(), h.end(), [](const PsClass &a, const PsClass &b) { return a.n > b.n; });
+    int re = 0, ce = 0;
+    for (auto &c : h) {
+      re += c.count * (c.n * c.n / 2048);
+      ce += c.count * 2 * c.ucol;
+      c.row_end = re;
+      c.col_end = ce;
+    }
+    nk = (int)h.size();
+    e->ps_rows = re;
+    e->ps_cols = ce;
+    HIPCHK(hipMemcpy(e->d_ps, h.data(), sizeof(PsClass) * h.size(), hipMemcpyHostToDevice));
+  }
   HIPCHK(hipStreamBeginCapture(e->es, hipStreamCaptureModeThreadLocal));
   int rc = 0;
   hipError_t he = hipEventRecord(e->fork, e->es);
+  if (e->persistent) {
+    for (int o = 4; o >= 1 && he == hipSuccess; --o) {
+      const int mode = o == 1 ? (FM_STATE | FM_DISK | FM_EMIT) : FM_EMIT;
+      if (e->persistent == 1) {
+        k_ens_stage<<<e->ps_grid, kPsT, 0, e->es>>>(e->d_ps, nk, e->d_bar, e->d_err, kPsSpin, o,
+                                                    mode);
+      } else {
+        k_ens_rows<<<round16(e->ps_rows), kPsT, 0, e->es>>>(e->d_ps, nk);
+        k_ens_cols<<<round16(e->ps_cols), kPsT, 0, e->es>>>(e->d_ps, nk, o, mode);
+      }
+      he = hipGetLastError();
+    }
+  }
   // largest sizes first: with concurrent branches theirs is the longest path
   std::vector<EnsClass *> by_size;
   for (auto &k : e->cls) by_size.push_back(&k);
@@ -2682,7 +2976,7 @@ static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, bool closures,
                    [](const EnsClass *a, const EnsClass *b) { return a->n > b->n; });
   for (EnsClass *kp : by_size) {
     EnsClass &k = *kp;
-    if (he != hipSuccess || rc) break;
+    if (he != hipSuccess || rc || e->persistent) break;
     if (!e->concurrent) {  // one class after another, each with the whole GPU
       rc = ens_dispatch(e, k, ftab, e->es, false, true, true, closures);
       continue;
@@ -2693,7 +2987,8 @@ static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, bool closures,
     if (he == hipSuccess && !rc) he = hipStreamWaitEvent(e->es, k.done, 0);
   }
   if (he == hipSuccess && !rc) {
-    k_ens_tick<<<1, 1, 0, e->es>>>(e->d_step);
+    k_ens_force<<<1, 256, 0, e->es>>>(e->d_step, 1, 0, ftab, e->count, e->d_fcls,
+                                      (int)e->cls.size());
     he = hipGetLastError();
   }
   hipGraph_t g = nullptr;
@@ -2744,6 +3039,10 @@ static void ens_free(r2d3c_ensemble *e) {
   }
   if (e->d_ftab) (void)hipFree(e->d_ftab);
   if (e->d_step) (void)hipFree(e->d_step);
+  if (e->d_fcls) (void)hipFree(e->d_fcls);
+  if (e->d_ps) (void)hipFree(e->d_ps);
+  if (e->d_bar) (void)hipFree(e->d_bar);
+  if (e->d_err) (void)hipFree(e->d_err);
   if (e->ev_in) (void)hipEventDestroy(e->ev_in);
   if (e->ev_out) (void)hipEventDestroy(e->ev_out);
   if (e->fork) (void)hipEventDestroy(e->fork);
@@ -2814,7 +3113,14 @@ int r2d3c_ensemble_create(r2d3c_ensemble **out, const r2d3c_member *members, int
   if (!rc && he == hipSuccess) he = hipEventCreateWithFlags(&e->ev_in, hipEventDisableTiming);
   if (!rc && he == hipSuccess) he = hipEventCreateWithFlags(&e->ev_out, hipEventDisableTiming);
   if (!rc && he == hipSuccess) he = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
-  if (!rc && he == hipSuccess) he = hipMalloc(&e->d_step, sizeof(int));
+  if (!rc && he == hipSuccess) he = hipMalloc(&e->d_step, 2 * sizeof(int));
+  if (!rc && he == hipSuccess) {
+    std::vector<EnsForceClass> fc;
+    for (auto &k : e->cls) fc.push_back(EnsForceClass{k.d_mem, k.first, k.count});
+    he = hipMalloc(&e->d_fcls, sizeof(EnsForceClass) * fc.size());
+    if (he == hipSuccess)
+      he = hipMemcpy(e->d_fcls, fc.data(), sizeof(EnsForceClass) * fc.size(), hipMemcpyHostToDevice);
+  }
   if (!rc && he != hipSuccess) rc = hip_err(he, "ensemble set-up");
   if (rc) { ens_free(e); return rc; }
   *out = e;
@@ -2836,6 +3142,69 @@ int r2d3c_ensemble_set_schedule(r2d3c_ensemble *e, int n, int row_rows, int row_
       return rc;
     }
   return set_err(R2D3C_EINVAL, "no run of that size in the ensemble");
+}
+
+int r2d3c_ensemble_set_stage_launch(r2d3c_ensemble *e, int form) {
+  NEED(e);
+  if (form < 0 || form > 2) return set_err(R2D3C_EINVAL, "stage launch form: 0, 1 or 2");
+  const int on = form;
+  if (on) {
+    if (e->cls.size() > (size_t)kPsMaxClasses)
+      return set_err(R2D3C_ENOTSUP, "all-size stage launch: at most 5 size classes");
+    for (auto &k : e->cls) {
+      if (k.n < 64 || k.n > 1024)
+        return set_err(R2D3C_ENOTSUP, "all-size stage launch: run sizes 64 to 1024 only");
+      if (!k.closures.empty())
+        return set_err(R2D3C_ENOTSUP, "all-size stage launch: runs without closures only");
+    }
+    HIPCHK(hipStreamSynchronize(e->es));
+    if (!e->d_ps) {
+      HIPCHK(hipMalloc(&e->d_ps, sizeof(PsClass) * kPsMaxClasses));
+      HIPCHK(hipMalloc(&e->d_bar, 2 * sizeof(unsigned)));
+      HIPCHK(hipMalloc(&e->d_err, sizeof(int)));
+    }
+    HIPCHK(hipMemset(e->d_bar, 0, 2 * sizeof(unsigned)));
+    HIPCHK(hipMemset(e->d_err, 0, sizeof(int)));
+    // every block resident: the occupancy limit, a multiple of 16 blocks
+    int per_cu = 0, cus = e->cls[0].plan->cus;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_ens_stage, kPsT, 0));
+    e->ps_grid = (per_cu * cus) & ~15;
+    if (e->ps_grid < 16) return set_err(R2D3C_ENOTSUP, "persistent stage: occupancy too low");
+    // the per-size kernels of a call's first step use the all-size
+    // launches' arithmetic: the sequential row kernel, Stockham columns
+    for (auto &k : e->cls) {
+      k.plan->row_rw = 2;
+      k.plan->row_bpc = 0;
+      k.plan->row_split = 0;
+      k.plan->col_cs = 1;
+    }
+  }
+  if (e->persistent != on) e->graph_valid = false;
+  e->persistent = on;
+  return 0;
+}
+
+int r2d3c_ensemble_get_stage_launch(const r2d3c_ensemble *e, int *form, int *grid) {
+  NEED(e);
+  if (form) *form = e->persistent;
+  if (grid) *grid = e->persistent == 1 ? e->ps_grid : 0;
+  return 0;
+}
+
+int r2d3c_ensemble_sync(r2d3c_ensemble *e) {
+  NEED(e);
+  HIPCHK(hipStreamSynchronize(e->es));
+  if (e->d_err) {
+    int err = 0;
+    HIPCHK(hipMemcpy(&err, e->d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) {
+      HIPCHK(hipMemset(e->d_bar, 0, 2 * sizeof(unsigned)));
+      HIPCHK(hipMemset(e->d_err, 0, sizeof(int)));
+      return set_err(R2D3C_EFAIL, "persistent stage: a grid barrier wait timed out (not every "
+                                  "block resident); the state of that run call is invalid");
+    }
+  }
+  return 0;
 }
 
 int r2d3c_ensemble_set_concurrent(r2d3c_ensemble *e, int concurrent) {
@@ -2977,7 +3346,8 @@ int r2d3c_ensemble_run(r2d3c_ensemble *e, const r2d3c_forcing *forcing, int n_st
   }
   HIPCHK(hipEventRecord(e->ev_in, s));
   HIPCHK(hipStreamWaitEvent(e->es, e->ev_in, 0));
-  k_set_i32<<<1, 1, 0, e->es>>>(e->d_step, 0);
+  k_ens_force<<<1, 256, 0, e->es>>>(e->d_step, 0, n_steps, ftab, e->count, e->d_fcls,
+                                    (int)e->cls.size());
   // the first step reads the state's own columns and zeroes what lies
   // outside the update mask, as the first step of r2d3c_run does
   for (auto &k : e->cls) {
@@ -2985,7 +3355,8 @@ int r2d3c_ensemble_run(r2d3c_ensemble *e, const r2d3c_forcing *forcing, int n_st
     if (rc) return rc;
   }
   if (n_steps > 1) {
-    k_ens_tick<<<1, 1, 0, e->es>>>(e->d_step);
+    k_ens_force<<<1, 256, 0, e->es>>>(e->d_step, 1, 0, ftab, e->count, e->d_fcls,
+                                      (int)e->cls.size());
     LAUNCHCHK("ensemble tick");
     if (!e->graph_valid || e->graph_ftab != ftab) {
       int rc = ens_capture_all(e, ftab);

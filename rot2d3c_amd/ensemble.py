@@ -2,9 +2,11 @@
 reference runs one simulation per MPI job).
 
 Runs of one grid size advance together: every kernel of the step covers all
-of them in one launch (the run is blockIdx.y), and each size class is an
-independent branch of one HIP graph per time step, captured once and replayed.
-The forcing of every step is drawn on the host before the steps start
+of them in one launch (the run is blockIdx.y), and the size classes follow
+each other (or run as concurrent branches) in one HIP graph per time step,
+captured once and replayed.  Two other launch forms cover every size at once
+(set_stage_launch): one persistent launch per RK4 stage, or one launch per
+phase.  The forcing of every step is drawn on the host before the steps start
 (rand_force's wavevectors and phases do not depend on the flow) and uploaded
 as one table, so the steps need no host round trip.  Each run equals the same
 run advanced alone by rk4_run with its fixed dt, bit for bit.
@@ -52,6 +54,8 @@ class Ensemble:
         self.handle = h
         self.sizes = sorted({m.n for m in self.members})
         self.concurrent = 0
+        self.stage_launch = 0
+        self.row_split = {n: -1 for n in self.sizes}
         self.schedules = {n: autotune.DEFAULT for n in self.sizes}
         for n in self.sizes:
             if schedules == "auto":
@@ -77,13 +81,15 @@ class Ensemble:
             self.schedules[n] = (rw, bpc, cs)
         return rc
 
-    def tune(self, steps=20, trials=5):
+    def tune(self, steps=20, trials=5, log=None):
         """Choose the order of the size classes (one after another or
-        concurrent) and every size's schedule by timing the whole ensemble
-        step graph (a solo plan's fastest schedule need not be the fastest
-        there): the order first, then one coordinate-descent sweep over the
-        sizes, largest first, each trying every valid schedule with the other
-        sizes fixed (best of `trials` timings of `steps` unforced steps).
+        concurrent), every size's schedule and row form, and the stage launch
+        form by timing the whole ensemble step graph (a solo plan's fastest
+        schedule need not be the fastest there): the order first, then one
+        coordinate-descent sweep over the sizes, largest first, each trying
+        every valid schedule and then the other row form with the other sizes
+        fixed, then the all-size stage launches (best of `trials` timings of
+        `steps` unforced steps).
         The runs' states are restored afterwards, so tuning leaves the
         ensemble's trajectories as they were.  Returns the schedules and the
         final milliseconds per ensemble step."""
@@ -98,6 +104,10 @@ class Ensemble:
                 self.run(steps)
                 torch.cuda.synchronize(self.device)
                 best = min(best, (time.perf_counter() - t0) * 1e3 / steps)
+            if log:
+                log("tune: %s concurrent=%d row_split=%s stage_launch=%d: %.4f ms" % (
+                    dict(self.schedules), self.concurrent, dict(self.row_split),
+                    self.stage_launch, best))
             return best
 
         best_ms = timed()
@@ -112,22 +122,83 @@ class Ensemble:
             else:
                 self.set_concurrent(keep)
         for n in sorted(self.sizes, reverse=True):
+            # every schedule, and with the non-persistent row kernel at
+            # n <= 512 both row forms (the parallel field pairs, automatic
+            # there and made for latency-bound launches, and the sequential
+            # one, which a large batched launch may prefer)
             for cand in autotune.CANDIDATES:
-                if cand == self.schedules[n] or (cand[2] and n < 512):
+                if cand[2] and n < 512:
                     continue
-                keep = self.schedules[n]
-                if self._set_schedule(n, cand) != 0:
-                    continue
+                for split in ((-1, 0) if n <= 512 and cand[1] == 0 else (-1,)):
+                    if tuple(cand) == tuple(self.schedules[n]) and split == self.row_split[n]:
+                        continue
+                    keep = (self.schedules[n], self.row_split[n])
+                    if self._set_schedule(n, cand) != 0:
+                        continue
+                    self.set_row_split(n, split)
+                    ms = timed()
+                    if ms < best_ms:
+                        best_ms = ms
+                    else:
+                        check(self._set_schedule(n, keep[0]), "r2d3c_ensemble_set_schedule")
+                        self.set_row_split(n, keep[1])
+        # the all-size stage launches (one persistent launch per stage, one
+        # launch per phase), where every size fits them
+        if all(64 <= n <= 1024 for n in self.sizes) and len(self.sizes) <= 5 and \
+                not getattr(self, "_nets", None):
+            keep = (dict(self.schedules), dict(self.row_split))
+            best_form = 0
+            for form in (1, 2):
+                self.set_stage_launch(form)
                 ms = timed()
                 if ms < best_ms:
-                    best_ms = ms
-                else:
-                    check(self._set_schedule(n, keep), "r2d3c_ensemble_set_schedule")
+                    best_ms, best_form = ms, form
+            self.set_stage_launch(best_form)
+            if best_form == 0:
+                for n in self.sizes:
+                    check(self._set_schedule(n, keep[0][n]), "r2d3c_ensemble_set_schedule")
+                    self.set_row_split(n, keep[1][n])
         for m, (ps, vz) in zip(self.members, saved):
             m.ps.copy_(ps)
             m.vz.copy_(vz)
         torch.cuda.synchronize(self.device)
         return dict(self.schedules), best_ms
+
+    def set_stage_launch(self, form):
+        """How a step's stages are launched: 0 the per-size kernels (default),
+        1 one persistent launch per RK4 stage for every size (a grid barrier
+        between the row and column phases), 2 one launch per phase for every
+        size.  Forms 1 and 2 take sizes 64..1024 and set every size's
+        schedule to theirs (sequential row kernel, radix-8 Stockham columns:
+        autotune schedule (2, 0, 1), (2, 0, 0) below 512, row split off)."""
+        with torch.cuda.device(self.device):
+            check(_lib.lib().r2d3c_ensemble_set_stage_launch(self.handle, int(form)),
+                  "r2d3c_ensemble_set_stage_launch")
+        self.stage_launch = int(form)
+        if form:  # (the column form is moot below 512)
+            self.schedules = {n: (2, 0, 1 if n >= 512 else 0) for n in self.sizes}
+            self.row_split = {n: 0 for n in self.sizes}
+
+    def set_row_split(self, n, mode):
+        """Row kernel form of size n (-1 automatic: parallel field pairs for
+        n <= 512; 0 sequential; 1 parallel)."""
+        check(_lib.lib().r2d3c_ensemble_set_row_split(self.handle, int(n), int(mode)),
+              "r2d3c_ensemble_set_row_split")
+        self.row_split[n] = int(mode)
+
+    def persistent_grid(self):
+        """Blocks of the persistent launch (form 1), else 0."""
+        form, grid = ctypes.c_int(0), ctypes.c_int(0)
+        check(_lib.lib().r2d3c_ensemble_get_stage_launch(self.handle, ctypes.byref(form),
+                                                         ctypes.byref(grid)),
+              "r2d3c_ensemble_get_stage_launch")
+        return int(grid.value)
+
+    def sync(self):
+        """Wait for the ensemble's work; raises if a grid barrier of the
+        persistent launch timed out (that run call's states are invalid)."""
+        with torch.cuda.device(self.device):
+            check(_lib.lib().r2d3c_ensemble_sync(self.handle), "r2d3c_ensemble_sync")
 
     def set_concurrent(self, concurrent):
         """Size classes of a step one after another (0, the default) or as

@@ -211,9 +211,10 @@ def test_ensemble_with_stockham_columns_equals_solo_runs(cuda):
 
 
 def test_ensemble_tune_keeps_state_and_runs_match_solo(cuda):
-    """Ensemble.tune times schedules inside the ensemble's step graph: it
-    leaves every run's state as it was (bitwise), keeps valid schedules, and
-    afterwards each run still equals its solo run under the chosen schedule."""
+    """Ensemble.tune times schedules, row forms and stage launch forms inside
+    the ensemble's step graph: it leaves every run's state as it was
+    (bitwise), keeps valid schedules, and afterwards each run still equals its
+    solo run under the chosen schedule and row form."""
     print("ensemble tune", flush=True)
     from rot2d3c_amd import ensemble as E, spectral as S, _lib, autotune
     from rot2d3c_amd._lib import check
@@ -236,6 +237,7 @@ def test_ensemble_tune_keeps_state_and_runs_match_solo(cuda):
             rw, bpc, cs = sch[m.n]
             check(_lib.lib().r2d3c_plan_set_schedule(pl.handle, rw, bpc), "set_schedule")
             check(_lib.lib().r2d3c_plan_set_column_radix(pl.handle, cs), "set_column_radix")
+            check(_lib.lib().r2d3c_plan_set_row_split(pl.handle, ens.row_split[m.n]), "row_split")
             ps, vz = _solo(m, h, None, nsteps, cuda)
             torch.cuda.synchronize()
             assert torch.equal(ps, m.ps) and torch.equal(vz, m.vz), m.n
@@ -244,6 +246,7 @@ def test_ensemble_tune_keeps_state_and_runs_match_solo(cuda):
             pl = S.Plan.get(n, cuda)
             check(_lib.lib().r2d3c_plan_set_schedule(pl.handle, 2, 0), "set_schedule")
             check(_lib.lib().r2d3c_plan_set_column_radix(pl.handle, 0), "set_column_radix")
+            check(_lib.lib().r2d3c_plan_set_row_split(pl.handle, -1), "row_split")
     ens.close()
 
 
@@ -430,3 +433,55 @@ def test_ensemble_order_of_size_classes_does_not_change_results(cuda):
     for r, (m, h) in enumerate(zip(ms, host)):
         ps, vz = _solo(m, h, [table[st][r] for st in range(nsteps)], nsteps, cuda)
         assert torch.equal(outs[0][r][0], ps) and torch.equal(outs[0][r][1], vz), r
+
+
+@pytest.mark.parametrize("form,nsteps", [(1, 1), (1, 6), (2, 6)])
+def test_all_size_stage_launches_equal_solo_runs_bitwise(form, nsteps, cuda):
+    """One persistent launch per RK4 stage for all sizes (form 1), and one
+    launch per phase for all sizes (form 2): with 64, 128, 256, 512 and 1024
+    runs interleaved, with forcing, every run equals its solo r2d3c_run with
+    these launches' schedule (sequential row kernel, Stockham columns) bit
+    for bit; no grid barrier timed out."""
+    print("all-size stage launch %d vs solo, %d steps" % (form, nsteps), flush=True)
+    from rot2d3c_amd import ensemble as E, spectral as S, _lib
+    spec = _spec([256, 64, 1024, 512, 128, 256, 64, 512], seed=21)
+    ms, host = _members(spec, cuda)
+    ens = E.Ensemble(ms, schedules=None)
+    ens.set_stage_launch(form)
+    if form == 1:
+        assert ens.persistent_grid() >= 256 and ens.persistent_grid() % 16 == 0
+    table = E.forcing_table(spec, nsteps)
+    ens.run(nsteps, table)
+    ens.sync()
+    for r, (m, h) in enumerate(zip(ms, host)):
+        pl = S.Plan.get(m.n, cuda).handle
+        old = (ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int())
+        _lib.check(_lib.lib().r2d3c_plan_get_schedule(pl, ctypes.byref(old[0]), ctypes.byref(old[1])), "get")
+        _lib.check(_lib.lib().r2d3c_plan_get_row_split(pl, ctypes.byref(old[2])), "get")
+        _lib.check(_lib.lib().r2d3c_plan_get_column_radix(pl, ctypes.byref(old[3])), "get")
+        try:
+            _lib.check(_lib.lib().r2d3c_plan_set_schedule(pl, 2, 0), "set")
+            _lib.check(_lib.lib().r2d3c_plan_set_row_split(pl, 0), "set")
+            _lib.check(_lib.lib().r2d3c_plan_set_column_radix(pl, 1), "set")
+            draws = [table[st][r] for st in range(nsteps)]
+            ps, vz = _solo(m, h, draws, nsteps, cuda)
+            torch.cuda.synchronize()
+        finally:
+            _lib.lib().r2d3c_plan_set_schedule(pl, old[0].value, old[1].value)
+            _lib.lib().r2d3c_plan_set_row_split(pl, old[2].value)
+            _lib.lib().r2d3c_plan_set_column_radix(pl, old[3].value)
+        assert torch.equal(m.ps, ps) and torch.equal(m.vz, vz), "run %d (n=%d)" % (r, m.n)
+    ens.close()
+
+
+def test_all_size_stage_launches_reject_unsupported_sizes(cuda):
+    print("all-size stage launch: sizes", flush=True)
+    from rot2d3c_amd import ensemble as E, _lib
+    spec = _spec([32, 64], seed=4)
+    ms, _ = _members(spec, cuda)
+    ens = E.Ensemble(ms, schedules=None)
+    for form in (1, 2):
+        with pytest.raises(_lib.R2D3CError):
+            ens.set_stage_launch(form)
+    assert ens.persistent_grid() == 0 and ens.stage_launch == 0
+    ens.close()
