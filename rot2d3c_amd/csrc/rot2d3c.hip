@@ -1611,43 +1611,9 @@ __global__ __launch_bounds__(kPsT) __attribute__((amdgpu_waves_per_eu(3))) void 
   }
 }
 
-// The forcing of a step, copied into every run's constants (EnsMember::c.fp,
-// c.fz) before the step's kernels run, so that they read it with the run's
-// other constants instead of through the step counter and the table (a
-// dependent load at the start of every column block).  step[0]: the step,
-// step[1]: the table's steps.  advance 0 (a run call's start): step 0;
-// advance 1 (the end of a step, inside the step graph): the next step.
-// Past the table, or without one, the runs are unforced.
-struct EnsForceClass {
-  EnsMember *mem;
-  int base, count;  // index of the class's first run in the table, runs
-};
-__global__ __launch_bounds__(256) void k_ens_force(int *step, int advance, int table_steps,
-                                                   const ForceSet *__restrict__ ftab, int stride,
-                                                   const EnsForceClass *__restrict__ cls, int nk) {
-  __shared__ int cur;
-  if (threadIdx.x == 0) {
-    const int v = advance ? step[0] + 1 : 0;
-    step[0] = v;
-    if (!advance) step[1] = table_steps;
-    cur = v < (advance ? step[1] : table_steps) ? v : -1;
-  }
-  __syncthreads();
-  const int st = cur;
-  for (int ci = 0; ci < nk; ++ci) {
-    const EnsForceClass k = cls[ci];
-    for (int m = threadIdx.x; m < k.count; m += blockDim.x) {
-      ForceSet fp{}, fz{};
-      if (ftab && st >= 0) {
-        const ForceSet *f = ftab + ((size_t)st * stride + k.base + m) * 2;
-        fp = f[0];
-        fz = f[1];
-      }
-      k.mem[m].c.fp = fp;
-      k.mem[m].c.fz = fz;
-    }
-  }
-}
+// advances the forcing-table step at the end of a time step
+__global__ void k_ens_tick(int *step) { step[0] = step[0] + 1; }
+__global__ void k_set_i32(int *p, int v) { p[0] = v; }
 
 // ---------------------------------------------------------------------------
 // host side: plan, dispatch, C ABI
@@ -2823,8 +2789,7 @@ struct r2d3c_ensemble {
   ForceSet *d_ftab = nullptr;
   size_t ftab_cap = 0;              // ForceSet entries allocated
   std::vector<ForceSet> h_ftab;
-  int *d_step = nullptr;             // [0] step of the forcing table, [1] its steps
-  EnsForceClass *d_fcls = nullptr;  // the classes' member tables (k_ens_force)
+  int *d_step = nullptr;
   int period = 1;                   // closures refreshed every period-th step of a run call
   hipGraph_t graph = nullptr;       // one step with the closures
   hipGraphExec_t exec = nullptr;
@@ -2865,8 +2830,7 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
   const r2d3c_plan *p = k.plan;
   const int nc = p->g.nc, ncp = (nc + 1) & ~1;
   const cplx *tw = p->tw;
-  (void)ftab;  // the step's forcing is in the runs' constants (k_ens_force)
-  EnsArgs a{k.d_mem, nullptr, e->d_step, e->count, k.first};
+  EnsArgs a{k.d_mem, ftab, e->d_step, e->count, k.first};
   const bool cs = N >= 512 && p->col_cs;  // the columns' radix schedule
   if (first) {
     if (cs) k_ens_colinv<N, true><<<dim3(round16(ncp + nc), k.count), N / 8, 0, s>>>(a, tw);
@@ -2932,7 +2896,7 @@ static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, bool closures,
     std::vector<PsClass> h;
     for (auto &k : e->cls) {
       PsClass c{};
-      c.a = EnsArgs{k.d_mem, nullptr, e->d_step, e->count, k.first};
+      c.a = EnsArgs{k.d_mem, ftab, e->d_step, e->count, k.first};
       c.tw = k.plan->tw;
       c.om = k.d_om;
       c.n = k.n;
@@ -2987,8 +2951,7 @@ static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, bool closures,
     if (he == hipSuccess && !rc) he = hipStreamWaitEvent(e->es, k.done, 0);
   }
   if (he == hipSuccess && !rc) {
-    k_ens_force<<<1, 256, 0, e->es>>>(e->d_step, 1, 0, ftab, e->count, e->d_fcls,
-                                      (int)e->cls.size());
+    k_ens_tick<<<1, 1, 0, e->es>>>(e->d_step);
     he = hipGetLastError();
   }
   hipGraph_t g = nullptr;
@@ -3039,7 +3002,6 @@ static void ens_free(r2d3c_ensemble *e) {
   }
   if (e->d_ftab) (void)hipFree(e->d_ftab);
   if (e->d_step) (void)hipFree(e->d_step);
-  if (e->d_fcls) (void)hipFree(e->d_fcls);
   if (e->d_ps) (void)hipFree(e->d_ps);
   if (e->d_bar) (void)hipFree(e->d_bar);
   if (e->d_err) (void)hipFree(e->d_err);
@@ -3113,14 +3075,7 @@ int r2d3c_ensemble_create(r2d3c_ensemble **out, const r2d3c_member *members, int
   if (!rc && he == hipSuccess) he = hipEventCreateWithFlags(&e->ev_in, hipEventDisableTiming);
   if (!rc && he == hipSuccess) he = hipEventCreateWithFlags(&e->ev_out, hipEventDisableTiming);
   if (!rc && he == hipSuccess) he = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
-  if (!rc && he == hipSuccess) he = hipMalloc(&e->d_step, 2 * sizeof(int));
-  if (!rc && he == hipSuccess) {
-    std::vector<EnsForceClass> fc;
-    for (auto &k : e->cls) fc.push_back(EnsForceClass{k.d_mem, k.first, k.count});
-    he = hipMalloc(&e->d_fcls, sizeof(EnsForceClass) * fc.size());
-    if (he == hipSuccess)
-      he = hipMemcpy(e->d_fcls, fc.data(), sizeof(EnsForceClass) * fc.size(), hipMemcpyHostToDevice);
-  }
+  if (!rc && he == hipSuccess) he = hipMalloc(&e->d_step, sizeof(int));
   if (!rc && he != hipSuccess) rc = hip_err(he, "ensemble set-up");
   if (rc) { ens_free(e); return rc; }
   *out = e;
@@ -3346,8 +3301,7 @@ int r2d3c_ensemble_run(r2d3c_ensemble *e, const r2d3c_forcing *forcing, int n_st
   }
   HIPCHK(hipEventRecord(e->ev_in, s));
   HIPCHK(hipStreamWaitEvent(e->es, e->ev_in, 0));
-  k_ens_force<<<1, 256, 0, e->es>>>(e->d_step, 0, n_steps, ftab, e->count, e->d_fcls,
-                                    (int)e->cls.size());
+  k_set_i32<<<1, 1, 0, e->es>>>(e->d_step, 0);
   // the first step reads the state's own columns and zeroes what lies
   // outside the update mask, as the first step of r2d3c_run does
   for (auto &k : e->cls) {
@@ -3355,8 +3309,7 @@ int r2d3c_ensemble_run(r2d3c_ensemble *e, const r2d3c_forcing *forcing, int n_st
     if (rc) return rc;
   }
   if (n_steps > 1) {
-    k_ens_force<<<1, 256, 0, e->es>>>(e->d_step, 1, 0, ftab, e->count, e->d_fcls,
-                                      (int)e->cls.size());
+    k_ens_tick<<<1, 1, 0, e->es>>>(e->d_step);
     LAUNCHCHK("ensemble tick");
     if (!e->graph_valid || e->graph_ftab != ftab) {
       int rc = ens_capture_all(e, ftab);
