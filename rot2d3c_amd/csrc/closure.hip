@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -94,11 +95,6 @@ __global__ __launch_bounds__(64 * kConvWaves) void k_conv3x3(const bf16 *__restr
   constexpr int CI16 = CI / 16, CO16 = CO / 16, WS = CI + (CI % 32 == 0 ? 8 : 4),
                 TW = kTilesPerWave;
   __shared__ bf16 ws[9 * CO * WS];
-  for (int e = threadIdx.x; e < 9 * CO * (CI / 4); e += 64 * kConvWaves) {
-    const int row = e / (CI / 4), c4 = e - row * (CI / 4);
-    *(uint2 *)(ws + row * WS + 4 * c4) = *(const uint2 *)(Wt + (size_t)row * CI + 4 * c4);
-  }
-  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, q = lane >> 4;
   const int tpr = W / 16;
@@ -115,6 +111,35 @@ __global__ __launch_bounds__(64 * kConvWaves) void k_conv3x3(const bf16 *__restr
     ty[i] = rem / tpr;
     tx[i] = (rem - ty[i] * tpr) * 16;
   }
+  // The input fragments of all nine taps are loaded before the weights are
+  // staged, so the two round trips to memory overlap.  16x16x32 path: lane l
+  // holds 8 consecutive channels 8(l/16)..8(l/16)+7 of pixel l%16 (A) and the
+  // same channels of the weights of output channel l%16 (B); A and B use one
+  // channel order, so the reduction over K is the layer's channel sum.
+  constexpr bool K32 = CI % 32 == 0;
+  constexpr int NC = K32 ? CI / 32 : CI16;
+  using frag = typename std::conditional<K32, s16x8, s16x4>::type;
+  frag a[9][NC][TW];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) {
+    const int dy = t / 3 - 1, dx = t % 3 - 1;
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int i = 0; i < TW; ++i) {
+        int yy = ty[i] + dy, xx = tx[i] + m + dx;
+        yy += yy < 0 ? H : (yy >= H ? -H : 0);
+        xx += xx < 0 ? W : (xx >= W ? -W : 0);
+        const bf16 *src = X + ((size_t)((size_t)tb[i] * H + yy) * W + xx) * CI +
+                          (K32 ? 32 * c + 8 * q : 16 * c + 4 * q);
+        a[t][c][i] = ok[i] ? *(const frag *)src : frag{};
+      }
+  }
+  for (int e = threadIdx.x; e < 9 * CO * (CI / 4); e += 64 * kConvWaves) {
+    const int row = e / (CI / 4), c4 = e - row * (CI / 4);
+    *(uint2 *)(ws + row * WS + 4 * c4) = *(const uint2 *)(Wt + (size_t)row * CI + 4 * c4);
+  }
+  __syncthreads();
   f32x4 acc[TW][CO16];
 #pragma unroll
   for (int i = 0; i < TW; ++i)
@@ -122,54 +147,24 @@ __global__ __launch_bounds__(64 * kConvWaves) void k_conv3x3(const bf16 *__restr
     for (int n = 0; n < CO16; ++n) acc[i][n] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int t = 0; t < 9; ++t) {
-    const int dy = t / 3 - 1, dx = t % 3 - 1;
-    if constexpr (CI % 32 == 0) {
-      // gfx950's 16x16x32 BF16 MFMA: lane l holds 8 consecutive channels
-      // 8(l/16)..8(l/16)+7 of pixel l%16 (A) and the same channels of the
-      // weights of output channel l%16 (B); A and B use one channel order,
-      // so the reduction over K is the layer's channel sum
 #pragma unroll
-      for (int c = 0; c < CI / 32; ++c) {
-        s16x8 a[TW];
+    for (int c = 0; c < NC; ++c) {
 #pragma unroll
-        for (int i = 0; i < TW; ++i) {
-          int yy = ty[i] + dy, xx = tx[i] + m + dx;
-          yy += yy < 0 ? H : (yy >= H ? -H : 0);
-          xx += xx < 0 ? W : (xx >= W ? -W : 0);
-          a[i] = ok[i] ? *(const s16x8 *)(X + ((size_t)((size_t)tb[i] * H + yy) * W + xx) * CI +
-                                          32 * c + 8 * q)
-                       : s16x8{0, 0, 0, 0, 0, 0, 0, 0};
-        }
-#pragma unroll
-        for (int n = 0; n < CO16; ++n) {
+      for (int n = 0; n < CO16; ++n) {
+        if constexpr (K32) {
           const s16x8 w = *(const s16x8 *)(ws + (t * CO + 16 * n + m) * WS + 32 * c + 8 * q);
 #pragma unroll
           for (int i = 0; i < TW; ++i)
             acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                __builtin_bit_cast(bf16x8, a[i]), __builtin_bit_cast(bf16x8, w), acc[i][n], 0, 0,
-                0);
+                __builtin_bit_cast(bf16x8, a[t][c][i]), __builtin_bit_cast(bf16x8, w), acc[i][n],
+                0, 0, 0);
+        } else {
+          const s16x4 w = *(const s16x4 *)(ws + (t * CO + 16 * n + m) * WS + 16 * c + 4 * q);
+#pragma unroll
+          for (int i = 0; i < TW; ++i)
+            acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t][c][i], w, acc[i][n], 0, 0,
+                                                                   0);
         }
-      }
-      continue;
-    }
-#pragma unroll
-    for (int c = 0; c < CI16; ++c) {
-      s16x4 a[TW];
-#pragma unroll
-      for (int i = 0; i < TW; ++i) {
-        int yy = ty[i] + dy, xx = tx[i] + m + dx;
-        yy += yy < 0 ? H : (yy >= H ? -H : 0);
-        xx += xx < 0 ? W : (xx >= W ? -W : 0);
-        a[i] = ok[i] ? *(const s16x4 *)(X + ((size_t)((size_t)tb[i] * H + yy) * W + xx) * CI +
-                                        16 * c + 4 * q)
-                     : s16x4{0, 0, 0, 0};
-      }
-#pragma unroll
-      for (int n = 0; n < CO16; ++n) {
-        const s16x4 w = *(const s16x4 *)(ws + (t * CO + 16 * n + m) * WS + 16 * c + 4 * q);
-#pragma unroll
-        for (int i = 0; i < TW; ++i)
-          acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], w, acc[i][n], 0, 0, 0);
       }
     }
   }

@@ -971,12 +971,20 @@ __device__ __forceinline__ void step_row_body(const cplx *__restrict__ M, cplx *
   unpack_store<N, RW, N / 3 + 1>(lds + r, b7, nc, kp, Q, y, Q + fs, y);
 }
 
+// tick (ensembles): a step counter the launch advances (block 0, thread 0);
+// the launch is the first of a time step, and the step's readers of the
+// counter (the column kernels) run after it.
+__device__ __forceinline__ void row_tick(int *tick) {
+  if (tick && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) tick[0] = tick[0] + 1;
+}
+
 template <int N>
 __global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
     const cplx *__restrict__ M, cplx *__restrict__ Q, StepConst c, unsigned long long *maxbits,
-    const cplx *__restrict__ tw, size_t mstride, const double *__restrict__ om_tab) {
+    const cplx *__restrict__ tw, size_t mstride, const double *__restrict__ om_tab, int *tick) {
   constexpr int RW = RowStep<N>::ROWS;
   __shared__ cplx lds[RW * Shape<N>::LDSN];
+  row_tick(tick);
   const int r = threadIdx.x % RW, b = threadIdx.x / RW;
   const int y = xcd_group<RowStep<N>::G>(blockIdx.x) * RW + r;
   M += blockIdx.y * mstride;
@@ -999,9 +1007,10 @@ __global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
 template <int N>
 __global__ __launch_bounds__(3 * (N / 8) * RowStep<N>::ROWS) void k_step_row3(
     const cplx *__restrict__ M, cplx *__restrict__ Q, StepConst c, unsigned long long *maxbits,
-    const cplx *__restrict__ tw, size_t mstride, const double *__restrict__ om_tab) {
+    const cplx *__restrict__ tw, size_t mstride, const double *__restrict__ om_tab, int *tick) {
   constexpr int T = N / 8, RW = RowStep<N>::ROWS, GT = T * RW, LN = RW * Shape<N>::LDSN;
   __shared__ cplx lds[3 * LN];
+  row_tick(tick);
   const int g = threadIdx.x / GT, t = threadIdx.x - g * GT;
   const int r = t % RW, b = t / RW;
   const int y = xcd_group<RowStep<N>::G>(blockIdx.x) * RW + r;
@@ -1058,7 +1067,8 @@ __global__ __launch_bounds__(3 * (N / 8) * RowStep<N>::ROWS) void k_step_row3(
 template <int N>
 static void launch_row_np(const r2d3c_plan *p, const cplx *M, cplx *Q, const StepConst &c,
                           unsigned long long *macc, const cplx *tw, size_t ms, const double *om,
-                          int count, hipStream_t s, hipEvent_t ea, hipEvent_t eb);
+                          int count, hipStream_t s, hipEvent_t ea, hipEvent_t eb,
+                          int *tick = nullptr);
 
 
 // Persistent variant of the row kernel for n >= 256.  A block walks its y
@@ -1142,11 +1152,12 @@ __device__ __forceinline__ void build_packed(cplx v[8], const cplx *sb, int r, i
 template <int N, int RWQ>
 __global__ __launch_bounds__((RowV2<N, RWQ>::THREADS)) void k_step_row2(
     const cplx *__restrict__ M, cplx *__restrict__ Q, StepConst c, unsigned long long *maxbits,
-    const cplx *__restrict__ tw, size_t mstride, const double *__restrict__ om_tab) {
+    const cplx *__restrict__ tw, size_t mstride, const double *__restrict__ om_tab, int *tick) {
   using RV = RowV2<N, RWQ>;
   constexpr int T = N / 8, RW = RV::RW, G = RV::G;
   __shared__ cplx xb[RW * Shape<N>::LDSN];
   __shared__ cplx sb[RV::SBN];
+  row_tick(tick);
   const int tid = threadIdx.x;
   const int r = tid % RW, b = tid / RW;
   cplx *P = xb + r;
@@ -1843,14 +1854,15 @@ static int make_step_const(r2d3c_plan *p, const r2d3c_params *q, int dt_mode, do
 template <int N>
 static void launch_row_np(const r2d3c_plan *p, const cplx *M, cplx *Q, const StepConst &c,
                           unsigned long long *macc, const cplx *tw, size_t ms, const double *om,
-                          int count, hipStream_t s, hipEvent_t ea, hipEvent_t eb) {
+                          int count, hipStream_t s, hipEvent_t ea, hipEvent_t eb, int *tick) {
   constexpr int RW = RowStep<N>::ROWS;
   if (N <= 512 && p->row_split != 0)
     hipExtLaunchKernelGGL(k_step_row3<(N <= 512 ? N : 512)>, dim3(N / RW, count),
-                          dim3(3 * (N / 8) * RW), 0, s, ea, eb, 0, M, Q, c, macc, tw, ms, om);
+                          dim3(3 * (N / 8) * RW), 0, s, ea, eb, 0, M, Q, c, macc, tw, ms, om,
+                          tick);
   else
     hipExtLaunchKernelGGL(k_step_row<N>, dim3(N / RW, count), dim3(N / 8 * RW), 0, s, ea, eb, 0,
-                          M, Q, c, macc, tw, ms, om);
+                          M, Q, c, macc, tw, ms, om, tick);
 }
 
 // Kernels of one time step.  first: the state's column transforms are made
@@ -1883,11 +1895,11 @@ static int step_seq(r2d3c_plan *p, StepConst &c, cplx *psi, cplx *vz, hipStream_
       else if (p->row_rw == 1 && RowV2<N, 1>::THREADS >= 64)
         hipExtLaunchKernelGGL(k_step_row2<N, 1>, dim3(row2_grid<N, 1>(p->cus, p->row_bpc)),
                               dim3(RowV2<N, 1>::THREADS), 0, s, ea, eb, 0, (const cplx *)M, Q, c,
-                              macc, tw, (size_t)0, (const double *)nullptr);
+                              macc, tw, (size_t)0, (const double *)nullptr, (int *)nullptr);
       else
         hipExtLaunchKernelGGL(k_step_row2<N, 2>, dim3(row2_grid<N, 2>(p->cus, p->row_bpc)),
                               dim3(RowV2<N, 2>::THREADS), 0, s, ea, eb, 0, (const cplx *)M, Q, c,
-                              macc, tw, (size_t)0, (const double *)nullptr);
+                              macc, tw, (size_t)0, (const double *)nullptr, (int *)nullptr);
     } else {
       launch_row_np<N>(p, M, Q, c, macc, tw, 0, nullptr, 1, s, ea, eb);
     }
@@ -2811,9 +2823,11 @@ struct r2d3c_ensemble {
   int ps_grid = 0, ps_rows = 0, ps_cols = 0;
 };
 
+// tick: the step counter this class's first row launch advances (the first
+// class of a step whose classes follow each other), or NULL.
 template <int N>
 static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, hipStream_t s,
-                          bool first, bool masked, bool chain, bool closures) {
+                          bool first, bool masked, bool chain, bool closures, int *tick) {
   // the closure forcing of this step, from the step's state (else the
   // forcing of the last evaluation is held)
   for (size_t ci = 0; closures && ci < k.closures.size(); ++ci) {
@@ -2841,17 +2855,21 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
     const size_t ms = 8 * p->fs;
     const cplx *M0 = k.work;
     cplx *Q0 = k.work + 6 * p->fs;
+    int *tk = o == 4 ? tick : nullptr;
     if constexpr (RowV2<N, 2>::THREADS >= 64) {
       if (p->row_bpc == 0)
-        launch_row_np<N>(p, M0, Q0, k.c0, nullptr, tw, ms, k.d_om, k.count, s, nullptr, nullptr);
+        launch_row_np<N>(p, M0, Q0, k.c0, nullptr, tw, ms, k.d_om, k.count, s, nullptr, nullptr,
+                         tk);
       else if (p->row_rw == 1 && RowV2<N, 1>::THREADS >= 64)
         k_step_row2<N, 1><<<dim3(row2_grid<N, 1>(p->cus, p->row_bpc), k.count),
-                            RowV2<N, 1>::THREADS, 0, s>>>(M0, Q0, k.c0, nullptr, tw, ms, k.d_om);
+                            RowV2<N, 1>::THREADS, 0, s>>>(M0, Q0, k.c0, nullptr, tw, ms, k.d_om,
+                                                          tk);
       else
         k_step_row2<N, 2><<<dim3(row2_grid<N, 2>(p->cus, p->row_bpc), k.count),
-                            RowV2<N, 2>::THREADS, 0, s>>>(M0, Q0, k.c0, nullptr, tw, ms, k.d_om);
+                            RowV2<N, 2>::THREADS, 0, s>>>(M0, Q0, k.c0, nullptr, tw, ms, k.d_om,
+                                                          tk);
     } else {
-      launch_row_np<N>(p, M0, Q0, k.c0, nullptr, tw, ms, k.d_om, k.count, s, nullptr, nullptr);
+      launch_row_np<N>(p, M0, Q0, k.c0, nullptr, tw, ms, k.d_om, k.count, s, nullptr, nullptr, tk);
     }
     int mode = FM_EMIT;
     if (o == 1) mode = FM_STATE | (masked ? FM_DISK : FM_ZERO) | (chain ? FM_EMIT : 0);
@@ -2864,18 +2882,19 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
 }
 
 static int ens_dispatch(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, hipStream_t s,
-                        bool first, bool masked, bool chain, bool closures = true) {
+                        bool first, bool masked, bool chain, bool closures = true,
+                        int *tick = nullptr) {
   int rc;
   switch (k.n) {
-    case 16: rc = ens_class_step<16>(e, k, ftab, s, first, masked, chain, closures); break;
-    case 32: rc = ens_class_step<32>(e, k, ftab, s, first, masked, chain, closures); break;
-    case 64: rc = ens_class_step<64>(e, k, ftab, s, first, masked, chain, closures); break;
-    case 128: rc = ens_class_step<128>(e, k, ftab, s, first, masked, chain, closures); break;
-    case 256: rc = ens_class_step<256>(e, k, ftab, s, first, masked, chain, closures); break;
-    case 512: rc = ens_class_step<512>(e, k, ftab, s, first, masked, chain, closures); break;
-    case 1024: rc = ens_class_step<1024>(e, k, ftab, s, first, masked, chain, closures); break;
-    case 2048: rc = ens_class_step<2048>(e, k, ftab, s, first, masked, chain, closures); break;
-    case 4096: rc = ens_class_step<4096>(e, k, ftab, s, first, masked, chain, closures); break;
+    case 16: rc = ens_class_step<16>(e, k, ftab, s, first, masked, chain, closures, tick); break;
+    case 32: rc = ens_class_step<32>(e, k, ftab, s, first, masked, chain, closures, tick); break;
+    case 64: rc = ens_class_step<64>(e, k, ftab, s, first, masked, chain, closures, tick); break;
+    case 128: rc = ens_class_step<128>(e, k, ftab, s, first, masked, chain, closures, tick); break;
+    case 256: rc = ens_class_step<256>(e, k, ftab, s, first, masked, chain, closures, tick); break;
+    case 512: rc = ens_class_step<512>(e, k, ftab, s, first, masked, chain, closures, tick); break;
+    case 1024: rc = ens_class_step<1024>(e, k, ftab, s, first, masked, chain, closures, tick); break;
+    case 2048: rc = ens_class_step<2048>(e, k, ftab, s, first, masked, chain, closures, tick); break;
+    case 4096: rc = ens_class_step<4096>(e, k, ftab, s, first, masked, chain, closures, tick); break;
     default: return set_err(R2D3C_ENOTSUP, "unsupported grid size");
   }
   if (rc) return rc;
@@ -2883,8 +2902,13 @@ static int ens_dispatch(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, hi
   return 0;
 }
 
-// Captures one steady time step of every class (their launches are independent
-// branches of the graph) followed by the forcing-table tick.
+// Whether a step's first row launch advances the forcing-table step (the
+// per-size classes one after another) rather than a tick kernel.
+static bool ens_row_ticks(const r2d3c_ensemble *e) { return !e->persistent && !e->concurrent; }
+
+// Captures one steady time step of every class: the classes one after another
+// or as concurrent branches, or the all-size stage launches; the forcing-table
+// step advances at the step's start (ens_row_ticks).
 static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, bool closures,
                        hipGraph_t *graph, hipGraphExec_t *exec) {
   HIPCHK(hipStreamSynchronize(e->es));  // no launch of the old graph may still run
@@ -2919,7 +2943,12 @@ static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, bool closures,
   }
   HIPCHK(hipStreamBeginCapture(e->es, hipStreamCaptureModeThreadLocal));
   int rc = 0;
-  hipError_t he = hipEventRecord(e->fork, e->es);
+  // the forcing-table step: advanced by the first row launch when the
+  // per-size classes follow each other, else by a tick at the step's start
+  const bool row_tick = ens_row_ticks(e);
+  if (!row_tick) k_ens_tick<<<1, 1, 0, e->es>>>(e->d_step);
+  hipError_t he = hipGetLastError();
+  if (he == hipSuccess) he = hipEventRecord(e->fork, e->es);
   if (e->persistent) {
     for (int o = 4; o >= 1 && he == hipSuccess; --o) {
       const int mode = o == 1 ? (FM_STATE | FM_DISK | FM_EMIT) : FM_EMIT;
@@ -2942,7 +2971,8 @@ static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, bool closures,
     EnsClass &k = *kp;
     if (he != hipSuccess || rc || e->persistent) break;
     if (!e->concurrent) {  // one class after another, each with the whole GPU
-      rc = ens_dispatch(e, k, ftab, e->es, false, true, true, closures);
+      rc = ens_dispatch(e, k, ftab, e->es, false, true, true, closures,
+                        row_tick && kp == by_size[0] ? e->d_step : nullptr);
       continue;
     }
     he = hipStreamWaitEvent(k.st, e->fork, 0);
@@ -2950,10 +2980,7 @@ static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, bool closures,
     if (he == hipSuccess && !rc) he = hipEventRecord(k.done, k.st);
     if (he == hipSuccess && !rc) he = hipStreamWaitEvent(e->es, k.done, 0);
   }
-  if (he == hipSuccess && !rc) {
-    k_ens_tick<<<1, 1, 0, e->es>>>(e->d_step);
-    he = hipGetLastError();
-  }
+  if (he == hipSuccess && !rc) he = hipGetLastError();
   hipGraph_t g = nullptr;
   const hipError_t ee = hipStreamEndCapture(e->es, &g);
   if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -3301,16 +3328,19 @@ int r2d3c_ensemble_run(r2d3c_ensemble *e, const r2d3c_forcing *forcing, int n_st
   }
   HIPCHK(hipEventRecord(e->ev_in, s));
   HIPCHK(hipStreamWaitEvent(e->es, e->ev_in, 0));
-  k_set_i32<<<1, 1, 0, e->es>>>(e->d_step, 0);
+  // step 0 during the first step: with row ticks its first row launch
+  // advances the counter from -1
+  const bool row_tick = ens_row_ticks(e);
+  k_set_i32<<<1, 1, 0, e->es>>>(e->d_step, row_tick ? -1 : 0);
   // the first step reads the state's own columns and zeroes what lies
   // outside the update mask, as the first step of r2d3c_run does
   for (auto &k : e->cls) {
-    int rc = ens_dispatch(e, k, ftab, e->es, true, false, n_steps > 1);
+    int rc = ens_dispatch(e, k, ftab, e->es, true, false, n_steps > 1, true,
+                          row_tick && &k == &e->cls[0] ? e->d_step : nullptr);
     if (rc) return rc;
   }
   if (n_steps > 1) {
-    k_ens_tick<<<1, 1, 0, e->es>>>(e->d_step);
-    LAUNCHCHK("ensemble tick");
+    LAUNCHCHK("ensemble first step");
     if (!e->graph_valid || e->graph_ftab != ftab) {
       int rc = ens_capture_all(e, ftab);
       if (rc) return rc;
@@ -3505,13 +3535,20 @@ __global__ __launch_bounds__(N / 8) void k_ci_colinv(const cplx *__restrict__ ps
 // Rows y, y+1 of the three fields to the network input bf16 [y][x][16]:
 // channel c = real-space field c times s[c] (s includes 1/n^2), 3..15 zero.
 // Three packed transforms: (f0, f1) of row y, (f0, f1) of row y+1, f2 of both.
+// When 3 n/8 threads fit a block (kCiPar), thread group g runs transform g in
+// LDS region g, in parallel (the launch is latency-bound), and the third
+// group's result is shared through its region; otherwise one group runs the
+// three in turn.  The arithmetic is the same either way.
+template <int N> struct CiRow {
+  static constexpr int G = 3 * (N / 8) <= 1024 ? 3 : 1;  // thread groups
+};
 template <int N>
-__global__ __launch_bounds__(N / 8) void k_ci_row(const cplx *__restrict__ mix, Geo g,
-                                                   const cplx *__restrict__ tw, double s0, double s1,
-                                                   double s2, unsigned short *__restrict__ out) {
-  constexpr int T = N / 8;
-  __shared__ cplx lds[Shape<N>::LDSN];
-  const int b = threadIdx.x, y = 2 * blockIdx.x;
+__global__ __launch_bounds__(CiRow<N>::G * (N / 8)) void k_ci_row(
+    const cplx *__restrict__ mix, Geo g, const cplx *__restrict__ tw, double s0, double s1,
+    double s2, unsigned short *__restrict__ out) {
+  constexpr int T = N / 8, G = CiRow<N>::G, LN = Shape<N>::LDSN;
+  __shared__ cplx lds[G * LN];
+  const int grp = threadIdx.x / T, b = threadIdx.x - grp * T, y = 2 * blockIdx.x;
   const size_t fs = (size_t)N * g.kpf;
   auto load = [&](cplx v[8], const cplx *fa, int ya, const cplx *fb, int yb) {
 #pragma unroll
@@ -3522,24 +3559,44 @@ __global__ __launch_bounds__(N / 8) void k_ci_row(const cplx *__restrict__ mix, 
       v[q] = packed_z<N>(k, fa[tix(ya, kk, g.kpf)], fb[tix(yb, kk, g.kpf)], mir);
     }
   };
-  cplx a[8], c[8], v[8];
-  load(a, mix, y, mix + fs, y);
-  fft_canon<N, +1, 1>(a, lds, b, tw);
-  load(c, mix, y + 1, mix + fs, y + 1);
-  fft_canon<N, +1, 1>(c, lds, b, tw);
-  load(v, mix + 2 * fs, y, mix + 2 * fs, y + 1);
-  fft_canon<N, +1, 1>(v, lds, b, tw);
+  auto emit = [&](int r, const cplx z[8], const cplx v[8]) {  // row y + r
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int x = b + T * q;
-    const cplx *zr[2] = {&a[q], &c[q]};
-    for (int r = 0; r < 2; ++r) {
-      const unsigned h0 = to_bf16((float)(zr[r]->x * s0)), h1 = to_bf16((float)(zr[r]->y * s1));
+    for (int q = 0; q < 8; ++q) {
+      const int x = b + T * q;
+      const unsigned h0 = to_bf16((float)(z[q].x * s0)), h1 = to_bf16((float)(z[q].y * s1));
       const unsigned h2 = to_bf16((float)((r == 0 ? v[q].x : v[q].y) * s2));
       uint4 *px = (uint4 *)(out + ((size_t)(y + r) * N + x) * 16);
       px[0] = make_uint4(h0 | (h1 << 16), h2, 0u, 0u);
       px[1] = make_uint4(0u, 0u, 0u, 0u);
     }
+  };
+  if constexpr (G == 3) {
+    cplx a[8];
+    if (grp < 2) load(a, mix, y + grp, mix + fs, y + grp);
+    else load(a, mix + 2 * fs, y, mix + 2 * fs, y + 1);
+    cplx *P = lds + grp * LN;
+    fft_canon<N, +1, 1>(a, P, b, tw);
+    if (grp == 2) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) P[pad(b + T * q)] = a[q];
+    }
+    __syncthreads();
+    if (grp < 2) {
+      cplx v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = lds[2 * LN + pad(b + T * q)];
+      emit(grp, a, v);
+    }
+  } else {
+    cplx a[8], c[8], v[8];
+    load(a, mix, y, mix + fs, y);
+    fft_canon<N, +1, 1>(a, lds, b, tw);
+    load(c, mix, y + 1, mix + fs, y + 1);
+    fft_canon<N, +1, 1>(c, lds, b, tw);
+    load(v, mix + 2 * fs, y, mix + 2 * fs, y + 1);
+    fft_canon<N, +1, 1>(v, lds, b, tw);
+    emit(0, a, v);
+    emit(1, c, v);
   }
 }
 
@@ -3635,8 +3692,8 @@ static int run_closure_io(r2d3c_plan *p, int dir, const cplx *psi, const cplx *v
   if (dir == 0) {
     k_ci_colinv<N><<<dim3(N / 2 + 1, 3), N / 8, 0, s>>>(psi, vz, p->mfull, p->g, p->tw);
     const double inv = 1.0 / ((double)N * N);
-    k_ci_row<N><<<N / 2, N / 8, 0, s>>>(p->mfull, p->g, p->tw, sc[0] * inv, sc[1] * inv,
-                                          sc[2] * inv, in_bf16);
+    k_ci_row<N><<<N / 2, CiRow<N>::G * (N / 8), 0, s>>>(p->mfull, p->g, p->tw, sc[0] * inv,
+                                                         sc[1] * inv, sc[2] * inv, in_bf16);
   } else {
     k_cf_row<N><<<N / 2, N / 8, 0, s>>>(z, p->g, p->tw, sc[0], sc[1], p->mfull);
     const double *gain = nullptr;
