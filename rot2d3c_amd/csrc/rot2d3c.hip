@@ -3601,16 +3601,22 @@ __global__ __launch_bounds__(CiRow<N>::G * (N / 8)) void k_ci_row(
 }
 
 // Rows y, y+1 of the prediction (channels 0, 1 of fp32 [y][x][16], times
-// s0, s1) forward transformed as packed real pairs into mixed fields 0, 1.
+// s0, s1) forward transformed as packed real pairs into mixed fields 0, 1:
+// thread group ch (two groups in parallel when 2 n/8 threads fit a block,
+// else one group for both in turn) handles channel ch in LDS region ch.
+template <int N> struct CfRow {
+  static constexpr int G = 2 * (N / 8) <= 1024 ? 2 : 1;
+};
 template <int N>
-__global__ __launch_bounds__(N / 8) void k_cf_row(const float *__restrict__ z, Geo g,
-                                                   const cplx *__restrict__ tw, double s0, double s1,
-                                                   cplx *__restrict__ mix) {
-  constexpr int T = N / 8;
-  __shared__ cplx lds[Shape<N>::LDSN];
-  const int b = threadIdx.x, y = 2 * blockIdx.x;
+__global__ __launch_bounds__(CfRow<N>::G * (N / 8)) void k_cf_row(
+    const float *__restrict__ z, Geo g, const cplx *__restrict__ tw, double s0, double s1,
+    cplx *__restrict__ mix) {
+  constexpr int T = N / 8, G = CfRow<N>::G, LN = Shape<N>::LDSN;
+  __shared__ cplx lds[G * LN];
+  const int grp = threadIdx.x / T, b = threadIdx.x - grp * T, y = 2 * blockIdx.x;
   const size_t fs = (size_t)N * g.kpf;
-  for (int ch = 0; ch < 2; ++ch) {
+  cplx *P = lds + grp * LN;
+  for (int ch = grp; ch < 2; ch += G) {
     const double sc = ch == 0 ? s0 : s1;
     cplx v[8];
 #pragma unroll
@@ -3619,11 +3625,11 @@ __global__ __launch_bounds__(N / 8) void k_cf_row(const float *__restrict__ z, G
       v[q] = mk((double)z[((size_t)y * N + x) * 16 + ch] * sc,
                 (double)z[((size_t)(y + 1) * N + x) * 16 + ch] * sc);
     }
-    fft_canon<N, -1, 1>(v, lds, b, tw);
+    fft_canon<N, -1, 1>(v, P, b, tw);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) lds[pad(b + T * q)] = v[q];
+    for (int q = 0; q < 8; ++q) P[pad(b + T * q)] = v[q];
     __syncthreads();
-    unpack_store<N, 1, N / 2 + 1>(lds, b, g.ncf, g.kpf, mix + ch * fs, y, mix + ch * fs, y + 1);
+    unpack_store<N, 1, N / 2 + 1>(P, b, g.ncf, g.kpf, mix + ch * fs, y, mix + ch * fs, y + 1);
     __syncthreads();
   }
 }
@@ -3695,7 +3701,7 @@ static int run_closure_io(r2d3c_plan *p, int dir, const cplx *psi, const cplx *v
     k_ci_row<N><<<N / 2, CiRow<N>::G * (N / 8), 0, s>>>(p->mfull, p->g, p->tw, sc[0] * inv,
                                                          sc[1] * inv, sc[2] * inv, in_bf16);
   } else {
-    k_cf_row<N><<<N / 2, N / 8, 0, s>>>(z, p->g, p->tw, sc[0], sc[1], p->mfull);
+    k_cf_row<N><<<N / 2, CfRow<N>::G * (N / 8), 0, s>>>(z, p->g, p->tw, sc[0], sc[1], p->mfull);
     const double *gain = nullptr;
     int ngain = 0;
     r2d3c_internal_cnn_forcing_gain(net, &gain, &ngain);
