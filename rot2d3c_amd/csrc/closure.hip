@@ -82,6 +82,43 @@ enum { CONV_GELU = 0, CONV_LINEAR = 1, CONV_BWD = 2 };
 #define R2D_CONV_WPB 16
 #endif
 constexpr int kTilesPerWave = R2D_CONV_TPW, kConvWaves = R2D_CONV_WPB;
+
+// Epilogue of one tile of 16 pixels (img, y, x0 .. x0+15): lane (m, q) holds
+// pixels 4q..4q+3 of channel 16n + m in acc[n].
+//   CONV_GELU / CONV_LINEAR: Z = D + bias (stored when Zout), Y = bf16(gelu(Z) or Z)
+//   CONV_BWD: Y = bf16(D * gelu'(Zprev))
+template <int CO, int MODE>
+__device__ __forceinline__ void conv_store(const f32x4 (&acc)[CO / 16], int img, int y, int x0,
+                                           int m, int q, int H, int W,
+                                           const float *__restrict__ bias,
+                                           const float *__restrict__ Zprev,
+                                           float *__restrict__ Zout, bf16 *__restrict__ Y,
+                                           float clamp, const bf16 *__restrict__ X0, float keep) {
+#pragma unroll
+  for (int n = 0; n < CO / 16; ++n) {
+    const int co = 16 * n + m;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const size_t o = ((size_t)((size_t)img * H + y) * W + x0 + 4 * q + r) * CO + co;
+      const float v = acc[n][r];
+      if constexpr (MODE == CONV_BWD) {
+        Y[o] = f2bf(v * gelu_grad(Zprev[o]));
+      } else {
+        float z = v + bias[co];
+        if constexpr (MODE == CONV_LINEAR) {
+          if (clamp > 0.f) z = fminf(fmaxf(z, -clamp), clamp);
+          // sign clipping: where output channel co < 2 shares the sign of
+          // input channel co (X0 is the network input, 16 channels) it is
+          // scaled by keep = 1 - strength
+          if (X0 && co < 2 && z * bf2f(X0[o / CO * 16 + co]) > 0.f) z *= keep;
+        }
+        if (Zout) Zout[o] = z;
+        Y[o] = f2bf(MODE == CONV_GELU ? gelu(z) : z);
+      }
+    }
+  }
+}
+
 template <int CI, int CO, int MODE>
 __global__ __launch_bounds__(64 * kConvWaves) void k_conv3x3(const bf16 *__restrict__ X,
                                                   const bf16 *__restrict__ Wt,
@@ -169,32 +206,10 @@ __global__ __launch_bounds__(64 * kConvWaves) void k_conv3x3(const bf16 *__restr
     }
   }
 #pragma unroll
-  for (int i = 0; i < TW; ++i) {
-    if (!ok[i]) continue;
-#pragma unroll
-    for (int n = 0; n < CO16; ++n) {
-      const int co = 16 * n + m;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const size_t o = ((size_t)((size_t)tb[i] * H + ty[i]) * W + tx[i] + 4 * q + r) * CO + co;
-        const float v = acc[i][n][r];
-        if constexpr (MODE == CONV_BWD) {
-          Y[o] = f2bf(v * gelu_grad(Zprev[o]));
-        } else {
-          float z = v + bias[co];
-          if constexpr (MODE == CONV_LINEAR) {
-            if (clamp > 0.f) z = fminf(fmaxf(z, -clamp), clamp);
-            // sign clipping: where output channel co < 2 shares the sign of
-            // input channel co (X0 is the network input, 16 channels) it is
-            // scaled by keep = 1 - strength
-            if (X0 && co < 2 && z * bf2f(X0[o / CO * 16 + co]) > 0.f) z *= keep;
-          }
-          if (Zout) Zout[o] = z;
-          Y[o] = f2bf(MODE == CONV_GELU ? gelu(z) : z);
-        }
-      }
-    }
-  }
+  for (int i = 0; i < TW; ++i)
+    if (ok[i])
+      conv_store<CO, MODE>(acc[i], tb[i], ty[i], tx[i], m, q, H, W, bias, Zprev, Zout, Y, clamp,
+                           X0, keep);
 }
 
 // Weight gradient of one layer over a slice of rows: dW[t][co][ci] =

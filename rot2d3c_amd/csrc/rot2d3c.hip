@@ -3234,6 +3234,9 @@ int r2d3c_ensemble_set_closure(r2d3c_ensemble *e, int run, r2d3c_cnn *net, const
                                const double *scale_out, double kmin, double kclip) {
   NEED(e);
   if (run < 0 || run >= e->count) return set_err(R2D3C_EINVAL, "no such run");
+  if (net && e->persistent)
+    return set_err(R2D3C_ENOTSUP, "closures need the per-size stage launches "
+                                  "(r2d3c_ensemble_set_stage_launch(ens, 0))");
   int j = 0;
   while (e->order[j] != run) ++j;
   EnsClass *kp = nullptr;

@@ -485,3 +485,15 @@ def test_all_size_stage_launches_reject_unsupported_sizes(cuda):
             ens.set_stage_launch(form)
     assert ens.persistent_grid() == 0 and ens.stage_launch == 0
     ens.close()
+    # a closure cannot join an ensemble whose stages run as all-size launches
+    from rot2d3c_amd import closure as L
+    spec = _spec([64, 128], seed=6)
+    ms, _ = _members(spec, cuda)
+    ens = E.Ensemble(ms, schedules=None)
+    ens.set_stage_launch(2)
+    net = L.Net(1, 64, 64, cuda, seed=1)
+    with pytest.raises(_lib.R2D3CError):
+        ens.set_closure(0, net, [1.0, 1.0, 1.0], [1.0, 1.0])
+    ens.set_stage_launch(0)
+    ens.set_closure(0, net, [1.0, 1.0, 1.0], [1.0, 1.0])
+    ens.close()
