@@ -78,6 +78,22 @@ __device__ __forceinline__ int xcd_pairs(int hb) {
 }
 __host__ __forceinline__ int round16(int x) { return (x + 15) & ~15; }
 
+// Sizes whose step kernels load each thread's last-pass twiddles once and
+// share them across its transforms (a bit mask of log2 n; 0: every transform
+// loads them).  The 2048 row kernel stays at 4 waves per SIMD with them and
+// gains 1%; the 2048 column kernels (also 4 waves per SIMD with them) measured
+// no gain, and at 512 and 4096 the extra registers cost occupancy or spill.
+// The column inverse of a call's first step and the fused column kernel's
+// emitted transforms take the same setting, so they emit the same bits.
+#ifndef R2D_ROW_TWLAST
+#define R2D_ROW_TWLAST (1 << 11)
+#endif
+#ifndef R2D_COL_TWLAST
+#define R2D_COL_TWLAST 0
+#endif
+template <int N> constexpr bool row_twlast() { return ((R2D_ROW_TWLAST) >> ilog2(N)) & 1; }
+template <int N> constexpr bool col_twlast() { return ((R2D_COL_TWLAST) >> ilog2(N)) & 1; }
+
 // spectral operators applied on load (derivk2 / laplak2 / inv_laplak2)
 enum { OP_RAW = 0, OP_DX = 1, OP_DY = 2, OP_MASK = 3, OP_LAP = 4, OP_INVLAP = 5, OP_LDX = 6,
        OP_LDY = 7 };
@@ -217,7 +233,7 @@ __device__ __forceinline__ void unpack_store(const cplx *lds, int b, int ncols, 
     if (KC > 0 && T * q >= KC) continue;
     const int k = b + T * q;
     if ((KC > 0 && T * q + T - 1 < KC) || k < ncols) {
-      const cplx w = lds[pad(k) * IL], wm = lds[pad((N - k) & (N - 1)) * IL];
+      const cplx w = lds[pad<N>(k) * IL], wm = lds[pad<N>((N - k) & (N - 1)) * IL];
       const cplx A = mk(0.5 * (w.x + wm.x), 0.5 * (w.y - wm.y));
       const cplx B = mk(0.5 * (w.y + wm.y), -0.5 * (w.x - wm.x));
       *at_bytes(mixa, tix_bytes(ya, k, kp)) = A;
@@ -243,7 +259,7 @@ __global__ __launch_bounds__(N / 8 * RowFull<N>::P) void k_row_r2c(const double 
   }
   fft_canon<N, -1, P>(v, lds + r, b, tw);
 #pragma unroll
-  for (int q = 0; q < 8; ++q) lds[pad(b + T * q) * P + r] = v[q];
+  for (int q = 0; q < 8; ++q) lds[pad<N>(b + T * q) * P + r] = v[q];
   __syncthreads();
   unpack_store<N, P, N / 2 + 1>(lds + r, b, g.ncf, g.kpf, mix, y, mix, y + 1);
 }
@@ -798,6 +814,10 @@ __device__ __forceinline__ void colinv_body(const cplx *__restrict__ psi,
   }
   const int nf = field == 0 ? 4 : 2;
   cplx *dst0 = M + (size_t)(field == 0 ? 0 : 4) * c.fs;
+  // the same twiddles as the fused column kernel's emitted transforms, so the
+  // first step of a call and the later ones emit the same bits
+  const TwLast tls = tw_last<N>(tw, b);
+  const TwLast *tl = col_twlast<N>() ? &tls : nullptr;
   for (int f = 0; f < nf; ++f) {
     // opaque copy of the thread index: keeps the per-q index and address math
     // from being hoisted out of this loop (it would pin ~120 VGPRs and halve
@@ -817,7 +837,7 @@ __device__ __forceinline__ void colinv_body(const cplx *__restrict__ psi,
       if (f & 1) a = mk(mul_rn(-(double)ky, a.y), mul_rn((double)ky, a.x));
       v[q] = a;
     }
-    fft_dit<N, +1, 1, CS>(v, lds, bb, tw);
+    fft_dit<N, +1, 1, CS>(v, lds, bb, tw, tl);
     cplx *dst = dst0 + (size_t)f * c.fs;
 #pragma unroll
     for (int q = 0; q < 8; ++q) dst[tix(bb + T * q, kx, c.g.kp)] = v[q];
@@ -904,12 +924,57 @@ __device__ __forceinline__ void load_packed_step(cplx v[8], const cplx *__restri
   }
 }
 
+// Slots q of a packed row transform's canonical register set that are zero
+// for every thread (all their columns lie in the band past n/3).
+template <int N> constexpr int zero_row_slots() {
+  constexpr int T = N / 8, NC = N / 3 + 1;
+  int z = 0;
+  for (int q = 0; q < 8; ++q)
+    if (T * q >= NC && T * q + T - 1 <= N - NC) z |= 1 << q;
+  return z;
+}
+
+// Unpack of a packed forward row transform whose radix-4 last pass was run
+// mirrored (fft_canon MIR): thread b holds X[j_u + 2T m] in v[u + 2m] with
+// j_0 = b, j_1 = 2T - b (T for b = 0), so each output's mirror X[N - k] is in
+// its own registers.  Writes the half spectra of the two real rows,
+// A = (X[k] + conj X[N-k])/2 to QA and B = (X[k] - conj X[N-k])/(2i) to QB,
+// at the kept columns k < n/3 + 1 of row y: the arithmetic of unpack_store
+// without the trip through LDS.
+template <int N>
+__device__ __forceinline__ void mirror_store(const cplx v[8], int b, int kp,
+                                             cplx *__restrict__ QA, cplx *__restrict__ QB,
+                                             int y) {
+  constexpr int T = N / 8, H = 2 * T, NC = N / 3 + 1;
+  const bool b0 = b == 0;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int lo = (u ? T : 0) + H * m, hi = (u ? H - 1 : T - 1) + H * m;
+      if (lo >= NC) continue;  // wholly past the kept columns
+      const int k = (u ? (b0 ? T : H - b) : b) + H * m;
+      if (hi >= NC && k >= NC) continue;
+      const cplx w = v[u + 2 * m];
+      // X[N - k]: (1 - u, 3 - m), and for b = 0 (j_0 = 0, j_1 = T) the
+      // self-mirrored columns (0, -m mod 4) and (1, 3 - m)
+      const cplx wm = u ? (b0 ? v[1 + 2 * (3 - m)] : v[2 * (3 - m)])
+                        : (b0 ? v[2 * ((4 - m) & 3)] : v[1 + 2 * (3 - m)]);
+      const cplx A = mk(0.5 * (w.x + wm.x), 0.5 * (w.y - wm.y));
+      const cplx B = mk(0.5 * (w.y + wm.y), -0.5 * (w.x - wm.x));
+      *at_bytes(QA, tix_bytes(y, k, kp)) = A;
+      *at_bytes(QB, tix_bytes(y, k, kp)) = B;
+    }
+  }
+}
+
 // Rows y (this thread's, r = y % RW) of one run: three packed inverse
 // transforms, the brackets and the rotation terms, one forward transform,
 // unpacked into the bracket fields Q.  b: the thread's index in its group of
 // T*RW threads; lds: the group's RW * LDSN slots.  Every thread of the block
-// must call it together (block barriers).
-template <int N>
+// must call it together (block barriers).  LAST: nothing follows in the
+// kernel (the forward transform skips its closing barrier).
+template <int N, bool LAST = false>
 __device__ __forceinline__ void step_row_body(const cplx *__restrict__ M, cplx *__restrict__ Q,
                                               const StepConst &c, unsigned long long *maxbits,
                                               const cplx *__restrict__ tw, double two_omega_n2,
@@ -921,6 +986,9 @@ __device__ __forceinline__ void step_row_body(const cplx *__restrict__ M, cplx *
   // instead of ~245.
   const int nc = c.g.nc, kp = c.g.kp;
   const size_t fs = c.fs;
+  // the last-pass twiddles of this thread, shared by the four transforms
+  const TwLast tls = tw_last<N>(tw, b);
+  const TwLast *tl = row_twlast<N>() ? &tls : nullptr;
   cplx v[8];
   double px[8], py[8], j1[8];
   int b0 = b;
@@ -928,7 +996,8 @@ __device__ __forceinline__ void step_row_body(const cplx *__restrict__ M, cplx *
   load_packed_step<N>(v, M, M + fs, y, b0, nc, kp);
   int b1 = b;
   asm volatile("" : "+v"(b1));
-  fft_canon<N, +1, RW>(v, lds + r, b1, tw);
+  constexpr int ZQ = zero_row_slots<N>();
+  fft_canon<N, +1, RW, 1, ZQ>(v, lds + r, b1, tw, tl);
 #pragma unroll
   for (int q = 0; q < 8; ++q) { px[q] = v[q].x; py[q] = v[q].y; }
   if (maxbits) {
@@ -944,7 +1013,7 @@ __device__ __forceinline__ void step_row_body(const cplx *__restrict__ M, cplx *
   load_packed_step<N>(v, M + 2 * fs, M + 3 * fs, y, b2, nc, kp);
   int b3 = b;
   asm volatile("" : "+v"(b3));
-  fft_canon<N, +1, RW>(v, lds + r, b3, tw);
+  fft_canon<N, +1, RW, 1, ZQ>(v, lds + r, b3, tw, tl);
 #pragma unroll
   for (int q = 0; q < 8; ++q) j1[q] = px[q] * v[q].y - py[q] * v[q].x;
   int b4 = b;
@@ -952,7 +1021,7 @@ __device__ __forceinline__ void step_row_body(const cplx *__restrict__ M, cplx *
   load_packed_step<N>(v, M + 4 * fs, M + 5 * fs, y, b4, nc, kp);
   int b5 = b;
   asm volatile("" : "+v"(b5));
-  fft_canon<N, +1, RW>(v, lds + r, b5, tw);
+  fft_canon<N, +1, RW, 1, ZQ>(v, lds + r, b5, tw, tl);
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const double vx = v[q].x, vy = v[q].y;
@@ -962,13 +1031,21 @@ __device__ __forceinline__ void step_row_body(const cplx *__restrict__ M, cplx *
   }
   int b6 = b;
   asm volatile("" : "+v"(b6));
-  fft_canon<N, -1, RW>(v, lds + r, b6, tw);
+  if constexpr (Shape<N>::RL == 4) {
+    // mirrored last pass: the unpack needs no exchange through LDS
+    fft_canon<N, -1, RW, 1, 0, true, !LAST>(v, lds + r, b6, tw, tl);
+    int b7 = b;
+    asm volatile("" : "+v"(b7));
+    mirror_store<N>(v, b7, kp, Q, Q + fs, y);
+  } else {
+    fft_canon<N, -1, RW>(v, lds + r, b6, tw, tl);
 #pragma unroll
-  for (int q = 0; q < 8; ++q) lds[pad(b + T * q) * RW + r] = v[q];
-  __syncthreads();
-  int b7 = b;
-  asm volatile("" : "+v"(b7));
-  unpack_store<N, RW, N / 3 + 1>(lds + r, b7, nc, kp, Q, y, Q + fs, y);
+    for (int q = 0; q < 8; ++q) lds[pad<N>(b + T * q) * RW + r] = v[q];
+    __syncthreads();
+    int b7 = b;
+    asm volatile("" : "+v"(b7));
+    unpack_store<N, RW, N / 3 + 1>(lds + r, b7, nc, kp, Q, y, Q + fs, y);
+  }
 }
 
 // tick (ensembles): a step counter the launch advances (block 0, thread 0);
@@ -990,7 +1067,7 @@ __global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
   M += blockIdx.y * mstride;
   Q += blockIdx.y * mstride;
   const double two_omega_n2 = om_tab ? om_tab[blockIdx.y] : c.two_omega_n2;
-  step_row_body<N>(M, Q, c, maxbits, tw, two_omega_n2, y, b, r, lds);
+  step_row_body<N, true>(M, Q, c, maxbits, tw, two_omega_n2, y, b, r, lds);
 }
 
 
@@ -1028,12 +1105,12 @@ __global__ __launch_bounds__(3 * (N / 8) * RowStep<N>::ROWS) void k_step_row3(
   asm volatile("" : "+v"(b1));
   fft_canon<N, +1, RW>(v, P, b1, tw);
 #pragma unroll
-  for (int q = 0; q < 8; ++q) P[pad(b + T * q) * RW] = v[q];
+  for (int q = 0; q < 8; ++q) P[pad<N>(b + T * q) * RW] = v[q];
   __syncthreads();
   double m = 0.0;
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const int o = pad(b + T * q) * RW + r;
+    const int o = pad<N>(b + T * q) * RW + r;
     const cplx z1 = lds[o], z2 = lds[LN + o], z3 = lds[2 * LN + o];
     const double px = z1.x, py = z1.y, vx = z3.x, vy = z3.y;
     m = fmax(m, px * px + py * py);
@@ -1052,7 +1129,7 @@ __global__ __launch_bounds__(3 * (N / 8) * RowStep<N>::ROWS) void k_step_row3(
   asm volatile("" : "+v"(b6));
   fft_canon<N, -1, RW>(v, P, b6, tw);
 #pragma unroll
-  for (int q = 0; q < 8; ++q) P[pad(b + T * q) * RW] = v[q];
+  for (int q = 0; q < 8; ++q) P[pad<N>(b + T * q) * RW] = v[q];
   __syncthreads();
   if (g == 0) {
     int b7 = b;
@@ -1075,7 +1152,7 @@ static void launch_row_np(const r2d3c_plan *p, const cplx *M, cplx *Q, const Ste
 // tiles; the inputs of each packed transform (a field pair of RW rows) are
 // copied straight into LDS with global_load_lds_dwordx4 while the previous
 // transform runs, so the block never waits on HBM between its four transforms.
-// LDS: RW*(n + n/16) complex exchange slots + one staging buffer of 2*nc*RW.
+// LDS: RW*LDSN complex exchange slots + one staging buffer of 2*nc*RW.
 // RWQ (rows per block, 1 or 2) and the grid size are schedule choices: they
 // change neither the arithmetic nor its order, only the thread and LDS
 // layout.  The grid size leaves the compiled kernel unchanged, so it gives
@@ -1215,7 +1292,7 @@ __global__ __launch_bounds__((RowV2<N, RWQ>::THREADS)) void k_step_row2(
     }
     fft_canon<N, -1, RW>(v, P, b, tw);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) P[pad(b + T * q) * RW] = v[q];
+    for (int q = 0; q < 8; ++q) P[pad<N>(b + T * q) * RW] = v[q];
     __syncthreads();
     unpack_store<N, RW, N / 3 + 1>(P, b, nc, kp, Q, y, Q + fs, y);
     if (tn >= NTILES) break;
@@ -1285,11 +1362,14 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
     if (mreset) *mreset = 0ull;
   }
 
+  // the last-pass twiddles of this thread, shared by the column's transforms
+  const TwLast tls = tw_last<N>(tw, b);
+  const TwLast *tl = col_twlast<N>() ? &tls : nullptr;
   cplx v[8];
   const cplx *Qf = Q + (size_t)field * c.fs;
 #pragma unroll
   for (int q = 0; q < 8; ++q) v[q] = Qf[tix(b + T * q, kx, c.g.kp)];
-  fft_dif<N, -1, 1, CS>(v, lds, b, tw);  // v[q] = spectrum at ky index pos(b, q)
+  fft_dif<N, -1, 1, CS>(v, lds, b, tw, tl);  // v[q] = spectrum at ky index pos(b, q)
 
   const ForceSet &fo = field == 0 ? c.fp : c.fz;
   // at most two forced modes per field (scalars: no private-memory array)
@@ -1340,7 +1420,7 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
   if (mode & FM_STATE) {
     // interleaved -> canonical through LDS, then coalesced stores
 #pragma unroll
-    for (int q = 0; q < 8; ++q) lds[pad(pos(b, q))] = s[q];
+    for (int q = 0; q < 8; ++q) lds[pad<N>(pos(b, q))] = s[q];
     __syncthreads();
     const bool disk = (mode & FM_DISK) != 0;
 #pragma unroll
@@ -1348,7 +1428,7 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
       const int j = b + T * q, ky = kyof(j, N);
       const double ka2 = (double)kx * kx + (double)ky * ky;
       if (act && (!disk || (ka2 <= c.g.kmax && ka2 > c.g.tiny)))
-        state[(size_t)kx * N + j] = lds[pad(j)];
+        state[(size_t)kx * N + j] = lds[pad<N>(j)];
     }
     __syncthreads();
   }
@@ -1371,7 +1451,7 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
         if (f & 1) a = mk(mul_rn(-(double)ky, a.y), mul_rn((double)ky, a.x));
         w[q] = a;
       }
-      fft_dit<N, +1, 1, CS>(w, lds, bb, tw);
+      fft_dit<N, +1, 1, CS>(w, lds, bb, tw, tl);
       cplx *dst = dst0 + (size_t)f * c.fs;
       if (act) {
 #pragma unroll
@@ -1450,7 +1530,7 @@ constexpr int kPsMaxClasses = 5;           // sizes 64..1024
 // the block's LDS, passed to the unit bodies as an LDS-space pointer
 typedef __attribute__((address_space(3))) cplx *PsLds;
 constexpr unsigned kPsSpin = 1u << 22;     // barrier polls before the time-out
-constexpr int kPsLds = 2176;  // complex slots: G * 2 * (n + n/16) rows, G * (n + n/16) columns
+constexpr int kPsLds = 2 * Shape<1024>::LDSN;  // complex slots: G * 2 * LDSN rows, G * LDSN columns
 struct PsClass {
   EnsArgs a;            // the class's runs, forcing table and step
   const cplx *tw;       // twiddles of size n
@@ -3581,13 +3661,13 @@ __global__ __launch_bounds__(CiRow<N>::G * (N / 8)) void k_ci_row(
     fft_canon<N, +1, 1>(a, P, b, tw);
     if (grp == 2) {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) P[pad(b + T * q)] = a[q];
+      for (int q = 0; q < 8; ++q) P[pad<N>(b + T * q)] = a[q];
     }
     __syncthreads();
     if (grp < 2) {
       cplx v[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) v[q] = lds[2 * LN + pad(b + T * q)];
+      for (int q = 0; q < 8; ++q) v[q] = lds[2 * LN + pad<N>(b + T * q)];
       emit(grp, a, v);
     }
   } else {
@@ -3630,7 +3710,7 @@ __global__ __launch_bounds__(CfRow<N>::G * (N / 8)) void k_cf_row(
     }
     fft_canon<N, -1, 1>(v, P, b, tw);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) P[pad(b + T * q)] = v[q];
+    for (int q = 0; q < 8; ++q) P[pad<N>(b + T * q)] = v[q];
     __syncthreads();
     unpack_store<N, 1, N / 2 + 1>(P, b, g.ncf, g.kpf, mix + ch * fs, y, mix + ch * fs, y + 1);
     __syncthreads();
