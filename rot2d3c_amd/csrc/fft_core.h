@@ -20,6 +20,11 @@
 
 namespace r2d {
 
+// Sizes (bit mask of log2 N) whose radix-8 transforms run their middle
+// passes in place (see fft_canon).
+#ifndef R2D_INPLACE
+#define R2D_INPLACE (1 << 11)
+#endif
 // Grids from this size up pad their LDS arrays with one slot per 8 entries,
 // smaller ones with one per 16 (see pad below).  Off by default: at 2048 the
 // row kernel measured 1.5 us slower with it and the column kernels gained
@@ -57,7 +62,9 @@ template <int N> struct Shape {
   static constexpr int NMID = (LOG - 4) / 3;               // middle radix-8 passes
   static constexpr int RL = 1 << (LOG - 3 - 3 * NMID);     // last radix: 2, 4 or 8
   static constexpr int PS = N >= R2D_PAD8_MIN_N ? 3 : 4;  // one pad slot per 2^PS
-  static constexpr int LDSN = N + (N >> PS);                // padded LDS slots
+  // in-place passes (fft_canon): sizes in the bit mask R2D_INPLACE of log2 N
+  static constexpr bool IP = ((R2D_INPLACE) >> LOG) & 1;
+  static constexpr int LDSN = N + (N >> PS) + (IP ? RL : 0);  // padded LDS slots
   static constexpr int WR = 512 + (512 >> PS);              // one wave's four-step region
 };
 
@@ -71,6 +78,32 @@ template <int N> struct Shape {
 // small-grid kernels resident, and the transforms here are not bound by LDS
 // bandwidth, so it is the default.
 template <int N> __device__ __forceinline__ int pad(int i) { return i + (i >> Shape<N>::PS); }
+
+// In-place form of the radix-8 passes (Shape<N>::IP): the same butterflies
+// with the same twiddles as the Stockham passes below, so the same results
+// bit for bit, but each middle pass writes its outputs back to the LDS slots
+// it read, so no barrier is needed between its reads and its writes.  The
+// first pass stores thread b's outputs m at position m + 8 rho(b), where
+// rho(b) reverses the digits of b (the input index b + T m has digits
+// m_1 = m of weight T, then m_2 .. m_{NMID+1} of weights N/8^s and a last
+// digit of radix RL and weight 1); middle pass s (span L) works on positions
+// k 8L + j + mL (thread b: k = b / L, j = b % L) and the last pass on
+// j + m N/RL.  Position P lies in slot pad(P) + P / (N/RL): the skew makes
+// the first pass's writes (N/RL apart across lanes) and every other pass's
+// accesses free of bank conflicts.
+template <int N> __device__ __forceinline__ int ip_rho(int b) {
+  using F = Shape<N>;
+  int rho = 0, f = 1;
+#pragma unroll
+  for (int s = 2; s <= F::NMID + 1; ++s) {
+    rho += ((b >> (F::LOG - 3 * s)) & 7) * f;
+    f *= 8;
+  }
+  return rho + (b & (F::RL - 1)) * f;
+}
+template <int N> __device__ __forceinline__ int ip_slot(int P) {
+  return pad<N>(P) + P / (N / Shape<N>::RL);
+}
 
 template <int S> __device__ __forceinline__ cplx twid(const cplx *__restrict__ tw, int t) {
   cplx w = tw[t];
@@ -190,10 +223,14 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
   constexpr bool LIN = T % PQ == 0 && (N / RL0) % PQ == 0 &&
                        16 * IL * 7 * (T + (T >> PS)) <= 65535 &&
                        16 * IL * (RL0 - 1) * (N / RL0 + ((N / RL0) >> PS)) <= 65535;
+  constexpr bool IP = F::IP;
+  static_assert(!IP || (LIN && 16 * IL * (RL0 - 1) * (N / RL0 + ((N / RL0) >> PS) + 1) <= 65535),
+                "in-place passes take the linear slot offsets");
   // pass 1: radix 8, span 1; inputs b + m*T are exactly v[m]
   dft8<S, ZQ>(v);
   {
-    cplx *w = lds + pad<N>(8 * b) * IL;  // pad(8b + m) = pad(8b) + m
+    // Stockham: pad(8b + m) = pad(8b) + m; in place: slot(8 rho + m) = slot(8 rho) + m
+    cplx *w = lds + (IP ? ip_slot<N>(8 * ip_rho<N>(b)) : pad<N>(8 * b)) * IL;
 #pragma unroll
     for (int m = 0; m < 8; ++m) w[m * IL] = v[m];
   }
@@ -207,7 +244,13 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
     // the LDS reads and the barrier
     const cplx w1 = twid<S>(tw, TS * (st)), w2 = twid<S>(tw, TS * (2 * st)), w4 = twid<S>(tw, TS * (4 * st));
     cplx x[8];
-    if constexpr (LIN) {
+    // in place: the butterfly's positions k 8L + j + mL, read and written back
+    cplx *const ipw = lds + ip_slot<N>((b - j) * 8 + j) * IL;
+    const int ipo = L >= PQ ? L + (L >> PS) : 0;  // slot step per m (L = 8: 8m + m/2)
+    if constexpr (IP) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) x[m] = ipw[(ipo ? m * ipo : 8 * m + ((8 * m) >> PS)) * IL];
+    } else if constexpr (LIN) {
       const cplx *r = lds + pad<N>(b) * IL;
 #pragma unroll
       for (int m = 0; m < 8; ++m) x[m] = r[m * (T + (T >> PS)) * IL];
@@ -215,7 +258,7 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
 #pragma unroll
       for (int m = 0; m < 8; ++m) x[m] = lds[pad<N>(b + m * T) * IL];
     }
-    __syncthreads();
+    if constexpr (!IP) __syncthreads();
     {  // three table loads; the other powers by products (error <= 2 ulp more)
       const cplx w3 = cmul(w1, w2);
       x[1] = cmul(x[1], w1); x[2] = cmul(x[2], w2); x[3] = cmul(x[3], w3);
@@ -224,7 +267,10 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
     }
     dft8<S>(x);
     const int base = (b - j) * 8 + j;
-    if constexpr (LIN) {
+    if constexpr (IP) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) ipw[(ipo ? m * ipo : 8 * m + ((8 * m) >> PS)) * IL] = x[m];
+    } else if constexpr (LIN) {
       cplx *w = lds + pad<N>(base) * IL;
 #pragma unroll
       for (int m = 0; m < 8; ++m)
@@ -262,7 +308,11 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
       w4 = RL == 8 ? twid<S>(tw, TS * (4 * j)) : w1;
     }
     cplx x[RL];
-    if constexpr (LIN) {
+    if constexpr (IP) {  // slot(j + m N/RL) = slot(j) + m (N/RL + N/RL/Q + 1)
+      const cplx *r = lds + ip_slot<N>(j) * IL;
+#pragma unroll
+      for (int m = 0; m < RL; ++m) x[m] = r[m * (N / RL + ((N / RL) >> PS) + 1) * IL];
+    } else if constexpr (LIN) {
       const cplx *r = lds + pad<N>(j) * IL;
 #pragma unroll
       for (int m = 0; m < RL; ++m) x[m] = r[m * (N / RL + ((N / RL) >> PS)) * IL];
