@@ -23,7 +23,7 @@ namespace r2d {
 // Sizes (bit mask of log2 N) whose radix-8 transforms run their middle
 // passes in place (see fft_canon).
 #ifndef R2D_INPLACE
-#define R2D_INPLACE (1 << 11)
+#define R2D_INPLACE ((1 << 11) | (1 << 13))
 #endif
 // Grids from this size up pad their LDS arrays with one slot per 8 entries,
 // smaller ones with one per 16 (see pad below).  Off by default: at 2048 the
@@ -64,7 +64,8 @@ template <int N> struct Shape {
   static constexpr int PS = N >= R2D_PAD8_MIN_N ? 3 : 4;  // one pad slot per 2^PS
   // in-place passes (fft_canon): sizes in the bit mask R2D_INPLACE of log2 N
   static constexpr bool IP = ((R2D_INPLACE) >> LOG) & 1;
-  static constexpr int LDSN = N + (N >> PS) + (IP ? RL : 0);  // padded LDS slots
+  static constexpr int SK2 = RL == 2 ? 2 : 0;  // in-place slot skew per N/16 positions
+  static constexpr int LDSN = N + (N >> PS) + (IP ? RL + 15 * SK2 : 0);  // padded LDS slots
   static constexpr int WR = 512 + (512 >> PS);              // one wave's four-step region
 };
 
@@ -88,9 +89,10 @@ template <int N> __device__ __forceinline__ int pad(int i) { return i + (i >> Sh
 // m_1 = m of weight T, then m_2 .. m_{NMID+1} of weights N/8^s and a last
 // digit of radix RL and weight 1); middle pass s (span L) works on positions
 // k 8L + j + mL (thread b: k = b / L, j = b % L) and the last pass on
-// j + m N/RL.  Position P lies in slot pad(P) + P / (N/RL): the skew makes
-// the first pass's writes (N/RL apart across lanes) and every other pass's
-// accesses free of bank conflicts.
+// j + m N/RL.  Position P lies in slot pad(P) + P / (N/RL), plus 2 P/(N/16)
+// when RL = 2: the skew makes the first pass's writes (whose positions are
+// multiples of N/16 apart across lanes) and every other pass's accesses free
+// of bank conflicts.
 template <int N> __device__ __forceinline__ int ip_rho(int b) {
   using F = Shape<N>;
   int rho = 0, f = 1;
@@ -102,7 +104,13 @@ template <int N> __device__ __forceinline__ int ip_rho(int b) {
   return rho + (b & (F::RL - 1)) * f;
 }
 template <int N> __device__ __forceinline__ int ip_slot(int P) {
-  return pad<N>(P) + P / (N / Shape<N>::RL);
+  return pad<N>(P) + P / (N / Shape<N>::RL) + Shape<N>::SK2 * (P / (N / 16));
+}
+// slot(P0 + m L) - slot(P0) for a middle pass of span L (P0 = k 8L + j)
+template <int N> constexpr int ip_step(int L, int m) {
+  constexpr int PS = Shape<N>::PS;
+  return (L >= (1 << PS) ? m * (L + (L >> PS)) : 8 * m + ((8 * m) >> PS)) +
+         (L == N / 16 ? Shape<N>::SK2 * m : 0);
 }
 
 template <int S> __device__ __forceinline__ cplx twid(const cplx *__restrict__ tw, int t) {
@@ -224,8 +232,6 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
                        16 * IL * 7 * (T + (T >> PS)) <= 65535 &&
                        16 * IL * (RL0 - 1) * (N / RL0 + ((N / RL0) >> PS)) <= 65535;
   constexpr bool IP = F::IP;
-  static_assert(!IP || (LIN && 16 * IL * (RL0 - 1) * (N / RL0 + ((N / RL0) >> PS) + 1) <= 65535),
-                "in-place passes take the linear slot offsets");
   // pass 1: radix 8, span 1; inputs b + m*T are exactly v[m]
   dft8<S, ZQ>(v);
   {
@@ -246,10 +252,9 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
     cplx x[8];
     // in place: the butterfly's positions k 8L + j + mL, read and written back
     cplx *const ipw = lds + ip_slot<N>((b - j) * 8 + j) * IL;
-    const int ipo = L >= PQ ? L + (L >> PS) : 0;  // slot step per m (L = 8: 8m + m/2)
     if constexpr (IP) {
 #pragma unroll
-      for (int m = 0; m < 8; ++m) x[m] = ipw[(ipo ? m * ipo : 8 * m + ((8 * m) >> PS)) * IL];
+      for (int m = 0; m < 8; ++m) x[m] = ipw[ip_step<N>(L, m) * IL];
     } else if constexpr (LIN) {
       const cplx *r = lds + pad<N>(b) * IL;
 #pragma unroll
@@ -269,7 +274,7 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
     const int base = (b - j) * 8 + j;
     if constexpr (IP) {
 #pragma unroll
-      for (int m = 0; m < 8; ++m) ipw[(ipo ? m * ipo : 8 * m + ((8 * m) >> PS)) * IL] = x[m];
+      for (int m = 0; m < 8; ++m) ipw[ip_step<N>(L, m) * IL] = x[m];
     } else if constexpr (LIN) {
       cplx *w = lds + pad<N>(base) * IL;
 #pragma unroll
@@ -308,10 +313,11 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
       w4 = RL == 8 ? twid<S>(tw, TS * (4 * j)) : w1;
     }
     cplx x[RL];
-    if constexpr (IP) {  // slot(j + m N/RL) = slot(j) + m (N/RL + N/RL/Q + 1)
+    if constexpr (IP) {  // slot(j + m N/RL) = slot(j) + m (N/RL + N/RL/Q + 1 + 8 SK2)
       const cplx *r = lds + ip_slot<N>(j) * IL;
 #pragma unroll
-      for (int m = 0; m < RL; ++m) x[m] = r[m * (N / RL + ((N / RL) >> PS) + 1) * IL];
+      for (int m = 0; m < RL; ++m)
+        x[m] = r[m * (N / RL + ((N / RL) >> PS) + 1 + 8 * F::SK2) * IL];
     } else if constexpr (LIN) {
       const cplx *r = lds + pad<N>(j) * IL;
 #pragma unroll

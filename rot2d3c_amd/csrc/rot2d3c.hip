@@ -1530,7 +1530,15 @@ constexpr int kPsMaxClasses = 5;           // sizes 64..1024
 // the block's LDS, passed to the unit bodies as an LDS-space pointer
 typedef __attribute__((address_space(3))) cplx *PsLds;
 constexpr unsigned kPsSpin = 1u << 22;     // barrier polls before the time-out
-constexpr int kPsLds = 2 * Shape<1024>::LDSN;  // complex slots: G * 2 * LDSN rows, G * LDSN columns
+// complex slots: the largest of G * 2 * LDSN (row units) and G * LDSN (column
+// units) over the sizes 64..1024
+template <int N> constexpr int ps_lds() {
+  constexpr int row = kPsT / ((N / 8) * 2) * 2 * Shape<N>::LDSN, col = kPsT / (N / 8) * Shape<N>::LDSN;
+  return row > col ? row : col;
+}
+constexpr int ps_lds_max(int a, int b) { return a > b ? a : b; }
+constexpr int kPsLds = ps_lds_max(ps_lds_max(ps_lds<64>(), ps_lds<128>()),
+                                  ps_lds_max(ps_lds_max(ps_lds<256>(), ps_lds<512>()), ps_lds<1024>()));
 struct PsClass {
   EnsArgs a;            // the class's runs, forcing table and step
   const cplx *tw;       // twiddles of size n
@@ -3843,19 +3851,40 @@ int r2d3c_closure_clip(r2d3c_plan *p, const double *psi, const double *vz, doubl
 
 // In: ev[q] = x[2(b + 1024 q)], od[q] = x[2(b + 1024 q) + 1].
 // Out: ev[q] = X[b + 1024 q], od[q] = X[b + 1024 q + 8192] (sign S).
+// With R2D_BIG_TWLAST the thread loads its twiddles once: w^(2b) for both
+// halves' last passes (tw_last) and w^b, w^(b + 1024) for the combine, whose
+// other factors w^(b + 1024 q) are these times exp(S i pi u / 4), q = 2u or
+// 2u + 1 (16 table loads fewer per transform).
+#ifndef R2D_BIG_TWLAST
+#define R2D_BIG_TWLAST 1
+#endif
 template <int S>
 __device__ __forceinline__ void fft16k(cplx ev[8], cplx od[8], cplx *lds, int b,
                                        const cplx *__restrict__ tw) {
+#if R2D_BIG_TWLAST
+  const TwLast tls = tw_last<kBigH, 2>(tw, b);
+  const TwLast *tl = &tls;
+#else
+  const TwLast *tl = nullptr;
+#endif
   // opaque copies of the thread index keep each half's index math from being
   // hoisted over the other half (at 1024 threads it would spill)
   int b1 = b, b2 = b;
   asm volatile("" : "+v"(b1));
-  fft_canon<kBigH, S, 1, 2>(ev, lds, b1, tw);
+  fft_canon<kBigH, S, 1, 2>(ev, lds, b1, tw, tl);
   asm volatile("" : "+v"(b2));
-  fft_canon<kBigH, S, 1, 2>(od, lds, b2, tw);
+  fft_canon<kBigH, S, 1, 2>(od, lds, b2, tw, tl);
+#if R2D_BIG_TWLAST
+  const cplx c0 = twid<S>(tw, b), c1 = twid<S>(tw, b + kBigT);
+#endif
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const cplx t = cmul(od[q], twid<S>(tw, b + kBigT * q)), e = ev[q];
+#if R2D_BIG_TWLAST
+    const cplx w = rot8<S>((q & 1) ? c1 : c0, q >> 1);
+#else
+    const cplx w = twid<S>(tw, b + kBigT * q);
+#endif
+    const cplx t = cmul(od[q], w), e = ev[q];
     ev[q] = cadd(e, t);
     od[q] = csub(e, t);
   }
@@ -4064,17 +4093,31 @@ constexpr int kBigNC = kBigN / 3 + 1;  // kept columns kx < nc
 template <int S>
 __device__ __forceinline__ void fft16k_dif(cplx ev[8], cplx od[8], cplx *lds, int b,
                                            const cplx *__restrict__ tw) {
+#if R2D_BIG_TWLAST
+  const cplx c0 = twid<S>(tw, b), c1 = twid<S>(tw, b + kBigT);
+#endif
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const cplx e = ev[q], o = od[q];
+#if R2D_BIG_TWLAST
+    const cplx w = rot8<S>((q & 1) ? c1 : c0, q >> 1);
+#else
+    const cplx w = twid<S>(tw, b + kBigT * q);
+#endif
     ev[q] = cadd(e, o);
-    od[q] = cmul(csub(e, o), twid<S>(tw, b + kBigT * q));
+    od[q] = cmul(csub(e, o), w);
   }
+#if R2D_BIG_TWLAST
+  const TwLast tls = tw_last<kBigH, 2>(tw, b);
+  const TwLast *tl = &tls;
+#else
+  const TwLast *tl = nullptr;
+#endif
   int b1 = b, b2 = b;
   asm volatile("" : "+v"(b1));
-  fft_canon<kBigH, S, 1, 2>(ev, lds, b1, tw);
+  fft_canon<kBigH, S, 1, 2>(ev, lds, b1, tw, tl);
   asm volatile("" : "+v"(b2));
-  fft_canon<kBigH, S, 1, 2>(od, lds, b2, tw);
+  fft_canon<kBigH, S, 1, 2>(od, lds, b2, tw, tl);
 }
 
 // psi_x + i psi_y along row y from the kept columns of its two column
