@@ -3844,9 +3844,11 @@ int r2d3c_closure_clip(r2d3c_plan *p, const double *psi, const double *vz, doubl
 // n = 16384.  A length-16384 transform exceeds one block's LDS and threads, so
 // it is done as two 8192-point halves (the even and the odd samples) in one
 // block of 1024 threads, 16 points per thread, then a radix-2 combine in
-// registers.  The fused step's row kernel would need more registers than
-// 1024 threads have, so the step runs stage by stage from full 2D transforms
-// through HBM (poisson by three real fields), like the iflow = 2 step.
+// registers.  The 2048 step's row kernel (all six derivative rows transformed
+// in one kernel, three products held in registers) would need more registers
+// than 1024 threads have, so four derivative fields go through HBM as real
+// fields and the bracket pass (k_big_row_jac2) transforms psi_x + i psi_y in
+// registers and reads them.
 // ---------------------------------------------------------------------------
 
 // In: ev[q] = x[2(b + 1024 q)], od[q] = x[2(b + 1024 q) + 1].
