@@ -94,6 +94,12 @@ __host__ __forceinline__ int round16(int x) { return (x + 15) & ~15; }
 template <int N> constexpr bool row_twlast() { return ((R2D_ROW_TWLAST) >> ilog2(N)) & 1; }
 template <int N> constexpr bool col_twlast() { return ((R2D_COL_TWLAST) >> ilog2(N)) & 1; }
 
+// Ensemble column kernels below n = 512 pack 2 or 4 columns per block so that
+// a block fills a wavefront (1: on, 0: one column per block).
+#ifndef R2D_ENS_COL_GROUPS
+#define R2D_ENS_COL_GROUPS 1
+#endif
+
 // spectral operators applied on load (derivk2 / laplak2 / inv_laplak2)
 enum { OP_RAW = 0, OP_DX = 1, OP_DY = 2, OP_MASK = 3, OP_LAP = 4, OP_INVLAP = 5, OP_LDX = 6,
        OP_LDY = 7 };
@@ -1511,6 +1517,37 @@ __global__ __launch_bounds__(N / 8) void k_ens_colfused(EnsArgs a, int o, int mo
   __shared__ cplx lds[Shape<N>::LDSN];
   colfused_body<N, CS>(e.M + 6 * c.fs, e.psi, e.vz, e.M, c, o, mode, nullptr, nullptr, nullptr, tw,
                        xcd_pairs(blockIdx.x), threadIdx.x, lds, blockIdx.z, gridDim.z, true);
+}
+// The same for n < 512, whose n/8-thread column groups fill less than a
+// wavefront: GC neighbouring columns per block (GC n/8 <= 64 threads), group g
+// = threadIdx.x / (n/8) taking logical column blockIdx.x GC + g.  GC divides
+// ncp = nc rounded up to even, so a block's groups are all psi columns, all vz
+// columns or all zeroed columns and follow one barrier sequence; a group past
+// nc in its range runs its field's last column without storing (act false).
+// Each column's arithmetic is k_ens_colfused's, so the results are the same
+// bits.
+template <int N, bool CS, int GC>
+__global__ __launch_bounds__(GC * (N / 8)) void k_ens_colfused_g(EnsArgs a, int o, int mode,
+                                                                 const cplx *__restrict__ tw) {
+  constexpr int T = N / 8;
+  static_assert(GC * T <= 64, "one wavefront per block");
+  const EnsMember &e = a.mem[blockIdx.y];
+  const StepConst c = ens_const(a, blockIdx.y);
+  __shared__ cplx lds[GC * Shape<N>::LDSN];
+  const int g = threadIdx.x / T, b = threadIdx.x - g * T;
+  const int nc = c.g.nc, ncp = (nc + 1) & ~1;
+  const int lg = xcd_pairs(blockIdx.x) * GC + g;
+  int lb = lg;
+  bool act = true;
+  if (lg < ncp) {
+    act = lg < nc;
+    lb = act ? lg : nc - 1;
+  } else if (lg < 2 * ncp) {
+    act = lg - ncp < nc;
+    lb = act ? lg : ncp + nc - 1;
+  }
+  colfused_body<N, CS>(e.M + 6 * c.fs, e.psi, e.vz, e.M, c, o, mode, nullptr, nullptr, nullptr, tw,
+                       lb, b, lds + g * Shape<N>::LDSN, blockIdx.z, gridDim.z, act);
 }
 // ---------------------------------------------------------------------------
 // ensembles: one persistent launch per RK4 stage for every size class
@@ -2963,6 +3000,20 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
     if (o == 1) mode = FM_STATE | (masked ? FM_DISK : FM_ZERO) | (chain ? FM_EMIT : 0);
     const int nblk = round16(2 * ncp + ((mode & FM_ZERO) ? (N / 2 + 1 - nc) : 0));
     const int nz = (mode & FM_STATE) ? 1 : col_slices(p, (long long)nblk * k.count * (N / 8));
+    // columns per block: fill a wavefront below n = 512 (GC divides ncp and nblk)
+    const int gc = (N / 8 >= 64 || !R2D_ENS_COL_GROUPS) ? 1 : (64 / (N / 8) >= 4 && ncp % 4 == 0 ? 4 : 2);
+    if constexpr (N >= 32 && 4 * (N / 8) <= 64) {
+      if (gc == 4) {
+        k_ens_colfused_g<N, false, 4><<<dim3(nblk / 4, k.count, nz), 4 * (N / 8), 0, s>>>(a, o, mode, tw);
+        continue;
+      }
+    }
+    if constexpr (N >= 32 && 2 * (N / 8) <= 64) {
+      if (gc == 2) {
+        k_ens_colfused_g<N, false, 2><<<dim3(nblk / 2, k.count, nz), 2 * (N / 8), 0, s>>>(a, o, mode, tw);
+        continue;
+      }
+    }
     if (cs) k_ens_colfused<N, true><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(a, o, mode, tw);
     else k_ens_colfused<N, false><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(a, o, mode, tw);
   }
