@@ -94,6 +94,20 @@ __host__ __forceinline__ int round16(int x) { return (x + 15) & ~15; }
 template <int N> constexpr bool row_twlast() { return ((R2D_ROW_TWLAST) >> ilog2(N)) & 1; }
 template <int N> constexpr bool col_twlast() { return ((R2D_COL_TWLAST) >> ilog2(N)) & 1; }
 
+// Latency-bound fused column launches (the emission split over slices) of
+// grids up to this size load the state and the dense forcing before the
+// column transform (colfused_body EARLY); 0: never.
+#ifndef R2D_COL_EARLY_MAXN
+#define R2D_COL_EARLY_MAXN 512
+#endif
+
+// Latency-bound ensemble column launches (emission split over 4 slices): the
+// slices as thread groups of one block in the last stage (1) or in every
+// stage (2); 0: the last stage unsplit and the others' slices as blocks.
+#ifndef R2D_ENS_GSLICES
+#define R2D_ENS_GSLICES 1
+#endif
+
 // Ensemble column kernels below n = 512 pack 2 or 4 columns per block so that
 // a block fills a wavefront (1: on, 0: one column per block).
 #ifndef R2D_ENS_COL_GROUPS
@@ -1337,7 +1351,14 @@ enum { FM_EMIT = 1, FM_STATE = 2, FM_DISK = 4, FM_ZERO = 8 };
 // With act false the group runs every transform and barrier on its (valid)
 // column but stores nothing: a group of a block that has no column of its
 // own keeps the block's barrier sequence.
-template <int N, bool CS = false>
+// Emission slices (nslices > 1): slice s emits field s only.  As separate
+// blocks (GS false), slices past the column's field count return at once and
+// the state is never written (every slice reads it); as thread groups of one
+// block (GS true), every slice runs the same transforms and barriers (slices
+// past the field count redo field s mod count without storing it), and slice
+// 0 writes the state once the block barrier after the update has ordered it
+// after every group's reads.
+template <int N, bool CS = false, bool GS = false, bool EARLY = false>
 __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *psi, cplx *vz,
                                               cplx *__restrict__ M, const StepConst &c, int o,
                                               int mode, const unsigned long long *mread,
@@ -1360,7 +1381,7 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
     }
     return;
   } else return;
-  if (nslices > 1 && slice >= (field == 0 ? 4 : 2)) return;  // no field left to emit
+  if (!GS && nslices > 1 && slice >= (field == 0 ? 4 : 2)) return;  // no field left to emit
 
   const double dt = c.cfl_mode ? dt_from_max(c.dc, *mread) : c.dt_fixed;
   if (lb == 0 && b == 0 && o == 4 && slice == 0) {
@@ -1375,6 +1396,23 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
   const cplx *Qf = Q + (size_t)field * c.fs;
 #pragma unroll
   for (int q = 0; q < 8; ++q) v[q] = Qf[tix(b + T * q, kx, c.g.kp)];
+  cplx *state = field == 0 ? psi : vz;
+  const cplx *fd = field == 0 ? c.fpd : c.fzd;
+  // EARLY (latency-bound launches): the state and dense-forcing loads of the
+  // update are issued with the column's loads, before the transform, so the
+  // chain has one round trip to memory instead of two.  Otherwise they follow
+  // it (about 50 fewer live registers: more waves per SIMD).
+  cplx s0e[EARLY ? 8 : 1], fde[EARLY ? 8 : 1];
+  if constexpr (EARLY) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int j = pos(b, q), ky = kyof(j, N);
+      const double ka2 = (double)kx * kx + (double)ky * ky;
+      const bool in = ka2 <= c.g.kmax && ka2 > c.g.tiny;
+      s0e[q] = in ? state[(size_t)kx * N + j] : mk(0.0, 0.0);
+      fde[q] = (in && fd) ? fd[(size_t)kx * N + j] : mk(0.0, 0.0);
+    }
+  }
   fft_dif<N, -1, 1, CS>(v, lds, b, tw, tl);  // v[q] = spectrum at ky index pos(b, q)
 
   const ForceSet &fo = field == 0 ? c.fp : c.fz;
@@ -1384,13 +1422,11 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
   const cplx fv1 = f1 ? force_value(fo, 1, dt, c.n2) : mk(0.0, 0.0);
   const int fj0 = f0 ? fo.e[0].j : -1, fj1 = f1 ? fo.e[1].j : -1;
 
-  cplx *state = field == 0 ? psi : vz;
   const double tmp1 = dt / (double)o;
   const double nu = field == 0 ? c.nu : c.nuv, hnu = field == 0 ? c.hnu : c.hnuv;
   const int nn = field == 0 ? c.nn : c.nnv, mm = field == 0 ? c.mm : c.mmv;
   // the state on the disc, in two groups of four slots: each group's loads
   // are issued before its arithmetic, so they are in flight together
-  const cplx *fd = field == 0 ? c.fpd : c.fzd;
   cplx s[8];
 #pragma unroll
   for (int g = 0; g < 8; g += 4) {
@@ -1400,7 +1436,8 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
       const int j = pos(b, g + u), ky = kyof(j, N);
       const double ka2 = (double)kx * kx + (double)ky * ky;
       s0v[u] = mk(0.0, 0.0);
-      if (ka2 <= c.g.kmax && ka2 > c.g.tiny) s0v[u] = state[(size_t)kx * N + j];
+      if constexpr (EARLY) s0v[u] = s0e[g + u];
+      else if (ka2 <= c.g.kmax && ka2 > c.g.tiny) s0v[u] = state[(size_t)kx * N + j];
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -1412,7 +1449,7 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
         const double tmp2 = 1.0 / ka2;
         const double tmp3 = 1.0 + (nu * ipow(ka2, nn) + hnu * ipow(tmp2, mm)) * tmp1;
         cplx f = j == fj0 ? fv0 : (j == fj1 ? fv1 : mk(0.0, 0.0));
-        if (fd) f = cadd(f, fd[(size_t)kx * N + j]);
+        if (fd) f = cadd(f, EARLY ? fde[q] : fd[(size_t)kx * N + j]);
         const cplx s0 = s0v[u];
         cplx t = field == 0 ? cscale(v[q], tmp2) : v[q];
         t = cadd(t, f);
@@ -1433,7 +1470,7 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
     for (int q = 0; q < 8; ++q) {
       const int j = b + T * q, ky = kyof(j, N);
       const double ka2 = (double)kx * kx + (double)ky * ky;
-      if (act && (!disk || (ka2 <= c.g.kmax && ka2 > c.g.tiny)))
+      if (act && slice == 0 && (!disk || (ka2 <= c.g.kmax && ka2 > c.g.tiny)))
         state[(size_t)kx * N + j] = lds[pad<N>(j)];
     }
     __syncthreads();
@@ -1443,7 +1480,7 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
     const int nf = field == 0 ? 4 : 2;
     cplx *dst0 = M + (size_t)(field == 0 ? 0 : 4) * c.fs;
     for (int f = 0; f < nf; ++f) {
-      if (nslices > 1 && f != slice) continue;
+      if (nslices > 1 && f != (GS ? slice % nf : slice)) continue;
       int bb = b;  // opaque thread index, as in k_step_colinv
       asm volatile("" : "+v"(bb));
       cplx w[8];
@@ -1459,7 +1496,7 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
       }
       fft_dit<N, +1, 1, CS>(w, lds, bb, tw, tl);
       cplx *dst = dst0 + (size_t)f * c.fs;
-      if (act) {
+      if (act && (!GS || slice < nf)) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) dst[tix(bb + T * q, kx, c.g.kp)] = w[q];
       }
@@ -1467,13 +1504,13 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
   }
 }
 
-template <int N, bool CS = false>
+template <int N, bool CS = false, bool EARLY = false>
 __global__ __launch_bounds__(N / 8) void k_step_colfused(
     const cplx *__restrict__ Q, cplx *psi, cplx *vz, cplx *__restrict__ M, StepConst c, int o,
     int mode, const unsigned long long *mread, unsigned long long *mreset, double *dt_out,
     const cplx *__restrict__ tw) {
   __shared__ cplx lds[Shape<N>::LDSN];
-  colfused_body<N, CS>(Q, psi, vz, M, c, o, mode, mread, mreset, dt_out, tw,
+  colfused_body<N, CS, false, EARLY>(Q, psi, vz, M, c, o, mode, mread, mreset, dt_out, tw,
                        xcd_pairs(blockIdx.x), threadIdx.x, lds, blockIdx.z, gridDim.z, true);
 }
 
@@ -1509,14 +1546,30 @@ __global__ __launch_bounds__(N / 8) void k_ens_colinv(EnsArgs a, const cplx *__r
   const StepConst c = ens_const(a, blockIdx.y);
   colinv_body<N, CS>(e.psi, e.vz, e.M, c, nullptr, tw, xcd_pairs(blockIdx.x));
 }
-template <int N, bool CS = false>
+template <int N, bool CS = false, bool EARLY = false>
 __global__ __launch_bounds__(N / 8) void k_ens_colfused(EnsArgs a, int o, int mode,
                                                          const cplx *__restrict__ tw) {
   const EnsMember &e = a.mem[blockIdx.y];
   const StepConst c = ens_const(a, blockIdx.y);
   __shared__ cplx lds[Shape<N>::LDSN];
-  colfused_body<N, CS>(e.M + 6 * c.fs, e.psi, e.vz, e.M, c, o, mode, nullptr, nullptr, nullptr, tw,
+  colfused_body<N, CS, false, EARLY>(e.M + 6 * c.fs, e.psi, e.vz, e.M, c, o, mode, nullptr, nullptr, nullptr, tw,
                        xcd_pairs(blockIdx.x), threadIdx.x, lds, blockIdx.z, gridDim.z, true);
+}
+// Emission slices as NS thread groups of one block (colfused_body, GS):
+// one column per block, each group emitting one field; unlike slices as
+// separate blocks this also splits the state-writing last stage.  For
+// latency-bound launches (a few small runs).
+template <int N, bool CS, int NS>
+__global__ __launch_bounds__(NS * (N / 8)) void k_ens_colfused_s(EnsArgs a, int o, int mode,
+                                                                 const cplx *__restrict__ tw) {
+  constexpr int T = N / 8;
+  const EnsMember &e = a.mem[blockIdx.y];
+  const StepConst c = ens_const(a, blockIdx.y);
+  __shared__ cplx lds[NS * Shape<N>::LDSN];
+  const int g = threadIdx.x / T, b = threadIdx.x - g * T;
+  colfused_body<N, CS, true, (N <= R2D_COL_EARLY_MAXN)>(
+      e.M + 6 * c.fs, e.psi, e.vz, e.M, c, o, mode, nullptr, nullptr, nullptr, tw,
+      xcd_pairs(blockIdx.x), b, lds + g * Shape<N>::LDSN, g, NS, true);
 }
 // The same for n < 512, whose n/8-thread column groups fill less than a
 // wavefront: GC neighbouring columns per block (GC n/8 <= 64 threads), group g
@@ -1526,7 +1579,7 @@ __global__ __launch_bounds__(N / 8) void k_ens_colfused(EnsArgs a, int o, int mo
 // nc in its range runs its field's last column without storing (act false).
 // Each column's arithmetic is k_ens_colfused's, so the results are the same
 // bits.
-template <int N, bool CS, int GC>
+template <int N, bool CS, int GC, bool EARLY = false>
 __global__ __launch_bounds__(GC * (N / 8)) void k_ens_colfused_g(EnsArgs a, int o, int mode,
                                                                  const cplx *__restrict__ tw) {
   constexpr int T = N / 8;
@@ -1546,7 +1599,7 @@ __global__ __launch_bounds__(GC * (N / 8)) void k_ens_colfused_g(EnsArgs a, int 
     act = lg - ncp < nc;
     lb = act ? lg : ncp + nc - 1;
   }
-  colfused_body<N, CS>(e.M + 6 * c.fs, e.psi, e.vz, e.M, c, o, mode, nullptr, nullptr, nullptr, tw,
+  colfused_body<N, CS, false, EARLY>(e.M + 6 * c.fs, e.psi, e.vz, e.M, c, o, mode, nullptr, nullptr, nullptr, tw,
                        lb, b, lds + g * Shape<N>::LDSN, blockIdx.z, gridDim.z, act);
 }
 // ---------------------------------------------------------------------------
@@ -2033,9 +2086,13 @@ static int step_seq(r2d3c_plan *p, StepConst &c, cplx *psi, cplx *vz, hipStream_
     const int nblk = round16(2 * ncp + ((mode & FM_ZERO) ? (N / 2 + 1 - nc) : 0));
     unsigned long long *mres = o == 4 ? mother : nullptr;
     ev_pair(p, o == 1 ? 3 : 2, &ea, &eb);
-    const int nz = (mode & FM_STATE) ? 1 : col_slices(p, (long long)nblk * (N / 8));
-    hipExtLaunchKernelGGL((N >= 512 && p->col_cs) ? k_step_colfused<N, true>
-                                                  : k_step_colfused<N, false>,
+    const int nsl = col_slices(p, (long long)nblk * (N / 8));
+    const int nz = (mode & FM_STATE) ? 1 : nsl;
+    // latency-bound (split) launches load the update's inputs early
+    constexpr bool ESM = N <= R2D_COL_EARLY_MAXN;
+    const bool cs = N >= 512 && p->col_cs, early = ESM && nsl > 1;
+    hipExtLaunchKernelGGL(early ? (cs ? k_step_colfused<N, true, ESM> : k_step_colfused<N, false, ESM>)
+                                : (cs ? k_step_colfused<N, true> : k_step_colfused<N, false>),
                           dim3(nblk, 1, nz), dim3(N / 8), 0, s, ea, eb, 0,
                           (const cplx *)Q, psi, vz, M, c, o, mode,
                           (const unsigned long long *)mb, mres, dt_out, tw);
@@ -2951,21 +3008,12 @@ struct r2d3c_ensemble {
 // tick: the step counter this class's first row launch advances (the first
 // class of a step whose classes follow each other), or NULL.
 template <int N>
+static int run_closure_inputs_mixed(r2d3c_plan *p, const cplx *M, const double *sc,
+                                    unsigned short *in_bf16, hipStream_t s);
+
+template <int N>
 static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, hipStream_t s,
                           bool first, bool masked, bool chain, bool closures, int *tick) {
-  // the closure forcing of this step, from the step's state (else the
-  // forcing of the last evaluation is held)
-  for (size_t ci = 0; closures && ci < k.closures.size(); ++ci) {
-    EnsClosure &cl = k.closures[ci];
-    int rc = r2d3c_closure_net_inputs(k.plan, cl.psi, cl.vz, cl.sin, cl.net, s);
-    if (!rc) rc = r2d3c_cnn_forward_buffers(cl.net, 1, s);
-    if (!rc)
-      rc = r2d3c_closure_net_forcing(k.plan, cl.net, cl.sout, cl.kmin, (double *)cl.fp,
-                                     (double *)cl.fz, s);
-    if (!rc && cl.kclip > 0.0)
-      rc = r2d3c_closure_clip(k.plan, cl.psi, cl.vz, cl.kclip, (double *)cl.fp, (double *)cl.fz, s);
-    if (rc) return rc;
-  }
   const r2d3c_plan *p = k.plan;
   const int nc = p->g.nc, ncp = (nc + 1) & ~1;
   const cplx *tw = p->tw;
@@ -2974,6 +3022,33 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
   if (first) {
     if (cs) k_ens_colinv<N, true><<<dim3(round16(ncp + nc), k.count), N / 8, 0, s>>>(a, tw);
     else k_ens_colinv<N, false><<<dim3(round16(ncp + nc), k.count), N / 8, 0, s>>>(a, tw);
+  }
+  // The closure forcing of this step, from the step's state (else the
+  // forcing of the last evaluation is held).  A steady step's state is zero
+  // outside the update mask and its column transforms are in the workspace
+  // (emitted by the last stage before), so the network inputs come from
+  // there.  (The chain on a side branch of the step graph beside the first
+  // row launch measured slower: the graph ran the branch's launches one after
+  // the other anyway, with a queue switch between.)
+  for (size_t ci = 0; closures && ci < k.closures.size(); ++ci) {
+    EnsClosure &cl = k.closures[ci];
+    int rc;
+    if (!first && masked) {
+      void *in = nullptr;
+      rc = r2d3c_cnn_buffers(cl.net, &in, nullptr);
+      if (!rc)
+        rc = run_closure_inputs_mixed<N>(k.plan, k.work + (size_t)cl.member * 8 * p->fs, cl.sin,
+                                         (unsigned short *)in, s);
+    } else {
+      rc = r2d3c_closure_net_inputs(k.plan, cl.psi, cl.vz, cl.sin, cl.net, s);
+    }
+    if (!rc) rc = r2d3c_cnn_forward_buffers(cl.net, 1, s);
+    if (!rc)
+      rc = r2d3c_closure_net_forcing(k.plan, cl.net, cl.sout, cl.kmin, (double *)cl.fp,
+                                     (double *)cl.fz, s);
+    if (!rc && cl.kclip > 0.0)
+      rc = r2d3c_closure_clip(k.plan, cl.psi, cl.vz, cl.kclip, (double *)cl.fp, (double *)cl.fz, s);
+    if (rc) return rc;
   }
   for (int o = 4; o >= 1; --o) {
     // one row launch covers the class: the runs' workspaces are 8 fs apart
@@ -2999,23 +3074,43 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
     int mode = FM_EMIT;
     if (o == 1) mode = FM_STATE | (masked ? FM_DISK : FM_ZERO) | (chain ? FM_EMIT : 0);
     const int nblk = round16(2 * ncp + ((mode & FM_ZERO) ? (N / 2 + 1 - nc) : 0));
-    const int nz = (mode & FM_STATE) ? 1 : col_slices(p, (long long)nblk * k.count * (N / 8));
+    const int nsl = col_slices(p, (long long)nblk * k.count * (N / 8));
+    // latency-bound columns: emission slices as thread groups of one block
+    // (the last stage too), or as separate blocks (not the last stage)
+    if constexpr (4 * (N / 8) <= 1024 && N <= 1024) {
+      if (nsl > 1 && R2D_ENS_GSLICES >= ((mode & FM_STATE) ? 1 : 2)) {
+        if (cs) k_ens_colfused_s<N, true, 4><<<dim3(nblk, k.count), 4 * (N / 8), 0, s>>>(a, o, mode, tw);
+        else k_ens_colfused_s<N, false, 4><<<dim3(nblk, k.count), 4 * (N / 8), 0, s>>>(a, o, mode, tw);
+        continue;
+      }
+    }
+    const int nz = (mode & FM_STATE) ? 1 : nsl;
+    // latency-bound (split) launches load the update's inputs early
+    constexpr bool ESM = N <= R2D_COL_EARLY_MAXN;
+    const bool early = ESM && nsl > 1;
     // columns per block: fill a wavefront below n = 512 (GC divides ncp and nblk)
     const int gc = (N / 8 >= 64 || !R2D_ENS_COL_GROUPS) ? 1 : (64 / (N / 8) >= 4 && ncp % 4 == 0 ? 4 : 2);
     if constexpr (N >= 32 && 4 * (N / 8) <= 64) {
       if (gc == 4) {
-        k_ens_colfused_g<N, false, 4><<<dim3(nblk / 4, k.count, nz), 4 * (N / 8), 0, s>>>(a, o, mode, tw);
+        if (early) k_ens_colfused_g<N, false, 4, ESM><<<dim3(nblk / 4, k.count, nz), 4 * (N / 8), 0, s>>>(a, o, mode, tw);
+        else k_ens_colfused_g<N, false, 4><<<dim3(nblk / 4, k.count, nz), 4 * (N / 8), 0, s>>>(a, o, mode, tw);
         continue;
       }
     }
     if constexpr (N >= 32 && 2 * (N / 8) <= 64) {
       if (gc == 2) {
-        k_ens_colfused_g<N, false, 2><<<dim3(nblk / 2, k.count, nz), 2 * (N / 8), 0, s>>>(a, o, mode, tw);
+        if (early) k_ens_colfused_g<N, false, 2, ESM><<<dim3(nblk / 2, k.count, nz), 2 * (N / 8), 0, s>>>(a, o, mode, tw);
+        else k_ens_colfused_g<N, false, 2><<<dim3(nblk / 2, k.count, nz), 2 * (N / 8), 0, s>>>(a, o, mode, tw);
         continue;
       }
     }
-    if (cs) k_ens_colfused<N, true><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(a, o, mode, tw);
-    else k_ens_colfused<N, false><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(a, o, mode, tw);
+    if (early) {
+      if (cs) k_ens_colfused<N, true, ESM><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(a, o, mode, tw);
+      else k_ens_colfused<N, false, ESM><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(a, o, mode, tw);
+    } else {
+      if (cs) k_ens_colfused<N, true><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(a, o, mode, tw);
+      else k_ens_colfused<N, false><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(a, o, mode, tw);
+    }
   }
   return 0;
 }
@@ -3676,6 +3771,10 @@ __global__ __launch_bounds__(N / 8) void k_ci_colinv(const cplx *__restrict__ ps
 
 // Rows y, y+1 of the three fields to the network input bf16 [y][x][16]:
 // channel c = real-space field c times s[c] (s includes 1/n^2), 3..15 zero.
+// The fields' column transforms f0, f1, f2 are tiled mixed fields of pitch kp
+// holding columns kx < ncol (zero beyond): the full transforms of a plan
+// (ncol = n/2 + 1), or a masked state's transforms in a step workspace
+// (ncol = nc: its columns past n/3 are zero).
 // Three packed transforms: (f0, f1) of row y, (f0, f1) of row y+1, f2 of both.
 // When 3 n/8 threads fit a block (kCiPar), thread group g runs transform g in
 // LDS region g, in parallel (the launch is latency-bound), and the third
@@ -3686,19 +3785,21 @@ template <int N> struct CiRow {
 };
 template <int N>
 __global__ __launch_bounds__(CiRow<N>::G * (N / 8)) void k_ci_row(
-    const cplx *__restrict__ mix, Geo g, const cplx *__restrict__ tw, double s0, double s1,
-    double s2, unsigned short *__restrict__ out) {
+    const cplx *__restrict__ f0, const cplx *__restrict__ f1, const cplx *__restrict__ f2,
+    int kp, int ncol, const cplx *__restrict__ tw, double s0, double s1, double s2,
+    unsigned short *__restrict__ out) {
   constexpr int T = N / 8, G = CiRow<N>::G, LN = Shape<N>::LDSN;
   __shared__ cplx lds[G * LN];
   const int grp = threadIdx.x / T, b = threadIdx.x - grp * T, y = 2 * blockIdx.x;
-  const size_t fs = (size_t)N * g.kpf;
   auto load = [&](cplx v[8], const cplx *fa, int ya, const cplx *fb, int yb) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int k = b + T * q;
       const bool mir = k > N / 2;
       const int kk = mir ? N - k : k;
-      v[q] = packed_z<N>(k, fa[tix(ya, kk, g.kpf)], fb[tix(yb, kk, g.kpf)], mir);
+      const bool in = kk < ncol;
+      v[q] = packed_z<N>(k, in ? fa[tix(ya, kk, kp)] : mk(0.0, 0.0),
+                         in ? fb[tix(yb, kk, kp)] : mk(0.0, 0.0), mir);
     }
   };
   auto emit = [&](int r, const cplx z[8], const cplx v[8]) {  // row y + r
@@ -3714,8 +3815,8 @@ __global__ __launch_bounds__(CiRow<N>::G * (N / 8)) void k_ci_row(
   };
   if constexpr (G == 3) {
     cplx a[8];
-    if (grp < 2) load(a, mix, y + grp, mix + fs, y + grp);
-    else load(a, mix + 2 * fs, y, mix + 2 * fs, y + 1);
+    if (grp < 2) load(a, f0, y + grp, f1, y + grp);
+    else load(a, f2, y, f2, y + 1);
     cplx *P = lds + grp * LN;
     fft_canon<N, +1, 1>(a, P, b, tw);
     if (grp == 2) {
@@ -3731,11 +3832,11 @@ __global__ __launch_bounds__(CiRow<N>::G * (N / 8)) void k_ci_row(
     }
   } else {
     cplx a[8], c[8], v[8];
-    load(a, mix, y, mix + fs, y);
+    load(a, f0, y, f1, y);
     fft_canon<N, +1, 1>(a, lds, b, tw);
-    load(c, mix, y + 1, mix + fs, y + 1);
+    load(c, f0, y + 1, f1, y + 1);
     fft_canon<N, +1, 1>(c, lds, b, tw);
-    load(v, mix + 2 * fs, y, mix + 2 * fs, y + 1);
+    load(v, f2, y, f2, y + 1);
     fft_canon<N, +1, 1>(v, lds, b, tw);
     emit(0, a, v);
     emit(1, c, v);
@@ -3840,8 +3941,10 @@ static int run_closure_io(r2d3c_plan *p, int dir, const cplx *psi, const cplx *v
   if (dir == 0) {
     k_ci_colinv<N><<<dim3(N / 2 + 1, 3), N / 8, 0, s>>>(psi, vz, p->mfull, p->g, p->tw);
     const double inv = 1.0 / ((double)N * N);
-    k_ci_row<N><<<N / 2, CiRow<N>::G * (N / 8), 0, s>>>(p->mfull, p->g, p->tw, sc[0] * inv,
-                                                         sc[1] * inv, sc[2] * inv, in_bf16);
+    const size_t fs = (size_t)N * p->g.kpf;
+    k_ci_row<N><<<N / 2, CiRow<N>::G * (N / 8), 0, s>>>(
+        p->mfull, p->mfull + fs, p->mfull + 2 * fs, p->g.kpf, N / 2 + 1, p->tw, sc[0] * inv,
+        sc[1] * inv, sc[2] * inv, in_bf16);
   } else {
     k_cf_row<N><<<N / 2, CfRow<N>::G * (N / 8), 0, s>>>(z, p->g, p->tw, sc[0], sc[1], p->mfull);
     const double *gain = nullptr;
@@ -3851,6 +3954,23 @@ static int run_closure_io(r2d3c_plan *p, int dir, const cplx *psi, const cplx *v
                                                       ngain, fp, fz);
   }
   LAUNCHCHK("closure io");
+  return 0;
+}
+
+// The network inputs of a masked state from its column transforms in a step
+// workspace (fields 0, 2, 4 of M: psi, -k^2 psi = omega, vz, as the step's
+// last stage emits them), instead of transforming the state again: the same
+// values as r2d3c_closure_net_inputs for a state that is zero outside the
+// update mask.
+template <int N>
+static int run_closure_inputs_mixed(r2d3c_plan *p, const cplx *M, const double *sc,
+                                    unsigned short *in_bf16, hipStream_t s) {
+  const double inv = 1.0 / ((double)N * N);
+  const size_t fs = p->fs;
+  k_ci_row<N><<<N / 2, CiRow<N>::G * (N / 8), 0, s>>>(M + 2 * fs, M + 4 * fs, M, p->g.kp,
+                                                       p->g.nc, p->tw, sc[0] * inv, sc[1] * inv,
+                                                       sc[2] * inv, in_bf16);
+  LAUNCHCHK("closure inputs (mixed)");
   return 0;
 }
 
