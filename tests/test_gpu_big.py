@@ -1,9 +1,12 @@
 """GPU: the n = 16384 path (two 8192-point halves per transform, stage-by-stage
-step).  The oracle cannot run at this size, so the checks are properties that
-do not depend on it: transforms of a few Fourier modes against their closed
-form, the r2c/c2r round trip, single-mode Jacobians, and a step from a
-band-limited state against the same step at 512^2 (whose path is checked
-against the oracle elsewhere), rescaled by the grid factor."""
+step).  The numpy oracle cannot run at this size, so most checks are
+properties that do not depend on it: transforms of a few Fourier modes against
+their closed form, the r2c/c2r round trip, single-mode Jacobians, and a step
+from a band-limited state against the same step at 512^2 (whose path is
+checked against the oracle elsewhere), rescaled by the grid factor.  The
+compiled CPU restatement of the step (oracle/cpu_step.cpp, itself checked
+against the numpy oracle) does run at this size: one forced step from a
+state with every mode set is compared with it."""
 import ctypes
 import math
 
@@ -190,3 +193,43 @@ def test_bracket_pass_forms_agree_on_broadband_state(cuda):
     finally:
         out.clear()
         _set_bracket_pass(cuda, 1)
+
+
+def test_broadband_step_matches_the_compiled_restatement(cuda):
+    """One forced fixed-dt step of the 16384^2 workload's parameters from a
+    state with every mode set, against the compiled CPU restatement of the
+    reference step (oracle/cpu_step.cpp on 16 threads, full 2D transforms in
+    the reference's order): the fields to 1e-12 relative L2, and the step's
+    increments (dt times the right-hand side, where the brackets live) to
+    1e-10."""
+    print("16384 step vs compiled restatement", flush=True)
+    import time
+    from rot2d3c_amd import spectral as S
+    from oracle import cpu_step as C
+    from oracle import spectral as O
+    # nu, hnu, nn, mm, nuv, hnuv, nnv, mmv, Omega; dt and kf of the workload
+    args = (5e-6, 1.0, 1, 2, 5e-6, 1.0, 1, 2, 10.0)
+    dt, kup = 2e-5, 256.0
+    kx, ky, ph, _ = O.rand_force_draw(kup, 11)
+    dp = (kx, ky, ph, 1.0, kup)
+    kx, ky, ph, _ = O.rand_force_draw(kup, 12)
+    dz = (kx, ky, ph, 1.0, 1.0)
+    ps, vz = _broadband(N, 21, cuda), _broadband(N, 22, cuda)
+    host0 = (ps.cpu().numpy(), vz.cpu().numpy())
+    S.rk4_run(ps, vz, S.Params(*args), [(S.ForceDraw(*dp), S.ForceDraw(*dz))], 1, dt=dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cpu = (host0[0].copy(), host0[1].copy())
+    C.threads(16)
+    C.run(cpu[0], cpu[1], O.Params(*args), 1, dt, [(dp, dz)])
+    print("  restatement: %.1f s" % (time.perf_counter() - t0), flush=True)
+    for g, c, x0 in zip((ps, vz), cpu, host0):
+        c_d = torch.from_numpy(c).to(cuda)
+        x0_d = torch.from_numpy(x0).to(cuda)
+        rel = float(torch.linalg.vector_norm(g - c_d) / torch.linalg.vector_norm(c_d))
+        inc = float(torch.linalg.vector_norm((g - x0_d) - (c_d - x0_d)) /
+                    torch.linalg.vector_norm(c_d - x0_d))
+        print("  rel %.3e, increment rel %.3e" % (rel, inc), flush=True)
+        assert rel < 1e-12 and inc < 1e-10
+        del c_d, x0_d
+    del ps, vz, cpu, host0
