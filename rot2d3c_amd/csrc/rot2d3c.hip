@@ -101,6 +101,12 @@ template <int N> constexpr bool col_twlast() { return ((R2D_COL_TWLAST) >> ilog2
 #define R2D_COL_EARLY_MAXN 512
 #endif
 
+// Lone closure runs fold the closure forcing's column pass into the first
+// stage's column launch (1) or run it as a launch of its own (0).
+#ifndef R2D_CL_FOLD
+#define R2D_CL_FOLD 1
+#endif
+
 // Latency-bound ensemble column launches (emission split over 4 slices): the
 // slices as thread groups of one block in the last stage (1) or in every
 // stage (2); 0: the last stage unsplit and the others' slices as blocks.
@@ -1351,6 +1357,20 @@ enum { FM_EMIT = 1, FM_STATE = 2, FM_DISK = 4, FM_ZERO = 8 };
 // With act false the group runs every transform and barrier on its (valid)
 // column but stores nothing: a group of a block that has no column of its
 // own keeps the block's barrier sequence.
+// The closure forcing's column pass folded into the first stage's fused
+// column kernel (latency-bound closure runs, colfused_body EARLY): mix holds
+// the forcing's row transforms (channels 0 and 1 as mixed fields of pitch
+// kpf, all columns); the kernel transforms its column as k_cf_col does,
+// writes the column's forcing to fp or fz (slice 0) and uses it in the
+// update.
+struct CfFuse {
+  const cplx *mix;
+  const double *gain;
+  cplx *fp, *fz;
+  double kmin2;
+  int kpf, ngain, member;
+};
+
 // Emission slices (nslices > 1): slice s emits field s only.  As separate
 // blocks (GS false), slices past the column's field count return at once and
 // the state is never written (every slice reads it); as thread groups of one
@@ -1364,8 +1384,12 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
                                               int mode, const unsigned long long *mread,
                                               unsigned long long *mreset, double *dt_out,
                                               const cplx *__restrict__ tw, int lb, int b,
-                                              cplx *lds, int slice, int nslices, bool act) {
+                                              cplx *lds, int slice, int nslices, bool act,
+                                              const CfFuse &cf = CfFuse{}, bool cfon = false) {
   auto pos = [](int b, int q) { return ilv<N, CS>(b, q); };
+  // the folded closure forcing pass: early loads and the canonical order
+  // (pos = b + T q); other instantiations ignore cfon
+  constexpr bool CFOK = EARLY && (N < 512 || CS);
   constexpr int T = N / 8;
   const int nc = c.g.nc, ncp = (nc + 1) & ~1;
   int field, kx;
@@ -1402,18 +1426,40 @@ __device__ __forceinline__ void colfused_body(const cplx *__restrict__ Q, cplx *
   // update are issued with the column's loads, before the transform, so the
   // chain has one round trip to memory instead of two.  Otherwise they follow
   // it (about 50 fewer live registers: more waves per SIMD).
-  cplx s0e[EARLY ? 8 : 1], fde[EARLY ? 8 : 1];
+  cplx s0e[EARLY ? 8 : 1], fde[EARLY ? 8 : 1], cfv[EARLY ? 8 : 1];
   if constexpr (EARLY) {
+    const cplx *cfm = (CFOK && cfon) ? cf.mix + (size_t)field * N * cf.kpf : nullptr;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int j = pos(b, q), ky = kyof(j, N);
       const double ka2 = (double)kx * kx + (double)ky * ky;
       const bool in = ka2 <= c.g.kmax && ka2 > c.g.tiny;
       s0e[q] = in ? state[(size_t)kx * N + j] : mk(0.0, 0.0);
-      fde[q] = (in && fd) ? fd[(size_t)kx * N + j] : mk(0.0, 0.0);
+      fde[q] = (in && fd && !(CFOK && cfon)) ? fd[(size_t)kx * N + j] : mk(0.0, 0.0);
+      cfv[q] = (CFOK && cfon) ? cfm[tix(b + T * q, kx, cf.kpf)] : mk(0.0, 0.0);
     }
   }
   fft_dif<N, -1, 1, CS>(v, lds, b, tw, tl);  // v[q] = spectrum at ky index pos(b, q)
+  if constexpr (CFOK) {
+    if (cfon) {  // uniform over the block (one run per block)
+      // the closure forcing of this column: k_cf_col's transform and arithmetic
+      fft_canon<N, -1, 1>(cfv, lds, b, tw);
+      cplx *dst = field == 0 ? cf.fp : cf.fz;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int j = b + T * q, ky = kyof(j, N);
+        const double ka2 = (double)kx * kx + (double)ky * ky;
+        cplx o = mk(0.0, 0.0);
+        if (ka2 <= c.g.kmax && ka2 > c.g.tiny && ka2 >= cf.kmin2) {
+          o = field == 0 ? mk(-cfv[q].x / ka2, -cfv[q].y / ka2) : cfv[q];
+          const int shell = (int)(sqrt(ka2) + 0.5);
+          if (shell < cf.ngain) o = mk(o.x * cf.gain[shell], o.y * cf.gain[shell]);
+        }
+        if (slice == 0 && act) dst[(size_t)kx * N + j] = o;
+        fde[q] = o;
+      }
+    }
+  }
 
   const ForceSet &fo = field == 0 ? c.fp : c.fz;
   // at most two forced modes per field (scalars: no private-memory array)
@@ -1529,6 +1575,7 @@ struct EnsArgs {
   const ForceSet *ftab;   // [step][run][field], NULL = no forcing
   const int *step;        // current step of the forcing table
   int stride, base;       // runs per table step, index of this class's first run
+  CfFuse cf;              // folded closure forcing pass of run cf.member (mix NULL: none)
 };
 __device__ __forceinline__ StepConst ens_const(const EnsArgs &a, int m) {
   StepConst c = a.mem[m].c;
@@ -1553,7 +1600,8 @@ __global__ __launch_bounds__(N / 8) void k_ens_colfused(EnsArgs a, int o, int mo
   const StepConst c = ens_const(a, blockIdx.y);
   __shared__ cplx lds[Shape<N>::LDSN];
   colfused_body<N, CS, false, EARLY>(e.M + 6 * c.fs, e.psi, e.vz, e.M, c, o, mode, nullptr, nullptr, nullptr, tw,
-                       xcd_pairs(blockIdx.x), threadIdx.x, lds, blockIdx.z, gridDim.z, true);
+                       xcd_pairs(blockIdx.x), threadIdx.x, lds, blockIdx.z, gridDim.z, true, a.cf,
+                       a.cf.mix && (int)blockIdx.y == a.cf.member);
 }
 // Emission slices as NS thread groups of one block (colfused_body, GS):
 // one column per block, each group emitting one field; unlike slices as
@@ -1569,7 +1617,8 @@ __global__ __launch_bounds__(NS * (N / 8)) void k_ens_colfused_s(EnsArgs a, int 
   const int g = threadIdx.x / T, b = threadIdx.x - g * T;
   colfused_body<N, CS, true, (N <= R2D_COL_EARLY_MAXN)>(
       e.M + 6 * c.fs, e.psi, e.vz, e.M, c, o, mode, nullptr, nullptr, nullptr, tw,
-      xcd_pairs(blockIdx.x), b, lds + g * Shape<N>::LDSN, g, NS, true);
+      xcd_pairs(blockIdx.x), b, lds + g * Shape<N>::LDSN, g, NS, true, a.cf,
+      a.cf.mix && (int)blockIdx.y == a.cf.member);
 }
 // The same for n < 512, whose n/8-thread column groups fill less than a
 // wavefront: GC neighbouring columns per block (GC n/8 <= 64 threads), group g
@@ -1600,7 +1649,8 @@ __global__ __launch_bounds__(GC * (N / 8)) void k_ens_colfused_g(EnsArgs a, int 
     lb = act ? lg : ncp + nc - 1;
   }
   colfused_body<N, CS, false, EARLY>(e.M + 6 * c.fs, e.psi, e.vz, e.M, c, o, mode, nullptr, nullptr, nullptr, tw,
-                       lb, b, lds + g * Shape<N>::LDSN, blockIdx.z, gridDim.z, act);
+                       lb, b, lds + g * Shape<N>::LDSN, blockIdx.z, gridDim.z, act,
+                       a.cf, a.cf.mix && (int)blockIdx.y == a.cf.member);
 }
 // ---------------------------------------------------------------------------
 // ensembles: one persistent launch per RK4 stage for every size class
@@ -3010,6 +3060,10 @@ struct r2d3c_ensemble {
 template <int N>
 static int run_closure_inputs_mixed(r2d3c_plan *p, const cplx *M, const double *sc,
                                     unsigned short *in_bf16, hipStream_t s);
+template <int N>
+static int run_closure_forcing_rows(r2d3c_plan *p, r2d3c_cnn *net, const double *sc,
+                                    hipStream_t s);
+void r2d3c_internal_cnn_forcing_gain(const r2d3c_cnn *c, const double **gain, int *n);
 
 template <int N>
 static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, hipStream_t s,
@@ -3030,9 +3084,33 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
   // there.  (The chain on a side branch of the step graph beside the first
   // row launch measured slower: the graph ran the branch's launches one after
   // the other anyway, with a queue switch between.)
+  // A lone closure run whose first column launch is latency-bound (early
+  // loads, canonical order) also folds the forcing's column pass into that
+  // launch (CfFuse): one launch fewer per evaluation.  Its forcing arrays
+  // hold zeros past the kept columns from the call's first step (unfolded).
+  const int nsl1 = col_slices(p, (long long)round16(2 * ncp) * k.count * (N / 8));
+  const bool fold = R2D_CL_FOLD && closures && k.closures.size() == 1 &&
+                    k.closures[0].kclip <= 0.0 && !first && masked && nsl1 > 1 &&
+                    N <= R2D_COL_EARLY_MAXN && (N < 512 || cs);
+  EnsArgs a4 = a;
   for (size_t ci = 0; closures && ci < k.closures.size(); ++ci) {
     EnsClosure &cl = k.closures[ci];
     int rc;
+    if (fold) {
+      void *in = nullptr;
+      rc = r2d3c_cnn_buffers(cl.net, &in, nullptr);
+      if (!rc)
+        rc = run_closure_inputs_mixed<N>(k.plan, k.work + (size_t)cl.member * 8 * p->fs, cl.sin,
+                                         (unsigned short *)in, s);
+      if (!rc) rc = r2d3c_cnn_forward_buffers(cl.net, 1, s);
+      if (!rc) rc = run_closure_forcing_rows<N>(k.plan, cl.net, cl.sout, s);
+      if (rc) return rc;
+      const double *gain = nullptr;
+      int ngain = 0;
+      r2d3c_internal_cnn_forcing_gain(cl.net, &gain, &ngain);
+      a4.cf = CfFuse{p->mfull, gain, cl.fp, cl.fz, cl.kmin * cl.kmin, p->g.kpf, ngain, cl.member};
+      continue;
+    }
     if (!first && masked) {
       void *in = nullptr;
       rc = r2d3c_cnn_buffers(cl.net, &in, nullptr);
@@ -3079,8 +3157,8 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
     // (the last stage too), or as separate blocks (not the last stage)
     if constexpr (4 * (N / 8) <= 1024 && N <= 1024) {
       if (nsl > 1 && R2D_ENS_GSLICES >= ((mode & FM_STATE) ? 1 : 2)) {
-        if (cs) k_ens_colfused_s<N, true, 4><<<dim3(nblk, k.count), 4 * (N / 8), 0, s>>>(a, o, mode, tw);
-        else k_ens_colfused_s<N, false, 4><<<dim3(nblk, k.count), 4 * (N / 8), 0, s>>>(a, o, mode, tw);
+        if (cs) k_ens_colfused_s<N, true, 4><<<dim3(nblk, k.count), 4 * (N / 8), 0, s>>>(o == 4 ? a4 : a, o, mode, tw);
+        else k_ens_colfused_s<N, false, 4><<<dim3(nblk, k.count), 4 * (N / 8), 0, s>>>(o == 4 ? a4 : a, o, mode, tw);
         continue;
       }
     }
@@ -3092,24 +3170,24 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
     const int gc = (N / 8 >= 64 || !R2D_ENS_COL_GROUPS) ? 1 : (64 / (N / 8) >= 4 && ncp % 4 == 0 ? 4 : 2);
     if constexpr (N >= 32 && 4 * (N / 8) <= 64) {
       if (gc == 4) {
-        if (early) k_ens_colfused_g<N, false, 4, ESM><<<dim3(nblk / 4, k.count, nz), 4 * (N / 8), 0, s>>>(a, o, mode, tw);
-        else k_ens_colfused_g<N, false, 4><<<dim3(nblk / 4, k.count, nz), 4 * (N / 8), 0, s>>>(a, o, mode, tw);
+        if (early) k_ens_colfused_g<N, false, 4, ESM><<<dim3(nblk / 4, k.count, nz), 4 * (N / 8), 0, s>>>(o == 4 ? a4 : a, o, mode, tw);
+        else k_ens_colfused_g<N, false, 4><<<dim3(nblk / 4, k.count, nz), 4 * (N / 8), 0, s>>>(o == 4 ? a4 : a, o, mode, tw);
         continue;
       }
     }
     if constexpr (N >= 32 && 2 * (N / 8) <= 64) {
       if (gc == 2) {
-        if (early) k_ens_colfused_g<N, false, 2, ESM><<<dim3(nblk / 2, k.count, nz), 2 * (N / 8), 0, s>>>(a, o, mode, tw);
-        else k_ens_colfused_g<N, false, 2><<<dim3(nblk / 2, k.count, nz), 2 * (N / 8), 0, s>>>(a, o, mode, tw);
+        if (early) k_ens_colfused_g<N, false, 2, ESM><<<dim3(nblk / 2, k.count, nz), 2 * (N / 8), 0, s>>>(o == 4 ? a4 : a, o, mode, tw);
+        else k_ens_colfused_g<N, false, 2><<<dim3(nblk / 2, k.count, nz), 2 * (N / 8), 0, s>>>(o == 4 ? a4 : a, o, mode, tw);
         continue;
       }
     }
     if (early) {
-      if (cs) k_ens_colfused<N, true, ESM><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(a, o, mode, tw);
-      else k_ens_colfused<N, false, ESM><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(a, o, mode, tw);
+      if (cs) k_ens_colfused<N, true, ESM><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(o == 4 ? a4 : a, o, mode, tw);
+      else k_ens_colfused<N, false, ESM><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(o == 4 ? a4 : a, o, mode, tw);
     } else {
-      if (cs) k_ens_colfused<N, true><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(a, o, mode, tw);
-      else k_ens_colfused<N, false><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(a, o, mode, tw);
+      if (cs) k_ens_colfused<N, true><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(o == 4 ? a4 : a, o, mode, tw);
+      else k_ens_colfused<N, false><<<dim3(nblk, k.count, nz), N / 8, 0, s>>>(o == 4 ? a4 : a, o, mode, tw);
     }
   }
   return 0;
@@ -3971,6 +4049,19 @@ static int run_closure_inputs_mixed(r2d3c_plan *p, const cplx *M, const double *
                                                        p->g.nc, p->tw, sc[0] * inv, sc[1] * inv,
                                                        sc[2] * inv, in_bf16);
   LAUNCHCHK("closure inputs (mixed)");
+  return 0;
+}
+
+// The forcing's row pass alone (k_cf_row into the plan's mixed fields); the
+// column pass then runs inside the first stage's column launch (CfFuse).
+template <int N>
+static int run_closure_forcing_rows(r2d3c_plan *p, r2d3c_cnn *net, const double *sc,
+                                    hipStream_t s) {
+  const float *z = nullptr;
+  const int rc = r2d3c_cnn_buffers(net, nullptr, &z);
+  if (rc) return rc;
+  k_cf_row<N><<<N / 2, CfRow<N>::G * (N / 8), 0, s>>>(z, p->g, p->tw, sc[0], sc[1], p->mfull);
+  LAUNCHCHK("closure forcing rows");
   return 0;
 }
 
