@@ -310,6 +310,14 @@ template <int G> __device__ __forceinline__ int xcd_group(int hb) {
   const int x = hb & 7, s = hb >> 3;
   return (s / G) * (8 * G) + x * G + (s % G);
 }
+// the same map over the first n blocks of a launch (n % (8 G) == 0: a
+// bijection of [0, n); otherwise the identity)
+template <int G> __device__ __forceinline__ int xcd_group_n(int hb, int n) {
+  if constexpr (G == 1) return hb;
+  if (n % (8 * G) != 0) return hb;
+  const int x = hb & 7, s = hb >> 3;
+  return (s / G) * (8 * G) + x * G + (s % G);
+}
 
 // Poisson bracket rows (full range): z1 = ax + i ay, z2 = bx + i by from four
 // mixed fields; out[y][x] = (ax*by - ay*bx)/n^4  (src/pseudospec2D_Rot2D3C.f90:195-216)
@@ -1108,15 +1116,14 @@ __global__ __launch_bounds__(N / 8 * RowStep<N>::ROWS) void k_step_row(
 // at a barrier the others skip); group 0 stores.  The arithmetic is
 // k_step_row's (results agree within the rounding of multiply-add fusion).
 template <int N>
-__global__ __launch_bounds__(3 * (N / 8) * RowStep<N>::ROWS) void k_step_row3(
-    const cplx *__restrict__ M, cplx *__restrict__ Q, StepConst c, unsigned long long *maxbits,
-    const cplx *__restrict__ tw, size_t mstride, const double *__restrict__ om_tab, int *tick) {
+__device__ __forceinline__ void row3_body(const cplx *__restrict__ M, cplx *__restrict__ Q,
+                                          const StepConst &c, unsigned long long *maxbits,
+                                          const cplx *__restrict__ tw, size_t mstride,
+                                          const double *__restrict__ om_tab, int lb, cplx *lds) {
   constexpr int T = N / 8, RW = RowStep<N>::ROWS, GT = T * RW, LN = RW * Shape<N>::LDSN;
-  __shared__ cplx lds[3 * LN];
-  row_tick(tick);
   const int g = threadIdx.x / GT, t = threadIdx.x - g * GT;
   const int r = t % RW, b = t / RW;
-  const int y = xcd_group<RowStep<N>::G>(blockIdx.x) * RW + r;
+  const int y = lb * RW + r;
   M += blockIdx.y * mstride;
   Q += blockIdx.y * mstride;
   const double two_omega_n2 = om_tab ? om_tab[blockIdx.y] : c.two_omega_n2;
@@ -1163,15 +1170,31 @@ __global__ __launch_bounds__(3 * (N / 8) * RowStep<N>::ROWS) void k_step_row3(
     unpack_store<N, RW, N / 3 + 1>(P, b7, nc, kp, Q, y, Q + fs, y);
   }
 }
+template <int N>
+__global__ __launch_bounds__(3 * (N / 8) * RowStep<N>::ROWS) void k_step_row3(
+    const cplx *__restrict__ M, cplx *__restrict__ Q, StepConst c, unsigned long long *maxbits,
+    const cplx *__restrict__ tw, size_t mstride, const double *__restrict__ om_tab, int *tick) {
+  __shared__ cplx lds[3 * RowStep<N>::ROWS * Shape<N>::LDSN];
+  row_tick(tick);
+  row3_body<N>(M, Q, c, maxbits, tw, mstride, om_tab, xcd_group<RowStep<N>::G>(blockIdx.x), lds);
+}
 
 // Row kernel of a stage: the parallel-pair form for small grids when the
 // plan's row split allows it (-1: automatic, n <= 512; 1: always; 0: never),
-// else the non-persistent kernel.  blockIdx.y: the run (count runs).
+// else the non-persistent kernel.  blockIdx.y: the run (count runs).  cfr:
+// the closure forcing's row pass rides along (k_step_row3_cf; only where the
+// parallel-pair form runs, row3_takes_cf).
+struct CfRowArgs {
+  const float *z;  // network output, fp32 [y][x][16]
+  cplx *mix;       // the plan's mixed fields 0, 1
+  double s0, s1;
+  int ncf, kpf, member, nrb;  // nrb: the row launch's own blocks
+};
 template <int N>
 static void launch_row_np(const r2d3c_plan *p, const cplx *M, cplx *Q, const StepConst &c,
                           unsigned long long *macc, const cplx *tw, size_t ms, const double *om,
                           int count, hipStream_t s, hipEvent_t ea, hipEvent_t eb,
-                          int *tick = nullptr);
+                          int *tick = nullptr, const CfRowArgs *cfr = nullptr);
 
 
 // Persistent variant of the row kernel for n >= 256.  A block walks its y
@@ -2079,10 +2102,14 @@ static int make_step_const(r2d3c_plan *p, const r2d3c_params *q, int dt_mode, do
   return 0;
 }
 
+template <int N> static bool row3_takes_cf(const r2d3c_plan *p) {
+  return N <= 512 && p->row_split != 0;
+}
 template <int N>
-static void launch_row_np(const r2d3c_plan *p, const cplx *M, cplx *Q, const StepConst &c,
-                          unsigned long long *macc, const cplx *tw, size_t ms, const double *om,
-                          int count, hipStream_t s, hipEvent_t ea, hipEvent_t eb, int *tick) {
+static void launch_row_np_impl(const r2d3c_plan *p, const cplx *M, cplx *Q, const StepConst &c,
+                               unsigned long long *macc, const cplx *tw, size_t ms,
+                               const double *om, int count, hipStream_t s, hipEvent_t ea,
+                               hipEvent_t eb, int *tick) {
   constexpr int RW = RowStep<N>::ROWS;
   if (N <= 512 && p->row_split != 0)
     hipExtLaunchKernelGGL(k_step_row3<(N <= 512 ? N : 512)>, dim3(N / RW, count),
@@ -3092,7 +3119,13 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
   const bool fold = R2D_CL_FOLD && closures && k.closures.size() == 1 &&
                     k.closures[0].kclip <= 0.0 && !first && masked && nsl1 > 1 &&
                     N <= R2D_COL_EARLY_MAXN && (N < 512 || cs);
+  // Where the first row launch is the parallel-pair kernel, the forcing's
+  // row pass rides along in extra blocks of that launch (k_step_row3_cf).
+  bool row_plain = true;
+  if constexpr (RowV2<N, 2>::THREADS >= 64) row_plain = p->row_bpc == 0;
+  const bool fold_row = fold && row_plain && row3_takes_cf<N>(p);
   EnsArgs a4 = a;
+  CfRowArgs cfr{};
   for (size_t ci = 0; closures && ci < k.closures.size(); ++ci) {
     EnsClosure &cl = k.closures[ci];
     int rc;
@@ -3103,7 +3136,12 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
         rc = run_closure_inputs_mixed<N>(k.plan, k.work + (size_t)cl.member * 8 * p->fs, cl.sin,
                                          (unsigned short *)in, s);
       if (!rc) rc = r2d3c_cnn_forward_buffers(cl.net, 1, s);
-      if (!rc) rc = run_closure_forcing_rows<N>(k.plan, cl.net, cl.sout, s);
+      if (!rc && !fold_row) rc = run_closure_forcing_rows<N>(k.plan, cl.net, cl.sout, s);
+      if (!rc && fold_row) {
+        const float *z = nullptr;
+        rc = r2d3c_cnn_buffers(cl.net, nullptr, &z);
+        cfr = CfRowArgs{z, p->mfull, cl.sout[0], cl.sout[1], p->g.ncf, p->g.kpf, cl.member, 0};
+      }
       if (rc) return rc;
       const double *gain = nullptr;
       int ngain = 0;
@@ -3137,7 +3175,7 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
     if constexpr (RowV2<N, 2>::THREADS >= 64) {
       if (p->row_bpc == 0)
         launch_row_np<N>(p, M0, Q0, k.c0, nullptr, tw, ms, k.d_om, k.count, s, nullptr, nullptr,
-                         tk);
+                         tk, (o == 4 && fold_row) ? &cfr : nullptr);
       else if (p->row_rw == 1 && RowV2<N, 1>::THREADS >= 64)
         k_step_row2<N, 1><<<dim3(row2_grid<N, 1>(p->cus, p->row_bpc), k.count),
                             RowV2<N, 1>::THREADS, 0, s>>>(M0, Q0, k.c0, nullptr, tw, ms, k.d_om,
@@ -3147,7 +3185,8 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
                             RowV2<N, 2>::THREADS, 0, s>>>(M0, Q0, k.c0, nullptr, tw, ms, k.d_om,
                                                           tk);
     } else {
-      launch_row_np<N>(p, M0, Q0, k.c0, nullptr, tw, ms, k.d_om, k.count, s, nullptr, nullptr, tk);
+      launch_row_np<N>(p, M0, Q0, k.c0, nullptr, tw, ms, k.d_om, k.count, s, nullptr, nullptr, tk,
+                       (o == 4 && fold_row) ? &cfr : nullptr);
     }
     int mode = FM_EMIT;
     if (o == 1) mode = FM_STATE | (masked ? FM_DISK : FM_ZERO) | (chain ? FM_EMIT : 0);
@@ -3928,6 +3967,30 @@ __global__ __launch_bounds__(CiRow<N>::G * (N / 8)) void k_ci_row(
 template <int N> struct CfRow {
   static constexpr int G = 2 * (N / 8) <= 1024 ? 2 : 1;
 };
+// Channel ch of rows y, y+1 (one thread group of n/8 threads, LDS area P;
+// every thread of the block calls it together: block barriers).  store:
+// false for a group that only keeps the block's barriers in step.
+template <int N>
+__device__ __forceinline__ void cf_rows_group(const float *__restrict__ z, int ch, int y, int b,
+                                              cplx *P, const cplx *__restrict__ tw, double sc,
+                                              int ncf, int kpf, cplx *__restrict__ mix,
+                                              bool store) {
+  constexpr int T = N / 8;
+  const size_t fs = (size_t)N * kpf;
+  cplx v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int x = b + T * q;
+    v[q] = mk((double)z[((size_t)y * N + x) * 16 + ch] * sc,
+              (double)z[((size_t)(y + 1) * N + x) * 16 + ch] * sc);
+  }
+  fft_canon<N, -1, 1>(v, P, b, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) P[pad<N>(b + T * q)] = v[q];
+  __syncthreads();
+  if (store) unpack_store<N, 1, N / 2 + 1>(P, b, ncf, kpf, mix + ch * fs, y, mix + ch * fs, y + 1);
+  __syncthreads();
+}
 template <int N>
 __global__ __launch_bounds__(CfRow<N>::G * (N / 8)) void k_cf_row(
     const float *__restrict__ z, Geo g, const cplx *__restrict__ tw, double s0, double s1,
@@ -3935,24 +3998,52 @@ __global__ __launch_bounds__(CfRow<N>::G * (N / 8)) void k_cf_row(
   constexpr int T = N / 8, G = CfRow<N>::G, LN = Shape<N>::LDSN;
   __shared__ cplx lds[G * LN];
   const int grp = threadIdx.x / T, b = threadIdx.x - grp * T, y = 2 * blockIdx.x;
-  const size_t fs = (size_t)N * g.kpf;
   cplx *P = lds + grp * LN;
-  for (int ch = grp; ch < 2; ch += G) {
-    const double sc = ch == 0 ? s0 : s1;
-    cplx v[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int x = b + T * q;
-      v[q] = mk((double)z[((size_t)y * N + x) * 16 + ch] * sc,
-                (double)z[((size_t)(y + 1) * N + x) * 16 + ch] * sc);
-    }
-    fft_canon<N, -1, 1>(v, P, b, tw);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) P[pad<N>(b + T * q)] = v[q];
-    __syncthreads();
-    unpack_store<N, 1, N / 2 + 1>(P, b, g.ncf, g.kpf, mix + ch * fs, y, mix + ch * fs, y + 1);
-    __syncthreads();
+  for (int ch = grp; ch < 2; ch += G)
+    cf_rows_group<N>(z, ch, y, b, P, tw, ch == 0 ? s0 : s1, g.ncf, g.kpf, mix, true);
+}
+
+// k_step_row3 plus the closure forcing's row pass of run cf.member: blocks
+// past cf.nrb run it, 3 row pairs per block (one group of n/8 threads per
+// channel); a launch fewer per evaluation for a lone closure run.
+template <int N>
+__global__ __launch_bounds__(3 * (N / 8) * RowStep<N>::ROWS) void k_step_row3_cf(
+    const cplx *__restrict__ M, cplx *__restrict__ Q, StepConst c, unsigned long long *maxbits,
+    const cplx *__restrict__ tw, size_t mstride, const double *__restrict__ om_tab, int *tick,
+    CfRowArgs cf) {
+  constexpr int T = N / 8, RW = RowStep<N>::ROWS, LN = Shape<N>::LDSN, NG = 3 * RW;
+  static_assert(NG % 2 == 0, "channel pairs per block");
+  __shared__ cplx lds[3 * RW * LN];
+  row_tick(tick);
+  const int hb = blockIdx.x;
+  if (hb < cf.nrb) {
+    row3_body<N>(M, Q, c, maxbits, tw, mstride, om_tab, xcd_group_n<RowStep<N>::G>(hb, cf.nrb),
+                 lds);
+    return;
   }
+  if ((int)blockIdx.y != cf.member) return;  // uniform over the block
+  const int grp = threadIdx.x / T, b = threadIdx.x - grp * T, ch = grp & 1;
+  const int task = (hb - cf.nrb) * (NG / 2) + grp / 2;  // row pair
+  const bool ok = task < N / 2;
+  cf_rows_group<N>(cf.z, ch, 2 * (ok ? task : 0), b, lds + grp * LN, tw, ch == 0 ? cf.s0 : cf.s1,
+                   cf.ncf, cf.kpf, cf.mix, ok);
+}
+
+template <int N>
+static void launch_row_np(const r2d3c_plan *p, const cplx *M, cplx *Q, const StepConst &c,
+                          unsigned long long *macc, const cplx *tw, size_t ms, const double *om,
+                          int count, hipStream_t s, hipEvent_t ea, hipEvent_t eb, int *tick,
+                          const CfRowArgs *cfr) {
+  if (!cfr || !row3_takes_cf<N>(p)) {
+    launch_row_np_impl<N>(p, M, Q, c, macc, tw, ms, om, count, s, ea, eb, tick);
+    return;
+  }
+  constexpr int NN = N <= 512 ? N : 512, RW = RowStep<NN>::ROWS, PB = 3 * RW / 2;
+  CfRowArgs a = *cfr;
+  a.nrb = NN / RW;
+  hipExtLaunchKernelGGL(k_step_row3_cf<NN>, dim3(a.nrb + (NN / 2 + PB - 1) / PB, count),
+                        dim3(3 * (NN / 8) * RW), 0, s, ea, eb, 0, M, Q, c, macc, tw, ms, om, tick,
+                        a);
 }
 
 // column forwards of the two predicted fields with the closure epilogue:
