@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/rot2d3c.h"
+#include "fft_core.h"
 
 int r2d3c_internal_set_error(int code, const char *msg);
 
@@ -210,6 +211,104 @@ __global__ __launch_bounds__(64 * kConvWaves) void k_conv3x3(const bf16 *__restr
     if (ok[i])
       conv_store<CO, MODE>(acc[i], tb[i], ty[i], tx[i], m, q, H, W, bias, Zprev, Zout, Y, clamp,
                            X0, keep);
+}
+
+// The first layer with the network's input pass folded in (a closure run's
+// steady steps): the block of image row y forms the input rows y-1, y, y+1
+// itself from the column transforms of omega, vz and psi in a step
+// workspace (tiled mixed fields, pitch kp, columns kx < nc) into an LDS
+// tile, with the row transforms and roundings of the input pass
+// (k_ci_row: omega + i vz of one row per transform, psi of the row pairs
+// (2m, 2m+1)), then runs the layer on it as k_conv3x3<16, 32, GELU> does on
+// the input image; it also writes row y of the input image.  Each transform
+// is one group of n/8 lanes inside a wavefront, ordered within its wave, so
+// the other waves meet it only at the one block barrier.
+template <int N>
+__global__ __launch_bounds__(64 * kConvWaves) void k_conv0_mixed(
+    const r2d::cplx *__restrict__ mw, const r2d::cplx *__restrict__ mv,
+    const r2d::cplx *__restrict__ mp, int kp, int nc, const r2d::cplx *__restrict__ tw,
+    double s0, double s1, double s2, bf16 *__restrict__ X0, const bf16 *__restrict__ Wt,
+    const float *__restrict__ bias, float *__restrict__ Zout, bf16 *__restrict__ Y) {
+  constexpr int CI = 16, CO = 32, CO16 = CO / 16, WS = CI + 4, T = N / 8,
+                LN = r2d::Shape<N>::LDSN;
+  static_assert(N == 16 * kConvWaves && kTilesPerWave == 1, "one image row per block");
+  static_assert(6 * T <= 64 * kConvWaves && 64 % T == 0, "transform groups inside waves");
+  __shared__ bf16 ws[9 * CO * WS];
+  __shared__ bf16 xt[3 * N * CI];   // input rows y-1, y, y+1: [r][x][channel]
+  __shared__ r2d::cplx fl[5 * LN];  // the five transforms' LDS areas
+  const int y = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, q = lane >> 4;
+  for (int e = threadIdx.x; e < 9 * CO * (CI / 4); e += 64 * kConvWaves) {
+    const int row = e / (CI / 4), c4 = e - row * (CI / 4);
+    *(uint2 *)(ws + row * WS + 4 * c4) = *(const uint2 *)(Wt + (size_t)row * CI + 4 * c4);
+  }
+  // channels 3..15 of the tile are zero
+  for (int e = threadIdx.x; e < 3 * N; e += 64 * kConvWaves) {
+    bf16 *px = xt + e * CI;
+    px[3] = 0;
+    *(uint2 *)(px + 4) = make_uint2(0u, 0u);
+    *(uint4 *)(px + 8) = make_uint4(0u, 0u, 0u, 0u);
+  }
+  const int g = threadIdx.x / T, b = threadIdx.x - g * T;
+  if (g < 5) {
+    // groups 0-2: omega + i vz of rows y-1, y, y+1; groups 3, 4: psi of the
+    // row pairs holding y-1 and y+1
+    const int ym = (y + N - 1) % N, yp = (y + 1) % N;
+    const int ra = g < 3 ? (g == 0 ? ym : (g == 1 ? y : yp)) : ((g == 3 ? ym : yp) & ~1);
+    const int rb = g < 3 ? ra : ra + 1;
+    const r2d::cplx *fa = g < 3 ? mw : mp, *fb = g < 3 ? mv : mp;
+    r2d::cplx v[8];
+#pragma unroll
+    for (int k8 = 0; k8 < 8; ++k8) {
+      const int k = b + T * k8;
+      const bool mir = k > N / 2;
+      const int kk = mir ? N - k : k;
+      const bool in = kk < nc;
+      v[k8] = r2d::packed_z<N>(k, in ? fa[r2d::tix(ra, kk, kp)] : r2d::mk(0.0, 0.0),
+                               in ? fb[r2d::tix(rb, kk, kp)] : r2d::mk(0.0, 0.0), mir);
+    }
+    r2d::fft_canon<N, +1, 1, 1, 0, false, true, true>(v, fl + g * LN, b, tw);
+#pragma unroll
+    for (int k8 = 0; k8 < 8; ++k8) {
+      const int x = b + T * k8;
+      if (g < 3) {
+        bf16 *px = xt + (g * N + x) * CI;
+        px[0] = f2bf((float)(v[k8].x * s0));
+        px[1] = f2bf((float)(v[k8].y * s1));
+      } else {
+        // tile row of image row ra (real part) and ra + 1 (imaginary part)
+        const int da = (ra - ym + N) % N, db = (ra + 1 - ym + N) % N;
+        if (da < 3) xt[(da * N + x) * CI + 2] = f2bf((float)(v[k8].x * s2));
+        if (db < 3) xt[(db * N + x) * CI + 2] = f2bf((float)(v[k8].y * s2));
+      }
+    }
+  }
+  __syncthreads();
+  // row y of the input image (the last layer's sign clipping reads it)
+  for (int e = threadIdx.x; e < N * CI / 8; e += 64 * kConvWaves)
+    *(uint4 *)(X0 + (size_t)y * N * CI + 8 * e) = *(const uint4 *)(xt + N * CI + 8 * e);
+  // the layer: one tile of 16 pixels per wave, A fragments from the tile
+  const int tx = 16 * wave;
+  s16x4 a[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) {
+    const int dy = t / 3 - 1, dx = t % 3 - 1;
+    int xx = tx + m + dx;
+    xx += xx < 0 ? N : (xx >= N ? -N : 0);
+    a[t] = *(const s16x4 *)(xt + ((dy + 1) * N + xx) * CI + 4 * q);
+  }
+  f32x4 acc[1][CO16];
+#pragma unroll
+  for (int n = 0; n < CO16; ++n) acc[0][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+#pragma unroll
+    for (int n = 0; n < CO16; ++n) {
+      const s16x4 w = *(const s16x4 *)(ws + (t * CO + 16 * n + m) * WS + 4 * q);
+      acc[0][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], w, acc[0][n], 0, 0, 0);
+    }
+  conv_store<CO, CONV_GELU>(acc[0], 0, y, tx, m, q, N, N, bias, nullptr, Zout, Y, 0.f, nullptr,
+                            1.f);
 }
 
 // Weight gradient of one layer over a slice of rows: dW[t][co][ci] =
@@ -467,11 +566,13 @@ void conv(const r2d3c_cnn *c, int l, const bf16 *X, const bf16 *Wt, const float 
 // they share the sign of input channels 0, 1
 // keep_z: store the pre-activations of layers 0-4 (the backward pass needs
 // them for GELU'); the last layer's Z (the prediction) is always stored.
+// first_done: layer 0 already ran (k_conv0_mixed)
 int forward(r2d3c_cnn *c, int B, hipStream_t s, float clamp = 0.f, float sign_clip = 0.f,
-            bool keep_z = false) {
+            bool keep_z = false, bool first_done = false) {
   const bf16 *wf = c->wf;
-  conv<16, 32, CONV_GELU>(c, 0, c->act[0], wf + woff(0), nullptr, keep_z ? c->z[0] : nullptr,
-                          c->act[1], B, s);
+  if (!first_done)
+    conv<16, 32, CONV_GELU>(c, 0, c->act[0], wf + woff(0), nullptr, keep_z ? c->z[0] : nullptr,
+                            c->act[1], B, s);
   for (int l = 1; l < 5; ++l)
     conv<32, 32, CONV_GELU>(c, l, c->act[l], wf + woff(l), nullptr, keep_z ? c->z[l] : nullptr,
                             c->act[l + 1], B, s);
@@ -648,6 +749,7 @@ int r2d3c_cnn_forward_buffers(r2d3c_cnn *c, int batch, void *stream) {
   return forward(c, batch, (hipStream_t)stream, c->out_clamp, c->sign_clip);
 }
 
+
 int r2d3c_cnn_set_sign_clip(r2d3c_cnn *c, double strength) {
   if (!c) return err(R2D3C_EINVAL, "cnn: NULL argument");
   if (!(strength >= 0.0 && strength <= 1.0)) return err(R2D3C_EINVAL, "cnn: strength in [0, 1]");
@@ -720,6 +822,30 @@ int r2d3c_cnn_train_step(r2d3c_cnn *c, const float *d_in, const float *d_target,
 }
 
 }  // extern "C"
+
+// Inference of one image whose input pass is folded into the first layer
+// (k_conv0_mixed): M is a closure run's step workspace (mixed fields 0, 2, 4:
+// psi, omega, vz; field stride fs, column pitch kp, columns kx < nc), tw the
+// grid's twiddle table, s0..s2 the input scales (times 1/n^2).  The input
+// image buffer receives the same values as r2d3c_closure_net_inputs would
+// write.  Networks of 256 x 256 images only (one image row per block of the
+// first layer); others return R2D3C_ENOTSUP.
+int r2d3c_internal_cnn_forward_mixed(r2d3c_cnn *c, const double *M, size_t fs, int kp, int nc,
+                                     const double *tw, double s0, double s1, double s2,
+                                     void *stream) {
+  constexpr int N = 16 * kConvWaves;
+  if (!c || !M || !tw) return err(R2D3C_EINVAL, "cnn: NULL argument");
+  if (c->H != N || c->W != N) return err(R2D3C_ENOTSUP, "cnn: folded input pass needs 256 x 256");
+  int rc = check_batch(c, 1);
+  if (rc) return rc;
+  const hipStream_t s = (hipStream_t)stream;
+  const r2d::cplx *m = (const r2d::cplx *)M;
+  k_conv0_mixed<N><<<N, 64 * kConvWaves, 0, s>>>(
+      m + 2 * fs, m + 4 * fs, m, kp, nc, (const r2d::cplx *)tw, s0, s1, s2, c->act[0],
+      c->wf + woff(0), c->params + poff(0) + wsize(0), nullptr, c->act[1]);
+  KCHK("cnn first layer (folded input pass)");
+  return forward(c, 1, s, c->out_clamp, c->sign_clip, false, true);
+}
 
 // The closure forcing gain table of a network (rot2d3c.hip's forcing epilogue).
 void r2d3c_internal_cnn_forcing_gain(const r2d3c_cnn *c, const double **gain, int *n) {

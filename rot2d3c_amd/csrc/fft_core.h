@@ -201,6 +201,21 @@ template <int R, int S> __device__ __forceinline__ void dftR(cplx *x) {
   else dft8<S>(x);
 }
 
+// Orders this wave's LDS writes before its later LDS reads (compiler and
+// memory model); LDS instructions of one wave execute in order.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// The transforms' LDS hand-offs: a block barrier, or (WAVE) only this
+// wave's ordering, for a transform whose threads and LDS area lie in one
+// wavefront (no s_barrier: other waves of the block go on).
+template <bool WAVE> __device__ __forceinline__ void lds_sync() {
+  if constexpr (WAVE) wave_lds_sync();
+  else __syncthreads();
+}
+
 // Unnormalised DFT of length N, sign S (-1 forward, +1 inverse), of the
 // canonical register set v of thread b (0 <= b < N/8).  ZQ: bit q set when
 // v[q] is zero on entry for every thread (its additions in the first pass are
@@ -214,8 +229,10 @@ template <int R, int S> __device__ __forceinline__ void dftR(cplx *x) {
 // (exp(-2 pi i t / (TS N))): a longer transform's table serves its halves.  All threads of the block
 // must call this together (it contains block barriers).  The last pass reads
 // the LDS area and a closing barrier follows it, so the area is free for any
-// thread to write again on return.
-template <int N, int S, int IL, int TS = 1, int ZQ = 0, bool MIR = false, bool CLOSE = true>
+// thread to write again on return.  WAVE: the transform's threads and LDS
+// area lie in one wavefront; wave-level ordering replaces the barriers.
+template <int N, int S, int IL, int TS = 1, int ZQ = 0, bool MIR = false, bool CLOSE = true,
+          bool WAVE = false>
 __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
                                           const cplx *__restrict__ tw,
                                           const TwLast *tl = nullptr) {
@@ -240,7 +257,7 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
 #pragma unroll
     for (int m = 0; m < 8; ++m) w[m * IL] = v[m];
   }
-  __syncthreads();
+  lds_sync<WAVE>();
   int L = 8;
 #pragma unroll
   for (int p = 0; p < F::NMID; ++p) {
@@ -263,7 +280,7 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
 #pragma unroll
       for (int m = 0; m < 8; ++m) x[m] = lds[pad<N>(b + m * T) * IL];
     }
-    if constexpr (!IP) __syncthreads();
+    if constexpr (!IP) lds_sync<WAVE>();
     {  // three table loads; the other powers by products (error <= 2 ulp more)
       const cplx w3 = cmul(w1, w2);
       x[1] = cmul(x[1], w1); x[2] = cmul(x[2], w2); x[3] = cmul(x[3], w3);
@@ -284,7 +301,7 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
 #pragma unroll
       for (int m = 0; m < 8; ++m) lds[pad<N>(base + m * L) * IL] = x[m];
     }
-    __syncthreads();
+    lds_sync<WAVE>();
     L *= 8;
   }
   // last pass: radix RL, span N/RL; outputs j + m*N/RL land in canonical order
@@ -340,7 +357,7 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
 #pragma unroll
     for (int m = 0; m < RL; ++m) v[u + m * U] = x[m];
   }
-  if constexpr (CLOSE) __syncthreads();
+  if constexpr (CLOSE) lds_sync<WAVE>();
 }
 
 // ---------------------------------------------------------------------------
@@ -358,13 +375,6 @@ __device__ __forceinline__ void fft_canon(cplx v[8], cplx *lds, int b,
 // = N + N/16 slots, contiguous per transform.
 // ---------------------------------------------------------------------------
 
-// Orders this wave's LDS writes before its later LDS reads (compiler and
-// memory model); LDS instructions of one wave execute in order.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // The three table twiddles w^st, w^2st, w^4st of a radix-8 pass; loaded
 // ahead of the pass's LDS reads so that they are in flight meanwhile.
@@ -497,6 +507,24 @@ __device__ __forceinline__ void fft_dit(cplx v[8], cplx *P, int b, const cplx *_
                                         const TwLast *tl = nullptr) {
   if constexpr (N >= 512 && !CS) fft_dit4<N, S>(v, P, b, tw);
   else fft_canon<N, S, IL>(v, P, b, tw, tl);
+}
+
+// Packed half-spectrum loader for a row transform.  Two real rows a, b with
+// half spectra A, B (kx = 0..n/2) are transformed as one complex row
+// z = a + i b, whose full spectrum is Z[k] = A[k] + i B[k] (k <= n/2) and
+// Z[n-k] = conj(A[k]) + i conj(B[k]).  Imaginary parts at k = 0 and k = n/2
+// are dropped first, as FFTW's c2r does.
+template <int N>
+__device__ __forceinline__ cplx packed_z(int k, cplx A, cplx B, bool mirrored) {
+  if (!mirrored) {
+    if (k == 0 || k == N / 2) { A.y = 0.0; B.y = 0.0; }
+    return mk(A.x - B.y, A.y + B.x);
+  }
+  return mk(A.x + B.y, B.x - A.y);
+}
+// Slot of (y, kx) in a tiled mixed field of column pitch kp: M[y/4][kx][y%4].
+__device__ __forceinline__ size_t tix(int y, int kx, int kp) {
+  return (size_t)(y >> 2) * (size_t)(kp * 4) + (size_t)kx * 4 + (y & 3);
 }
 
 }  // namespace r2d

@@ -43,9 +43,6 @@ using namespace r2d;
 __device__ __forceinline__ int kyof(int j, int n) { return j < n / 2 ? j : j - n; }
 // half-plane index i -> ka(i): 0..n/2-1, and -n/2 for i = n/2 (src/Rot2D3C.para.f90:220-223)
 __device__ __forceinline__ int kxof(int i, int n) { return i < n / 2 ? i : i - n; }
-__device__ __forceinline__ size_t tix(int y, int kx, int kp) {
-  return (size_t)(y >> 2) * (size_t)(kp * 4) + (size_t)kx * 4 + (y & 3);
-}
 // tix as a byte offset in 32 bits, for n <= 4096 (a tiled field then holds
 // fewer than 2^24 16-byte slots): the access takes the field's uniform base
 // address plus a 32-bit lane offset instead of a 64-bit address per lane.
@@ -101,10 +98,14 @@ template <int N> constexpr bool col_twlast() { return ((R2D_COL_TWLAST) >> ilog2
 #define R2D_COL_EARLY_MAXN 512
 #endif
 
-// Lone closure runs fold the closure forcing's column pass into the first
-// stage's column launch (1) or run it as a launch of its own (0).
+// Lone closure runs fold the closure forcing's passes into the first stage's
+// launches (1) or run them as launches of their own (0); at 256 the
+// network's input pass also runs inside its first layer (R2D_CL_FOLD0).
 #ifndef R2D_CL_FOLD
 #define R2D_CL_FOLD 1
+#endif
+#ifndef R2D_CL_FOLD0
+#define R2D_CL_FOLD0 1
 #endif
 
 // Latency-bound ensemble column launches (emission split over 4 slices): the
@@ -153,19 +154,6 @@ __device__ __forceinline__ cplx apply_op(int op, cplx a, int kx, int ky, double 
   }
 }
 
-// Packed half-spectrum loader for a row transform.  Two real rows a, b with
-// half spectra A, B (kx = 0..n/2) are transformed as one complex row
-// z = a + i b, whose full spectrum is Z[k] = A[k] + i B[k] (k <= n/2) and
-// Z[n-k] = conj(A[k]) + i conj(B[k]).  Imaginary parts at k = 0 and k = n/2
-// are dropped first, as FFTW's c2r does.  Columns k >= nc are zero.
-template <int N>
-__device__ __forceinline__ cplx packed_z(int k, cplx A, cplx B, bool mirrored) {
-  if (!mirrored) {
-    if (k == 0 || k == N / 2) { A.y = 0.0; B.y = 0.0; }
-    return mk(A.x - B.y, A.y + B.x);
-  }
-  return mk(A.x + B.y, B.x - A.y);
-}
 
 // ---------------------------------------------------------------------------
 // generic transforms (full kx range, for r2d3c_c2r / r2c / poisson / cfl)
@@ -3091,6 +3079,9 @@ template <int N>
 static int run_closure_forcing_rows(r2d3c_plan *p, r2d3c_cnn *net, const double *sc,
                                     hipStream_t s);
 void r2d3c_internal_cnn_forcing_gain(const r2d3c_cnn *c, const double **gain, int *n);
+int r2d3c_internal_cnn_forward_mixed(r2d3c_cnn *c, const double *M, size_t fs, int kp, int nc,
+                                     const double *tw, double s0, double s1, double s2,
+                                     void *stream);
 
 template <int N>
 static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, hipStream_t s,
@@ -3130,12 +3121,19 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
     EnsClosure &cl = k.closures[ci];
     int rc;
     if (fold) {
-      void *in = nullptr;
-      rc = r2d3c_cnn_buffers(cl.net, &in, nullptr);
-      if (!rc)
-        rc = run_closure_inputs_mixed<N>(k.plan, k.work + (size_t)cl.member * 8 * p->fs, cl.sin,
-                                         (unsigned short *)in, s);
-      if (!rc) rc = r2d3c_cnn_forward_buffers(cl.net, 1, s);
+      const cplx *Mw = k.work + (size_t)cl.member * 8 * p->fs;
+      if (N == 256 && R2D_CL_FOLD0) {
+        // the input pass inside the network's first layer (k_conv0_mixed)
+        const double inv = 1.0 / ((double)N * N);
+        rc = r2d3c_internal_cnn_forward_mixed(cl.net, (const double *)Mw, p->fs, p->g.kp,
+                                              p->g.nc, (const double *)p->tw, cl.sin[0] * inv,
+                                              cl.sin[1] * inv, cl.sin[2] * inv, s);
+      } else {
+        void *in = nullptr;
+        rc = r2d3c_cnn_buffers(cl.net, &in, nullptr);
+        if (!rc) rc = run_closure_inputs_mixed<N>(k.plan, Mw, cl.sin, (unsigned short *)in, s);
+        if (!rc) rc = r2d3c_cnn_forward_buffers(cl.net, 1, s);
+      }
       if (!rc && !fold_row) rc = run_closure_forcing_rows<N>(k.plan, cl.net, cl.sout, s);
       if (!rc && fold_row) {
         const float *z = nullptr;

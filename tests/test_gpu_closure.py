@@ -232,14 +232,16 @@ def test_forcing_gain_scales_each_shell(cuda):
     net.close()
 
 
-def test_ensemble_closure_run_tracks_the_stepwise_closure_run(cuda):
+@pytest.mark.parametrize("n", [64, 256])
+def test_ensemble_closure_run_tracks_the_stepwise_closure_run(cuda, n):
     """A closure run inside the ensemble's step graph against ClosureRun
     (separate kernels, same network and forcing): equal up to the inputs'
-    bf16 rounding flips; a network of the wrong size is refused."""
-    print("ensemble closure run", flush=True)
+    bf16 rounding flips; a network of the wrong size is refused.  At 256 the
+    ensemble's steady steps form the network's input inside its first layer
+    and fold the forcing's passes into the first stage's launches."""
+    print("ensemble closure run", n, flush=True)
     from rot2d3c_amd import closure as L, ensemble as E, spectral as S
     from rot2d3c_amd._lib import R2D3CError
-    n = 64
     ps0, vz0 = random_state(n, 33)
     p = S.Params(1e-3, 0.1, 1, 2, 1e-3, 0.1, 1, 2, 2.0)
     net_a = L.Net(1, n, n, cuda, seed=4)
@@ -249,7 +251,7 @@ def test_ensemble_closure_run_tracks_the_stepwise_closure_run(cuda):
     vz = torch.from_numpy(vz0).to(cuda)
     ens = E.Ensemble([E.Member(ps, vz, p, 1e-3)], schedules=None)
     with pytest.raises(R2D3CError):
-        ens.set_closure(0, L.Net(1, 128, 128, cuda), sin, sout)
+        ens.set_closure(0, L.Net(1, 2 * n, 2 * n, cuda), sin, sout)
     ens.set_closure(0, net_a, sin, sout)
     spec = [dict(kf=6.0, f0=1.0, seed=5)]
     table = E.forcing_table(spec, 5)
