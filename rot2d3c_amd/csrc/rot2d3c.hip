@@ -108,6 +108,12 @@ template <int N> constexpr bool col_twlast() { return ((R2D_COL_TWLAST) >> ilog2
 #define R2D_CL_FOLD0 1
 #endif
 
+// Steady ensemble steps run as graphs of about this many steps (a multiple of
+// the closure period); 1: one graph launch per step.
+#ifndef R2D_ENS_BLOCK
+#define R2D_ENS_BLOCK 8
+#endif
+
 // Latency-bound ensemble column launches (emission split over 4 slices): the
 // slices as thread groups of one block in the last stage (1) or in every
 // stage (2); 0: the last stage unsplit and the others' slices as blocks.
@@ -3054,6 +3060,9 @@ struct r2d3c_ensemble {
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph2 = nullptr;      // one step holding the closures' last forcing
   hipGraphExec_t exec2 = nullptr;
+  hipGraph_t graphk = nullptr;      // a block of kblock steps (a multiple of period),
+  hipGraphExec_t execk = nullptr;   // closures on its steps 0, period, 2 period, ...
+  int kblock = 0;
   const ForceSet *graph_ftab = nullptr;
   bool graph_valid = false;
   // size classes of a step: 0 (default) one after another on the ensemble's
@@ -3258,7 +3267,10 @@ static bool ens_row_ticks(const r2d3c_ensemble *e) { return !e->persistent && !e
 // Captures one steady time step of every class: the classes one after another
 // or as concurrent branches, or the all-size stage launches; the forcing-table
 // step advances at the step's start (ens_row_ticks).
-static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, bool closures,
+// nsteps steps, closures evaluated on steps j with j % every == 0 (every 0:
+// none), in one graph: consecutive replays of a one-step graph leave a gap
+// between graphs on the device that the steps of one graph do not.
+static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, int nsteps, int every,
                        hipGraph_t *graph, hipGraphExec_t *exec) {
   HIPCHK(hipStreamSynchronize(e->es));  // no launch of the old graph may still run
   if (*exec) { (void)hipGraphExecDestroy(*exec); *exec = nullptr; }
@@ -3292,43 +3304,47 @@ static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, bool closures,
   }
   HIPCHK(hipStreamBeginCapture(e->es, hipStreamCaptureModeThreadLocal));
   int rc = 0;
-  // the forcing-table step: advanced by the first row launch when the
-  // per-size classes follow each other, else by a tick at the step's start
-  const bool row_tick = ens_row_ticks(e);
-  if (!row_tick) k_ens_tick<<<1, 1, 0, e->es>>>(e->d_step);
-  hipError_t he = hipGetLastError();
-  if (he == hipSuccess) he = hipEventRecord(e->fork, e->es);
-  if (e->persistent) {
-    for (int o = 4; o >= 1 && he == hipSuccess; --o) {
-      const int mode = o == 1 ? (FM_STATE | FM_DISK | FM_EMIT) : FM_EMIT;
-      if (e->persistent == 1) {
-        k_ens_stage<<<e->ps_grid, kPsT, 0, e->es>>>(e->d_ps, nk, e->d_bar, e->d_err, kPsSpin, o,
-                                                    mode);
-      } else {
-        k_ens_rows<<<round16(e->ps_rows), kPsT, 0, e->es>>>(e->d_ps, nk);
-        k_ens_cols<<<round16(e->ps_cols), kPsT, 0, e->es>>>(e->d_ps, nk, o, mode);
+  hipError_t he = hipSuccess;
+  for (int j = 0; j < nsteps && he == hipSuccess && !rc; ++j) {
+    const bool closures = every > 0 && j % every == 0;
+    // the forcing-table step: advanced by the first row launch when the
+    // per-size classes follow each other, else by a tick at the step's start
+    const bool row_tick = ens_row_ticks(e);
+    if (!row_tick) k_ens_tick<<<1, 1, 0, e->es>>>(e->d_step);
+    he = hipGetLastError();
+    if (he == hipSuccess) he = hipEventRecord(e->fork, e->es);
+    if (e->persistent) {
+      for (int o = 4; o >= 1 && he == hipSuccess; --o) {
+        const int mode = o == 1 ? (FM_STATE | FM_DISK | FM_EMIT) : FM_EMIT;
+        if (e->persistent == 1) {
+          k_ens_stage<<<e->ps_grid, kPsT, 0, e->es>>>(e->d_ps, nk, e->d_bar, e->d_err, kPsSpin, o,
+                                                      mode);
+        } else {
+          k_ens_rows<<<round16(e->ps_rows), kPsT, 0, e->es>>>(e->d_ps, nk);
+          k_ens_cols<<<round16(e->ps_cols), kPsT, 0, e->es>>>(e->d_ps, nk, o, mode);
+        }
+        he = hipGetLastError();
       }
-      he = hipGetLastError();
     }
-  }
-  // largest sizes first: with concurrent branches theirs is the longest path
-  std::vector<EnsClass *> by_size;
-  for (auto &k : e->cls) by_size.push_back(&k);
-  std::stable_sort(by_size.begin(), by_size.end(),
-                   [](const EnsClass *a, const EnsClass *b) { return a->n > b->n; });
-  for (EnsClass *kp : by_size) {
-    EnsClass &k = *kp;
-    if (he != hipSuccess || rc || e->persistent) break;
-    if (!e->concurrent) {  // one class after another, each with the whole GPU
-      rc = ens_dispatch(e, k, ftab, e->es, false, true, true, closures,
-                        row_tick && kp == by_size[0] ? e->d_step : nullptr);
-      continue;
+    // largest sizes first: with concurrent branches theirs is the longest path
+    std::vector<EnsClass *> by_size;
+    for (auto &k : e->cls) by_size.push_back(&k);
+    std::stable_sort(by_size.begin(), by_size.end(),
+                     [](const EnsClass *a, const EnsClass *b) { return a->n > b->n; });
+    for (EnsClass *kp : by_size) {
+      EnsClass &k = *kp;
+      if (he != hipSuccess || rc || e->persistent) break;
+      if (!e->concurrent) {  // one class after another, each with the whole GPU
+        rc = ens_dispatch(e, k, ftab, e->es, false, true, true, closures,
+                          row_tick && kp == by_size[0] ? e->d_step : nullptr);
+        continue;
+      }
+      he = hipStreamWaitEvent(k.st, e->fork, 0);
+      if (he == hipSuccess) rc = ens_dispatch(e, k, ftab, k.st, false, true, true, closures);
+      if (he == hipSuccess && !rc) he = hipEventRecord(k.done, k.st);
+      if (he == hipSuccess && !rc) he = hipStreamWaitEvent(e->es, k.done, 0);
     }
-    he = hipStreamWaitEvent(k.st, e->fork, 0);
-    if (he == hipSuccess) rc = ens_dispatch(e, k, ftab, k.st, false, true, true, closures);
-    if (he == hipSuccess && !rc) he = hipEventRecord(k.done, k.st);
-    if (he == hipSuccess && !rc) he = hipStreamWaitEvent(e->es, k.done, 0);
-  }
+  }  // steps of the graph
   if (he == hipSuccess && !rc) he = hipGetLastError();
   hipGraph_t g = nullptr;
   const hipError_t ee = hipStreamEndCapture(e->es, &g);
@@ -3345,13 +3361,17 @@ static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, bool closures,
 static int ens_capture_all(r2d3c_ensemble *e, const ForceSet *ftab) {
   bool any = false;
   for (auto &k : e->cls) any |= !k.closures.empty();
-  int rc = ens_capture(e, ftab, true, &e->graph, &e->exec);
+  int rc = ens_capture(e, ftab, 1, 1, &e->graph, &e->exec);
   if (!rc && any && e->period > 1) {
-    rc = ens_capture(e, ftab, false, &e->graph2, &e->exec2);
+    rc = ens_capture(e, ftab, 1, 0, &e->graph2, &e->exec2);
   } else {  // no held-forcing steps: drop a graph captured for an earlier set-up
     if (e->exec2) { (void)hipGraphExecDestroy(e->exec2); e->exec2 = nullptr; }
     if (e->graph2) { (void)hipGraphDestroy(e->graph2); e->graph2 = nullptr; }
   }
+  // the block graph: about R2D_ENS_BLOCK steps, a multiple of the period
+  e->kblock = e->period * std::max(1, R2D_ENS_BLOCK / e->period);
+  if (!rc && R2D_ENS_BLOCK > 1)
+    rc = ens_capture(e, ftab, e->kblock, any ? e->period : 0, &e->graphk, &e->execk);
   if (rc) return rc;
   e->graph_ftab = ftab;
   e->graph_valid = true;
@@ -3364,6 +3384,8 @@ static void ens_free(r2d3c_ensemble *e) {
   if (e->graph) (void)hipGraphDestroy(e->graph);
   if (e->exec2) (void)hipGraphExecDestroy(e->exec2);
   if (e->graph2) (void)hipGraphDestroy(e->graph2);
+  if (e->execk) (void)hipGraphExecDestroy(e->execk);
+  if (e->graphk) (void)hipGraphDestroy(e->graphk);
   for (auto &k : e->cls) {
     for (auto &cl : k.closures) {
       (void)hipFree(cl.fp);
@@ -3697,8 +3719,16 @@ int r2d3c_ensemble_run(r2d3c_ensemble *e, const r2d3c_forcing *forcing, int n_st
       int rc = ens_capture_all(e, ftab);
       if (rc) return rc;
     }
-    for (int st = 1; st < n_steps; ++st)
-      HIPCHK(hipGraphLaunch(st % e->period && e->exec2 ? e->exec2 : e->exec, e->es));
+    // single steps up to a multiple of the period, then blocks of kblock
+    // steps, then single steps again (the same kernels either way)
+    auto single = [&](int st) {
+      return hipGraphLaunch(st % e->period && e->exec2 ? e->exec2 : e->exec, e->es);
+    };
+    int st = 1;
+    for (; st < n_steps && st % e->period; ++st) HIPCHK(single(st));
+    for (; e->execk && n_steps - st >= e->kblock; st += e->kblock)
+      HIPCHK(hipGraphLaunch(e->execk, e->es));
+    for (; st < n_steps; ++st) HIPCHK(single(st));
   }
   HIPCHK(hipEventRecord(e->ev_out, e->es));
   HIPCHK(hipStreamWaitEvent(s, e->ev_out, 0));

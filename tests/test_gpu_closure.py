@@ -382,11 +382,13 @@ def test_output_clamp_saturates_inference_only(cuda):
     net.close()
 
 
-def test_ensemble_closure_period_tracks_the_stepwise_run(cuda):
+@pytest.mark.parametrize("nsteps", [7, 20])
+def test_ensemble_closure_period_tracks_the_stepwise_run(cuda, nsteps):
     """Closure forcing refreshed every 2nd step (held in between) inside the
     ensemble's step graphs against ClosureRun(period=2), and the held forcing
-    changes the run against refreshing every step."""
-    print("ensemble closure period", flush=True)
+    changes the run against refreshing every step.  20 steps also run the
+    steady steps as graphs of several steps."""
+    print("ensemble closure period, %d steps" % nsteps, flush=True)
     from rot2d3c_amd import closure as L, ensemble as E
     n = 64
     ps0, vz0 = random_state(n, 37)
@@ -394,7 +396,7 @@ def test_ensemble_closure_period_tracks_the_stepwise_run(cuda):
     p = S.Params(1e-3, 0.1, 1, 2, 1e-3, 0.1, 1, 2, 2.0)
     sin, sout = (0.02, 0.5, 3.0), (2.0, 0.2)
     spec = [dict(kf=6.0, f0=1.0, seed=9)]
-    table = E.forcing_table(spec, 7)
+    table = E.forcing_table(spec, nsteps)
     outs = {}
     for period in (2, 1):
         ps = torch.from_numpy(ps0).to(cuda)
@@ -403,14 +405,14 @@ def test_ensemble_closure_period_tracks_the_stepwise_run(cuda):
         net = L.Net(1, n, n, cuda, seed=8)
         ens.set_closure(0, net, sin, sout)
         ens.set_closure_period(period)
-        ens.run(7, table)
+        ens.run(nsteps, table)
         torch.cuda.synchronize()
         outs[period] = (ps.cpu().numpy(), vz.cpu().numpy())
         ens.close()
     b = (torch.from_numpy(ps0).to(cuda), torch.from_numpy(vz0).to(cuda))
     run = L.ClosureRun(b[0], b[1], p, L.Net(1, n, n, cuda, seed=8), sin, sout, guard_every=0,
                        period=2)
-    for st in range(7):
+    for st in range(nsteps):
         run.step(*table[st][0], dt=1e-3)
     torch.cuda.synchronize()
     assert rel_l2(outs[2][0], b[0].cpu().numpy()) < 1e-6

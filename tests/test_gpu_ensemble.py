@@ -43,7 +43,7 @@ def _solo(m, host, draws, nsteps, cuda):
     return ps, vz
 
 
-@pytest.mark.parametrize("nsteps", [1, 6])
+@pytest.mark.parametrize("nsteps", [1, 6, 20])
 def test_ensemble_runs_equal_solo_runs_bitwise(nsteps, cuda):
     """Mixed sizes (three classes, interleaved), per-run Omega, nu, dt and
     forcing; one step (no graph) and several (captured step graph)."""
