@@ -3,8 +3,9 @@ reference runs one simulation per MPI job).
 
 Runs of one grid size advance together: every kernel of the step covers all
 of them in one launch (the run is blockIdx.y), and the size classes follow
-each other (or run as concurrent branches) in one HIP graph per time step,
-captured once and replayed.  Two other launch forms cover every size at once
+each other (or run as concurrent branches) in HIP graphs captured once and
+replayed: one of 8 steps for the steady steps (a multiple of the closure
+period) and one-step graphs for the steps around them.  Two other launch forms cover every size at once
 (set_stage_launch): one persistent launch per RK4 stage, or one launch per
 phase.  The forcing of every step is drawn on the host before the steps start
 (rand_force's wavevectors and phases do not depend on the flow) and uploaded
