@@ -4620,22 +4620,6 @@ __device__ __forceinline__ int big_col_of_block() {
   return ((l >> 3) << 6) + (xcd << 3) + (l & 7);
 }
 
-// FFT(J1) of the kept columns -> spec[i][:] (the reference's C5)
-__global__ __launch_bounds__(kBigT) void k_big_col_fwd_j1(const cplx *__restrict__ fa, int kp,
-                                                           cplx *__restrict__ spec,
-                                                           const cplx *__restrict__ tw) {
-  constexpr int N = kBigN;
-  __shared__ cplx lds[Shape<kBigH>::LDSN];
-  const int i = big_col_of_block(), b = threadIdx.x;
-  if (i >= kBigNC) return;
-  cplx ev[8], od[8];
-  big_col_fwd_tiled(ev, od, fa, i, kp, b, lds, tw);
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    spec[(size_t)i * N + b + kBigT * q] = ev[q];
-    spec[(size_t)i * N + b + kBigT * q + kBigH] = od[q];
-  }
-}
 
 // a = a * b;  a = (a - b * c) / n4;  max of a^2 + b^2 (bit pattern, atomicMax)
 __global__ void k_real_mul(double *__restrict__ a, const double *__restrict__ b, size_t nn) {
@@ -4715,14 +4699,65 @@ __global__ void k_big_update(const cplx *ps, const cplx *vz, const cplx *c1, con
                          mk(0.0, 0.0), c, dt, o);
 }
 
-// FFT(J2) of kept column i fused with the stage update of that column (C6 =
-// FFT(J2) stays in registers; C5 = FFT(J1) from k_big_col_fwd_j1).  With
+
+// The two halves of big_update_at for one element: psi's update from C5
+// (reads c2 for C4 = d/dy c2) and vz's from C6 (reads c1 for C3 = d/dy c1),
+// the same arithmetic.
+__device__ __forceinline__ void big_update_psi_at(size_t e, int i, int j, const cplx *ps,
+                                                  const cplx *c2, cplx *o1, cplx t5,
+                                                  const StepConst &c, double dt, int o) {
+  const int n = kBigN;
+  const double kx = kxof(i, n), ky = kyof(j, n), ka2 = kx * kx + ky * ky;
+  cplx a = mk(0.0, 0.0);
+  if (ka2 <= c.g.kmax && ka2 > c.g.tiny) {
+    cplx f = mk(0.0, 0.0);
+    for (int q = 0; q < c.fp.nent; ++q)
+      if (c.fp.e[q].i == i && c.fp.e[q].j == j) f = force_value(c.fp, q, dt, c.n2);
+    if (c.fpd) f = cadd(f, c.fpd[e]);
+    const cplx s2 = c2[e];
+    const cplx t4 = mk(-ky * s2.y, ky * s2.x);
+    const double tmp1 = dt / (double)o, tmp2 = 1.0 / ka2;
+    const double tmp3 = 1.0 + (c.nu * ipow(ka2, c.nn) + c.hnu * ipow(tmp2, c.mm)) * tmp1;
+    cplx t = mk((-t5.x + c.two_omega * t4.x) * tmp2 + f.x, (-t5.y + c.two_omega * t4.y) * tmp2 + f.y);
+    t = cadd(ps[e], cscale(t, tmp1));
+    a = mk(t.x / tmp3, t.y / tmp3);
+  }
+  o1[e] = a;
+}
+__device__ __forceinline__ void big_update_vz_at(size_t e, int i, int j, const cplx *vz,
+                                                 const cplx *c1, cplx *o2, cplx t6,
+                                                 const StepConst &c, double dt, int o) {
+  const int n = kBigN;
+  const double kx = kxof(i, n), ky = kyof(j, n), ka2 = kx * kx + ky * ky;
+  cplx b = mk(0.0, 0.0);
+  if (ka2 <= c.g.kmax && ka2 > c.g.tiny) {
+    cplx g = mk(0.0, 0.0);
+    for (int q = 0; q < c.fz.nent; ++q)
+      if (c.fz.e[q].i == i && c.fz.e[q].j == j) g = force_value(c.fz, q, dt, c.n2);
+    if (c.fzd) g = cadd(g, c.fzd[e]);
+    const cplx s1 = c1[e];
+    const cplx t3 = mk(-ky * s1.y, ky * s1.x);
+    const double tmp1 = dt / (double)o, tmp2 = 1.0 / ka2;
+    const double tmp3 = 1.0 + (c.nuv * ipow(ka2, c.nnv) + c.hnuv * ipow(tmp2, c.mmv)) * tmp1;
+    cplx t = mk(t6.x + c.two_omega * t3.x + g.x, t6.y + c.two_omega * t3.y + g.y);
+    t = cadd(vz[e], cscale(t, tmp1));
+    b = mk(t.x / tmp3, t.y / tmp3);
+  }
+  o2[e] = b;
+}
+
+// Both brackets' column forwards with the stage update (the fused form's
+// last kernel of a stage): column i of FFT(J1) (fa) carries psi's update,
+// then column i of FFT(J2) (fb) vz's, so neither spectrum goes through HBM.
+// psi's update writes o1 before vz's reads c1, so o1 must not alias c1 (the
+// stages alternate between two pairs of stage fields).  Each transform's
+// output goes half by half to LDS slots only its thread reads back.  With
 // zero_tail the grid also covers the columns nc..n/2, which it sets to zero
 // in o1, o2 (the last stage, which writes the state: the update mask).
-__global__ __launch_bounds__(kBigT) void k_big_col_update(
-    const cplx *__restrict__ fb, int kp, const cplx *ps, const cplx *vz, const cplx *c1, const cplx *c2,
-    cplx *o1, cplx *o2, const cplx *__restrict__ c5, StepConst c, const double *__restrict__ dtp,
-    int o, int zero_tail, const cplx *__restrict__ tw) {
+__global__ __launch_bounds__(kBigT) void k_big_col_update2(
+    const cplx *__restrict__ fa, const cplx *__restrict__ fb, int kp, const cplx *ps,
+    const cplx *vz, const cplx *c1, const cplx *c2, cplx *o1, cplx *o2, StepConst c,
+    const double *__restrict__ dtp, int o, int zero_tail, const cplx *__restrict__ tw) {
   constexpr int N = kBigN;
   __shared__ cplx lds[Shape<kBigH>::LDSN];
   const int i = big_col_of_block(), b = threadIdx.x;
@@ -4737,21 +4772,25 @@ __global__ __launch_bounds__(kBigT) void k_big_col_update(
     }
     return;
   }
-  cplx ev[8], od[8];
-  big_col_fwd_tiled(ev, od, fb, i, kp, b, lds, tw);
   const double dt = dtp ? *dtp : c.dt_fixed;
-  // The transform's closing barrier has freed the LDS.  Each half of the
-  // column goes to slots that only this thread reads back, so the element
-  // loop runs one element at a time (unrolled it would spill) with no barrier.
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-#pragma unroll
-    for (int q = 0; q < 8; ++q) lds[b + kBigT * q] = h ? od[q] : ev[q];
 #pragma unroll 1
-    for (int q = 0; q < 8; ++q) {
-      const int j = b + kBigT * q + h * kBigH;
-      big_update_at<true>((size_t)i * N + j, i, j, ps, vz, c1, c2, o1, o2, c5, nullptr,
-                          lds[b + kBigT * q], c, dt, o);
+  for (int f = 0; f < 2; ++f) {
+    cplx ev[8], od[8];
+    if (f) __syncthreads();  // every thread's psi slots read before the transform reuses them
+    int bb = b;  // opaque copy: no index math hoisted across the two transforms
+    asm volatile("" : "+v"(bb));
+    big_col_fwd_tiled(ev, od, f ? fb : fa, i, kp, bb, lds, tw);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) lds[b + kBigT * q] = h ? od[q] : ev[q];
+#pragma unroll 1
+      for (int q = 0; q < 8; ++q) {
+        const int j = b + kBigT * q + h * kBigH;
+        const size_t e = (size_t)i * N + j;
+        if (f == 0) big_update_psi_at(e, i, j, ps, c2, o1, lds[b + kBigT * q], c, dt, o);
+        else big_update_vz_at(e, i, j, vz, c1, o2, lds[b + kBigT * q], c, dt, o);
+      }
     }
   }
 }
@@ -4862,8 +4901,10 @@ static int big_step(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
     cplx *MX = p->mfull, *MY = MX + p->fsf, *MS = MY + p->fsf, *FA = MS, *FB = FA + p->fs;
     const double n4 = (double)kBigN * kBigN * (double)kBigN * kBigN;
     const bool cfl = dt_mode == R2D3C_DT_CFL;
+    // stage fields: C1, C2 after stages 4 and 2, C5, C6 after stage 3 (the
+    // merged update must not write the fields it reads)
     for (int o = 4; o >= 1; --o) {
-      const cplx *a = o == 4 ? psi : C1, *v = o == 4 ? vz : C2;
+      const cplx *a = o == 4 ? psi : (o == 2 ? C5 : C1), *v = o == 4 ? vz : (o == 2 ? C6 : C2);
       if ((rc = big_c2r(p, a, OP_LDX, R1, s, nc, MS)) || (rc = big_c2r(p, a, OP_LDY, R2, s, nc, MS)) ||
           (rc = big_c2r(p, v, OP_DX, R3, s, nc, MS)) || (rc = big_c2r(p, v, OP_DY, R4, s, nc, MS)))
         return rc;
@@ -4876,11 +4917,10 @@ static int big_step(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
                                              o == 4 && cfl ? p->maxbits : nullptr);
       if (o == 4 && cfl)
         k_dt_from_max<<<1, 1, 0, s>>>(p->maxbits, make_dt_const(kBigN, q, dt_or_cfl), dtp);
-      k_big_col_fwd_j1<<<(kBigNC + 63) / 64 * 64, kBigT, 0, s>>>(FA, p->g.kp, C5, p->tw);
       const int ncol = o == 1 ? kBigN / 2 + 1 : kBigNC;
-      k_big_col_update<<<(ncol + 63) / 64 * 64, kBigT, 0, s>>>(
-          FB, p->g.kp, psi, vz, a, v, o == 1 ? psi : C1, o == 1 ? vz : C2, C5, c, dtp, o, o == 1,
-          p->tw);
+      cplx *o1 = o == 1 ? psi : (o == 3 ? C5 : C1), *o2 = o == 1 ? vz : (o == 3 ? C6 : C2);
+      k_big_col_update2<<<(ncol + 63) / 64 * 64, kBigT, 0, s>>>(
+          FA, FB, p->g.kp, psi, vz, a, v, o1, o2, c, dtp, o, o == 1, p->tw);
     }
     LAUNCHCHK("step 16384");
     return 0;
