@@ -4379,6 +4379,144 @@ __global__ __launch_bounds__(kBigT) void k_big_row_c2r(const cplx *__restrict__ 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Split column inverse.  A column's 16384-point transform is X[k] = E[k] +
+// w^k O[k], X[k + 8192] = E[k] - w^k O[k] (w = exp(S 2 pi i / 16384)) with E,
+// O the 8192-point transforms of the even and the odd samples.  k_big_col_inv_h
+// computes E or O in a block of 512 threads (16 points per thread, two
+// 4096-point halves through 70 KB of LDS), so two such blocks are resident per
+// CU and one block's loads and stores overlap the other's transform (one
+// 1024-thread block per CU, as k_big_col_inv runs, leaves them in turn).  The
+// halves go to a field of the same size in the layout hix: at each column the
+// 64-byte chunk of E[4t..4t+3] is followed by that of O[4t..4t+3], so each
+// block writes whole chunks and E[k], O[k] share one 128-byte line.  The row
+// pass that reads it (k_big_row_c2r_h) transforms rows k and k + 8192 as its
+// packed pair, forming both from E[k] and O[k] with one twiddle per row.
+// ---------------------------------------------------------------------------
+constexpr int kBigT2 = 512;
+// slot of half p, row k < 8192, column kx in a split field of column pitch kp
+__device__ __forceinline__ size_t hix(int k, int p, int kx, int kp) {
+  return (size_t)(k >> 2) * (size_t)(kp * 8) + (size_t)kx * 8 + p * 4 + (k & 3);
+}
+
+// In: ev[q] = u[2(b + 512 q)], od[q] = u[2(b + 512 q) + 1] (u of length 8192).
+// Out: ev[q] = U[b + 512 q], od[q] = U[b + 512 q + 4096] (sign S).  tw is the
+// 16384-point table.
+template <int S>
+__device__ __forceinline__ void fft8k(cplx ev[8], cplx od[8], cplx *lds, int b,
+                                      const cplx *__restrict__ tw) {
+  constexpr int H = 4096;
+  const TwLast tls = tw_last<H, 4>(tw, b);
+  int b1 = b, b2 = b;
+  asm volatile("" : "+v"(b1));
+  fft_canon<H, S, 1, 4>(ev, lds, b1, tw, &tls);
+  asm volatile("" : "+v"(b2));
+  fft_canon<H, S, 1, 4>(od, lds, b2, tw, &tls);
+  // combine factors exp(S 2 pi i (b + 512 q) / 8192): the loaded b and
+  // b + 512 times exp(S i pi u / 4), q = 2u or 2u + 1
+  const cplx c0 = twid<S>(tw, 2 * b), c1 = twid<S>(tw, 2 * (b + kBigT2));
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const cplx w = rot8<S>((q & 1) ? c1 : c0, q >> 1);
+    const cplx t = cmul(od[q], w), e = ev[q];
+    ev[q] = cadd(e, t);
+    od[q] = csub(e, t);
+  }
+}
+
+// half p (0: even samples, 1: odd) of the column inverse of S[i][:] (with op,
+// times scale) -> H[hix(k, p, i)], k < 8192.  Block 8 l +
+// x (x = 0..7) takes column 8 (l / 2) + x, half l % 2: both halves of a column
+// go to one XCD back to back, so the second finds the first's lines (each
+// 64-byte line holds samples of both) in that XCD's L2.
+__global__ __launch_bounds__(kBigT2) __attribute__((amdgpu_waves_per_eu(4)))
+void k_big_col_inv_h(const cplx *__restrict__ spec, cplx *__restrict__ mix, int op, double scale,
+                     Geo g, const cplx *__restrict__ tw, int ncols) {
+  constexpr int N = kBigN;
+  __shared__ cplx lds[Shape<4096>::LDSN];
+  const int xcd = blockIdx.x & 7, l = blockIdx.x >> 3, p = l & 1;
+  const int i = ((l >> 1) << 3) + xcd, b = threadIdx.x;
+  if (i >= ncols) return;  // whole block, before any barrier
+  const int kx = kxof(i, N);
+  cplx ev[8], od[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int j = 4 * (b + kBigT2 * q) + p;
+    ev[q] = cscale(apply_op(op, spec[(size_t)i * N + j], kx, kyof(j, N), g.kmax, g.tiny), scale);
+    od[q] = cscale(apply_op(op, spec[(size_t)i * N + j + 2], kx, kyof(j + 2, N), g.kmax, g.tiny),
+                   scale);
+  }
+  fft8k<+1>(ev, od, lds, b, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int k = b + kBigT2 * q;
+    mix[hix(k, p, i, g.kpf)] = ev[q];
+    mix[hix(k + 4096, p, i, g.kpf)] = od[q];
+  }
+}
+
+// Packed inputs of rows y and y + 8192 from a split mixed field H (E[y] and
+// O[y] at hix(y, 0 or 1, kx)): A = E + w O, B = E - w O with
+// w = exp(2 pi i y / 16384), then as big_load_packed.
+template <int KC>
+__device__ __forceinline__ void big_load_packed_split(cplx ev[8], cplx od[8],
+                                                      const cplx *__restrict__ H, int y, cplx w,
+                                                      int b, int kp) {
+  constexpr int N = kBigN, ncols = KC;
+#pragma unroll
+  for (int q0 = 0; q0 < 8; q0 += 2) {
+    cplx ra[4], rb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = q0 + (u >> 1), par = u & 1;
+      if (big_kk_min(q, par) >= ncols) continue;  // no thread reads this slot
+      const int k = 2 * (b + kBigT * q) + par;
+      const int kk = k > N / 2 ? N - k : k;
+      const int kl = (big_kk_max(q, par) < ncols || kk < ncols) ? kk : 0;
+      ra[u] = H[hix(y, 0, kl, kp)];
+      rb[u] = H[hix(y, 1, kl, kp)];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = q0 + (u >> 1), par = u & 1;
+      const int k = 2 * (b + kBigT * q) + par;
+      const bool mir = k > N / 2;
+      const int kk = mir ? N - k : k;
+      const bool in = big_kk_min(q, par) < ncols && (big_kk_max(q, par) < ncols || kk < ncols);
+      const cplx t = cmul(rb[u], w);
+      const cplx zz = in ? packed_z<N>(k, cadd(ra[u], t), csub(ra[u], t), mir) : mk(0.0, 0.0);
+      if (u & 1) od[q0 + (u >> 1)] = zz;
+      else ev[q0 + (u >> 1)] = zz;
+    }
+    __builtin_amdgcn_sched_barrier(0);  // the next group's loads stay below
+  }
+}
+
+// rows y, y + 8192 of M, from the split field H -> R[y][:], R[y + 8192][:] as
+// one packed transform; columns kx >= KC are taken as zero (not read).  Block
+// 8 l + x takes y = 4 (8 (l / 4) + x) + l % 4: the four blocks that share the
+// 128-byte lines of H's rows 4t..4t+3 run back to back on one XCD.
+template <int KC>
+__global__ __launch_bounds__(kBigT) void k_big_row_c2r_h(const cplx *__restrict__ H,
+                                                          double *__restrict__ out, Geo g,
+                                                          const cplx *__restrict__ tw) {
+  constexpr int N = kBigN;
+  __shared__ cplx lds[Shape<kBigH>::LDSN];
+  const int xcd = blockIdx.x & 7, l = blockIdx.x >> 3;
+  const int b = threadIdx.x, y = 4 * (((l >> 2) << 3) + xcd) + (l & 3);
+  cplx ev[8], od[8];
+  big_load_packed_split<KC>(ev, od, H, y, twid<+1>(tw, y), b, g.kpf);
+  fft16k<+1>(ev, od, lds, b, tw);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int x = b + kBigT * q;
+    out[(size_t)y * N + x] = ev[q].x;
+    out[(size_t)(y + kBigH) * N + x] = ev[q].y;
+    out[(size_t)y * N + x + kBigH] = od[q].x;
+    out[(size_t)(y + kBigH) * N + x + kBigH] = od[q].y;
+  }
+}
+
 // rows y, y+1 of R as one packed forward transform: Z[y/2][k], k = 0..N-1
 __global__ __launch_bounds__(kBigT) void k_big_row_r2c(const double *__restrict__ in,
                                                         cplx *__restrict__ zrows,
@@ -4810,6 +4948,19 @@ static int big_c2r(r2d3c_plan *p, const cplx *in, int op, double *out, hipStream
   LAUNCHCHK("c2r 16384");
   return 0;
 }
+// The step's dealiased c2r through the split column inverse: the column
+// halves into mix, rows y and y + 8192 combined in the row pass.
+#ifndef R2D_BIG_SPLIT
+#define R2D_BIG_SPLIT 1
+#endif
+static int big_c2r_split(r2d3c_plan *p, const cplx *in, int op, double *out, hipStream_t s,
+                         cplx *mix) {
+  k_big_col_inv_h<<<16 * ((kBigNC + 7) / 8), kBigT2, 0, s>>>(in, mix, op, 1.0, p->g, p->tw,
+                                                             kBigNC);
+  k_big_row_c2r_h<kBigNC><<<kBigN / 2, kBigT, 0, s>>>(mix, out, p->g, p->tw);
+  LAUNCHCHK("c2r 16384 (split)");
+  return 0;
+}
 // columns kx < ncols of r2c((a b - c d) / n^4)
 static int big_r2c_jac(r2d3c_plan *p, const double *a, const double *b, const double *c,
                        const double *d, cplx *out, int ncols, hipStream_t s) {
@@ -4905,9 +5056,18 @@ static int big_step(r2d3c_plan *p, const r2d3c_params *q, cplx *psi, cplx *vz,
     // merged update must not write the fields it reads)
     for (int o = 4; o >= 1; --o) {
       const cplx *a = o == 4 ? psi : (o == 2 ? C5 : C1), *v = o == 4 ? vz : (o == 2 ? C6 : C2);
-      if ((rc = big_c2r(p, a, OP_LDX, R1, s, nc, MS)) || (rc = big_c2r(p, a, OP_LDY, R2, s, nc, MS)) ||
-          (rc = big_c2r(p, v, OP_DX, R3, s, nc, MS)) || (rc = big_c2r(p, v, OP_DY, R4, s, nc, MS)))
+      if (R2D_BIG_SPLIT) {
+        if ((rc = big_c2r_split(p, a, OP_LDX, R1, s, MS)) ||
+            (rc = big_c2r_split(p, a, OP_LDY, R2, s, MS)) ||
+            (rc = big_c2r_split(p, v, OP_DX, R3, s, MS)) ||
+            (rc = big_c2r_split(p, v, OP_DY, R4, s, MS)))
+          return rc;
+      } else if ((rc = big_c2r(p, a, OP_LDX, R1, s, nc, MS)) ||
+                 (rc = big_c2r(p, a, OP_LDY, R2, s, nc, MS)) ||
+                 (rc = big_c2r(p, v, OP_DX, R3, s, nc, MS)) ||
+                 (rc = big_c2r(p, v, OP_DY, R4, s, nc, MS))) {
         return rc;
+      }
       k_big_col_inv<<<kBigNC, kBigT, 0, s>>>(a, MX, OP_DX, 1.0, p->g, p->tw);
       k_big_col_inv<<<kBigNC, kBigT, 0, s>>>(a, MY, OP_DY, 1.0, p->g, p->tw);
       if (o == 4 && cfl) k_set_u64<<<1, 1, 0, s>>>(p->maxbits, 0ull);
