@@ -308,7 +308,8 @@ def ensemble_bench(steps, dev, tuned, warmup=10):
             "frac_of_peak": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
             "model_76S_frac_of_peak": round(b76 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
             "finite": finite, "schedules": schedules,
-            "size_classes": "concurrent graph branches" if ens.concurrent else "one after another",
+            "size_classes": {0: "one after another", 1: "concurrent graph branches",
+                             2: "each on a stream of its own"}[ens.concurrent],
             "row_split": row_split, "stage_launch": names[form],
             "stage_launch_forms_ms_per_step": forms_ms}
 

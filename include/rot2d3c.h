@@ -322,8 +322,9 @@ typedef struct r2d3c_ensemble r2d3c_ensemble;
 
 /* Groups the runs by grid size and allocates each run's workspace.  Runs of
  * one size advance together: one launch per kernel and stage covers all of
- * them, and the sizes follow each other (or run as concurrent branches,
- * r2d3c_ensemble_set_concurrent) in one captured HIP graph per time step.
+ * them; each size runs on a stream of its own, replaying captured HIP
+ * graphs of its steps (or the sizes follow each other in one graph per time
+ * step, r2d3c_ensemble_set_concurrent).
  * Each run's result equals r2d3c_run of that run alone (fixed dt, same
  * forcing) bit for bit. */
 int r2d3c_ensemble_create(r2d3c_ensemble **ens, const r2d3c_member *members, int count);
@@ -332,9 +333,12 @@ int r2d3c_ensemble_destroy(r2d3c_ensemble *ens);
 int r2d3c_ensemble_set_schedule(r2d3c_ensemble *ens, int n, int row_rows, int row_blocks_per_cu);
 /* r2d3c_plan_set_column_radix for the runs of size n. */
 int r2d3c_ensemble_set_column_radix(r2d3c_ensemble *ens, int n, int form);
-/* How the size classes of a time step are ordered: 0 (default) one after
- * another, each class's launches having the whole GPU; 1 as concurrent
- * branches of the step graph.  Results are identical either way. */
+/* How the size classes of a time step are ordered: 0 one after another,
+ * each class's launches having the whole GPU; 1 as concurrent branches of
+ * the step graph; 2 (default) each class on a stream of its own for the
+ * whole run call, replaying graphs of its own steps (classes then overlap on
+ * the device; ensembles with a closure run, and the all-size stage launches,
+ * take 0 instead).  Results are identical in every mode. */
 int r2d3c_ensemble_set_concurrent(r2d3c_ensemble *ens, int concurrent);
 /* How a time step's stages are launched: 0 (default) the per-size kernels
  * (a row and a fused column launch per size and stage); 1 one persistent

@@ -3042,6 +3042,11 @@ struct EnsClass {
   StepConst c0{};                   // geometry shared by the class (row kernel)
   hipStream_t st = nullptr;         // branch stream of this class in the step graph
   hipEvent_t done = nullptr;
+  // own streams (concurrent 2): the forcing-table step of this class and its
+  // graphs of one step and of kblock steps, replayed on st
+  int *d_step = nullptr;
+  hipGraph_t g1 = nullptr, gk = nullptr;
+  hipGraphExec_t x1 = nullptr, xk = nullptr;
 };
 
 struct r2d3c_ensemble {
@@ -3065,9 +3070,10 @@ struct r2d3c_ensemble {
   int kblock = 0;
   const ForceSet *graph_ftab = nullptr;
   bool graph_valid = false;
-  // size classes of a step: 0 (default) one after another on the ensemble's
-  // stream, 1 as concurrent branches of the step graph
-  int concurrent = 0;
+  // size classes of a step: 0 one after another on the ensemble's stream, 1
+  // as concurrent branches of the step graph, 2 (default) each class on its
+  // own stream for a whole run call (no closures; else as 0)
+  int concurrent = 2;
   // stage launch form: 0 per-size kernels; 1 one persistent launch per RK4
   // stage for every class (k_ens_stage); 2 one launch per phase for every
   // class (k_ens_rows, k_ens_cols).  The class table, the grid barrier, its
@@ -3078,6 +3084,17 @@ struct r2d3c_ensemble {
   int *d_err = nullptr;
   int ps_grid = 0, ps_rows = 0, ps_cols = 0;
 };
+
+// The size-class mode in effect: own streams (2) only with the per-size
+// kernels and for ensembles without closures (their graphs alternate steps
+// with and without the network); else one class after another.
+static int ens_mode(const r2d3c_ensemble *e) {
+  if (e->concurrent != 2) return e->concurrent;
+  if (e->persistent) return 0;
+  for (const auto &k : e->cls)
+    if (!k.closures.empty()) return 0;
+  return 2;
+}
 
 // tick: the step counter this class's first row launch advances (the first
 // class of a step whose classes follow each other), or NULL.
@@ -3098,7 +3115,7 @@ static int ens_class_step(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, 
   const r2d3c_plan *p = k.plan;
   const int nc = p->g.nc, ncp = (nc + 1) & ~1;
   const cplx *tw = p->tw;
-  EnsArgs a{k.d_mem, ftab, e->d_step, e->count, k.first};
+  EnsArgs a{k.d_mem, ftab, ens_mode(e) == 2 ? k.d_step : e->d_step, e->count, k.first};
   const bool cs = N >= 512 && p->col_cs;  // the columns' radix schedule
   if (first) {
     if (cs) k_ens_colinv<N, true><<<dim3(round16(ncp + nc), k.count), N / 8, 0, s>>>(a, tw);
@@ -3262,7 +3279,28 @@ static int ens_dispatch(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, hi
 
 // Whether a step's first row launch advances the forcing-table step (the
 // per-size classes one after another) rather than a tick kernel.
-static bool ens_row_ticks(const r2d3c_ensemble *e) { return !e->persistent && !e->concurrent; }
+static bool ens_row_ticks(const r2d3c_ensemble *e) {
+  return !e->persistent && ens_mode(e) != 1;
+}
+
+// Mode 2: nsteps steady steps of class k alone, captured on its own stream;
+// its first row launch of each step advances its own forcing-table step.
+static int ens_capture_cls(r2d3c_ensemble *e, EnsClass &k, const ForceSet *ftab, int nsteps,
+                           hipGraph_t *graph, hipGraphExec_t *exec) {
+  if (*exec) { (void)hipGraphExecDestroy(*exec); *exec = nullptr; }
+  if (*graph) { (void)hipGraphDestroy(*graph); *graph = nullptr; }
+  HIPCHK(hipStreamBeginCapture(k.st, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int j = 0; j < nsteps && !rc; ++j)
+    rc = ens_dispatch(e, k, ftab, k.st, false, true, true, false, k.d_step);
+  hipGraph_t g = nullptr;
+  const hipError_t ee = hipStreamEndCapture(k.st, &g);
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (ee != hipSuccess) return hip_err(ee, "hipStreamEndCapture");
+  *graph = g;
+  HIPCHK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+  return 0;
+}
 
 // Captures one steady time step of every class: the classes one after another
 // or as concurrent branches, or the all-size stage launches; the forcing-table
@@ -3334,7 +3372,7 @@ static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, int nsteps, int 
     for (EnsClass *kp : by_size) {
       EnsClass &k = *kp;
       if (he != hipSuccess || rc || e->persistent) break;
-      if (!e->concurrent) {  // one class after another, each with the whole GPU
+      if (ens_mode(e) != 1) {  // one class after another, each with the whole GPU
         rc = ens_dispatch(e, k, ftab, e->es, false, true, true, closures,
                           row_tick && kp == by_size[0] ? e->d_step : nullptr);
         continue;
@@ -3361,6 +3399,19 @@ static int ens_capture(r2d3c_ensemble *e, const ForceSet *ftab, int nsteps, int 
 static int ens_capture_all(r2d3c_ensemble *e, const ForceSet *ftab) {
   bool any = false;
   for (auto &k : e->cls) any |= !k.closures.empty();
+  if (ens_mode(e) == 2) {
+    HIPCHK(hipStreamSynchronize(e->es));  // and so every class stream's last replay
+    e->graph_valid = false;
+    e->kblock = std::max(1, R2D_ENS_BLOCK);
+    for (auto &k : e->cls) {
+      int rc = ens_capture_cls(e, k, ftab, 1, &k.g1, &k.x1);
+      if (!rc && e->kblock > 1) rc = ens_capture_cls(e, k, ftab, e->kblock, &k.gk, &k.xk);
+      if (rc) return rc;
+    }
+    e->graph_ftab = ftab;
+    e->graph_valid = true;
+    return 0;
+  }
   int rc = ens_capture(e, ftab, 1, 1, &e->graph, &e->exec);
   if (!rc && any && e->period > 1) {
     rc = ens_capture(e, ftab, 1, 0, &e->graph2, &e->exec2);
@@ -3394,6 +3445,11 @@ static void ens_free(r2d3c_ensemble *e) {
     if (k.d_mem) (void)hipFree(k.d_mem);
     if (k.work) (void)hipFree(k.work);
     if (k.d_om) (void)hipFree(k.d_om);
+    if (k.x1) (void)hipGraphExecDestroy(k.x1);
+    if (k.g1) (void)hipGraphDestroy(k.g1);
+    if (k.xk) (void)hipGraphExecDestroy(k.xk);
+    if (k.gk) (void)hipGraphDestroy(k.gk);
+    if (k.d_step) (void)hipFree(k.d_step);
     if (k.st) (void)hipStreamDestroy(k.st);
     if (k.done) (void)hipEventDestroy(k.done);
     if (k.plan) r2d3c_plan_destroy(k.plan);
@@ -3468,6 +3524,7 @@ int r2d3c_ensemble_create(r2d3c_ensemble **out, const r2d3c_member *members, int
     if ((he = hipMemcpy(k.d_om, om.data(), sizeof(double) * k.count, hipMemcpyHostToDevice)) != hipSuccess) break;
     if ((he = hipStreamCreateWithFlags(&k.st, hipStreamNonBlocking)) != hipSuccess) break;
     if ((he = hipEventCreateWithFlags(&k.done, hipEventDisableTiming)) != hipSuccess) break;
+    if ((he = hipMalloc(&k.d_step, sizeof(int))) != hipSuccess) break;
   }
   if (!rc && he == hipSuccess) he = hipStreamCreateWithFlags(&e->es, hipStreamNonBlocking);
   if (!rc && he == hipSuccess) he = hipEventCreateWithFlags(&e->ev_in, hipEventDisableTiming);
@@ -3562,7 +3619,7 @@ int r2d3c_ensemble_sync(r2d3c_ensemble *e) {
 
 int r2d3c_ensemble_set_concurrent(r2d3c_ensemble *e, int concurrent) {
   NEED(e);
-  if (concurrent != 0 && concurrent != 1) return set_err(R2D3C_EINVAL, "concurrent: 0 or 1");
+  if (concurrent < 0 || concurrent > 2) return set_err(R2D3C_EINVAL, "concurrent: 0, 1 or 2");
   if (e->concurrent != concurrent) e->graph_valid = false;
   e->concurrent = concurrent;
   return 0;
@@ -3705,6 +3762,34 @@ int r2d3c_ensemble_run(r2d3c_ensemble *e, const r2d3c_forcing *forcing, int n_st
   // step 0 during the first step: with row ticks its first row launch
   // advances the counter from -1
   const bool row_tick = ens_row_ticks(e);
+  if (ens_mode(e) == 2) {
+    // every class on its own stream, from its own graphs, for the whole call
+    for (auto &k : e->cls) k_set_i32<<<1, 1, 0, e->es>>>(k.d_step, -1);
+    for (auto &k : e->cls) {
+      int rc = ens_dispatch(e, k, ftab, e->es, true, false, n_steps > 1, true, k.d_step);
+      if (rc) return rc;
+    }
+    if (n_steps > 1) {
+      LAUNCHCHK("ensemble first step");
+      if (!e->graph_valid || e->graph_ftab != ftab) {
+        int rc = ens_capture_all(e, ftab);
+        if (rc) return rc;
+      }
+      HIPCHK(hipEventRecord(e->fork, e->es));
+      for (auto &k : e->cls) {
+        HIPCHK(hipStreamWaitEvent(k.st, e->fork, 0));
+        int st = 1;
+        for (; k.xk && n_steps - st >= e->kblock; st += e->kblock)
+          HIPCHK(hipGraphLaunch(k.xk, k.st));
+        for (; st < n_steps; ++st) HIPCHK(hipGraphLaunch(k.x1, k.st));
+        HIPCHK(hipEventRecord(k.done, k.st));
+        HIPCHK(hipStreamWaitEvent(e->es, k.done, 0));
+      }
+    }
+    HIPCHK(hipEventRecord(e->ev_out, e->es));
+    HIPCHK(hipStreamWaitEvent(s, e->ev_out, 0));
+    return 0;
+  }
   k_set_i32<<<1, 1, 0, e->es>>>(e->d_step, row_tick ? -1 : 0);
   // the first step reads the state's own columns and zeroes what lies
   // outside the update mask, as the first step of r2d3c_run does

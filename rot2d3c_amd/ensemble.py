@@ -2,8 +2,9 @@
 reference runs one simulation per MPI job).
 
 Runs of one grid size advance together: every kernel of the step covers all
-of them in one launch (the run is blockIdx.y), and the size classes follow
-each other (or run as concurrent branches) in HIP graphs captured once and
+of them in one launch (the run is blockIdx.y), and each size class runs on a
+stream of its own, so that the classes overlap on the device (or the classes
+follow each other, set_concurrent), from HIP graphs captured once and
 replayed: one of 8 steps for the steady steps (a multiple of the closure
 period) and one-step graphs for the steps around them.  Two other launch forms cover every size at once
 (set_stage_launch): one persistent launch per RK4 stage, or one launch per
@@ -54,7 +55,7 @@ class Ensemble:
                   "r2d3c_ensemble_create")
         self.handle = h
         self.sizes = sorted({m.n for m in self.members})
-        self.concurrent = 0
+        self.concurrent = 2
         self.stage_launch = 0
         self.row_split = {n: -1 for n in self.sizes}
         self.schedules = {n: autotune.DEFAULT for n in self.sizes}
@@ -113,15 +114,17 @@ class Ensemble:
 
         best_ms = timed()
         # the order of the size classes in a step: one after another (the
-        # default) or as concurrent graph branches
+        # default), as concurrent graph branches, or each on its own stream
         if len(self.sizes) > 1:
             keep = self.concurrent
-            self.set_concurrent(1 - keep)
-            ms = timed()
-            if ms < best_ms:
-                best_ms = ms
-            else:
-                self.set_concurrent(keep)
+            for mode in (0, 1, 2):
+                if mode == keep:
+                    continue
+                self.set_concurrent(mode)
+                ms = timed()
+                if ms < best_ms:
+                    best_ms, keep = ms, mode
+            self.set_concurrent(keep)
         for n in sorted(self.sizes, reverse=True):
             # every schedule, and with the non-persistent row kernel at
             # n <= 512 both row forms (the parallel field pairs, automatic
@@ -202,8 +205,10 @@ class Ensemble:
             check(_lib.lib().r2d3c_ensemble_sync(self.handle), "r2d3c_ensemble_sync")
 
     def set_concurrent(self, concurrent):
-        """Size classes of a step one after another (0, the default) or as
-        concurrent branches of the step graph (1); results are identical."""
+        """Size classes of a step one after another (0), as concurrent
+        branches of the step graph (1), or each on a stream of its own for a
+        whole run call (2, the default; ensembles with a closure run and the
+        all-size stage launches take 0); results are identical."""
         check(_lib.lib().r2d3c_ensemble_set_concurrent(self.handle, int(concurrent)),
               "r2d3c_ensemble_set_concurrent")
         self.concurrent = int(concurrent)

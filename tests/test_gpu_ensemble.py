@@ -410,25 +410,28 @@ def test_row_split_forms_agree(n, cuda):
 
 
 def test_ensemble_order_of_size_classes_does_not_change_results(cuda):
-    """Size classes one after another (the default) and as concurrent graph
-    branches give the same bits, each run equal to its solo run."""
+    """Size classes one after another, as concurrent graph branches and each
+    on its own stream (the default) give the same bits, each run equal to
+    its solo run (12 steps: the first, a replay of the graph of 8 steps, three
+    replays of the one-step graph)."""
     print("ensemble class order", flush=True)
     from rot2d3c_amd import ensemble as E
     spec = _spec([32, 64, 32, 128, 64], seed=13)
-    nsteps = 4
+    nsteps = 12
     table = E.forcing_table(spec, nsteps)
     outs = []
-    for conc in (0, 1):
+    for conc in (0, 1, 2):
         ms, host = _members(spec, cuda)
         ens = E.Ensemble(ms, schedules=None)
-        assert ens.concurrent == 0
+        assert ens.concurrent == 2
         ens.set_concurrent(conc)
         ens.run(nsteps, table)
         torch.cuda.synchronize()
         outs.append([(m.ps.clone(), m.vz.clone()) for m in ms])
         ens.close()
-    for r, ((a0, b0), (a1, b1)) in enumerate(zip(*outs)):
+    for r, ((a0, b0), (a1, b1), (a2, b2)) in enumerate(zip(*outs)):
         assert torch.equal(a0, a1) and torch.equal(b0, b1), r
+        assert torch.equal(a0, a2) and torch.equal(b0, b2), r
     ms, host = _members(spec, cuda)
     for r, (m, h) in enumerate(zip(ms, host)):
         ps, vz = _solo(m, h, [table[st][r] for st in range(nsteps)], nsteps, cuda)
