@@ -3775,8 +3775,16 @@ int r2d3c_ensemble_run(r2d3c_ensemble *e, const r2d3c_forcing *forcing, int n_st
         int rc = ens_capture_all(e, ftab);
         if (rc) return rc;
       }
+      // each class's graph launches in one go, smallest size first (on the
+      // 24/12/4 mix 0.94-0.96 ms per step against 1.04 largest first and
+      // 1.03 with the classes' launches interleaved)
+      std::vector<EnsClass *> by_size;
+      for (auto &k : e->cls) by_size.push_back(&k);
+      std::stable_sort(by_size.begin(), by_size.end(),
+                       [](const EnsClass *a, const EnsClass *b) { return a->n < b->n; });
       HIPCHK(hipEventRecord(e->fork, e->es));
-      for (auto &k : e->cls) {
+      for (EnsClass *kp : by_size) {
+        EnsClass &k = *kp;
         HIPCHK(hipStreamWaitEvent(k.st, e->fork, 0));
         int st = 1;
         for (; k.xk && n_steps - st >= e->kblock; st += e->kblock)
